@@ -1,0 +1,220 @@
+/*
+ * jaxpm_hip.h -- C ABI of libjaxpm_hip.so, the MI355X (gfx950) particle-mesh
+ * step behind the jaxpm_amd Python package.
+ *
+ * The reference (JaxPM) is pure Python over JAX/XLA; it has no FFI of its own.
+ * Each entry point below replaces the XLA computation that a reference
+ * function lowers to on one device, cited as reference file:line.  A JAX
+ * binding would wrap these as XLA-FFI custom calls (one handler per entry
+ * point); the shipped binding is ctypes from jaxpm_amd/_lib.py.
+ *
+ * Conventions
+ *   - Every pointer is device memory owned by the caller, except where noted.
+ *   - `stream` is a hipStream_t passed as void* (0 = default stream).  All
+ *     work is stream-ordered; no entry point synchronises the host unless
+ *     its comment says so.
+ *   - Mesh layout: C order [nx][ny][nz] float32 (z fastest), as the
+ *     reference's jnp arrays.  Particle arrays: [n][3] float32 in grid units
+ *     (cell = 1).  Half spectra: C order [nx][ny][nz/2+1] complex64
+ *     (interleaved re, im), the hipFFT R2C layout.
+ *   - Return value: 0 on success, a negative JPM_E* code on failure; the
+ *     message of the last failure on the calling thread is jpm_last_error().
+ *     No C++ exception crosses the ABI.
+ *   - Re-entrant; the FFT plan and wave-number table caches are mutex-guarded.
+ */
+#ifndef JAXPM_HIP_H
+#define JAXPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JPM_ABI_VERSION 1
+
+#define JPM_OK 0
+#define JPM_E_INVALID -1   /* bad shape, size, flag or null pointer */
+#define JPM_E_HIP -2       /* HIP runtime error */
+#define JPM_E_FFT -3       /* hipFFT error */
+#define JPM_E_NOMEM -4     /* workspace too small or allocation failed */
+
+/* paint / read flags */
+#define JPM_RELATIVE 1u      /* positions are displacements from the particle
+                                lattice (i,j,k) of shape (nx,ny,nz), n == nx*ny*nz
+                                (painting_utils.enmesh semantics) */
+#define JPM_DETERMINISTIC 2u /* bitwise-reproducible paint (fixed-point sums) */
+#define JPM_ZERO_MESH 4u     /* clear the output mesh before painting */
+
+/* k-space flags */
+#define JPM_K_GRAD_EXACT 1u  /* gradient_kernel order=0 (i k, len//2 zeroed)
+                                instead of the default 4-point order=1 */
+#define JPM_K_LAPL_FD 2u     /* invlaplace_kernel(fd=True) */
+#define JPM_K_DECONV 4u      /* multiply by cic_compensation (opt-in) */
+
+/* Library lifetime and diagnostics. */
+int jpm_init(void);
+int jpm_shutdown(void);                 /* frees cached FFT plans and tables */
+int jpm_abi_version(void);
+const char *jpm_last_error(void);
+
+/* ---------------------------------------------------------------- painting */
+
+/* Bytes of workspace jpm_cic_paint_f32 needs for this call (0 unless
+ * JPM_DETERMINISTIC: 8 bytes per mesh cell). */
+size_t jpm_cic_paint_workspace_size(int nx, int ny, int nz, int64_t n,
+                                    uint32_t flags);
+
+/* CIC deposit of n particles into mesh[nx][ny][nz] (accumulates unless
+ * JPM_ZERO_MESH).  weight: per-particle weights or NULL for the scalar
+ * `wscalar`.
+ * Replaces painting.py:15-45 (_cic_paint_impl: 8 corners, weight
+ * prod(1-|x-c|), int32 cast then periodic mod, scatter_add) and, with
+ * JPM_RELATIVE, painting.py:161-189 + painting_utils.py:28-131 (enmesh +
+ * chunked scatter; corners that round onto index L are dropped). */
+int jpm_cic_paint_f32(void *stream, int nx, int ny, int nz, const float *pos,
+                      int64_t n, const float *weight, float wscalar,
+                      float *mesh, uint32_t flags, void *ws, size_t ws_bytes);
+
+/* CIC gather of `ncomp` (1..4) meshes stored `comp_stride` floats apart:
+ * out[p*ncomp + c] = sum_o mesh_c[idx_o(p)] W_o(p).
+ * Replaces painting.py:78-106 (_cic_read_impl) and, with JPM_RELATIVE,
+ * painting.py:218-236 + painting_utils.py:144-187 (mode='drop' gather);
+ * ncomp = 3 is the fused read of pm.py:54-56. */
+int jpm_cic_read_f32(void *stream, int nx, int ny, int nz, const float *mesh,
+                     int ncomp, int64_t comp_stride, const float *pos,
+                     int64_t n, float *out, uint32_t flags);
+
+/* Bytes of workspace jpm_cic_paint_multi_f32 needs (0 unless deterministic). */
+size_t jpm_cic_paint_multi_workspace_size(int nx, int ny, int nz, int nc,
+                                          uint32_t flags);
+
+/* nc-channel paint (nc = 1..4): channel c deposits weights[p*nc + c] into
+ * mesh + c*nx*ny*nz.  The transpose of the nc-component jpm_cic_read_f32;
+ * it is the mesh cotangent of cic_read in the hand-written adjoint (the
+ * reference gets it from XLA's transpose of the gather, painting.py:104-106). */
+int jpm_cic_paint_multi_f32(void *stream, int nx, int ny, int nz,
+                            const float *pos, int64_t n, const float *weights,
+                            int nc, float *mesh, uint32_t flags, void *ws,
+                            size_t ws_bytes);
+
+/* Position cotangent of CIC sampling:
+ *   gpos[p][e] (+)= w_p * sum_c cot[p*ncomp+c] * sum_o mesh_c[idx_o] dW_o/dx_e
+ * with dW/dx from d|u|/du = +1 at u >= 0 (the jnp.abs convention).  cot NULL
+ * means 1; weight NULL means the scalar wscalar; accumulate != 0 adds to gpos.
+ * The position gradient of cic_read (painting.py:78-106) and cic_paint
+ * (painting.py:15-45) under jax.grad. */
+int jpm_cic_grad_pos_f32(void *stream, int nx, int ny, int nz,
+                         const float *mesh, int ncomp, int64_t comp_stride,
+                         const float *pos, int64_t n, const float *cot,
+                         const float *weight, float wscalar, float *gpos,
+                         uint32_t flags, int accumulate);
+
+/* ------------------------------------------------------------------- FFT */
+
+/* R2C of `batch` meshes (stride nx*ny*nz floats) into half spectra (stride
+ * nx*ny*(nz/2+1) complex), unnormalised.  Replaces the forward c2c
+ * jaxdecomp.pfft3d of distributed.py:37-38 for real input. */
+int jpm_fft_r2c_f32(void *stream, int nx, int ny, int nz, int batch,
+                    const float *in, void *out);
+
+/* C2R of `batch` half spectra, unnormalised (the caller folds 1/M into the
+ * k-space multiplier).  `in` is clobbered.  Replaces pifft3d(...).real of
+ * distributed.py:41-42 for Hermitian spectra. */
+int jpm_fft_c2r_f32(void *stream, int nx, int ny, int nz, int batch,
+                    void *in, float *out);
+
+/* Complex-to-complex transform of `batch` [nx][ny][nz] complex64 arrays,
+ * direction -1 forward / +1 inverse, unnormalised; in == out allowed.
+ * The c2c jaxdecomp.pfft3d / pifft3d of distributed.py:37-42 for complex
+ * input. */
+int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch,
+                    const void *in, void *out, int direction);
+
+/* ------------------------------------------------------------ k-space */
+
+/* out3[d] = scale * (-g_d(k)) * G(k) * Lr(k) * [C(k)] * delta_k for d=0,1,2
+ * (three consecutive half spectra).  G = invlaplace_kernel, g_d =
+ * gradient_kernel, Lr = longrange_kernel(r_split), C = cic_compensation
+ * (JPM_K_DECONV only).  Replaces pm.py:49-56 with kernels.py:41-115. */
+int jpm_kspace_force_f32(void *stream, int nx, int ny, int nz,
+                         const void *delta_k, void *out3, float r_split,
+                         float scale, uint32_t flags);
+
+/* Multiply a half spectrum in place by scale * C(k) (cic_compensation,
+ * kernels.py:118-136): the k-space step of compensate_cic,
+ * painting.py:263-275. */
+int jpm_kspace_compensate_f32(void *stream, int nx, int ny, int nz,
+                              void *field_k, float scale);
+
+/* field_k *= scale * sqrt(pk(|k| n/L) * M / V) with pk the linear
+ * interpolation of the table (k_tab[ntab] ascending, pk_tab[ntab]) clamped at
+ * the ends (jnp.interp).  k_tab/pk_tab are device arrays.  Replaces
+ * pm.py:135-143 (linear_field). */
+int jpm_kspace_linear_field_f32(void *stream, int nx, int ny, int nz,
+                                void *field_k, float Lx, float Ly, float Lz,
+                                const float *k_tab, const float *pk_tab,
+                                int ntab, float scale);
+
+/* Six half spectra scale * g_i g_j G(k) delta_k for (i,j) in
+ * (0,0),(1,1),(2,2),(0,1),(0,2),(1,2): the 2LPT shear terms of pm.py:89-109. */
+int jpm_kspace_shear6_f32(void *stream, int nx, int ny, int nz,
+                          const void *delta_k, void *out6, float scale,
+                          uint32_t flags);
+
+/* delta2 = sum_{i<j} (phi_ii phi_jj - phi_ij^2) from six meshes stored
+ * `M` floats apart in the order of jpm_kspace_shear6_f32, accumulated in the
+ * order of pm.py:92-109. */
+int jpm_lpt2_source_f32(void *stream, int64_t M, const float *phi6,
+                        float *delta2);
+
+/* Adjoint of jpm_kspace_force_f32 as a real-linear map of half spectra:
+ * out = scale * sum_d conj(-g_d G Lr [C]) in3[d].  Composed with R2C/C2R it
+ * is the transpose of the force field operator of pm.py:49-56. */
+int jpm_kspace_force_adjoint_f32(void *stream, int nx, int ny, int nz,
+                                 const void *in3, void *out, float r_split,
+                                 float scale, uint32_t flags);
+
+/* Transpose of jpm_kspace_shear6_f32 (the multipliers are real and even):
+ * out = scale * sum_m (-g_a g_b G) in6[m]. */
+int jpm_kspace_shear6_adjoint_f32(void *stream, int nx, int ny, int nz,
+                                  const void *in6, void *out, float scale,
+                                  uint32_t flags);
+
+/* Vector-Jacobian product of jpm_lpt2_source_f32: out6[m] = dL/dphi_m given
+ * gbar = dL/d(delta2) (the adjoint of pm.py:92-109). */
+int jpm_lpt2_source_vjp_f32(void *stream, int64_t M, const float *phi6,
+                            const float *gbar, float *out6);
+
+/* Hermitian part of a full c2c spectrum [nx][ny][nz] on the half grid,
+ * (X(k) + conj(X(-k)))/2, so that C2R(half)/M == Re(ifftn(X)) for any X
+ * (ifft3d of distributed.py:41-42; pm_forces with a complex delta,
+ * pm.py:46-47). */
+int jpm_spectrum_full_to_half_f32(void *stream, int nx, int ny, int nz,
+                                  const void *full, void *half);
+
+/* Full c2c spectrum of a real field from its half spectrum (fft3d of
+ * distributed.py:37-38 for real input). */
+int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz,
+                                  const void *half, void *full);
+
+/* ------------------------------------------------------------- pipeline */
+
+/* Workspace bytes for jpm_pm_forces_f32 on this mesh. */
+size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, uint32_t flags);
+
+/* forces[n][3] = pm_forces(pos): paint (or use delta_k if non-NULL), R2C,
+ * jpm_kspace_force_f32, 3 C2R, 3-component read.  `scale` multiplies the
+ * result (1 for pm_forces, 1.5*Omega_m for the ODE).  paint flags
+ * (JPM_RELATIVE, JPM_DETERMINISTIC) in the low byte, k-space flags shifted
+ * left by 8.  Replaces pm.py:12-58. */
+int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos,
+                      int64_t n, const void *delta_k, float *forces,
+                      float r_split, float scale, uint32_t flags, void *ws,
+                      size_t ws_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JAXPM_HIP_H */

@@ -1,0 +1,122 @@
+"""ctypes binding of libjaxpm_hip.so (C ABI declared in include/jaxpm_hip.h).
+
+Loading fails loudly when the library is not built; there is no CPU
+fallback anywhere in the package.
+"""
+import ctypes
+import os
+import threading
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libjaxpm_hip.so")
+
+# Flags, mirrored from include/jaxpm_hip.h.
+RELATIVE = 1
+DETERMINISTIC = 2
+ZERO_MESH = 4
+K_GRAD_EXACT = 1
+K_LAPL_FD = 2
+K_DECONV = 4
+
+E_INVALID = -1
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_i64 = ctypes.c_int64
+_u32 = ctypes.c_uint32
+_f = ctypes.c_float
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "jpm_init": (_i, []),
+    "jpm_shutdown": (_i, []),
+    "jpm_abi_version": (_i, []),
+    "jpm_last_error": (ctypes.c_char_p, []),
+    "jpm_cic_paint_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
+    "jpm_cic_paint_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _vp, _u32, _vp, _sz]),
+    "jpm_cic_read_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _u32]),
+    "jpm_cic_paint_multi_workspace_size": (_sz, [_i, _i, _i, _i, _u32]),
+    "jpm_cic_paint_multi_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _i, _vp, _u32, _vp, _sz]),
+    "jpm_cic_grad_pos_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _vp, _f, _vp, _u32, _i]),
+    "jpm_fft_r2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "jpm_fft_c2r_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
+    "jpm_fft_c2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i]),
+    "jpm_kspace_force_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
+    "jpm_kspace_force_adjoint_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
+    "jpm_kspace_compensate_f32": (_i, [_vp, _i, _i, _i, _vp, _f]),
+    "jpm_kspace_linear_field_f32": (_i, [_vp, _i, _i, _i, _vp, _f, _f, _f, _vp, _vp, _i, _f]),
+    "jpm_kspace_shear6_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _u32]),
+    "jpm_lpt2_source_f32": (_i, [_vp, _i64, _vp, _vp]),
+    "jpm_kspace_shear6_adjoint_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _u32]),
+    "jpm_lpt2_source_vjp_f32": (_i, [_vp, _i64, _vp, _vp, _vp]),
+    "jpm_spectrum_full_to_half_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "jpm_spectrum_half_to_full_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_pm_forces_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class JpmError(RuntimeError):
+    """A libjaxpm_hip call failed on the device side."""
+
+
+def load():
+    """Load the shared library (no device access); raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is not built; run `python -m jaxpm_amd.build` "
+                    "(jaxpm_amd has no CPU fallback)")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = lib
+    return _lib
+
+
+def call(name, *args):
+    """Call an entry point and raise ValueError / JpmError on failure."""
+    rc = getattr(load(), name)(*args)
+    if rc != 0:
+        msg = load().jpm_last_error().decode(errors="replace")
+        if rc == E_INVALID:
+            raise ValueError(f"{name}: {msg}")
+        raise JpmError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def size(name, *args):
+    return int(getattr(load(), name)(*args))
+
+
+_device_ready = False
+
+
+def device():
+    """The HIP device the library runs on; raises if none is available."""
+    global _device_ready
+    if not torch.cuda.is_available():
+        raise RuntimeError("jaxpm_amd needs a ROCm GPU (torch.cuda.is_available() is False); "
+                           "there is no CPU fallback")
+    if not _device_ready:
+        call("jpm_init")
+        _device_ready = True
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    """Raw hipStream_t of torch's current stream."""
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None

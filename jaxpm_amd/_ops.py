@@ -1,0 +1,261 @@
+"""Thin tensor-level wrappers over the C ABI (one function per entry point).
+
+Every wrapper takes contiguous float32 / complex64 device tensors, allocates
+its outputs with torch on the same device and launches on torch's current
+stream.  Shapes are validated here and again by the library.
+"""
+import torch
+
+from . import _lib
+from ._lib import call, ptr, size, stream
+
+
+def half_shape(shape):
+    nx, ny, nz = shape
+    return (nx, ny, nz // 2 + 1)
+
+
+def _f32(t, name):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32, got {t.dtype}")
+    return t.contiguous()
+
+
+def _c64(t, name):
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a device tensor")
+    if t.dtype != torch.complex64:
+        raise TypeError(f"{name} must be complex64, got {t.dtype}")
+    return t.contiguous()
+
+
+def _dims(shape):
+    if len(shape) != 3:
+        raise ValueError(f"mesh shape must have 3 dimensions, got {tuple(shape)}")
+    return [int(s) for s in shape]
+
+
+# ---------------------------------------------------------------- painting
+
+def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=False, zero=False):
+    """mesh (nx,ny,nz) f32 updated in place with CIC deposits of pos (n,3)."""
+    mesh = _f32(mesh, "mesh")
+    pos = _f32(pos, "positions").reshape(-1, 3)
+    nx, ny, nz = _dims(mesh.shape)
+    n = pos.shape[0]
+    if weight is not None:
+        weight = _f32(weight, "weight").reshape(-1)
+        if weight.numel() != n:
+            raise ValueError(f"weight has {weight.numel()} entries for {n} particles")
+    flags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
+        (_lib.ZERO_MESH if zero else 0)
+    wsb = size("jpm_cic_paint_workspace_size", nx, ny, nz, n, flags)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=mesh.device) if wsb else None
+    call("jpm_cic_paint_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weight), float(wscalar), ptr(mesh),
+         flags, ptr(ws), wsb)
+    return mesh
+
+
+def paint_multi(shape, pos, weights, relative=False, deterministic=False):
+    """nc meshes (nc,nx,ny,nz): channel c painted with weights[:, c]."""
+    nx, ny, nz = _dims(shape)
+    pos = _f32(pos, "positions").reshape(-1, 3)
+    weights = _f32(weights, "weights")
+    n = pos.shape[0]
+    weights = weights.reshape(n, -1)
+    nc = weights.shape[1]
+    out = torch.empty((nc, nx, ny, nz), dtype=torch.float32, device=pos.device)
+    flags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | _lib.ZERO_MESH
+    wsb = size("jpm_cic_paint_multi_workspace_size", nx, ny, nz, nc, flags)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device) if wsb else None
+    call("jpm_cic_paint_multi_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weights), nc, ptr(out), flags,
+         ptr(ws), wsb)
+    return out
+
+
+def read(meshes, pos, relative=False):
+    """meshes (nc,nx,ny,nz) or (nx,ny,nz) -> values (n,nc) or (n,)."""
+    meshes = _f32(meshes, "mesh")
+    single = meshes.dim() == 3
+    m4 = meshes.unsqueeze(0) if single else meshes
+    nc = m4.shape[0]
+    nx, ny, nz = _dims(m4.shape[1:])
+    pos = _f32(pos, "positions").reshape(-1, 3)
+    n = pos.shape[0]
+    out = torch.empty((n, nc), dtype=torch.float32, device=pos.device)
+    call("jpm_cic_read_f32", stream(), nx, ny, nz, ptr(m4), nc, nx * ny * nz, ptr(pos), n, ptr(out),
+         _lib.RELATIVE if relative else 0)
+    return out.reshape(n) if single else out
+
+
+def grad_pos(meshes, pos, cot=None, weight=None, wscalar=1.0, relative=False, out=None):
+    """Position cotangent (n,3) of CIC sampling (see jpm_cic_grad_pos_f32)."""
+    meshes = _f32(meshes, "mesh")
+    m4 = meshes.unsqueeze(0) if meshes.dim() == 3 else meshes
+    nc = m4.shape[0]
+    nx, ny, nz = _dims(m4.shape[1:])
+    pos = _f32(pos, "positions").reshape(-1, 3)
+    n = pos.shape[0]
+    if cot is not None:
+        cot = _f32(cot, "cotangent").reshape(n, nc)
+    if weight is not None:
+        weight = _f32(weight, "weight").reshape(n)
+    acc = out is not None
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float32, device=pos.device)
+    call("jpm_cic_grad_pos_f32", stream(), nx, ny, nz, ptr(m4), nc, nx * ny * nz, ptr(pos), n, ptr(cot),
+         ptr(weight), float(wscalar), ptr(out), _lib.RELATIVE if relative else 0, int(acc))
+    return out
+
+
+# --------------------------------------------------------------------- FFT
+
+def r2c(x):
+    """(…, nx, ny, nz) f32 -> (…, nx, ny, nz//2+1) c64, unnormalised."""
+    x = _f32(x, "field")
+    nx, ny, nz = _dims(x.shape[-3:])
+    batch = x.numel() // (nx * ny * nz)
+    out = torch.empty((*x.shape[:-3], *half_shape((nx, ny, nz))), dtype=torch.complex64, device=x.device)
+    call("jpm_fft_r2c_f32", stream(), nx, ny, nz, batch, ptr(x), ptr(out))
+    return out
+
+
+def c2r(xk, shape, clobber=False):
+    """Half spectra -> real meshes of `shape` (unnormalised).  The input is
+    copied first unless ``clobber`` (the transform overwrites its input)."""
+    xk = _c64(xk, "spectrum")
+    nx, ny, nz = _dims(shape)
+    if tuple(xk.shape[-3:]) != half_shape((nx, ny, nz)):
+        raise ValueError(f"spectrum shape {tuple(xk.shape)} does not match mesh {tuple(shape)}")
+    if not clobber:
+        xk = xk.clone()
+    batch = xk.numel() // xk.shape[-3:].numel()
+    out = torch.empty((*xk.shape[:-3], nx, ny, nz), dtype=torch.float32, device=xk.device)
+    call("jpm_fft_c2r_f32", stream(), nx, ny, nz, batch, ptr(xk), ptr(out))
+    return out
+
+
+def c2c(xk, inverse=False):
+    """Unnormalised complex-to-complex transform of (…, nx, ny, nz) c64."""
+    xk = _c64(xk, "spectrum")
+    nx, ny, nz = _dims(xk.shape[-3:])
+    batch = xk.numel() // (nx * ny * nz)
+    out = torch.empty_like(xk)
+    call("jpm_fft_c2c_f32", stream(), nx, ny, nz, batch, ptr(xk), ptr(out), 1 if inverse else -1)
+    return out
+
+
+def full_to_half(xk):
+    xk = _c64(xk, "spectrum")
+    nx, ny, nz = _dims(xk.shape)
+    out = torch.empty(half_shape((nx, ny, nz)), dtype=torch.complex64, device=xk.device)
+    call("jpm_spectrum_full_to_half_f32", stream(), nx, ny, nz, ptr(xk), ptr(out))
+    return out
+
+
+def half_to_full(hk, shape):
+    hk = _c64(hk, "spectrum")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty((nx, ny, nz), dtype=torch.complex64, device=hk.device)
+    call("jpm_spectrum_half_to_full_f32", stream(), nx, ny, nz, ptr(hk), ptr(out))
+    return out
+
+
+# ----------------------------------------------------------------- k-space
+
+def kflags(order=1, fd=False, deconv=False):
+    return (_lib.K_GRAD_EXACT if order == 0 else 0) | (_lib.K_LAPL_FD if fd else 0) | \
+        (_lib.K_DECONV if deconv else 0)
+
+
+def kspace_force(dk, shape, r_split=0.0, scale=1.0, flags=0):
+    dk = _c64(dk, "delta_k")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty((3, *half_shape(shape)), dtype=torch.complex64, device=dk.device)
+    call("jpm_kspace_force_f32", stream(), nx, ny, nz, ptr(dk), ptr(out), float(r_split), float(scale), flags)
+    return out
+
+
+def kspace_force_adjoint(in3, shape, r_split=0.0, scale=1.0, flags=0):
+    in3 = _c64(in3, "spectra")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty(half_shape(shape), dtype=torch.complex64, device=in3.device)
+    call("jpm_kspace_force_adjoint_f32", stream(), nx, ny, nz, ptr(in3), ptr(out), float(r_split), float(scale),
+         flags)
+    return out
+
+
+def kspace_compensate(fk, shape, scale=1.0):
+    fk = _c64(fk, "spectrum")
+    nx, ny, nz = _dims(shape)
+    call("jpm_kspace_compensate_f32", stream(), nx, ny, nz, ptr(fk), float(scale))
+    return fk
+
+
+def kspace_linear_field(fk, shape, box, ktab, pktab, scale=1.0):
+    fk = _c64(fk, "spectrum")
+    nx, ny, nz = _dims(shape)
+    ktab = _f32(ktab, "k table")
+    pktab = _f32(pktab, "pk table")
+    if ktab.numel() != pktab.numel():
+        raise ValueError("k and P(k) tables differ in length")
+    call("jpm_kspace_linear_field_f32", stream(), nx, ny, nz, ptr(fk), float(box[0]), float(box[1]), float(box[2]),
+         ptr(ktab), ptr(pktab), ktab.numel(), float(scale))
+    return fk
+
+
+def kspace_shear6(dk, shape, scale=1.0, flags=0):
+    dk = _c64(dk, "delta_k")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty((6, *half_shape(shape)), dtype=torch.complex64, device=dk.device)
+    call("jpm_kspace_shear6_f32", stream(), nx, ny, nz, ptr(dk), ptr(out), float(scale), flags)
+    return out
+
+
+def kspace_shear6_adjoint(in6, shape, scale=1.0, flags=0):
+    in6 = _c64(in6, "spectra")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty(half_shape(shape), dtype=torch.complex64, device=in6.device)
+    call("jpm_kspace_shear6_adjoint_f32", stream(), nx, ny, nz, ptr(in6), ptr(out), float(scale), flags)
+    return out
+
+
+def lpt2_source(phi6):
+    phi6 = _f32(phi6, "shear meshes")
+    m = phi6[0].numel()
+    out = torch.empty(phi6.shape[1:], dtype=torch.float32, device=phi6.device)
+    call("jpm_lpt2_source_f32", stream(), m, ptr(phi6), ptr(out))
+    return out
+
+
+def lpt2_source_vjp(phi6, gbar):
+    phi6 = _f32(phi6, "shear meshes")
+    gbar = _f32(gbar, "cotangent")
+    m = phi6[0].numel()
+    out = torch.empty_like(phi6)
+    call("jpm_lpt2_source_vjp_f32", stream(), m, ptr(phi6), ptr(gbar), ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- pipeline
+
+def pm_forces_fused(pos, shape, delta_k=None, r_split=0.0, scale=1.0, relative=False, deterministic=False,
+                    kf=0, out=None, ws=None):
+    """forces (n,3) through the single-call pipeline jpm_pm_forces_f32."""
+    pos = _f32(pos, "positions").reshape(-1, 3)
+    nx, ny, nz = _dims(shape)
+    n = pos.shape[0]
+    pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0)
+    flags = pflags | (kf << 8)
+    wsb = size("jpm_pm_forces_workspace_size", nx, ny, nz, flags)
+    if ws is None or ws.numel() < wsb:
+        ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device)
+    if delta_k is not None:
+        delta_k = _c64(delta_k, "delta_k")
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float32, device=pos.device)
+    call("jpm_pm_forces_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(delta_k), ptr(out), float(r_split),
+         float(scale), flags, ptr(ws), ws.numel())
+    return out

@@ -1,0 +1,237 @@
+// extern "C" entry points of libjaxpm_hip.so (declared in jaxpm_hip.h).
+// Each wrapper clears the thread's error, forwards to the jpm:: routine and
+// turns any C++ exception into JPM_E_NOMEM / JPM_E_INVALID.
+#include <exception>
+#include <new>
+#include <string>
+
+#include "common.hpp"
+
+namespace jpm {
+
+static thread_local std::string t_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+void clear_error() { t_err.clear(); }
+const char *last_error() { return t_err.c_str(); }
+
+int cic_paint(hipStream_t, int, int, int, const float *, int64_t, const float *, float, float *, uint32_t,
+              void *, size_t);
+int cic_read(hipStream_t, int, int, int, const float *, int, int64_t, const float *, int64_t, float *,
+             uint32_t);
+int fft_r2c(hipStream_t, int, int, int, int, const float *, void *);
+int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
+int fft_c2c(hipStream_t, int, int, int, int, const void *, void *, int);
+void free_plans();
+void free_tabs();
+int kspace_force(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
+int kspace_compensate(hipStream_t, int, int, int, void *, float);
+int kspace_linear_field(hipStream_t, int, int, int, void *, float, float, float, const float *, const float *,
+                        int, float);
+int kspace_shear6(hipStream_t, int, int, int, const void *, void *, float, uint32_t);
+int lpt2_source(hipStream_t, int64_t, const float *, float *);
+int cic_paint_multi(hipStream_t, int, int, int, const float *, int64_t, const float *, int, float *, uint32_t, void *,
+                    size_t);
+int cic_grad_pos(hipStream_t, int, int, int, const float *, int, int64_t, const float *, int64_t, const float *,
+                 const float *, float, float *, uint32_t, int);
+int kspace_force_adjoint(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
+int spectrum_full_to_half(hipStream_t, int, int, int, const void *, void *);
+int kspace_shear6_adjoint(hipStream_t, int, int, int, const void *, void *, float, uint32_t);
+int lpt2_source_vjp(hipStream_t, int64_t, const float *, const float *, float *);
+int spectrum_half_to_full(hipStream_t, int, int, int, const void *, void *);
+
+// Workspace layout of pm_forces: [3 meshes][1 half spectrum][3 half
+// spectra][deterministic-paint scratch].  The painted mesh lives in the first
+// of the three force meshes; it is dead once the R2C has read it.
+struct ForceWs {
+  size_t meshes, spec, spec3, paint, total;
+};
+
+static ForceWs force_ws(int nx, int ny, int nz, uint32_t flags) {
+  const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
+  ForceWs w;
+  w.meshes = 0;
+  w.spec = w.meshes + align_up(3 * m * sizeof(float));
+  w.spec3 = w.spec + align_up(h * 8);
+  w.paint = w.spec3 + align_up(3 * h * 8);
+  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? jpm_cic_paint_workspace_size(nx, ny, nz, int64_t(m), flags) : 0);
+  return w;
+}
+
+static int pm_forces(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
+                     float *forces, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const ForceWs w = force_ws(nx, ny, nz, pflags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "pm_forces needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  JPM_REQUIRE(forces != nullptr, "forces is null");
+  char *base = static_cast<char *>(ws);
+  float *meshes = reinterpret_cast<float *>(base + w.meshes);
+  void *spec = base + w.spec;
+  void *spec3 = base + w.spec3;
+  const int64_t m = cells(nx, ny, nz);
+  if (!delta_k) {
+    rc = cic_paint(s, nx, ny, nz, pos, n, nullptr, 1.0f, meshes, (pflags & ~JPM_ZERO_MESH) | JPM_ZERO_MESH,
+                   base + w.paint, w.total - w.paint);
+    if (rc) return rc;
+    rc = fft_r2c(s, nx, ny, nz, 1, meshes, spec);
+    if (rc) return rc;
+    delta_k = spec;
+  }
+  rc = kspace_force(s, nx, ny, nz, delta_k, spec3, r_split, float(double(scale) / double(m)), kflags);
+  if (rc) return rc;
+  rc = fft_c2r(s, nx, ny, nz, 3, spec3, meshes);
+  if (rc) return rc;
+  return cic_read(s, nx, ny, nz, meshes, 3, m, pos, n, forces, pflags & JPM_RELATIVE);
+}
+
+}  // namespace jpm
+
+using namespace jpm;
+
+#define JPM_GUARD(expr)                                                         \
+  do {                                                                          \
+    clear_error();                                                              \
+    try {                                                                       \
+      return (expr);                                                            \
+    } catch (const std::bad_alloc &) {                                          \
+      return fail(JPM_E_NOMEM, "host allocation failed");                       \
+    } catch (const std::exception &ex) {                                        \
+      return fail(JPM_E_INVALID, "internal error: %s", ex.what());              \
+    } catch (...) {                                                             \
+      return fail(JPM_E_INVALID, "internal error");                             \
+    }                                                                           \
+  } while (0)
+
+extern "C" {
+
+int jpm_init(void) {
+  clear_error();
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(JPM_E_HIP, "no HIP device available");
+  return JPM_OK;
+}
+
+int jpm_shutdown(void) {
+  clear_error();
+  free_plans();
+  free_tabs();
+  return JPM_OK;
+}
+
+int jpm_abi_version(void) { return JPM_ABI_VERSION; }
+
+const char *jpm_last_error(void) { return last_error(); }
+
+size_t jpm_cic_paint_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
+  (void)n;
+  if (!(flags & JPM_DETERMINISTIC) || nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return align_up(size_t(cells(nx, ny, nz)) * 8) + 256;
+}
+
+int jpm_cic_paint_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
+                      float wscalar, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(cic_paint(as_stream(stream), nx, ny, nz, pos, n, weight, wscalar, mesh, flags, ws, ws_bytes));
+}
+
+int jpm_cic_read_f32(void *stream, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+                     const float *pos, int64_t n, float *out, uint32_t flags) {
+  JPM_GUARD(cic_read(as_stream(stream), nx, ny, nz, mesh, ncomp, comp_stride, pos, n, out, flags));
+}
+
+int jpm_fft_r2c_f32(void *stream, int nx, int ny, int nz, int batch, const float *in, void *out) {
+  JPM_GUARD(fft_r2c(as_stream(stream), nx, ny, nz, batch, in, out));
+}
+
+int jpm_fft_c2r_f32(void *stream, int nx, int ny, int nz, int batch, void *in, float *out) {
+  JPM_GUARD(fft_c2r(as_stream(stream), nx, ny, nz, batch, in, out));
+}
+
+int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch, const void *in, void *out, int direction) {
+  JPM_GUARD(fft_c2c(as_stream(stream), nx, ny, nz, batch, in, out, direction));
+}
+
+int jpm_kspace_force_f32(void *stream, int nx, int ny, int nz, const void *delta_k, void *out3, float r_split,
+                         float scale, uint32_t flags) {
+  JPM_GUARD(kspace_force(as_stream(stream), nx, ny, nz, delta_k, out3, r_split, scale, flags));
+}
+
+int jpm_kspace_compensate_f32(void *stream, int nx, int ny, int nz, void *field_k, float scale) {
+  JPM_GUARD(kspace_compensate(as_stream(stream), nx, ny, nz, field_k, scale));
+}
+
+int jpm_kspace_linear_field_f32(void *stream, int nx, int ny, int nz, void *field_k, float Lx, float Ly, float Lz,
+                                const float *k_tab, const float *pk_tab, int ntab, float scale) {
+  JPM_GUARD(kspace_linear_field(as_stream(stream), nx, ny, nz, field_k, Lx, Ly, Lz, k_tab, pk_tab, ntab, scale));
+}
+
+int jpm_kspace_shear6_f32(void *stream, int nx, int ny, int nz, const void *delta_k, void *out6, float scale,
+                          uint32_t flags) {
+  JPM_GUARD(kspace_shear6(as_stream(stream), nx, ny, nz, delta_k, out6, scale, flags));
+}
+
+int jpm_lpt2_source_f32(void *stream, int64_t M, const float *phi6, float *delta2) {
+  JPM_GUARD(lpt2_source(as_stream(stream), M, phi6, delta2));
+}
+
+size_t jpm_cic_paint_multi_workspace_size(int nx, int ny, int nz, int nc, uint32_t flags) {
+  if (!(flags & JPM_DETERMINISTIC) || nx <= 0 || ny <= 0 || nz <= 0 || nc <= 0) return 0;
+  return align_up(size_t(nc) * size_t(cells(nx, ny, nz)) * 8) + 256;
+}
+
+int jpm_cic_paint_multi_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n,
+                            const float *weights, int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(cic_paint_multi(as_stream(stream), nx, ny, nz, pos, n, weights, nc, mesh, flags, ws, ws_bytes));
+}
+
+int jpm_cic_grad_pos_f32(void *stream, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+                         const float *pos, int64_t n, const float *cot, const float *weight, float wscalar,
+                         float *gpos, uint32_t flags, int accumulate) {
+  JPM_GUARD(cic_grad_pos(as_stream(stream), nx, ny, nz, mesh, ncomp, comp_stride, pos, n, cot, weight, wscalar, gpos,
+                         flags, accumulate));
+}
+
+int jpm_kspace_force_adjoint_f32(void *stream, int nx, int ny, int nz, const void *in3, void *out, float r_split,
+                                 float scale, uint32_t flags) {
+  JPM_GUARD(kspace_force_adjoint(as_stream(stream), nx, ny, nz, in3, out, r_split, scale, flags));
+}
+
+int jpm_kspace_shear6_adjoint_f32(void *stream, int nx, int ny, int nz, const void *in6, void *out, float scale,
+                                  uint32_t flags) {
+  JPM_GUARD(kspace_shear6_adjoint(as_stream(stream), nx, ny, nz, in6, out, scale, flags));
+}
+
+int jpm_lpt2_source_vjp_f32(void *stream, int64_t M, const float *phi6, const float *gbar, float *out6) {
+  JPM_GUARD(lpt2_source_vjp(as_stream(stream), M, phi6, gbar, out6));
+}
+
+int jpm_spectrum_full_to_half_f32(void *stream, int nx, int ny, int nz, const void *full, void *half) {
+  JPM_GUARD(spectrum_full_to_half(as_stream(stream), nx, ny, nz, full, half));
+}
+
+int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz, const void *half, void *full) {
+  JPM_GUARD(spectrum_half_to_full(as_stream(stream), nx, ny, nz, half, full));
+}
+
+size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return force_ws(nx, ny, nz, flags & 0xffu).total;
+}
+
+int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
+                      float *forces, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(pm_forces(as_stream(stream), nx, ny, nz, pos, n, delta_k, forces, r_split, scale, flags, ws, ws_bytes));
+}
+
+}  // extern "C"

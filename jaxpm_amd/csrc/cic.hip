@@ -1,0 +1,573 @@
+// CIC paint and read kernels for gfx950.
+//
+// Corner maths follows the reference exactly in float32:
+//   absolute (painting.py:21-37): f = floor(x); corner c = f + o; weight
+//     1 - |x - c| per axis, product over axes; index int32(c) mod n.
+//   relative (painting_utils.py:43-69): p = lattice + disp; q = (p + o) mod L
+//     (jnp.remainder: fmod, +L when negative); q = floor(q); nd = p - q;
+//     nd -= rint(nd / L) * L; weight 1 - |nd|; index q, dropped unless
+//     0 <= q < L (scatter drops, gather mode='drop' reads 0).
+// The library is built with -ffp-contract=off so these sequences round as
+// written.
+#include "common.hpp"
+
+namespace jpm {
+
+struct Axis {
+  int i0, i1;    // wrapped indices of the low / high corner
+  float w0, w1;  // CIC weights 1 - |x - c|
+  float d0, d1;  // their derivatives d w / d x (|u|' = +1 at u = 0, as jnp.abs)
+};
+
+__device__ __forceinline__ int wrap_index(int c, int n) {
+  int r = c % n;
+  return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ Axis axis_abs(float x, int n) {
+  Axis a;
+  float f = floorf(x);
+  float f1 = f + 1.0f;
+  const float u0 = x - f, u1 = x - f1;
+  a.w0 = 1.0f - fabsf(u0);
+  a.w1 = 1.0f - fabsf(u1);
+  a.d0 = u0 >= 0.0f ? -1.0f : 1.0f;
+  a.d1 = u1 >= 0.0f ? -1.0f : 1.0f;
+  a.i0 = wrap_index(int(f), n);
+  a.i1 = wrap_index(int(f1), n);
+  return a;
+}
+
+__device__ __forceinline__ float jnp_mod(float x, float L) {
+  float m = fmodf(x, L);
+  if (m != 0.0f && m < 0.0f) m = m + L;
+  return m;
+}
+
+__device__ __forceinline__ void corner_rel(float p, float o, float L, int &idx, float &w, float &dw) {
+  float q = jnp_mod(p + o, L);
+  q = floorf(q);
+  float nd = p - q;
+  nd = nd - rintf(nd / L) * L;
+  // Out-of-range corners (q == L after rounding, or NaN) carry zero weight at
+  // a valid index, which is what dropping them does to a sum.
+  bool ok = (q >= 0.0f) && (q < L);
+  idx = ok ? int(q) : 0;
+  w = ok ? 1.0f - fabsf(nd) : 0.0f;
+  dw = ok ? (nd >= 0.0f ? -1.0f : 1.0f) : 0.0f;
+}
+
+__device__ __forceinline__ Axis axis_rel(float p, float L) {
+  Axis a;
+  corner_rel(p, 0.0f, L, a.i0, a.w0, a.d0);
+  corner_rel(p, 1.0f, L, a.i1, a.w1, a.d1);
+  return a;
+}
+
+struct Cic {
+  uint32_t idx[8];
+  float w[8];
+};
+
+// Corner order painting.py:24-25: 000,100,010,001,110,101,011,111
+// (x is the first axis).
+__device__ __forceinline__ void corners_abs_order(const Axis &ax, const Axis &ay, const Axis &az,
+                                                  int ny, int nz, Cic &c) {
+  const int xs[8] = {0, 1, 0, 0, 1, 1, 0, 1};
+  const int ys[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+  const int zs[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    int ix = xs[o] ? ax.i1 : ax.i0;
+    int iy = ys[o] ? ay.i1 : ay.i0;
+    int iz = zs[o] ? az.i1 : az.i0;
+    float wx = xs[o] ? ax.w1 : ax.w0;
+    float wy = ys[o] ? ay.w1 : ay.w0;
+    float wz = zs[o] ? az.w1 : az.w0;
+    c.idx[o] = (uint32_t(ix) * uint32_t(ny) + uint32_t(iy)) * uint32_t(nz) + uint32_t(iz);
+    c.w[o] = wx * wy * wz;
+  }
+}
+
+// Corner order painting_utils.py:43-45: bit d of the corner number is the
+// offset along axis d.
+__device__ __forceinline__ void corners_bit_order(const Axis &ax, const Axis &ay, const Axis &az,
+                                                  int ny, int nz, Cic &c) {
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+    int ix = bx ? ax.i1 : ax.i0;
+    int iy = by ? ay.i1 : ay.i0;
+    int iz = bz ? az.i1 : az.i0;
+    float wx = bx ? ax.w1 : ax.w0;
+    float wy = by ? ay.w1 : ay.w0;
+    float wz = bz ? az.w1 : az.w0;
+    c.idx[o] = (uint32_t(ix) * uint32_t(ny) + uint32_t(iy)) * uint32_t(nz) + uint32_t(iz);
+    c.w[o] = wx * wy * wz;
+  }
+}
+
+struct Grid {
+  int nx, ny, nz;
+  FastDiv div_nz, div_ny;
+};
+
+template <bool REL>
+__device__ __forceinline__ void particle_axes(const float *__restrict__ pos, uint32_t p, const Grid &g,
+                                              Axis &ax, Axis &ay, Axis &az) {
+  float x = pos[3 * size_t(p) + 0];
+  float y = pos[3 * size_t(p) + 1];
+  float z = pos[3 * size_t(p) + 2];
+  if (REL) {
+    uint32_t t = g.div_nz.div(p);
+    uint32_t k = p - t * uint32_t(g.nz);
+    uint32_t i = g.div_ny.div(t);
+    uint32_t j = t - i * uint32_t(g.ny);
+    // enmesh: base_indices * cell_size + displacements (float32 add).
+    ax = axis_rel(float(i) + x, float(g.nx));
+    ay = axis_rel(float(j) + y, float(g.ny));
+    az = axis_rel(float(k) + z, float(g.nz));
+  } else {
+    ax = axis_abs(x, g.nx);
+    ay = axis_abs(y, g.ny);
+    az = axis_abs(z, g.nz);
+  }
+}
+
+template <bool REL>
+__device__ __forceinline__ void particle_cic(const float *__restrict__ pos, uint32_t p,
+                                             const Grid &g, Cic &c) {
+  Axis ax, ay, az;
+  particle_axes<REL>(pos, p, g, ax, ay, az);
+  if (REL)
+    corners_bit_order(ax, ay, az, g.ny, g.nz, c);
+  else
+    corners_abs_order(ax, ay, az, g.ny, g.nz, c);
+}
+
+// ------------------------------------------------------------------ paint
+
+// 2^e with e the largest exponent such that n * wmax * 2^e < 2^62.  Every
+// particle spreads |w| over its corners (the CIC weights sum to 1), so no
+// cell of the fixed-point sum can leave the int64 range.
+__host__ __device__ inline double fixed_point_scale(float wmax, uint32_t n) {
+  double bound = double(wmax) * double(n > 0 ? n : 1);
+  if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
+  int e;
+  frexp(bound, &e);  // bound < 2^e
+  return ldexp(1.0, 62 - e);
+}
+
+template <bool REL, bool HAS_W>
+__global__ void __launch_bounds__(256)
+paint_atomic_kernel(const float *__restrict__ pos, const float *__restrict__ weight, float wscalar,
+                    uint32_t n, Grid g, float *__restrict__ mesh) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+    float wp = HAS_W ? weight[p] : wscalar;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) unsafeAtomicAdd(&mesh[c.idx[o]], wp * c.w[o]);
+  }
+}
+
+// Deterministic paint: every deposit is rounded to a 64-bit fixed-point
+// integer with a power-of-two scale and summed with integer atomics, which
+// are associative, so the result does not depend on the order in which
+// particles land.  The scale 2^e keeps sum |w| * 2^e below 2^62.
+template <bool REL, bool HAS_W>
+__global__ void __launch_bounds__(256)
+paint_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weight, float wscalar,
+                   uint32_t n, Grid g, const float *__restrict__ wmax_dev, float wmax_host,
+                   unsigned long long *__restrict__ acc) {
+  float wmax = HAS_W ? *wmax_dev : wmax_host;
+  double scale = fixed_point_scale(wmax, n);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+    float wp = HAS_W ? weight[p] : wscalar;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      double v = double(wp * c.w[o]) * scale;
+      long long q = (long long)rint(v);
+      atomicAdd(&acc[c.idx[o]], (unsigned long long)q);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+abs_max_kernel(const float *__restrict__ w, uint32_t n, float *__restrict__ out) {
+  float m = 0.0f;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(w[p]));
+  // wave reduction, then one atomic per wave (values are >= 0, so the
+  // unsigned bit pattern orders like the float)
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int *>(out), __float_as_uint(m));
+}
+
+__global__ void __launch_bounds__(256)
+fixed_to_float_kernel(const unsigned long long *__restrict__ acc, uint32_t m, uint32_t n,
+                      const float *__restrict__ wmax_dev, float wmax_host, int has_w,
+                      int accumulate, float *__restrict__ mesh) {
+  float wmax = has_w ? *wmax_dev : wmax_host;
+  double inv = 1.0 / fixed_point_scale(wmax, n);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+    float v = float(double((long long)acc[i]) * inv);
+    mesh[i] = accumulate ? mesh[i] + v : v;
+  }
+}
+
+__global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long *__restrict__ p, uint32_t m) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) zero_f32_kernel(float *__restrict__ p, uint32_t m) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) p[i] = 0.0f;
+}
+
+// ------------------------------------------------------------------- read
+
+template <bool REL, int NC>
+__global__ void __launch_bounds__(256)
+read_kernel(const float *__restrict__ mesh, uint32_t cstride, const float *__restrict__ pos,
+            uint32_t n, Grid g, float *__restrict__ out) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      const float *m = mesh + size_t(comp) * cstride;
+      float s = 0.0f;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) s += m[c.idx[o]] * c.w[o];
+      out[size_t(p) * NC + comp] = s;
+    }
+  }
+}
+
+// Multi-channel paint: channel c of particle p deposits weight[p*nc + c]
+// into mesh + c*M.  This is the transpose of the nc-component read.
+template <bool REL, int NC>
+__global__ void __launch_bounds__(256)
+paint_multi_atomic_kernel(const float *__restrict__ pos, const float *__restrict__ weight, uint32_t n, Grid g,
+                          uint32_t m, float *__restrict__ mesh) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+      const float wp = weight[size_t(p) * NC + ch];
+      float *mc = mesh + size_t(ch) * m;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) unsafeAtomicAdd(&mc[c.idx[o]], wp * c.w[o]);
+    }
+  }
+}
+
+template <bool REL, int NC>
+__global__ void __launch_bounds__(256)
+paint_multi_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weight, uint32_t n, Grid g,
+                         uint32_t m, const float *__restrict__ wmax_dev, unsigned long long *__restrict__ acc) {
+  const double scale = fixed_point_scale(*wmax_dev, n);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+      const float wp = weight[size_t(p) * NC + ch];
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        long long q = (long long)rint(double(wp * c.w[o]) * scale);
+        atomicAdd(&acc[size_t(ch) * m + c.idx[o]], (unsigned long long)q);
+      }
+    }
+  }
+}
+
+// Gradient of CIC sampling with respect to the particle coordinate:
+//   gpos[p, e] (+)= w_p * sum_c cot[p, c] * sum_o mesh_c[idx_o] dW_o/dx_e
+// cot == NULL means cot = 1.  This is the position cotangent of both
+// cic_read (mesh = the sampled field, cot = output cotangent) and cic_paint
+// (mesh = cotangent of the painted mesh, w = particle weight).
+template <bool REL, int NC>
+__global__ void __launch_bounds__(256)
+grad_pos_kernel(const float *__restrict__ mesh, uint32_t cstride, const float *__restrict__ pos, uint32_t n,
+                Grid g, const float *__restrict__ cot, const float *__restrict__ weight, float wscalar,
+                int accumulate, float *__restrict__ gpos) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Axis ax, ay, az;
+    particle_axes<REL>(pos, p, g, ax, ay, az);
+    const uint32_t ixs[2] = {uint32_t(ax.i0), uint32_t(ax.i1)};
+    const uint32_t iys[2] = {uint32_t(ay.i0), uint32_t(ay.i1)};
+    const uint32_t izs[2] = {uint32_t(az.i0), uint32_t(az.i1)};
+    const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
+    const float dx[2] = {ax.d0, ax.d1}, dy[2] = {ay.d0, ay.d1}, dz[2] = {az.d0, az.d1};
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+    for (int comp = 0; comp < NC; ++comp) {
+      const float *mc = mesh + size_t(comp) * cstride;
+      float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+        const float v = mc[(ixs[bx] * uint32_t(g.ny) + iys[by]) * uint32_t(g.nz) + izs[bz]];
+        sx += v * (dx[bx] * wy[by] * wz[bz]);
+        sy += v * (wx[bx] * dy[by] * wz[bz]);
+        sz += v * (wx[bx] * wy[by] * dz[bz]);
+      }
+      const float cc = cot ? cot[size_t(p) * NC + comp] : 1.0f;
+      gx += cc * sx;
+      gy += cc * sy;
+      gz += cc * sz;
+    }
+    const float wp = weight ? weight[p] : wscalar;
+    gx *= wp;
+    gy *= wp;
+    gz *= wp;
+    float *o = gpos + 3 * size_t(p);
+    if (accumulate) {
+      gx += o[0];
+      gy += o[1];
+      gz += o[2];
+    }
+    o[0] = gx;
+    o[1] = gy;
+    o[2] = gz;
+  }
+}
+
+// ------------------------------------------------------------ host side
+
+static Grid make_grid(int nx, int ny, int nz) {
+  Grid g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.div_nz = FastDiv(uint32_t(nz));
+  g.div_ny = FastDiv(uint32_t(ny));
+  return g;
+}
+
+int zero_f32(hipStream_t s, float *p, int64_t m) {
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, p, uint32_t(m));
+  JPM_LAUNCHED("zero_f32_kernel");
+  return JPM_OK;
+}
+
+int cic_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
+              float wscalar, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  JPM_REQUIRE(n == 0 || pos != nullptr, "positions are null");
+  const bool rel = flags & JPM_RELATIVE;
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(!rel || n == m, "relative painting needs one particle per lattice node (n=%lld, mesh %lld)",
+              (long long)n, (long long)m);
+  if (flags & JPM_ZERO_MESH) {
+    rc = zero_f32(s, mesh, m);
+    if (rc) return rc;
+  }
+  if (n == 0) return JPM_OK;
+  Grid g = make_grid(nx, ny, nz);
+  const unsigned grid = grid_for(n, 256, 256 * 128);
+  const bool hw = weight != nullptr;
+  if (!(flags & JPM_DETERMINISTIC)) {
+#define JPM_PAINT(R, W)                                                                          \
+  hipLaunchKernelGGL((paint_atomic_kernel<R, W>), dim3(grid), dim3(256), 0, s, pos, weight, wscalar, \
+                     uint32_t(n), g, mesh)
+    if (rel) {
+      if (hw) JPM_PAINT(true, true); else JPM_PAINT(true, false);
+    } else {
+      if (hw) JPM_PAINT(false, true); else JPM_PAINT(false, false);
+    }
+#undef JPM_PAINT
+    JPM_LAUNCHED("paint_atomic_kernel");
+    return JPM_OK;
+  }
+  const size_t need = jpm_cic_paint_workspace_size(nx, ny, nz, n, flags);
+  JPM_REQUIRE(ws != nullptr && ws_bytes >= need, "deterministic paint needs %zu workspace bytes, got %zu",
+              need, ws_bytes);
+  auto *acc = static_cast<unsigned long long *>(ws);
+  float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(m) * 8));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, acc, uint32_t(m));
+  JPM_LAUNCHED("zero_u64_kernel");
+  if (hw) {
+    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
+    hipLaunchKernelGGL(abs_max_kernel, dim3(grid), dim3(256), 0, s, weight, uint32_t(n), wmax);
+    JPM_LAUNCHED("abs_max_kernel");
+  }
+  const float wh = fabsf(wscalar);
+#define JPM_PAINT(R, W)                                                                          \
+  hipLaunchKernelGGL((paint_fixed_kernel<R, W>), dim3(grid), dim3(256), 0, s, pos, weight, wscalar,  \
+                     uint32_t(n), g, wmax, wh, acc)
+  if (rel) {
+    if (hw) JPM_PAINT(true, true); else JPM_PAINT(true, false);
+  } else {
+    if (hw) JPM_PAINT(false, true); else JPM_PAINT(false, false);
+  }
+#undef JPM_PAINT
+  JPM_LAUNCHED("paint_fixed_kernel");
+  hipLaunchKernelGGL(fixed_to_float_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, acc, uint32_t(m),
+                     uint32_t(n), wmax, wh, int(hw), int(!(flags & JPM_ZERO_MESH)), mesh);
+  JPM_LAUNCHED("fixed_to_float_kernel");
+  return JPM_OK;
+}
+
+int cic_read(hipStream_t s, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+             const float *pos, int64_t n, float *out, uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(ncomp >= 1 && ncomp <= 4, "ncomp=%d must be 1..4", ncomp);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(ncomp == 1 || (comp_stride >= m && comp_stride <= kMaxCells * 2),
+              "component stride %lld smaller than the mesh", (long long)comp_stride);
+  JPM_REQUIRE(int64_t(ncomp - 1) * comp_stride + m <= (int64_t(1) << 32),
+              "components exceed the 32-bit offset range");
+  const bool rel = flags & JPM_RELATIVE;
+  JPM_REQUIRE(!rel || n == m, "relative reading needs one particle per lattice node (n=%lld, mesh %lld)",
+              (long long)n, (long long)m);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(mesh && pos && out, "null pointer argument");
+  Grid g = make_grid(nx, ny, nz);
+  const unsigned grid = grid_for(n, 256, 256 * 128);
+  const uint32_t cs = uint32_t(comp_stride);
+#define JPM_READ(R, C)                                                                      \
+  hipLaunchKernelGGL((read_kernel<R, C>), dim3(grid), dim3(256), 0, s, mesh, cs, pos, uint32_t(n), g, \
+                     out)
+#define JPM_READ_NC(R)                 \
+  switch (ncomp) {                     \
+    case 1: JPM_READ(R, 1); break;     \
+    case 2: JPM_READ(R, 2); break;     \
+    case 3: JPM_READ(R, 3); break;     \
+    default: JPM_READ(R, 4); break;    \
+  }
+  if (rel) {
+    JPM_READ_NC(true);
+  } else {
+    JPM_READ_NC(false);
+  }
+#undef JPM_READ_NC
+#undef JPM_READ
+  JPM_LAUNCHED("read_kernel");
+  return JPM_OK;
+}
+
+int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
+                    int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(nc >= 1 && nc <= 4, "channel count %d must be 1..4", nc);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(int64_t(nc) * m <= (int64_t(1) << 32), "channels exceed the 32-bit offset range");
+  const bool rel = flags & JPM_RELATIVE;
+  JPM_REQUIRE(!rel || n == m, "relative painting needs one particle per lattice node (n=%lld, mesh %lld)",
+              (long long)n, (long long)m);
+  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  if (flags & JPM_ZERO_MESH) {
+    rc = zero_f32(s, mesh, nc * m);
+    if (rc) return rc;
+  }
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(pos && weight, "null pointer argument");
+  Grid g = make_grid(nx, ny, nz);
+  const unsigned grid = grid_for(n, 256, 256 * 128);
+  if (!(flags & JPM_DETERMINISTIC)) {
+#define JPM_PM(R, C)                                                                                   \
+  hipLaunchKernelGGL((paint_multi_atomic_kernel<R, C>), dim3(grid), dim3(256), 0, s, pos, weight, uint32_t(n), \
+                     g, uint32_t(m), mesh)
+#define JPM_PM_NC(R)              \
+  switch (nc) {                   \
+    case 1: JPM_PM(R, 1); break;  \
+    case 2: JPM_PM(R, 2); break;  \
+    case 3: JPM_PM(R, 3); break;  \
+    default: JPM_PM(R, 4); break; \
+  }
+    if (rel) {
+      JPM_PM_NC(true);
+    } else {
+      JPM_PM_NC(false);
+    }
+#undef JPM_PM_NC
+#undef JPM_PM
+    JPM_LAUNCHED("paint_multi_atomic_kernel");
+    return JPM_OK;
+  }
+  const size_t need = align_up(size_t(nc) * size_t(m) * 8) + 256;
+  JPM_REQUIRE(ws != nullptr && ws_bytes >= need, "deterministic paint needs %zu workspace bytes, got %zu", need,
+              ws_bytes);
+  auto *acc = static_cast<unsigned long long *>(ws);
+  float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(nc) * size_t(m) * 8));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m));
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
+  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 256 * 128)), dim3(256), 0, s, weight,
+                     uint32_t(n * nc), wmax);
+  JPM_LAUNCHED("abs_max_kernel");
+#define JPM_PM(R, C)                                                                                  \
+  hipLaunchKernelGGL((paint_multi_fixed_kernel<R, C>), dim3(grid), dim3(256), 0, s, pos, weight, uint32_t(n), \
+                     g, uint32_t(m), wmax, acc)
+#define JPM_PM_NC(R)              \
+  switch (nc) {                   \
+    case 1: JPM_PM(R, 1); break;  \
+    case 2: JPM_PM(R, 2); break;  \
+    case 3: JPM_PM(R, 3); break;  \
+    default: JPM_PM(R, 4); break; \
+  }
+  if (rel) {
+    JPM_PM_NC(true);
+  } else {
+    JPM_PM_NC(false);
+  }
+#undef JPM_PM_NC
+#undef JPM_PM
+  JPM_LAUNCHED("paint_multi_fixed_kernel");
+  hipLaunchKernelGGL(fixed_to_float_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m),
+                     uint32_t(n), wmax, 0.0f, 1, int(!(flags & JPM_ZERO_MESH)), mesh);
+  JPM_LAUNCHED("fixed_to_float_kernel");
+  return JPM_OK;
+}
+
+int cic_grad_pos(hipStream_t s, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+                 const float *pos, int64_t n, const float *cot, const float *weight, float wscalar, float *gpos,
+                 uint32_t flags, int accumulate) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(ncomp >= 1 && ncomp <= 4, "ncomp=%d must be 1..4", ncomp);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(ncomp == 1 || comp_stride >= m, "component stride %lld smaller than the mesh", (long long)comp_stride);
+  JPM_REQUIRE(int64_t(ncomp - 1) * comp_stride + m <= (int64_t(1) << 32), "components exceed the 32-bit offset range");
+  const bool rel = flags & JPM_RELATIVE;
+  JPM_REQUIRE(!rel || n == m, "relative mode needs one particle per lattice node (n=%lld, mesh %lld)",
+              (long long)n, (long long)m);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(mesh && pos && gpos, "null pointer argument");
+  Grid g = make_grid(nx, ny, nz);
+  const unsigned grid = grid_for(n, 256, 256 * 128);
+  const uint32_t cs = uint32_t(ncomp == 1 ? m : comp_stride);
+#define JPM_G(R, C)                                                                                          \
+  hipLaunchKernelGGL((grad_pos_kernel<R, C>), dim3(grid), dim3(256), 0, s, mesh, cs, pos, uint32_t(n), g, cot, \
+                     weight, wscalar, accumulate, gpos)
+#define JPM_G_NC(R)              \
+  switch (ncomp) {               \
+    case 1: JPM_G(R, 1); break;  \
+    case 2: JPM_G(R, 2); break;  \
+    case 3: JPM_G(R, 3); break;  \
+    default: JPM_G(R, 4); break; \
+  }
+  if (rel) {
+    JPM_G_NC(true);
+  } else {
+    JPM_G_NC(false);
+  }
+#undef JPM_G_NC
+#undef JPM_G
+  JPM_LAUNCHED("grad_pos_kernel");
+  return JPM_OK;
+}
+
+}  // namespace jpm

@@ -1,0 +1,89 @@
+// Shared host/device helpers for libjaxpm_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+
+#include "jaxpm_hip.h"
+
+namespace jpm {
+
+// Records a formatted message as the calling thread's last error and
+// returns `code`, so entry points can `return fail(JPM_E_INVALID, ...)`.
+int fail(int code, const char *fmt, ...);
+void clear_error();
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define JPM_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return ::jpm::fail(JPM_E_HIP, "%s failed: %s (%s:%d)", #call,            \
+                         hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+#define JPM_REQUIRE(cond, ...)                                                 \
+  do {                                                                         \
+    if (!(cond)) return ::jpm::fail(JPM_E_INVALID, __VA_ARGS__);               \
+  } while (0)
+
+#define JPM_LAUNCHED(name)                                                     \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess)                                                      \
+      return ::jpm::fail(JPM_E_HIP, "launch of %s failed: %s", name,           \
+                         hipGetErrorString(e_));                               \
+  } while (0)
+
+// Largest mesh the 32-bit flat cell index supports (1290^3 cells).
+constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
+
+inline int check_dims(int nx, int ny, int nz) {
+  JPM_REQUIRE(nx > 0 && ny > 0 && nz > 0, "mesh shape (%d,%d,%d) must be positive", nx, ny, nz);
+  JPM_REQUIRE(int64_t(nx) * ny * nz <= kMaxCells, "mesh of %lld cells exceeds the 2^31-1 limit",
+              (long long)(int64_t(nx) * ny * nz));
+  return JPM_OK;
+}
+
+inline int64_t cells(int nx, int ny, int nz) { return int64_t(nx) * ny * nz; }
+inline int64_t half_cells(int nx, int ny, int nz) { return int64_t(nx) * ny * (nz / 2 + 1); }
+
+// Unsigned division by a divisor fixed at launch time with one multiply-high
+// (Granlund & Montgomery 1994, fig. 4.1); exact for all 32-bit numerators and
+// divisors 1 <= d < 2^31.
+struct FastDiv {
+  uint32_t d, m, sh1, sh2;
+  FastDiv() = default;
+  explicit FastDiv(uint32_t div) : d(div) {
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < div) ++l;
+    m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << l) - div)) / div + 1);
+    sh1 = l < 1 ? l : 1;
+    sh2 = l > 1 ? l - 1 : 0;
+  }
+  __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t t = __umulhi(n, m);
+#else
+    uint32_t t = uint32_t((uint64_t(n) * m) >> 32);
+#endif
+    return (t + ((n - t) >> sh1)) >> sh2;
+  }
+};
+
+inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
+
+// Number of blocks for a grid-stride loop over `n` items, capped so every
+// XCD gets many workgroups without launching more than the loop needs.
+inline unsigned grid_for(int64_t n, int block, int64_t cap = 256 * 64) {
+  int64_t g = (n + block - 1) / block;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return unsigned(g);
+}
+
+}  // namespace jpm

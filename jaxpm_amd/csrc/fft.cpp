@@ -1,0 +1,116 @@
+// hipFFT plan cache: one R2C / C2R plan per (shape, direction, batch),
+// created on first use and kept until jpm_shutdown.  Plans are bound to the
+// caller's stream under the cache mutex right before execution, so two
+// threads never race on one plan's stream.
+#include <hipfft/hipfft.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "common.hpp"
+
+namespace jpm {
+
+namespace {
+std::mutex g_mu;
+std::map<std::tuple<int, int, int, int, int>, hipfftHandle> g_plans;
+
+const char *fft_err(hipfftResult r) {
+  switch (r) {
+    case HIPFFT_SUCCESS: return "success";
+    case HIPFFT_INVALID_PLAN: return "invalid plan";
+    case HIPFFT_ALLOC_FAILED: return "allocation failed";
+    case HIPFFT_INVALID_TYPE: return "invalid type";
+    case HIPFFT_INVALID_VALUE: return "invalid value";
+    case HIPFFT_INTERNAL_ERROR: return "internal error";
+    case HIPFFT_EXEC_FAILED: return "exec failed";
+    case HIPFFT_SETUP_FAILED: return "setup failed";
+    case HIPFFT_INVALID_SIZE: return "invalid size";
+    default: return "hipFFT error";
+  }
+}
+
+// Caller holds g_mu.
+int plan_locked(int nx, int ny, int nz, hipfftType type, int batch, hipfftHandle &out) {
+  auto key = std::make_tuple(nx, ny, nz, int(type), batch);
+  auto it = g_plans.find(key);
+  if (it != g_plans.end()) {
+    out = it->second;
+    return JPM_OK;
+  }
+  hipfftHandle h;
+  hipfftResult r = hipfftCreate(&h);
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftCreate: %s", fft_err(r));
+  int n[3] = {nx, ny, nz};
+  const long long M = cells(nx, ny, nz), H = half_cells(nx, ny, nz);
+  const int idist = int(type == HIPFFT_C2R ? H : M);
+  const int odist = int(type == HIPFFT_R2C ? H : M);
+  size_t work = 0;
+  r = hipfftMakePlanMany(h, 3, n, nullptr, 1, idist, nullptr, 1, odist, type, batch, &work);
+  if (r != HIPFFT_SUCCESS) {
+    hipfftDestroy(h);
+    return fail(JPM_E_FFT, "hipfftMakePlanMany(%d,%d,%d, batch %d): %s", nx, ny, nz, batch, fft_err(r));
+  }
+  g_plans.emplace(key, h);
+  out = h;
+  return JPM_OK;
+}
+}  // namespace
+
+int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(in && out, "null FFT buffer");
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipfftHandle h;
+  rc = plan_locked(nx, ny, nz, HIPFFT_R2C, batch, h);
+  if (rc) return rc;
+  hipfftResult r = hipfftSetStream(h, s);
+  if (r == HIPFFT_SUCCESS)
+    r = hipfftExecR2C(h, const_cast<hipfftReal *>(in), static_cast<hipfftComplex *>(out));
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecR2C: %s", fft_err(r));
+  return JPM_OK;
+}
+
+int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(in && out, "null FFT buffer");
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipfftHandle h;
+  rc = plan_locked(nx, ny, nz, HIPFFT_C2R, batch, h);
+  if (rc) return rc;
+  hipfftResult r = hipfftSetStream(h, s);
+  if (r == HIPFFT_SUCCESS) r = hipfftExecC2R(h, static_cast<hipfftComplex *>(in), out);
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2R: %s", fft_err(r));
+  return JPM_OK;
+}
+
+int fft_c2c(hipStream_t s, int nx, int ny, int nz, int batch, const void *in, void *out, int direction) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(direction == -1 || direction == 1, "direction must be -1 (forward) or +1 (inverse)");
+  JPM_REQUIRE(in && out, "null FFT buffer");
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipfftHandle h;
+  rc = plan_locked(nx, ny, nz, HIPFFT_C2C, batch, h);
+  if (rc) return rc;
+  hipfftResult r = hipfftSetStream(h, s);
+  if (r == HIPFFT_SUCCESS)
+    r = hipfftExecC2C(h, static_cast<hipfftComplex *>(const_cast<void *>(in)), static_cast<hipfftComplex *>(out),
+                      direction == -1 ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2C: %s", fft_err(r));
+  return JPM_OK;
+}
+
+void free_plans() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &kv : g_plans) hipfftDestroy(kv.second);
+  g_plans.clear();
+}
+
+}  // namespace jpm

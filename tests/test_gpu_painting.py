@@ -1,0 +1,167 @@
+"""GPU parity of CIC paint / read and their adjoints against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import painting as op
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    return torch.as_tensor(np.asarray(x), dtype=torch.float32, device="cuda")
+
+
+def host(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def lattice(shape):
+    return np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1).astype(np.float32)
+
+
+def close(got, ref, rel=2e-6):
+    """FP32 sums of O(10) deposits: error relative to the largest value."""
+    scale = max(np.abs(ref).max(), 1.0)
+    err = np.abs(got - ref).max() / scale
+    assert err < rel, f"max error {err:.3e} (x{scale:.3g})"
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (32, 32, 64)])
+def test_paint_absolute(gpu, shape):
+    from jaxpm_amd import painting
+    print(f"paint absolute {shape}", flush=True)
+    r = np.random.default_rng(10)
+    n = int(np.prod(shape))
+    L = np.array(shape)
+    pos = r.uniform(-0.5, 1.5, size=(n, 3)).astype(np.float32) * L  # wrap on both sides
+    w = r.normal(size=n).astype(np.float32)
+    base = r.normal(size=shape).astype(np.float32)
+    got = host(painting.cic_paint(dev(base), dev(pos)))
+    close(got, op.cic_paint(base, pos))
+    got = host(painting.cic_paint(dev(np.zeros(shape)), dev(pos), weight=dev(w)))
+    close(got, op.cic_paint(np.zeros(shape), pos, w))
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (32, 32, 64)])
+def test_paint_relative(gpu, shape):
+    from jaxpm_amd import painting
+    print(f"paint relative {shape}", flush=True)
+    r = np.random.default_rng(11)
+    disp = r.normal(scale=3.0, size=(*shape, 3)).astype(np.float32)
+    disp[0, 0, 0] = (-1e-9, 0.0, 0.0)   # rounds onto index L in float32: dropped
+    got = host(painting.cic_paint_dx(dev(disp)))
+    ref = op.cic_paint_dx(disp, dtype=np.float32).astype(np.float64)
+    close(got, ref)
+    w = r.uniform(0.5, 2.0, size=shape).astype(np.float32)
+    close(host(painting.cic_paint_dx(dev(disp), weight=dev(w))), op.cic_paint_dx(disp, w, dtype=np.float32))
+
+
+def test_paint_weight_shape_error(gpu):
+    from jaxpm_amd import painting
+    with pytest.raises(ValueError, match="Weight shape must match particle shape"):
+        painting.cic_paint_dx(dev(np.zeros((4, 4, 4, 3))), weight=dev(np.ones(5)))
+
+
+def test_deterministic_paint_is_bitwise_reproducible(gpu):
+    from jaxpm_amd import painting
+    shape = (32, 32, 32)
+    r = np.random.default_rng(12)
+    # clustered: every particle in a few cells -> heavy atomic contention
+    pos = (16 + r.normal(scale=1.5, size=(32768, 3))).astype(np.float32)
+    a = painting.cic_paint(dev(np.zeros(shape)), dev(pos), deterministic=True)
+    b = painting.cic_paint(dev(np.zeros(shape)), dev(pos), deterministic=True)
+    assert torch.equal(a, b)
+    plain = painting.cic_paint(dev(np.zeros(shape)), dev(pos))
+    close(host(a), host(plain), rel=1e-5)  # float32 atomics lose ~1e-7 per deposit
+    close(host(a), op.cic_paint(np.zeros(shape), pos), rel=5e-7)
+
+
+def test_paint_large_mass_conservation(gpu):
+    """256^3 particles on a 256^3 mesh: sum(mesh) == N, uniform and clustered."""
+    from jaxpm_amd import painting
+    shape = (256, 256, 256)
+    n = 256 ** 3
+    g = torch.Generator(device="cuda").manual_seed(1)
+    pos = torch.rand((n, 3), generator=g, device="cuda") * 256
+    for p in (pos, 128 + 4 * torch.randn((n, 3), generator=g, device="cuda")):
+        m = painting.cic_paint(torch.zeros(shape, device="cuda"), p)
+        total = m.double().sum().item()
+        assert abs(total - n) < 1e-5 * n, total
+        md = painting.cic_paint(torch.zeros(shape, device="cuda"), p, deterministic=True)
+        assert abs(md.double().sum().item() - n) < 1e-6 * n
+        assert (m - md).abs().max().item() < 1e-3 * max(1.0, md.abs().max().item())
+
+
+def test_paint_128_vs_oracle(gpu):
+    from jaxpm_amd import painting
+    shape = (128, 128, 128)
+    r = np.random.default_rng(13)
+    pos = (lattice(shape).reshape(-1, 3) + r.normal(scale=2.0, size=(128 ** 3, 3))).astype(np.float32)
+    close(host(painting.cic_paint(dev(np.zeros(shape)), dev(pos))), op.cic_paint(np.zeros(shape), pos))
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (32, 32, 64)])
+def test_read_absolute_and_relative(gpu, shape):
+    from jaxpm_amd import painting
+    print(f"read {shape}", flush=True)
+    r = np.random.default_rng(14)
+    mesh = r.normal(size=shape).astype(np.float32)
+    n = int(np.prod(shape))
+    pos = (r.uniform(-0.5, 1.5, size=(n, 3)) * np.array(shape)).astype(np.float32)
+    close(host(painting.cic_read(dev(mesh), dev(pos))), op.cic_read(mesh, pos))
+    disp = r.normal(scale=3.0, size=(*shape, 3)).astype(np.float32)
+    close(host(painting.cic_read_dx(dev(mesh), dev(disp))), op.cic_read_dx(mesh, disp, dtype=np.float32))
+    # lattice read returns node values exactly
+    np.testing.assert_array_equal(host(painting.cic_read(dev(mesh), dev(lattice(shape)))), mesh)
+    # three meshes in one call
+    m3 = r.normal(size=(3, *shape)).astype(np.float32)
+    got = host(painting.cic_read(dev(m3), dev(pos)))
+    for c in range(3):
+        close(got[:, c], op.cic_read(m3[c], pos))
+
+
+def test_read_paint_transpose_and_position_gradients(gpu):
+    from jaxpm_amd import painting
+    shape = (16, 16, 16)
+    r = np.random.default_rng(15)
+    n = 4096
+    pos = r.uniform(0, 16, size=(n, 3))
+    # keep away from cell faces so that central differences stay on one piece
+    frac = pos - np.floor(pos)
+    pos = np.floor(pos) + np.clip(frac, 0.01, 0.99)
+    pos = pos.astype(np.float32)
+    g = r.normal(size=shape).astype(np.float32)
+    w = r.normal(size=n).astype(np.float32)
+    lhs = (painting.cic_paint(dev(np.zeros(shape)), dev(pos), weight=dev(w)).double() * dev(g).double()).sum()
+    rhs = (dev(w).double() * painting.cic_read(dev(g), dev(pos)).double()).sum()
+    assert abs(lhs.item() - rhs.item()) < 1e-5 * abs(lhs.item())
+
+    # d/dpos of sum(u * read(g, pos)) against float64 central differences
+    u = r.normal(size=n)
+    pt = dev(pos).requires_grad_(True)
+    (painting.cic_read(dev(g), pt) * dev(u)).sum().backward()
+    eps = 1e-4
+    dread = np.zeros((n, 3))
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = eps
+        dread[:, d] = (op.cic_read(g, pos + e) - op.cic_read(g, pos - e)) / (2 * eps)
+    close(host(pt.grad), dread * u[:, None], rel=1e-4)
+
+    # d/dpos of <paint(pos, w), g>: same derivative with u = w
+    pt = dev(pos).requires_grad_(True)
+    (painting.cic_paint(dev(np.zeros(shape)), pt, weight=dev(w)) * dev(g)).sum().backward()
+    close(host(pt.grad), dread * w[:, None], rel=1e-4)
+
+    # mesh cotangent of read is the paint of the cotangent
+    mt = dev(g).requires_grad_(True)
+    (painting.cic_read(mt, dev(pos)) * dev(u)).sum().backward()
+    close(host(mt.grad), op.cic_paint(np.zeros(shape), pos, u))
+
+
+def test_compensate_cic(gpu):
+    from jaxpm_amd import painting
+    r = np.random.default_rng(16)
+    f = r.normal(size=(16, 16, 32)).astype(np.float32)
+    close(host(painting.compensate_cic(dev(f))), op.compensate_cic(f), rel=5e-6)
