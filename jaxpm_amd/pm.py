@@ -63,7 +63,7 @@ def pm_forces(positions, mesh_shape=None, delta=None, r_split=0, paint_absolute_
             return _autograd.CicRead.apply(phi, pflat, rel, deterministic).reshape(out_shape)
         return _ops.pm_forces_fused(pflat, mesh_shape, r_split=float(r_split), relative=rel,
                                     deterministic=deterministic, kf=kf).reshape(out_shape)
-    is_complex = (isinstance(delta, torch.Tensor) and delta.is_complex()) or np.iscomplexobj(delta)
+    is_complex = delta.is_complex() if isinstance(delta, torch.Tensor) else np.iscomplexobj(delta)
     if is_complex:
         dk = _ops.full_to_half(to_device(delta, torch.complex64))
         return _ops.pm_forces_fused(pflat, mesh_shape, delta_k=dk, r_split=float(r_split), relative=rel,
