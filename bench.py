@@ -1,0 +1,184 @@
+#!/usr/bin/env python
+"""Forward + adjoint PM steps per second on one MI355X.
+
+Workload (default): 256^3 particles on a 256^3 mesh, periodic box of
+1024 Mpc/h, cosmology Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96,
+sigma8=0.8; Gaussian initial conditions from seeded white noise and the
+Eisenstein-Hu P(k), 2LPT at a=0.1, then the kick-drift-kick leapfrog to
+a=1.0.  One timed "step" is one leapfrog step through the forward pass and
+the hand-written adjoint (checkpoints every 5 steps, so the backward sweep
+recomputes each segment), with the loss sum((rho - rho_t)^2)/sigma^2 of the
+final painted density and its gradient back to the initial positions and
+velocities.  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line with the throughput, the roofline of the dominant
+kernel (timed with HIP events on the launch stream) and a CPU baseline (the
+float32 NumPy/SciPy port of the same step timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="timed leapfrog steps (forward + adjoint)")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed warm-up steps")
+    ap.add_argument("--mesh", type=int, default=256)
+    ap.add_argument("--box", type=float, default=1024.0)
+    ap.add_argument("--checkpoint", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    return ap.parse_args()
+
+
+def kernel_time_ms(fn, reps):
+    """Average duration of fn's launches, HIP events on torch's current stream
+    (the stream libjaxpm_hip launches on)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def cpu_baseline(n, box_cells_per_side, x, v):
+    """One forward + adjoint step of the float32 NumPy/SciPy port on the host."""
+    from oracle import cpu_step
+    cores = len(os.sched_getaffinity(0))
+    shape = (n, n, n)
+    r = np.random.default_rng(0)
+    cx = r.normal(size=x.shape).astype(np.float32)
+    cv = r.normal(size=x.shape).astype(np.float32)
+    t0 = time.perf_counter()
+    cpu_step.step_fwd_adj(x, v, cx, cv, shape, 0.01, 0.01, cores)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": f"1 forward+adjoint leapfrog step at {n}^3 (drift, force, kick, recomputed force, force "
+                      f"VJP, drift adjoint) of oracle/cpu_step.py, float32 NumPy/SciPy c2c FFT, {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the reference's sharded "
+              "halo exchange and pencil FFTs are out of scope); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from jaxpm_amd import _lib, _ops, integrate, painting, pm
+    from jaxpm_amd import cosmology as pc
+
+    dev = _lib.device()
+    n = args.mesh
+    shape = (n, n, n)
+    N = n ** 3
+    box = (args.box,) * 3
+    cosmo = pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    k, pk = pc.pk_table(cosmo)
+    gen = torch.Generator(device=dev).manual_seed(2)
+    noise = torch.randn(shape, generator=gen, device=dev)
+    with torch.no_grad():
+        ic = pm.linear_field(shape, box, pm.TabulatedPk(k, pk), noise)
+        lattice = torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
+                                             indexing="ij"), -1)
+        dx, p, _ = pm.lpt(cosmo, ic, lattice, a=0.1, order=2)
+        x0 = (lattice + dx).reshape(-1, 3).contiguous()
+        v0 = p.reshape(-1, 3).contiguous()
+        target = painting.cic_paint(torch.zeros(shape, device=dev),
+                                    lattice.reshape(-1, 3) + 0.5 * torch.randn((N, 3), generator=gen, device=dev))
+    sigma = 0.1
+
+    def fwd_adj(S):
+        steps = np.linspace(0.1, 1.0, S + 1)
+        xt = x0.clone().requires_grad_(True)
+        vt = v0.clone().requires_grad_(True)
+        x, v = integrate.leapfrog(xt, vt, steps, cosmo, shape, checkpoint_every=args.checkpoint)
+        rho = painting.cic_paint(torch.zeros(shape, device=dev), x)
+        loss = ((rho - target) ** 2).sum() / sigma ** 2
+        loss.backward()
+        return x.detach(), xt.grad
+
+    if args.warmup > 0:
+        fwd_adj(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xS, g0 = fwd_adj(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert torch.isfinite(g0).all(), "non-finite gradient"
+    steps_per_s = args.steps / elapsed
+
+    # Kernel roofline at the final (clustered) state.
+    M = N
+    H = n * n * (n // 2 + 1)
+    mesh = torch.zeros(shape, device=dev)
+    phi = torch.randn((3, *shape), device=dev)
+    vel = torch.zeros((N, 3), device=dev)
+    dk = _ops.r2c(mesh)
+    st = integrate.Stepper(shape, N)
+    cot = torch.randn((N, 3), device=dev)
+    gpos = torch.zeros((N, 3), device=dev)
+    reps = args.kernel_reps
+    kernels = {
+        # name: (ms per launch, algorithmic bytes per launch, launches per timed step)
+        "cic_paint": (kernel_time_ms(lambda: _ops.paint(mesh, xS), reps), 12 * N + 4 * M, 3),
+        "cic_read3_kick": (kernel_time_ms(lambda: st.kick(phi, xS, vel, 0.0), reps), 24 * N + 12 * M + 12 * N, 2),
+        "kspace_force": (kernel_time_ms(lambda: _ops.kspace_force(dk, shape, 0.0, 1.0 / M), reps), 32 * H, 2),
+        "cic_paint3": (kernel_time_ms(lambda: _ops.paint_multi(shape, xS, cot), reps), 24 * N + 12 * M, 1),
+        "cic_grad_pos3": (kernel_time_ms(lambda: _ops.grad_pos(phi, xS, cot=cot, out=gpos), reps),
+                          24 * N + 12 * M + 24 * N, 1),
+    }
+    dom = max(kernels, key=lambda kk: kernels[kk][0] * kernels[kk][2])
+    t_dom, b_dom, _ = kernels[dom]
+    achieved = b_dom / (t_dom * 1e-3) / 1e9
+    kinfo = {name: {"ms": round(t, 4), "GB_s": round(b / (t * 1e-3) / 1e9, 1),
+                    "frac_hbm": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for name, (t, b, _) in kernels.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(n, n, x0.cpu().numpy(), v0.cpu().numpy())
+
+    out = {
+        "metric": "fwd+adjoint PM steps/s at 256^3 mesh; CIC paint/read GB/s as % HBM peak",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * elapsed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"leapfrog forward+adjoint, {n}^3 particles on {n}^3 mesh, box {args.box:g} Mpc/h, "
+                               f"2LPT IC at a=0.1, {args.steps} steps to a=1, checkpoint every {args.checkpoint}",
+                   "mesh": n, "particles": N, "checkpoint_every": args.checkpoint},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": b_dom, "ms_per_launch": round(t_dom, 4)},
+        "kernels": kinfo,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -214,6 +214,39 @@ int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos,
                       float r_split, float scale, uint32_t flags, void *ws,
                       size_t ws_bytes);
 
+/* Workspace bytes for jpm_force_field_f32. */
+size_t jpm_force_field_workspace_size(int nx, int ny, int nz, uint32_t flags);
+
+/* phi3[d] (three meshes, 3*nx*ny*nz floats) = scale * Re IFFT[-g_d G Lr FFT
+ * rho], rho = paint(pos) (or delta_k when non-NULL): the force meshes of
+ * pm.py:49-56 before the read.  Flags as jpm_pm_forces_f32. */
+int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos,
+                        int64_t n, const void *delta_k, float *phi3,
+                        float r_split, float scale, uint32_t flags, void *ws,
+                        size_t ws_bytes);
+
+/* Workspace bytes for jpm_force_vjp_f32. */
+size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, uint32_t flags);
+
+/* gpos[n][3] += scale * J(pos)^T cot, J the Jacobian of the pm_forces output
+ * (pm.py:12-58) with respect to the particle positions, given the force
+ * meshes phi3 of those positions (jpm_force_field_f32 with scale 1).  This is
+ * the hand-written counterpart of jax.grad through pm_forces. */
+int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3,
+                      const float *pos, int64_t n, const float *cot,
+                      float scale, float *gpos, float r_split, uint32_t flags,
+                      void *ws, size_t ws_bytes);
+
+/* y += a * x over n floats (16-byte aligned): the drift x += h v / (a^3 E)
+ * of the leapfrog built on make_ode_fn (ode.py:139-140) and its adjoint. */
+int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y);
+
+/* vel[n][3] += gamma * read(phi3, pos): the kick of ode.py:142-143 fused
+ * into the 3-component CIC read. */
+int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3,
+                 const float *pos, int64_t n, float gamma, float *vel,
+                 uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,12 @@ SIGNATURES = {
     "jpm_lpt2_source_vjp_f32": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "jpm_spectrum_full_to_half_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "jpm_spectrum_half_to_full_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "jpm_force_field_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_force_field_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
+    "jpm_force_vjp_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_force_vjp_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz]),
+    "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
+    "jpm_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _f, _vp, _u32]),
     "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _u32]),
     "jpm_pm_forces_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
 }

@@ -49,6 +49,106 @@ int kspace_shear6_adjoint(hipStream_t, int, int, int, const void *, void *, floa
 int lpt2_source_vjp(hipStream_t, int64_t, const float *, const float *, float *);
 int spectrum_half_to_full(hipStream_t, int, int, int, const void *, void *);
 
+int axpy(hipStream_t, int64_t, float, const float *, float *);
+int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, float, float *, uint32_t);
+
+// Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
+struct FieldWs {
+  size_t mesh, spec, spec3, paint, total;
+};
+
+static FieldWs field_ws(int nx, int ny, int nz, uint32_t flags) {
+  const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
+  FieldWs w;
+  w.mesh = 0;
+  w.spec = align_up(m * sizeof(float));
+  w.spec3 = w.spec + align_up(h * 8);
+  w.paint = w.spec3 + align_up(3 * h * 8);
+  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? jpm_cic_paint_workspace_size(nx, ny, nz, int64_t(m), flags) : 0);
+  return w;
+}
+
+// phi3 = scale * Re IFFT[-g_d G Lr FFT(paint(pos) or delta_k)], the three
+// force meshes that the read / kick sample.
+static int force_field(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
+                       float *phi3, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const FieldWs w = field_ws(nx, ny, nz, pflags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "force_field needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  JPM_REQUIRE(phi3 != nullptr, "phi3 is null");
+  char *base = static_cast<char *>(ws);
+  float *mesh = reinterpret_cast<float *>(base + w.mesh);
+  void *spec = base + w.spec;
+  void *spec3 = base + w.spec3;
+  const int64_t m = cells(nx, ny, nz);
+  if (!delta_k) {
+    rc = cic_paint(s, nx, ny, nz, pos, n, nullptr, 1.0f, mesh, pflags | JPM_ZERO_MESH, base + w.paint,
+                   w.total - w.paint);
+    if (rc) return rc;
+    rc = fft_r2c(s, nx, ny, nz, 1, mesh, spec);
+    if (rc) return rc;
+    delta_k = spec;
+  }
+  rc = kspace_force(s, nx, ny, nz, delta_k, spec3, r_split, float(double(scale) / double(m)), kflags);
+  if (rc) return rc;
+  return fft_c2r(s, nx, ny, nz, 3, spec3, phi3);
+}
+
+// Workspace of force_vjp: [3 cotangent meshes][3 half spectra][half
+// spectrum][density cotangent mesh][paint scratch].
+struct VjpWs {
+  size_t g3, spec3, spec, rho, paint, total;
+};
+
+static VjpWs vjp_ws(int nx, int ny, int nz, uint32_t flags) {
+  const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
+  VjpWs w;
+  w.g3 = 0;
+  w.spec3 = align_up(3 * m * sizeof(float));
+  w.spec = w.spec3 + align_up(3 * h * 8);
+  w.rho = w.spec + align_up(h * 8);
+  w.paint = w.rho + align_up(m * sizeof(float));
+  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? align_up(3 * m * 8) + 256 : 0);
+  return w;
+}
+
+// gpos += scale * J^T cot, J the Jacobian of the forces read at pos from the
+// field of the particles at pos (pm.py:12-58): the direct term samples the
+// gradient of phi3, the density term paints cot (the transpose of the read),
+// applies the transposed k-space operator and samples the gradient of the
+// resulting density cotangent.
+static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n,
+                     const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
+                     size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const VjpWs w = vjp_ws(nx, ny, nz, pflags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "force_vjp needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  JPM_REQUIRE(phi3 && pos && cot && gpos, "null pointer argument");
+  char *base = static_cast<char *>(ws);
+  float *g3 = reinterpret_cast<float *>(base + w.g3);
+  void *spec3 = base + w.spec3;
+  void *spec = base + w.spec;
+  float *rho = reinterpret_cast<float *>(base + w.rho);
+  const int64_t m = cells(nx, ny, nz);
+  const uint32_t rel = pflags & JPM_RELATIVE;
+  rc = cic_paint_multi(s, nx, ny, nz, pos, n, cot, 3, g3, (pflags & (JPM_RELATIVE | JPM_DETERMINISTIC)) | JPM_ZERO_MESH,
+                       base + w.paint, w.total - w.paint);
+  if (rc) return rc;
+  rc = fft_r2c(s, nx, ny, nz, 3, g3, spec3);
+  if (rc) return rc;
+  rc = kspace_force_adjoint(s, nx, ny, nz, spec3, spec, r_split, float(double(scale) / double(m)), kflags);
+  if (rc) return rc;
+  rc = fft_c2r(s, nx, ny, nz, 1, spec, rho);
+  if (rc) return rc;
+  rc = cic_grad_pos(s, nx, ny, nz, phi3, 3, m, pos, n, cot, nullptr, scale, gpos, rel, 1);
+  if (rc) return rc;
+  return cic_grad_pos(s, nx, ny, nz, rho, 1, m, pos, n, nullptr, nullptr, 1.0f, gpos, rel, 1);
+}
+
 // Workspace layout of pm_forces: [3 meshes][1 half spectrum][3 half
 // spectra][deterministic-paint scratch].  The painted mesh lives in the first
 // of the three force meshes; it is dead once the R2C has read it.
@@ -222,6 +322,36 @@ int jpm_spectrum_full_to_half_f32(void *stream, int nx, int ny, int nz, const vo
 
 int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz, const void *half, void *full) {
   JPM_GUARD(spectrum_half_to_full(as_stream(stream), nx, ny, nz, half, full));
+}
+
+size_t jpm_force_field_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return field_ws(nx, ny, nz, flags & 0xffu).total;
+}
+
+int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
+                        float *phi3, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(force_field(as_stream(stream), nx, ny, nz, pos, n, delta_k, phi3, r_split, scale, flags, ws, ws_bytes));
+}
+
+size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return vjp_ws(nx, ny, nz, flags & 0xffu).total;
+}
+
+int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n,
+                      const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
+                      size_t ws_bytes) {
+  JPM_GUARD(force_vjp(as_stream(stream), nx, ny, nz, phi3, pos, n, cot, scale, gpos, r_split, flags, ws, ws_bytes));
+}
+
+int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y) {
+  JPM_GUARD(axpy(as_stream(stream), n, a, x, y));
+}
+
+int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n, float gamma,
+                 float *vel, uint32_t flags) {
+  JPM_GUARD(kick(as_stream(stream), nx, ny, nz, phi3, pos, n, gamma, vel, flags));
 }
 
 size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, uint32_t flags) {

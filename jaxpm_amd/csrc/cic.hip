@@ -571,3 +571,89 @@ int cic_grad_pos(hipStream_t s, int nx, int ny, int nz, const float *mesh, int n
 }
 
 }  // namespace jpm
+
+namespace jpm {
+
+// ----------------------------------------------------- integrator kernels
+
+// y += a * x over n floats (drift x += beta v; adjoint lambda_v += beta lambda_x).
+__global__ void __launch_bounds__(256) axpy_kernel(float a, const float4 *__restrict__ x, float4 *__restrict__ y,
+                                                   uint32_t n4) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+    float4 u = x[i], v = y[i];
+    v.x += a * u.x;
+    v.y += a * u.y;
+    v.z += a * u.z;
+    v.w += a * u.w;
+    y[i] = v;
+  }
+}
+
+__global__ void axpy_tail_kernel(float a, const float *__restrict__ x, float *__restrict__ y, uint32_t start,
+                                 uint32_t n) {
+  uint32_t i = start + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+
+// Kick fused into the 3-component read: vel[p] += gamma * sum_o phi[idx_o] W_o.
+template <bool REL>
+__global__ void __launch_bounds__(256)
+kick_kernel(const float *__restrict__ phi, uint32_t cstride, const float *__restrict__ pos, uint32_t n, Grid g,
+            float gamma, float *__restrict__ vel) {
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const float *m = phi + size_t(comp) * cstride;
+      float s = 0.0f;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) s += m[c.idx[o]] * c.w[o];
+      vel[size_t(p) * 3 + comp] += gamma * s;
+    }
+  }
+}
+
+int axpy(hipStream_t s, int64_t n, float a, const float *x, float *y) {
+  JPM_REQUIRE(n >= 0 && n <= (int64_t(1) << 32) - 1, "length %lld out of range", (long long)n);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(x && y, "null pointer argument");
+  JPM_REQUIRE((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) % 16 == 0,
+              "axpy operands must be 16-byte aligned");
+  const uint32_t n4 = uint32_t(n / 4);
+  if (n4) {
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n4, 256, 256 * 128)), dim3(256), 0, s, a,
+                       reinterpret_cast<const float4 *>(x), reinterpret_cast<float4 *>(y), n4);
+    JPM_LAUNCHED("axpy_kernel");
+  }
+  if (uint32_t(n) > 4 * n4) {
+    hipLaunchKernelGGL(axpy_tail_kernel, dim3(1), dim3(64), 0, s, a, x, y, 4 * n4, uint32_t(n));
+    JPM_LAUNCHED("axpy_tail_kernel");
+  }
+  return JPM_OK;
+}
+
+int kick(hipStream_t s, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n, float gamma,
+         float *vel, uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(3 * m <= (int64_t(1) << 32), "mesh too large");
+  const bool rel = flags & JPM_RELATIVE;
+  JPM_REQUIRE(!rel || n == m, "relative mode needs one particle per lattice node");
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(phi3 && pos && vel, "null pointer argument");
+  Grid g = make_grid(nx, ny, nz);
+  const unsigned grid = grid_for(n, 256, 256 * 128);
+  if (rel)
+    hipLaunchKernelGGL(kick_kernel<true>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g, gamma,
+                       vel);
+  else
+    hipLaunchKernelGGL(kick_kernel<false>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g,
+                       gamma, vel);
+  JPM_LAUNCHED("kick_kernel");
+  return JPM_OK;
+}
+
+}  // namespace jpm

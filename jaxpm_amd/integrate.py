@@ -1,0 +1,160 @@
+"""Fixed-step kick-drift-kick leapfrog and its checkpointed adjoint.
+
+The reference drives ``make_ode_fn`` (ode.py:122-147) with external
+integrators (diffrax / odeint) and differentiates them with
+``RecursiveCheckpointAdjoint`` (tests/test_gradients.py:27-72).  This module
+fixes one integrator, a kick-drift-kick leapfrog in the scale factor over the
+same right-hand side ``dx/da = v / (a^3 E)``, ``dv/da = 1.5 Om F(x) / (a^2 E)``:
+
+    v += h/2 * dv/da(a, x);  x += h v / (am^3 E(am)), am = a + h/2;
+    v += h/2 * dv/da(a + h, x)
+
+with the force at ``a + h`` reused for the next step's first half kick, so a
+run of S steps costs S + 1 force evaluations.  Its adjoint is written by hand
+(:class:`NBody`): the backward pass restores (x, v) from checkpoints taken
+every ``checkpoint_every`` steps, recomputes one segment at a time keeping
+the segment's positions and force meshes, and walks it backwards with
+``jpm_force_vjp_f32`` (J^T of the force) and ``jpm_axpy_f32`` (the drift).
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from . import growth as gr
+from ._convert import to_device
+from ._lib import call, ptr, size, stream
+
+
+def kdk_schedule(cosmo, a_steps):
+    """Host float64 coefficients of the leapfrog over ``a_steps``.
+
+    Returns (k0, beta, gamma): first half kick ``v += k0 F(x0)``, then for
+    step i ``x += beta[i] v`` and ``v += gamma[i] F(x)``.
+    """
+    a = np.asarray(a_steps, np.float64)
+    if a.ndim != 1 or a.size < 2 or np.any(np.diff(a) == 0):
+        raise ValueError("a_steps must be a 1-D sequence of at least two distinct scale factors")
+    om = cosmo.Omega_m
+    c = 1.5 * om / (a ** 2 * np.sqrt(gr.Esqr(cosmo, a)))
+    h = np.diff(a)
+    am = a[:-1] + 0.5 * h
+    beta = h / (am ** 3 * np.sqrt(gr.Esqr(cosmo, am)))
+    gamma = 0.5 * h * c[1:]
+    gamma[:-1] += 0.5 * h[1:] * c[1:-1]
+    k0 = 0.5 * h[0] * c[0]
+    return float(k0), beta, gamma
+
+
+class Stepper:
+    """Device buffers and C-ABI calls of one mesh / particle configuration."""
+
+    def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0):
+        self.shape = tuple(int(s) for s in mesh_shape)
+        self.nx, self.ny, self.nz = self.shape
+        self.n = int(n)
+        self.m = self.nx * self.ny * self.nz
+        if relative and self.n != self.m:
+            raise ValueError("relative mode needs one particle per mesh cell")
+        self.r_split = float(r_split)
+        pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0)
+        self.pflags = pflags
+        self.flags = pflags | (kflags << 8)
+        self.dev = _lib.device()
+        fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.flags)
+        vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.flags)
+        self.ws = torch.empty(max(fb, vb), dtype=torch.uint8, device=self.dev)
+
+    def new_phi(self):
+        return torch.empty((3, *self.shape), dtype=torch.float32, device=self.dev)
+
+    def field(self, x, phi):
+        call("jpm_force_field_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n, None, ptr(phi),
+             self.r_split, 1.0, self.flags, ptr(self.ws), self.ws.numel())
+
+    def kick(self, phi, x, v, gamma):
+        call("jpm_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n, float(gamma), ptr(v),
+             self.pflags & _lib.RELATIVE)
+
+    def axpy(self, a, x, y):
+        call("jpm_axpy_f32", stream(), 3 * self.n, float(a), ptr(x), ptr(y))
+
+    def vjp(self, phi, x, cot, scale, gpos):
+        call("jpm_force_vjp_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n, ptr(cot),
+             float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws), self.ws.numel())
+
+
+def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None):
+    """Advance steps [start, stop) in place; ``keep`` collects (x, phi) after
+    each step's force evaluation."""
+    for i in range(start, stop):
+        stepper.axpy(beta[i], v, x)
+        stepper.field(x, phi)
+        stepper.kick(phi, x, v, gamma[i])
+        if keep is not None:
+            keep.append((x.clone(), phi.clone()))
+
+
+class NBody(torch.autograd.Function):
+    """(x0, v0) -> (xS, vS) by the leapfrog, with a hand-written adjoint."""
+
+    @staticmethod
+    def forward(ctx, x0, v0, stepper, sched, checkpoint_every):
+        k0, beta, gamma = sched
+        S = len(beta)
+        x = x0.detach().clone().contiguous()
+        v = v0.detach().clone().contiguous()
+        phi = stepper.new_phi()
+        stepper.field(x, phi)
+        stepper.kick(phi, x, v, k0)
+        k = S if checkpoint_every is None else max(1, int(checkpoint_every))
+        ckpts = {0: (x.clone(), v.clone())}
+        for s0 in range(0, S, k):
+            _run(stepper, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi)
+            if s0 + k < S:
+                ckpts[s0 + k] = (x.clone(), v.clone())
+        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = stepper, sched, k, ckpts
+        ctx.x0 = x0.detach()
+        return x, v
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        st = ctx.stepper
+        k0, beta, gamma = ctx.sched
+        S = len(beta)
+        lx = (gx if gx is not None else torch.zeros_like(ctx.x0)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(ctx.x0)).detach().clone().contiguous()
+        phi = st.new_phi()
+        for s0 in reversed(range(0, S, ctx.k)):
+            s1 = min(S, s0 + ctx.k)
+            xc, vc = ctx.ckpts[s0]
+            x, v = xc.clone(), vc.clone()
+            keep = []
+            _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep)
+            for i in reversed(range(s0, s1)):
+                xi, phii = keep[i - s0]
+                st.vjp(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
+                st.axpy(beta[i], lx, lv)            # drift i: x += beta v
+            del keep
+        x0 = ctx.ckpts[0][0]
+        # the first half kick used x0 before any drift: recompute its field
+        xs = ctx.x0.contiguous()
+        st.field(xs, phi)
+        st.vjp(phi, xs, lv, k0, lx)
+        del x0
+        ctx.ckpts = None
+        return lx, lv, None, None, None
+
+
+def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, deterministic=False, r_split=0.0,
+             checkpoint_every=None):
+    """Integrate ``(pos, vel)`` over ``a_steps`` (absolute grid-unit positions,
+    or displacements with ``paint_absolute_pos=False``).  Differentiable in
+    both inputs through :class:`NBody`."""
+    x0 = to_device(pos).reshape(-1, 3)
+    v0 = to_device(vel).reshape(-1, 3)
+    st = Stepper(mesh_shape, x0.shape[0], relative=not paint_absolute_pos, deterministic=deterministic,
+                 r_split=r_split)
+    sched = kdk_schedule(cosmo, a_steps)
+    x, v = NBody.apply(x0, v0, st, sched, checkpoint_every)
+    shape = to_device(pos).shape
+    return x.reshape(shape), v.reshape(shape)
