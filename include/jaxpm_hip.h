@@ -46,6 +46,8 @@ extern "C" {
                                 (painting_utils.enmesh semantics) */
 #define JPM_DETERMINISTIC 2u /* bitwise-reproducible paint (fixed-point sums) */
 #define JPM_ZERO_MESH 4u     /* clear the output mesh before painting */
+#define JPM_ATOMIC 8u        /* pipelines: paint with global atomics instead
+                                of the binned LDS paint */
 
 /* k-space flags */
 #define JPM_K_GRAD_EXACT 1u  /* gradient_kernel order=0 (i k, len//2 zeroed)
@@ -98,6 +100,24 @@ int jpm_cic_paint_multi_f32(void *stream, int nx, int ny, int nz,
                             const float *pos, int64_t n, const float *weights,
                             int nc, float *mesh, uint32_t flags, void *ws,
                             size_t ws_bytes);
+
+/* Workspace bytes for jpm_cic_paint_binned_f32 (nc = 1 or 3). */
+size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n,
+                                           int nc, uint32_t flags);
+
+/* The binned paint, the default inside every pipeline: a counting sort of
+ * the particles into 16^3-cell tiles (LDS histograms, scan, LDS-ranked
+ * scatter), one workgroup per chunk of a tile's particles accumulating into
+ * the tile's 17^3 LDS window, and a gather-reduce of the windows into the
+ * mesh.  No global float atomics; JPM_DETERMINISTIC makes the window sums
+ * 64-bit fixed point (bitwise reproducible).  nc = 1 (weights per particle or
+ * wscalar) or 3 (weights[p*3 + c], the adjoint of the 3-component read).
+ * Replaces the scatter_add of painting.py:39-44 / painting_utils.py:110;
+ * meshes up to 32768 tiles (544^3). */
+int jpm_cic_paint_binned_f32(void *stream, int nx, int ny, int nz,
+                             const float *pos, int64_t n, const float *weights,
+                             float wscalar, int nc, float *mesh,
+                             uint32_t flags, void *ws, size_t ws_bytes);
 
 /* Position cotangent of CIC sampling:
  *   gpos[p][e] (+)= w_p * sum_c cot[p*ncomp+c] * sum_o mesh_c[idx_o] dW_o/dx_e
@@ -201,8 +221,9 @@ int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz,
 
 /* ------------------------------------------------------------- pipeline */
 
-/* Workspace bytes for jpm_pm_forces_f32 on this mesh. */
-size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, uint32_t flags);
+/* Workspace bytes for jpm_pm_forces_f32 on this mesh and particle count. */
+size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, int64_t n,
+                                    uint32_t flags);
 
 /* forces[n][3] = pm_forces(pos): paint (or use delta_k if non-NULL), R2C,
  * jpm_kspace_force_f32, 3 C2R, 3-component read.  `scale` multiplies the
@@ -215,7 +236,8 @@ int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos,
                       size_t ws_bytes);
 
 /* Workspace bytes for jpm_force_field_f32. */
-size_t jpm_force_field_workspace_size(int nx, int ny, int nz, uint32_t flags);
+size_t jpm_force_field_workspace_size(int nx, int ny, int nz, int64_t n,
+                                      uint32_t flags);
 
 /* phi3[d] (three meshes, 3*nx*ny*nz floats) = scale * Re IFFT[-g_d G Lr FFT
  * rho], rho = paint(pos) (or delta_k when non-NULL): the force meshes of
@@ -226,7 +248,8 @@ int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos,
                         size_t ws_bytes);
 
 /* Workspace bytes for jpm_force_vjp_f32. */
-size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, uint32_t flags);
+size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, int64_t n,
+                                    uint32_t flags);
 
 /* gpos[n][3] += scale * J(pos)^T cot, J the Jacobian of the pm_forces output
  * (pm.py:12-58) with respect to the particle positions, given the force

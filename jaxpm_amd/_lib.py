@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libjaxpm_hi
 RELATIVE = 1
 DETERMINISTIC = 2
 ZERO_MESH = 4
+ATOMIC = 8
 K_GRAD_EXACT = 1
 K_LAPL_FD = 2
 K_DECONV = 4
@@ -53,13 +54,15 @@ SIGNATURES = {
     "jpm_lpt2_source_vjp_f32": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "jpm_spectrum_full_to_half_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "jpm_spectrum_half_to_full_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
-    "jpm_force_field_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_force_field_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_field_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
-    "jpm_force_vjp_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_force_vjp_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_vjp_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
     "jpm_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _f, _vp, _u32]),
-    "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _u32]),
+    "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
+    "jpm_cic_paint_binned_workspace_size": (_sz, [_i, _i, _i, _i64, _i, _u32]),
+    "jpm_cic_paint_binned_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _i, _vp, _u32, _vp, _sz]),
     "jpm_pm_forces_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
 }
 

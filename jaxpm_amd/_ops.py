@@ -39,8 +39,21 @@ def _dims(shape):
 
 # ---------------------------------------------------------------- painting
 
-def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=False, zero=False):
-    """mesh (nx,ny,nz) f32 updated in place with CIC deposits of pos (n,3)."""
+MAX_BINNED_TILES = 32768
+
+
+def binned_ok(shape):
+    t = 1
+    for s in shape:
+        t *= -(-int(s) // 16)
+    return t <= MAX_BINNED_TILES
+
+
+def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=False, zero=False, method="binned"):
+    """mesh (nx,ny,nz) f32 updated in place with CIC deposits of pos (n,3).
+
+    method "binned" (tile sort + LDS windows, the default) or "atomic"
+    (one global atomic per corner)."""
     mesh = _f32(mesh, "mesh")
     pos = _f32(pos, "positions").reshape(-1, 3)
     nx, ny, nz = _dims(mesh.shape)
@@ -51,6 +64,14 @@ def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=Fal
             raise ValueError(f"weight has {weight.numel()} entries for {n} particles")
     flags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
         (_lib.ZERO_MESH if zero else 0)
+    if method == "binned" and binned_ok(mesh.shape):
+        wsb = size("jpm_cic_paint_binned_workspace_size", nx, ny, nz, n, 1, flags)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=mesh.device)
+        call("jpm_cic_paint_binned_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weight), float(wscalar), 1,
+             ptr(mesh), flags, ptr(ws), wsb)
+        return mesh
+    if method not in ("binned", "atomic"):
+        raise ValueError(f"unknown paint method {method!r}")
     wsb = size("jpm_cic_paint_workspace_size", nx, ny, nz, n, flags)
     ws = torch.empty(wsb, dtype=torch.uint8, device=mesh.device) if wsb else None
     call("jpm_cic_paint_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weight), float(wscalar), ptr(mesh),
@@ -58,7 +79,7 @@ def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=Fal
     return mesh
 
 
-def paint_multi(shape, pos, weights, relative=False, deterministic=False):
+def paint_multi(shape, pos, weights, relative=False, deterministic=False, method="binned"):
     """nc meshes (nc,nx,ny,nz): channel c painted with weights[:, c]."""
     nx, ny, nz = _dims(shape)
     pos = _f32(pos, "positions").reshape(-1, 3)
@@ -68,6 +89,12 @@ def paint_multi(shape, pos, weights, relative=False, deterministic=False):
     nc = weights.shape[1]
     out = torch.empty((nc, nx, ny, nz), dtype=torch.float32, device=pos.device)
     flags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | _lib.ZERO_MESH
+    if method == "binned" and nc in (1, 3) and binned_ok(shape):
+        wsb = size("jpm_cic_paint_binned_workspace_size", nx, ny, nz, n, nc, flags)
+        ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device)
+        call("jpm_cic_paint_binned_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weights), 1.0, nc, ptr(out),
+             flags, ptr(ws), wsb)
+        return out
     wsb = size("jpm_cic_paint_multi_workspace_size", nx, ny, nz, nc, flags)
     ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device) if wsb else None
     call("jpm_cic_paint_multi_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(weights), nc, ptr(out), flags,
@@ -242,14 +269,15 @@ def lpt2_source_vjp(phi6, gbar):
 # ---------------------------------------------------------------- pipeline
 
 def pm_forces_fused(pos, shape, delta_k=None, r_split=0.0, scale=1.0, relative=False, deterministic=False,
-                    kf=0, out=None, ws=None):
+                    kf=0, out=None, ws=None, atomic=False):
     """forces (n,3) through the single-call pipeline jpm_pm_forces_f32."""
     pos = _f32(pos, "positions").reshape(-1, 3)
     nx, ny, nz = _dims(shape)
     n = pos.shape[0]
-    pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0)
+    pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
+        (_lib.ATOMIC if atomic else 0)
     flags = pflags | (kf << 8)
-    wsb = size("jpm_pm_forces_workspace_size", nx, ny, nz, flags)
+    wsb = size("jpm_pm_forces_workspace_size", nx, ny, nz, n, flags)
     if ws is None or ws.numel() < wsb:
         ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device)
     if delta_k is not None:

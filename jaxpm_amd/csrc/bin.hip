@@ -1,0 +1,630 @@
+// Binned CIC paint: counting sort of particles into 16^3-cell tiles, LDS
+// accumulation of each chunk of a tile's particles into the tile's (17^3)
+// window, and a gather-reduce that sums the windows covering each cell.
+// No global float atomics; with JPM_DETERMINISTIC the LDS sums are 64-bit
+// fixed point, so the result does not depend on the order of deposits.
+//
+// Passes (all stream-ordered, no host synchronisation):
+//   hist     per-workgroup LDS histogram of tile keys -> hist[key][wg]
+//   scan     exclusive scan of hist (key-major) -> scatter offsets; tile
+//            start/count; chunk offsets (ceil(count / CH) chunks per tile)
+//   chunks   chunk descriptors (tile, start, count)
+//   scatter  each workgroup re-reads its slice and writes its particles to
+//            their tile's range (positions, original index, weights)
+//   accum    one workgroup per chunk: LDS window, deposits, window -> partial
+//   reduce   one thread per cell: sum the partial windows (up to 8 tiles,
+//            all their chunks) covering the cell, in a fixed order
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace jpm {
+
+constexpr int kTile = 16;                       // tile edge in cells
+constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
+constexpr int kWinCells = kWin * kWin * kWin;   // 4913
+constexpr int kChunk = 4096;                    // max particles per chunk
+constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
+constexpr int kScanItems = 8;                   // scan: elements per thread
+constexpr int kScanBlock = 256 * kScanItems;
+
+struct TileGrid {
+  int nx, ny, nz;   // mesh
+  int tx, ty, tz;   // tiles per axis
+  uint32_t K;       // tiles
+  FastDiv div_nz, div_ny, div_tz, div_ty;
+};
+
+static TileGrid make_tiles(int nx, int ny, int nz) {
+  TileGrid t;
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.tx = (nx + kTile - 1) / kTile;
+  t.ty = (ny + kTile - 1) / kTile;
+  t.tz = (nz + kTile - 1) / kTile;
+  t.K = uint32_t(t.tx) * t.ty * t.tz;
+  t.div_nz = FastDiv(uint32_t(nz));
+  t.div_ny = FastDiv(uint32_t(ny));
+  t.div_tz = FastDiv(uint32_t(t.tz));
+  t.div_ty = FastDiv(uint32_t(t.ty));
+  return t;
+}
+
+__device__ __forceinline__ int wrapi(int c, int n) {
+  int r = c % n;
+  return r < 0 ? r + n : r;
+}
+
+// Floor cell of a particle: absolute -> int32(floor(x)) mod n (painting.py:
+// 23, 35-37); relative -> the low corner of enmesh (painting_utils.py:48-57),
+// or, when that corner is dropped, the high corner minus one.
+__device__ __forceinline__ int floor_cell_abs(float x, int n) { return wrapi(int(floorf(x)), n); }
+
+__device__ __forceinline__ float jmod(float x, float L) {
+  float m = fmodf(x, L);
+  if (m != 0.0f && m < 0.0f) m = m + L;
+  return m;
+}
+
+__device__ __forceinline__ int floor_cell_rel(float p, int n) {
+  const float L = float(n);
+  float q = floorf(jmod(p, L));
+  if (q >= 0.0f && q < L) return int(q);
+  float q1 = floorf(jmod(p + 1.0f, L));
+  if (q1 >= 0.0f && q1 < L) return wrapi(int(q1) - 1, n);
+  return 0;  // NaN: every corner is dropped, any tile will do
+}
+
+template <bool REL>
+__device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, const TileGrid &t, int &cx,
+                                            int &cy, int &cz) {
+  const float x = pos[3 * size_t(p)], y = pos[3 * size_t(p) + 1], z = pos[3 * size_t(p) + 2];
+  if (REL) {
+    uint32_t r = t.div_nz.div(p);
+    uint32_t k = p - r * uint32_t(t.nz);
+    uint32_t i = t.div_ny.div(r);
+    uint32_t j = r - i * uint32_t(t.ny);
+    cx = floor_cell_rel(float(i) + x, t.nx);
+    cy = floor_cell_rel(float(j) + y, t.ny);
+    cz = floor_cell_rel(float(k) + z, t.nz);
+  } else {
+    cx = floor_cell_abs(x, t.nx);
+    cy = floor_cell_abs(y, t.ny);
+    cz = floor_cell_abs(z, t.nz);
+  }
+}
+
+__device__ __forceinline__ uint32_t tile_key(int cx, int cy, int cz, const TileGrid &t) {
+  return (uint32_t(cx / kTile) * uint32_t(t.ty) + uint32_t(cy / kTile)) * uint32_t(t.tz) + uint32_t(cz / kTile);
+}
+
+// ---------------------------------------------------------------- hist
+
+template <bool REL>
+__global__ void __launch_bounds__(256)
+bin_hist_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t, uint32_t *__restrict__ hist,
+                uint32_t G) {
+  extern __shared__ uint32_t sh[];
+  const uint32_t g = blockIdx.x;
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) sh[k] = 0;
+  __syncthreads();
+  const uint32_t p0 = g * slice;
+  const uint32_t p1 = min(n, p0 + slice);
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    int cx, cy, cz;
+    floor_cells<REL>(pos, p, t, cx, cy, cz);
+    atomicAdd(&sh[tile_key(cx, cy, cz, t)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) hist[size_t(k) * G + g] = sh[k];
+}
+
+// ---------------------------------------------------------------- scan
+
+// In-place exclusive scan of blocks of kScanBlock elements; block totals to
+// sums[blockIdx.x].
+__global__ void __launch_bounds__(256) scan_blocks_kernel(uint32_t *__restrict__ a, uint32_t n,
+                                                          uint32_t *__restrict__ sums) {
+  __shared__ uint32_t sh[256];
+  const size_t base = size_t(blockIdx.x) * kScanBlock + size_t(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = (base + i < n) ? a[base + i] : 0u;
+    tot += v[i];
+  }
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+    __syncthreads();
+    sh[threadIdx.x] += x;
+    __syncthreads();
+  }
+  uint32_t run = sh[threadIdx.x] - tot;  // exclusive prefix of this thread
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) a[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 255) sums[blockIdx.x] = sh[255];
+}
+
+// Single-workgroup exclusive scan of `sums` (any length); the grand total
+// goes to *total.
+__global__ void __launch_bounds__(256) scan_sums_kernel(uint32_t *__restrict__ sums, uint32_t nb,
+                                                        uint32_t *__restrict__ total) {
+  __shared__ uint32_t sh[256];
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < nb; c0 += 256) {
+    const uint32_t i = c0 + threadIdx.x;
+    const uint32_t v = i < nb ? sums[i] : 0u;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      uint32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0u;
+      __syncthreads();
+      sh[threadIdx.x] += x;
+      __syncthreads();
+    }
+    if (i < nb) sums[i] = carry + sh[threadIdx.x] - v;
+    const uint32_t chunk_total = sh[255];
+    __syncthreads();
+    carry += chunk_total;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *__restrict__ a, uint32_t n,
+                                                       const uint32_t *__restrict__ sums) {
+  const size_t base = size_t(blockIdx.x) * kScanBlock + size_t(threadIdx.x) * kScanItems;
+  const uint32_t add = sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < n) a[base + i] += add;
+}
+
+static int exclusive_scan(hipStream_t s, uint32_t *a, uint32_t n, uint32_t *sums, uint32_t *total) {
+  const uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(nb), dim3(256), 0, s, a, n, sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(256), 0, s, sums, nb, total);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(nb), dim3(256), 0, s, a, n, sums);
+  JPM_LAUNCHED("exclusive_scan");
+  return JPM_OK;
+}
+
+// tile_start[k] = offsets[k*G] (after the scan), tile_count[k] = next start
+// - start; nchunk[k] = ceil(count / kChunk).
+__global__ void __launch_bounds__(256)
+tile_table_kernel(const uint32_t *__restrict__ offs, uint32_t G, uint32_t K, uint32_t n,
+                  uint32_t *__restrict__ tile_start, uint32_t *__restrict__ tile_count, uint32_t *__restrict__ nchunk) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
+    const uint32_t st = offs[size_t(k) * G];
+    const uint32_t en = (k + 1 < K) ? offs[size_t(k + 1) * G] : n;
+    tile_start[k] = st;
+    tile_count[k] = en - st;
+    nchunk[k] = (en - st + kChunk - 1) / kChunk;
+  }
+}
+
+struct ChunkDesc {
+  uint32_t tile, start, count, pad;
+};
+
+__global__ void __launch_bounds__(256)
+chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ tile_count,
+                  const uint32_t *__restrict__ chunk_off, uint32_t K, ChunkDesc *__restrict__ desc) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
+    const uint32_t st = tile_start[k], cnt = tile_count[k];
+    uint32_t c = chunk_off[k];
+    for (uint32_t o = 0; o < cnt; o += kChunk, ++c) {
+      ChunkDesc d;
+      d.tile = k;
+      d.start = st + o;
+      d.count = min(uint32_t(kChunk), cnt - o);
+      d.pad = 0;
+      desc[c] = d;
+    }
+  }
+}
+
+// -------------------------------------------------------------- scatter
+
+template <bool REL>
+__global__ void __launch_bounds__(256)
+bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t,
+                   const uint32_t *__restrict__ offs, uint32_t G, float *__restrict__ spos, uint32_t *__restrict__ sidx) {
+  extern __shared__ uint32_t cur[];
+  const uint32_t g = blockIdx.x;
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) cur[k] = offs[size_t(k) * G + g];
+  __syncthreads();
+  const uint32_t p0 = g * slice;
+  const uint32_t p1 = min(n, p0 + slice);
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
+    int cx, cy, cz;
+    floor_cells<REL>(pos, p, t, cx, cy, cz);
+    const uint32_t dst = atomicAdd(&cur[tile_key(cx, cy, cz, t)], 1u);
+    spos[3 * size_t(dst) + 0] = pos[3 * size_t(p) + 0];
+    spos[3 * size_t(dst) + 1] = pos[3 * size_t(p) + 1];
+    spos[3 * size_t(dst) + 2] = pos[3 * size_t(p) + 2];
+    sidx[dst] = p;
+  }
+}
+
+// ----------------------------------------------------------------- accum
+
+// One corner axis of a particle relative to its tile origin: local window
+// index (or -1 when the corner is dropped / outside the window) and weight.
+struct LAxis {
+  int l0, l1;
+  float w0, w1;
+};
+
+__device__ __forceinline__ LAxis laxis_abs(float x, int origin, int n) {
+  LAxis a;
+  const float f = floorf(x);
+  const float f1 = f + 1.0f;
+  a.w0 = 1.0f - fabsf(x - f);
+  a.w1 = 1.0f - fabsf(x - f1);
+  const int c0 = wrapi(int(f), n);
+  a.l0 = c0 - origin;  // the floor cell lies in the tile by construction
+  a.l1 = a.l0 + 1;
+  return a;
+}
+
+// enmesh corner (painting_utils.py:48-69) mapped into the window; corners
+// that land outside [origin, origin + 16] (rounding edge cases) report -1 and
+// are deposited by the caller through the overflow path.
+__device__ __forceinline__ void lcorner_rel(float p, float o, int n, int origin, int &l, float &w, int &cell) {
+  const float L = float(n);
+  float q = floorf(jmod(p + o, L));
+  float nd = p - q;
+  nd = nd - rintf(nd / L) * L;
+  const bool ok = (q >= 0.0f) && (q < L);
+  cell = ok ? int(q) : -1;
+  w = ok ? 1.0f - fabsf(nd) : 0.0f;
+  if (!ok) {
+    l = -2;  // dropped: no deposit at all
+    return;
+  }
+  int d = int(q) - origin;
+  if (d < 0) d += n;  // window may wrap past the mesh end
+  l = (d >= 0 && d < kWin) ? d : -1;
+}
+
+template <bool REL, bool FIXED, int NC>
+__global__ void __launch_bounds__(256)
+bin_accum_kernel(const float *__restrict__ spos, const uint32_t *__restrict__ sidx,
+                 const float *__restrict__ weight, float wscalar, const ChunkDesc *__restrict__ desc,
+                 const uint32_t *__restrict__ nchunks_total, TileGrid t, const float *__restrict__ wmax_dev,
+                 float wmax_host, uint32_t n, float *__restrict__ partial, float *__restrict__ overflow) {
+  extern __shared__ unsigned char smem[];
+  const uint32_t c = blockIdx.x;
+  if (c >= *nchunks_total) return;
+  const ChunkDesc d = desc[c];
+  const uint32_t tz = t.div_tz.div(d.tile);
+  const uint32_t tzr = d.tile - tz * uint32_t(t.tz);
+  const uint32_t ty = t.div_ty.div(tz);
+  const uint32_t tyr = tz - ty * uint32_t(t.ty);
+  const int ox = int(ty) * kTile, oy = int(tyr) * kTile, oz = int(tzr) * kTile;
+  typedef typename std::conditional<FIXED, unsigned long long, float>::type Acc;
+  Acc *win = reinterpret_cast<Acc *>(smem);
+  for (int i = threadIdx.x; i < NC * kWinCells; i += blockDim.x) win[i] = Acc(0);
+  double scale = 1.0;
+  if (FIXED) scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
+    const uint32_t s = d.start + q;
+    const float x = spos[3 * size_t(s)], y = spos[3 * size_t(s) + 1], z = spos[3 * size_t(s) + 2];
+    const uint32_t pidx = sidx[s];
+    int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
+    float wx[2], wy[2], wz[2];
+    if (REL) {
+      uint32_t r = t.div_nz.div(pidx);
+      uint32_t k = pidx - r * uint32_t(t.nz);
+      uint32_t i = t.div_ny.div(r);
+      uint32_t j = r - i * uint32_t(t.ny);
+      const float px = float(i) + x, py = float(j) + y, pz = float(k) + z;
+      lcorner_rel(px, 0.0f, t.nx, ox, lx[0], wx[0], gx[0]);
+      lcorner_rel(px, 1.0f, t.nx, ox, lx[1], wx[1], gx[1]);
+      lcorner_rel(py, 0.0f, t.ny, oy, ly[0], wy[0], gy[0]);
+      lcorner_rel(py, 1.0f, t.ny, oy, ly[1], wy[1], gy[1]);
+      lcorner_rel(pz, 0.0f, t.nz, oz, lz[0], wz[0], gz[0]);
+      lcorner_rel(pz, 1.0f, t.nz, oz, lz[1], wz[1], gz[1]);
+    } else {
+      LAxis ax = laxis_abs(x, ox, t.nx), ay = laxis_abs(y, oy, t.ny), az = laxis_abs(z, oz, t.nz);
+      lx[0] = ax.l0; lx[1] = ax.l1; wx[0] = ax.w0; wx[1] = ax.w1;
+      ly[0] = ay.l0; ly[1] = ay.l1; wy[0] = ay.w0; wy[1] = ay.w1;
+      lz[0] = az.l0; lz[1] = az.l1; wz[0] = az.w0; wz[1] = az.w1;
+    }
+    float wp[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch)
+      wp[ch] = weight ? weight[size_t(pidx) * NC + ch] : wscalar;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+      const float wc = wx[bx] * wy[by] * wz[bz];
+      const int Lx = lx[bx], Ly = ly[by], Lz = lz[bz];
+      if (REL && (Lx == -2 || Ly == -2 || Lz == -2)) continue;   // dropped corner
+      if (!REL || (Lx >= 0 && Ly >= 0 && Lz >= 0)) {
+        const int li = (Lx * kWin + Ly) * kWin + Lz;
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch) {
+          const float v = wp[ch] * wc;
+          if (FIXED)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&win[ch * kWinCells + li]),
+                      (unsigned long long)(long long)rint(double(v) * scale));
+          else
+            atomicAdd(reinterpret_cast<float *>(&win[ch * kWinCells + li]), v);
+        }
+      } else {
+        // rounding put a relative corner outside the window: global path
+        const uint32_t cell = (uint32_t(gx[bx]) * uint32_t(t.ny) + uint32_t(gy[by])) * uint32_t(t.nz) + uint32_t(gz[bz]);
+#pragma unroll
+        for (int ch = 0; ch < NC; ++ch)
+          atomicAdd(&overflow[size_t(ch) * t.nx * t.ny * t.nz + cell], wp[ch] * wc);
+      }
+    }
+  }
+  __syncthreads();
+  Acc *out = reinterpret_cast<Acc *>(partial) + size_t(c) * NC * kWinCells;
+  for (int i = threadIdx.x; i < NC * kWinCells; i += blockDim.x) out[i] = win[i];
+}
+
+// --------------------------------------------------------------- reduce
+
+// Tiles (and window offsets) covering cell c along one axis: the tile that
+// owns c at local index c - origin, and, when c starts a tile (or is 0), the
+// tile owning c - 1 (mod n) at local index tile_len (its +1 halo).
+__device__ __forceinline__ int axis_contrib(int c, int n, int ntile, int tid[2], int loc[2]) {
+  const int t0 = c / kTile;
+  tid[0] = t0;
+  loc[0] = c - t0 * kTile;
+  if (loc[0] != 0) return 1;
+  const int pc = (c == 0) ? n - 1 : c - 1;
+  const int t1 = pc / kTile;
+  tid[1] = t1;
+  loc[1] = pc - t1 * kTile + 1;
+  (void)ntile;
+  return 2;
+}
+
+template <bool FIXED, int NC>
+__global__ void __launch_bounds__(256)
+bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
+                  const uint32_t *__restrict__ nchunk, TileGrid t, const float *__restrict__ overflow, int has_overflow,
+                  const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n, int accumulate,
+                  float *__restrict__ mesh) {
+  const uint32_t M = uint32_t(t.nx) * t.ny * t.nz;
+  typedef typename std::conditional<FIXED, unsigned long long, float>::type Acc;
+  const Acc *part = reinterpret_cast<const Acc *>(partial);
+  double inv = 1.0;
+  if (FIXED) inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
+    uint32_t r = t.div_nz.div(e);
+    const int k = int(e - r * uint32_t(t.nz));
+    const uint32_t i = t.div_ny.div(r);
+    const int j = int(r - i * uint32_t(t.ny));
+    int tx[2], lx[2], ty[2], ly[2], tz[2], lz[2];
+    const int nxc = axis_contrib(int(i), t.nx, t.tx, tx, lx);
+    const int nyc = axis_contrib(j, t.ny, t.ty, ty, ly);
+    const int nzc = axis_contrib(k, t.nz, t.tz, tz, lz);
+    Acc acc[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) acc[ch] = Acc(0);
+    for (int a = 0; a < nxc; ++a)
+      for (int b = 0; b < nyc; ++b)
+        for (int cz = 0; cz < nzc; ++cz) {
+          const uint32_t tile = (uint32_t(tx[a]) * uint32_t(t.ty) + uint32_t(ty[b])) * uint32_t(t.tz) + uint32_t(tz[cz]);
+          const int li = (lx[a] * kWin + ly[b]) * kWin + lz[cz];
+          const uint32_t c0 = chunk_off[tile], nc = nchunk[tile];
+          for (uint32_t q = 0; q < nc; ++q) {
+            const Acc *w = part + size_t(c0 + q) * NC * kWinCells + li;
+#pragma unroll
+            for (int ch = 0; ch < NC; ++ch) acc[ch] += w[ch * kWinCells];
+          }
+        }
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+      float v;
+      if (FIXED)
+        v = float(double((long long)acc[ch]) * inv);
+      else
+        v = float(acc[ch]);
+      if (has_overflow) v += overflow[size_t(ch) * M + e];
+      float *dst = mesh + size_t(ch) * M + e;
+      *dst = accumulate ? *dst + v : v;
+    }
+  }
+}
+
+}  // namespace jpm
+
+namespace jpm {
+
+int abs_max(hipStream_t s, const float *w, uint32_t n, float *out);
+int zero_f32(hipStream_t s, float *p, int64_t m);
+
+// Workspace layout of the binned paint (byte offsets).
+struct BinWs {
+  uint32_t G, slice, maxchunks;
+  size_t hist, sums, tstart, tcount, nchunk, choff, totals, desc, spos, sidx, partial, overflow, wmax, total;
+};
+
+static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
+  BinWs w;
+  const uint64_t nn = uint64_t(n > 0 ? n : 1);
+  w.G = uint32_t(std::min<uint64_t>(512, (nn + 2047) / 2048));
+  if (w.G < 1) w.G = 1;
+  w.slice = uint32_t((nn + w.G - 1) / w.G);
+  w.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
+  const size_t KG = size_t(t.K) * w.G;
+  const size_t nsums = (std::max<size_t>(KG, t.K) + kScanBlock - 1) / kScanBlock + 1;
+  const size_t acc = (flags & JPM_DETERMINISTIC) ? 8 : 4;
+  const size_t M = size_t(t.nx) * t.ny * t.nz;
+  size_t o = 0;
+  w.hist = o;     o += align_up(KG * 4);
+  w.sums = o;     o += align_up(nsums * 4);
+  w.tstart = o;   o += align_up(size_t(t.K) * 4);
+  w.tcount = o;   o += align_up(size_t(t.K) * 4);
+  w.nchunk = o;   o += align_up(size_t(t.K) * 4);
+  w.choff = o;    o += align_up(size_t(t.K) * 4);
+  w.totals = o;   o += align_up(16);
+  w.desc = o;     o += align_up(size_t(w.maxchunks) * sizeof(ChunkDesc));
+  w.spos = o;     o += align_up(size_t(nn) * 12);
+  w.sidx = o;     o += align_up(size_t(nn) * 4);
+  w.partial = o;  o += align_up(size_t(w.maxchunks) * nc * kWinCells * acc);
+  w.overflow = o; o += (flags & JPM_RELATIVE) ? align_up(size_t(nc) * M * 4) : 0;
+  w.wmax = o;     o += align_up(4);
+  w.total = o;
+  return w;
+}
+
+bool binned_paint_supported(int nx, int ny, int nz) {
+  return nx > 0 && ny > 0 && nz > 0 && make_tiles(nx, ny, nz).K <= uint32_t(kMaxKeysLds);
+}
+
+size_t binned_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
+  return bin_ws(make_tiles(nx, ny, nz), n, nc, flags).total;
+}
+
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    JPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(bytes)));
+  return JPM_OK;
+}
+
+template <bool REL, bool FIXED, int NC>
+static int launch_accum(hipStream_t s, const BinWs &w, char *base, const float *weight, float wscalar,
+                        const TileGrid &t, float wmax_host, uint32_t n) {
+  const size_t lds = size_t(NC) * kWinCells * (FIXED ? 8 : 4);
+  auto kern = bin_accum_kernel<REL, FIXED, NC>;
+  int rc = set_lds(kern, lds);
+  if (rc) return rc;
+  hipLaunchKernelGGL(kern, dim3(w.maxchunks), dim3(256), lds, s, reinterpret_cast<const float *>(base + w.spos),
+                     reinterpret_cast<const uint32_t *>(base + w.sidx), weight, wscalar,
+                     reinterpret_cast<const ChunkDesc *>(base + w.desc),
+                     reinterpret_cast<const uint32_t *>(base + w.totals) + 1, t,
+                     reinterpret_cast<const float *>(base + w.wmax), wmax_host, n,
+                     reinterpret_cast<float *>(base + w.partial), reinterpret_cast<float *>(base + w.overflow));
+  JPM_LAUNCHED("bin_accum_kernel");
+  return JPM_OK;
+}
+
+template <bool FIXED, int NC>
+static int launch_reduce(hipStream_t s, const BinWs &w, char *base, const TileGrid &t, int rel, float wmax_host,
+                         int has_w, uint32_t n, int accumulate, float *mesh) {
+  const int64_t M = int64_t(t.nx) * t.ny * t.nz;
+  hipLaunchKernelGGL((bin_reduce_kernel<FIXED, NC>), dim3(grid_for(M, 256, 256 * 128)), dim3(256), 0, s,
+                     reinterpret_cast<const float *>(base + w.partial), reinterpret_cast<const uint32_t *>(base + w.choff),
+                     reinterpret_cast<const uint32_t *>(base + w.nchunk), t,
+                     reinterpret_cast<const float *>(base + w.overflow), rel,
+                     reinterpret_cast<const float *>(base + w.wmax), wmax_host, has_w, n, accumulate, mesh);
+  JPM_LAUNCHED("bin_reduce_kernel");
+  return JPM_OK;
+}
+
+// nc meshes (stride nx*ny*nz) = [mesh +] paint of n particles, channel c with
+// weight[p*nc + c] (or wscalar when weight is NULL).
+int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
+                 float wscalar, int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  const TileGrid t = make_tiles(nx, ny, nz);
+  JPM_REQUIRE(t.K <= uint32_t(kMaxKeysLds), "mesh (%d,%d,%d) has %u tiles, above the binned paint's %d",
+              nx, ny, nz, t.K, kMaxKeysLds);
+  const int64_t M = cells(nx, ny, nz);
+  JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
+  const bool rel = flags & JPM_RELATIVE;
+  const bool fixed = flags & JPM_DETERMINISTIC;
+  JPM_REQUIRE(!rel || n == M, "relative painting needs one particle per lattice node (n=%lld, mesh %lld)",
+              (long long)n, (long long)M);
+  const BinWs w = bin_ws(t, n, nc, flags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "binned paint needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  const bool accumulate = !(flags & JPM_ZERO_MESH);
+  if (n == 0) {
+    if (!accumulate) return zero_f32(s, mesh, nc * M);
+    return JPM_OK;
+  }
+  JPM_REQUIRE(pos != nullptr, "positions are null");
+  char *base = static_cast<char *>(ws);
+  uint32_t *hist = reinterpret_cast<uint32_t *>(base + w.hist);
+  uint32_t *sums = reinterpret_cast<uint32_t *>(base + w.sums);
+  uint32_t *tstart = reinterpret_cast<uint32_t *>(base + w.tstart);
+  uint32_t *tcount = reinterpret_cast<uint32_t *>(base + w.tcount);
+  uint32_t *nchunk = reinterpret_cast<uint32_t *>(base + w.nchunk);
+  uint32_t *choff = reinterpret_cast<uint32_t *>(base + w.choff);
+  uint32_t *totals = reinterpret_cast<uint32_t *>(base + w.totals);  // [0] particles, [1] chunks
+  const uint32_t KG = t.K * w.G;
+  const size_t lds_hist = size_t(t.K) * 4;
+
+  // 1. histogram of tile keys per workgroup slice
+  {
+    auto kern = rel ? bin_hist_kernel<true> : bin_hist_kernel<false>;
+    rc = set_lds(kern, lds_hist);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(256), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G);
+    JPM_LAUNCHED("bin_hist_kernel");
+  }
+  // 2. scatter offsets, tile table, chunk table
+  rc = exclusive_scan(s, hist, KG, sums, totals);
+  if (rc) return rc;
+  hipLaunchKernelGGL(tile_table_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, hist, w.G, t.K, uint32_t(n),
+                     tstart, tcount, nchunk);
+  JPM_LAUNCHED("tile_table_kernel");
+  JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
+  rc = exclusive_scan(s, choff, t.K, sums, totals + 1);
+  if (rc) return rc;
+  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, tstart, tcount, choff, t.K,
+                     reinterpret_cast<ChunkDesc *>(base + w.desc));
+  JPM_LAUNCHED("chunk_desc_kernel");
+  // 3. scatter particles into tile order
+  {
+    auto kern = rel ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
+    rc = set_lds(kern, lds_hist);
+    if (rc) return rc;
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(256), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G,
+                       reinterpret_cast<float *>(base + w.spos), reinterpret_cast<uint32_t *>(base + w.sidx));
+    JPM_LAUNCHED("bin_scatter_kernel");
+  }
+  // 4. per-chunk LDS accumulation
+  if (rel) {
+    rc = zero_f32(s, reinterpret_cast<float *>(base + w.overflow), nc * M);
+    if (rc) return rc;
+  }
+  float wmax_host = fabsf(wscalar);
+  if (fixed && weight) {
+    rc = abs_max(s, weight, uint32_t(n * nc), reinterpret_cast<float *>(base + w.wmax));
+    if (rc) return rc;
+  }
+#define JPM_ACC(R, F, C) launch_accum<R, F, C>(s, w, base, weight, wscalar, t, wmax_host, uint32_t(n))
+  if (nc == 1) {
+    if (rel) rc = fixed ? JPM_ACC(true, true, 1) : JPM_ACC(true, false, 1);
+    else rc = fixed ? JPM_ACC(false, true, 1) : JPM_ACC(false, false, 1);
+  } else {
+    if (rel) rc = fixed ? JPM_ACC(true, true, 3) : JPM_ACC(true, false, 3);
+    else rc = fixed ? JPM_ACC(false, true, 3) : JPM_ACC(false, false, 3);
+  }
+#undef JPM_ACC
+  if (rc) return rc;
+  // 5. gather-reduce of the windows
+  const int hw = weight != nullptr;
+  if (nc == 1)
+    rc = fixed ? launch_reduce<true, 1>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh)
+               : launch_reduce<false, 1>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh);
+  else
+    rc = fixed ? launch_reduce<true, 3>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh)
+               : launch_reduce<false, 3>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh);
+  return rc;
+}
+
+}  // namespace jpm

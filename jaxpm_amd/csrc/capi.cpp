@@ -50,6 +50,32 @@ int lpt2_source_vjp(hipStream_t, int64_t, const float *, const float *, float *)
 int spectrum_half_to_full(hipStream_t, int, int, int, const void *, void *);
 
 int axpy(hipStream_t, int64_t, float, const float *, float *);
+int binned_paint(hipStream_t, int, int, int, const float *, int64_t, const float *, float, int, float *, uint32_t,
+                 void *, size_t);
+size_t binned_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
+bool binned_paint_supported(int, int, int);
+
+// Paint used inside the pipelines: the binned LDS paint unless JPM_ATOMIC or
+// the mesh has more tiles than the binned paint's LDS histogram holds.
+static bool use_atomic(int nx, int ny, int nz, uint32_t pflags) {
+  return (pflags & JPM_ATOMIC) || !binned_paint_supported(nx, ny, nz);
+}
+
+static size_t pipeline_paint_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pflags) {
+  if (use_atomic(nx, ny, nz, pflags))
+    return (pflags & JPM_DETERMINISTIC) ? align_up(size_t(nc) * size_t(cells(nx, ny, nz)) * 8) + 256 : 0;
+  return binned_paint_workspace_size(nx, ny, nz, n, nc, pflags);
+}
+
+static int pipeline_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *w, int nc,
+                          float *mesh, uint32_t pflags, void *ws, size_t ws_bytes) {
+  const uint32_t f = (pflags & (JPM_RELATIVE | JPM_DETERMINISTIC)) | JPM_ZERO_MESH;
+  if (use_atomic(nx, ny, nz, pflags)) {
+    if (nc == 1) return cic_paint(s, nx, ny, nz, pos, n, w, 1.0f, mesh, f, ws, ws_bytes);
+    return cic_paint_multi(s, nx, ny, nz, pos, n, w, nc, mesh, f, ws, ws_bytes);
+  }
+  return binned_paint(s, nx, ny, nz, pos, n, w, 1.0f, nc, mesh, f, ws, ws_bytes);
+}
 int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, float, float *, uint32_t);
 
 // Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
@@ -57,14 +83,14 @@ struct FieldWs {
   size_t mesh, spec, spec3, paint, total;
 };
 
-static FieldWs field_ws(int nx, int ny, int nz, uint32_t flags) {
+static FieldWs field_ws(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
   FieldWs w;
   w.mesh = 0;
   w.spec = align_up(m * sizeof(float));
   w.spec3 = w.spec + align_up(h * 8);
   w.paint = w.spec3 + align_up(3 * h * 8);
-  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? jpm_cic_paint_workspace_size(nx, ny, nz, int64_t(m), flags) : 0);
+  w.total = w.paint + pipeline_paint_ws(nx, ny, nz, n, 1, flags);
   return w;
 }
 
@@ -75,7 +101,7 @@ static int force_field(hipStream_t s, int nx, int ny, int nz, const float *pos, 
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
-  const FieldWs w = field_ws(nx, ny, nz, pflags);
+  const FieldWs w = field_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "force_field needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(phi3 != nullptr, "phi3 is null");
   char *base = static_cast<char *>(ws);
@@ -84,8 +110,7 @@ static int force_field(hipStream_t s, int nx, int ny, int nz, const float *pos, 
   void *spec3 = base + w.spec3;
   const int64_t m = cells(nx, ny, nz);
   if (!delta_k) {
-    rc = cic_paint(s, nx, ny, nz, pos, n, nullptr, 1.0f, mesh, pflags | JPM_ZERO_MESH, base + w.paint,
-                   w.total - w.paint);
+    rc = pipeline_paint(s, nx, ny, nz, pos, n, nullptr, 1, mesh, pflags, base + w.paint, w.total - w.paint);
     if (rc) return rc;
     rc = fft_r2c(s, nx, ny, nz, 1, mesh, spec);
     if (rc) return rc;
@@ -102,7 +127,7 @@ struct VjpWs {
   size_t g3, spec3, spec, rho, paint, total;
 };
 
-static VjpWs vjp_ws(int nx, int ny, int nz, uint32_t flags) {
+static VjpWs vjp_ws(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
   VjpWs w;
   w.g3 = 0;
@@ -110,7 +135,7 @@ static VjpWs vjp_ws(int nx, int ny, int nz, uint32_t flags) {
   w.spec = w.spec3 + align_up(3 * h * 8);
   w.rho = w.spec + align_up(h * 8);
   w.paint = w.rho + align_up(m * sizeof(float));
-  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? align_up(3 * m * 8) + 256 : 0);
+  w.total = w.paint + pipeline_paint_ws(nx, ny, nz, n, 3, flags);
   return w;
 }
 
@@ -125,7 +150,7 @@ static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, c
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
-  const VjpWs w = vjp_ws(nx, ny, nz, pflags);
+  const VjpWs w = vjp_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "force_vjp needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(phi3 && pos && cot && gpos, "null pointer argument");
   char *base = static_cast<char *>(ws);
@@ -135,8 +160,7 @@ static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, c
   float *rho = reinterpret_cast<float *>(base + w.rho);
   const int64_t m = cells(nx, ny, nz);
   const uint32_t rel = pflags & JPM_RELATIVE;
-  rc = cic_paint_multi(s, nx, ny, nz, pos, n, cot, 3, g3, (pflags & (JPM_RELATIVE | JPM_DETERMINISTIC)) | JPM_ZERO_MESH,
-                       base + w.paint, w.total - w.paint);
+  rc = pipeline_paint(s, nx, ny, nz, pos, n, cot, 3, g3, pflags, base + w.paint, w.total - w.paint);
   if (rc) return rc;
   rc = fft_r2c(s, nx, ny, nz, 3, g3, spec3);
   if (rc) return rc;
@@ -156,14 +180,14 @@ struct ForceWs {
   size_t meshes, spec, spec3, paint, total;
 };
 
-static ForceWs force_ws(int nx, int ny, int nz, uint32_t flags) {
+static ForceWs force_ws(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
   ForceWs w;
   w.meshes = 0;
   w.spec = w.meshes + align_up(3 * m * sizeof(float));
   w.spec3 = w.spec + align_up(h * 8);
   w.paint = w.spec3 + align_up(3 * h * 8);
-  w.total = w.paint + ((flags & JPM_DETERMINISTIC) ? jpm_cic_paint_workspace_size(nx, ny, nz, int64_t(m), flags) : 0);
+  w.total = w.paint + pipeline_paint_ws(nx, ny, nz, n, 1, flags);
   return w;
 }
 
@@ -172,7 +196,7 @@ static int pm_forces(hipStream_t s, int nx, int ny, int nz, const float *pos, in
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
-  const ForceWs w = force_ws(nx, ny, nz, pflags);
+  const ForceWs w = force_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "pm_forces needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(forces != nullptr, "forces is null");
   char *base = static_cast<char *>(ws);
@@ -181,8 +205,7 @@ static int pm_forces(hipStream_t s, int nx, int ny, int nz, const float *pos, in
   void *spec3 = base + w.spec3;
   const int64_t m = cells(nx, ny, nz);
   if (!delta_k) {
-    rc = cic_paint(s, nx, ny, nz, pos, n, nullptr, 1.0f, meshes, (pflags & ~JPM_ZERO_MESH) | JPM_ZERO_MESH,
-                   base + w.paint, w.total - w.paint);
+    rc = pipeline_paint(s, nx, ny, nz, pos, n, nullptr, 1, meshes, pflags, base + w.paint, w.total - w.paint);
     if (rc) return rc;
     rc = fft_r2c(s, nx, ny, nz, 1, meshes, spec);
     if (rc) return rc;
@@ -324,9 +347,9 @@ int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz, const vo
   JPM_GUARD(spectrum_half_to_full(as_stream(stream), nx, ny, nz, half, full));
 }
 
-size_t jpm_force_field_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+size_t jpm_force_field_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return field_ws(nx, ny, nz, flags & 0xffu).total;
+  return field_ws(nx, ny, nz, n, flags & 0xffu).total;
 }
 
 int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
@@ -334,9 +357,9 @@ int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos, 
   JPM_GUARD(force_field(as_stream(stream), nx, ny, nz, pos, n, delta_k, phi3, r_split, scale, flags, ws, ws_bytes));
 }
 
-size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return vjp_ws(nx, ny, nz, flags & 0xffu).total;
+  return vjp_ws(nx, ny, nz, n, flags & 0xffu).total;
 }
 
 int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n,
@@ -354,9 +377,18 @@ int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3, const 
   JPM_GUARD(kick(as_stream(stream), nx, ny, nz, phi3, pos, n, gamma, vel, flags));
 }
 
-size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, uint32_t flags) {
+size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return force_ws(nx, ny, nz, flags & 0xffu).total;
+  return force_ws(nx, ny, nz, n, flags & 0xffu).total;
+}
+
+size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
+  return binned_paint_workspace_size(nx, ny, nz, n, nc, flags);
+}
+
+int jpm_cic_paint_binned_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const float *weights,
+                             float wscalar, int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(binned_paint(as_stream(stream), nx, ny, nz, pos, n, weights, wscalar, nc, mesh, flags, ws, ws_bytes));
 }
 
 int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,

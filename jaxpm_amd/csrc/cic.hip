@@ -147,17 +147,6 @@ __device__ __forceinline__ void particle_cic(const float *__restrict__ pos, uint
 
 // ------------------------------------------------------------------ paint
 
-// 2^e with e the largest exponent such that n * wmax * 2^e < 2^62.  Every
-// particle spreads |w| over its corners (the CIC weights sum to 1), so no
-// cell of the fixed-point sum can leave the int64 range.
-__host__ __device__ inline double fixed_point_scale(float wmax, uint32_t n) {
-  double bound = double(wmax) * double(n > 0 ? n : 1);
-  if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
-  int e;
-  frexp(bound, &e);  // bound < 2^e
-  return ldexp(1.0, 62 - e);
-}
-
 template <bool REL, bool HAS_W>
 __global__ void __launch_bounds__(256)
 paint_atomic_kernel(const float *__restrict__ pos, const float *__restrict__ weight, float wscalar,
@@ -347,6 +336,14 @@ static Grid make_grid(int nx, int ny, int nz) {
   g.div_nz = FastDiv(uint32_t(nz));
   g.div_ny = FastDiv(uint32_t(ny));
   return g;
+}
+
+// *out = max |w[i]| over n floats (out is zeroed first).
+int abs_max(hipStream_t s, const float *w, uint32_t n, float *out) {
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, out, 1u);
+  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 256 * 128)), dim3(256), 0, s, w, n, out);
+  JPM_LAUNCHED("abs_max_kernel");
+  return JPM_OK;
 }
 
 int zero_f32(hipStream_t s, float *p, int64_t m) {

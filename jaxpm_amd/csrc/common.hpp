@@ -77,6 +77,18 @@ struct FastDiv {
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 
+// 2^e with e the largest exponent such that n * wmax * 2^e < 2^62: the scale
+// of the 64-bit fixed-point sums of the deterministic paint.  Every particle
+// spreads |w| over its corners (the CIC weights sum to 1), so no cell of the
+// fixed-point sum can leave the int64 range.
+__host__ __device__ inline double fixed_point_scale(float wmax, uint32_t n) {
+  double bound = double(wmax) * double(n > 0 ? n : 1);
+  if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
+  int e;
+  frexp(bound, &e);  // bound < 2^e
+  return ldexp(1.0, 62 - e);
+}
+
 // Number of blocks for a grid-stride loop over `n` items, capped so every
 // XCD gets many workgroups without launching more than the loop needs.
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 256 * 64) {

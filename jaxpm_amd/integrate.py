@@ -48,7 +48,7 @@ def kdk_schedule(cosmo, a_steps):
 class Stepper:
     """Device buffers and C-ABI calls of one mesh / particle configuration."""
 
-    def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0):
+    def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0, atomic=False):
         self.shape = tuple(int(s) for s in mesh_shape)
         self.nx, self.ny, self.nz = self.shape
         self.n = int(n)
@@ -56,12 +56,13 @@ class Stepper:
         if relative and self.n != self.m:
             raise ValueError("relative mode needs one particle per mesh cell")
         self.r_split = float(r_split)
-        pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0)
+        pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
+            (_lib.ATOMIC if atomic else 0)
         self.pflags = pflags
         self.flags = pflags | (kflags << 8)
         self.dev = _lib.device()
-        fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.flags)
-        vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.flags)
+        fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
+        vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         self.ws = torch.empty(max(fb, vb), dtype=torch.uint8, device=self.dev)
 
     def new_phi(self):
@@ -113,7 +114,6 @@ class NBody(torch.autograd.Function):
             if s0 + k < S:
                 ckpts[s0 + k] = (x.clone(), v.clone())
         ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = stepper, sched, k, ckpts
-        ctx.x0 = x0.detach()
         return x, v
 
     @staticmethod
@@ -121,8 +121,9 @@ class NBody(torch.autograd.Function):
         st = ctx.stepper
         k0, beta, gamma = ctx.sched
         S = len(beta)
-        lx = (gx if gx is not None else torch.zeros_like(ctx.x0)).detach().clone().contiguous()
-        lv = (gv if gv is not None else torch.zeros_like(ctx.x0)).detach().clone().contiguous()
+        x_ref = ctx.ckpts[0][0]
+        lx = (gx if gx is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
         phi = st.new_phi()
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
@@ -135,12 +136,10 @@ class NBody(torch.autograd.Function):
                 st.vjp(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
                 st.axpy(beta[i], lx, lv)            # drift i: x += beta v
             del keep
-        x0 = ctx.ckpts[0][0]
         # the first half kick used x0 before any drift: recompute its field
-        xs = ctx.x0.contiguous()
+        xs = ctx.ckpts[0][0]
         st.field(xs, phi)
         st.vjp(phi, xs, lv, k0, lx)
-        del x0
         ctx.ckpts = None
         return lx, lv, None, None, None
 

@@ -165,3 +165,94 @@ def test_compensate_cic(gpu):
     r = np.random.default_rng(16)
     f = r.normal(size=(16, 16, 32)).astype(np.float32)
     close(host(painting.compensate_cic(dev(f))), op.compensate_cic(f), rel=5e-6)
+
+
+# ----------------------------------------------------------- binned paint
+
+BINNED_SHAPES = [(8, 8, 8), (12, 10, 9), (17, 33, 20), (48, 48, 48), (32, 32, 64)]
+
+
+@pytest.mark.parametrize("shape", BINNED_SHAPES)
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_binned_paint_absolute(gpu, shape, deterministic):
+    from jaxpm_amd import _ops
+    print(f"binned abs {shape} det={deterministic}", flush=True)
+    r = np.random.default_rng(40)
+    n = int(np.prod(shape))
+    pos = (r.uniform(-0.5, 1.5, size=(n, 3)) * np.array(shape)).astype(np.float32)
+    w = r.normal(size=n).astype(np.float32)
+    base = r.normal(size=shape).astype(np.float32)
+    m = dev(base)
+    _ops.paint(m, dev(pos), deterministic=deterministic)                       # accumulate
+    close(host(m), op.cic_paint(base, pos))
+    m = dev(np.zeros(shape))
+    _ops.paint(m, dev(pos), weight=dev(w), deterministic=deterministic, zero=True)
+    close(host(m), op.cic_paint(np.zeros(shape), pos, w))
+
+
+@pytest.mark.parametrize("shape", BINNED_SHAPES)
+def test_binned_paint_relative(gpu, shape):
+    from jaxpm_amd import _ops
+    r = np.random.default_rng(41)
+    disp = r.normal(scale=3.0, size=(*shape, 3)).astype(np.float32)
+    disp[0, 0, 0] = (-1e-9, 0.0, 0.0)
+    m = dev(np.zeros(shape))
+    _ops.paint(m, dev(disp), relative=True, zero=True)
+    close(host(m), op.cic_paint_dx(disp, dtype=np.float32))
+    m2 = dev(np.zeros(shape))
+    _ops.paint(m2, dev(disp), relative=True, zero=True, deterministic=True)
+    close(host(m2), op.cic_paint_dx(disp, dtype=np.float32))
+
+
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_binned_paint_three_channels(gpu, deterministic):
+    from jaxpm_amd import _ops
+    shape = (24, 40, 32)
+    r = np.random.default_rng(42)
+    n = 30000
+    pos = (r.uniform(0, 1, size=(n, 3)) * np.array(shape)).astype(np.float32)
+    w3 = r.normal(size=(n, 3)).astype(np.float32)
+    got = host(_ops.paint_multi(shape, dev(pos), dev(w3), deterministic=deterministic))
+    for c in range(3):
+        close(got[c], op.cic_paint(np.zeros(shape), pos, w3[:, c]))
+    atomic = host(_ops.paint_multi(shape, dev(pos), dev(w3), method="atomic"))
+    close(got, atomic, rel=1e-5)
+
+
+def test_binned_paint_heavy_tile_and_empty(gpu):
+    """One tile holding 50k particles (13 chunks), most tiles empty, and the
+    bitwise reproducibility of the fixed-point mode."""
+    from jaxpm_amd import _ops
+    shape = (64, 64, 64)
+    r = np.random.default_rng(43)
+    pos = (20 + r.uniform(0, 8, size=(50000, 3))).astype(np.float32)
+    pos[:10] = r.uniform(0, 64, size=(10, 3))
+    m = dev(np.zeros(shape))
+    _ops.paint(m, dev(pos), zero=True)
+    close(host(m), op.cic_paint(np.zeros(shape), pos), rel=2e-6)
+    a = dev(np.zeros(shape))
+    b = dev(np.zeros(shape))
+    _ops.paint(a, dev(pos), zero=True, deterministic=True)
+    _ops.paint(b, dev(pos), zero=True, deterministic=True)
+    assert torch.equal(a, b)
+    close(host(a), op.cic_paint(np.zeros(shape), pos), rel=5e-7)
+    # no particles: zero mesh / untouched mesh
+    z = dev(np.ones(shape))
+    _ops.paint(z, dev(np.zeros((0, 3))), zero=True)
+    assert float(z.abs().max()) == 0.0
+
+
+def test_binned_matches_atomic_at_256(gpu):
+    from jaxpm_amd import _ops
+    shape = (256, 256, 256)
+    n = 256 ** 3
+    g = torch.Generator(device="cuda").manual_seed(3)
+    for p in (torch.rand((n, 3), generator=g, device="cuda") * 256,
+              128 + 6 * torch.randn((n, 3), generator=g, device="cuda")):
+        a = torch.zeros(shape, device="cuda")
+        b = torch.zeros(shape, device="cuda")
+        _ops.paint(a, p, method="binned")
+        _ops.paint(b, p, method="atomic")
+        scale = max(1.0, float(b.abs().max()))
+        assert float((a - b).abs().max()) < 2e-5 * scale
+        assert abs(a.double().sum().item() - n) < 1e-5 * n
