@@ -1,19 +1,24 @@
 // Binned CIC paint: counting sort of particles into 16^3-cell tiles, LDS
 // accumulation of each chunk of a tile's particles into the tile's (17^3)
 // window, and a gather-reduce that sums the windows covering each cell.
-// No global float atomics; with JPM_DETERMINISTIC the LDS sums are 64-bit
-// fixed point, so the result does not depend on the order of deposits.
+// No global float atomics.  The window sums are 64-bit fixed point (on gfx950
+// ds_add_u64 is several times faster than ds_add_f32), so the result is
+// bitwise reproducible whatever the order of deposits.
 //
 // Passes (all stream-ordered, no host synchronisation):
-//   hist     per-workgroup LDS histogram of tile keys -> hist[key][wg]
+//   hist     per-workgroup LDS histogram of tile keys -> hist[key][wg]; lanes
+//            read consecutive particles and each run of equal keys within a
+//            wavefront adds one count (Lagrangian order keeps runs long)
 //   scan     exclusive scan of hist (key-major) -> scatter offsets; tile
 //            start/count; chunk offsets (ceil(count / CH) chunks per tile)
 //   chunks   chunk descriptors (tile, start, count)
 //   scatter  each workgroup re-reads its slice and writes its particles to
-//            their tile's range (positions, original index, weights)
-//   accum    one workgroup per chunk: LDS window, deposits, window -> partial
-//   reduce   one thread per cell: sum the partial windows (up to 8 tiles,
-//            all their chunks) covering the cell, in a fixed order
+//            their tile's range as float4 (x, y, z, original index bits)
+//   accum    one workgroup per (chunk, channel): LDS window of int64, the
+//            deposits, window -> partial
+//   reduce   one workgroup per tile: for each of its cells, the tile's own
+//            chunk windows plus the +1 halos of the (up to 7) lower
+//            neighbour tiles, summed in a fixed order and converted
 #include <type_traits>
 
 #include "common.hpp"
@@ -23,8 +28,9 @@ namespace jpm {
 constexpr int kTile = 16;                       // tile edge in cells
 constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
 constexpr int kWinCells = kWin * kWin * kWin;   // 4913
-constexpr int kChunk = 4096;                    // max particles per chunk
+constexpr int kChunk = 8192;                    // max particles per chunk
 constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
+constexpr int kBinThreads = 256;                // hist / scatter workgroup size
 constexpr int kScanItems = 8;                   // scan: elements per thread
 constexpr int kScanBlock = 256 * kScanItems;
 
@@ -101,8 +107,28 @@ __device__ __forceinline__ uint32_t tile_key(int cx, int cy, int cz, const TileG
 
 // ---------------------------------------------------------------- hist
 
+// Runs of equal keys among the 64 lanes of a wavefront: `heads` has bit l set
+// when lane l starts a run (its key differs from lane l-1's, or l is the
+// first active lane); a head's run ends where the next head begins.
+__device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool active) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(key, 1);
+  const bool head = active && (lane == 0 || key != prev);
+  return __ballot(head);
+}
+
+__device__ __forceinline__ uint32_t run_length(unsigned long long heads, unsigned long long act) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+  const int next = above ? __ffsll((long long)above) - 1 : 64;
+  // active lanes are a prefix of the wavefront (p < p1 tail), so the run
+  // stops at the first inactive lane too
+  const int last_active = 63 - __clzll((long long)act);
+  return uint32_t(min(next, last_active + 1) - lane);
+}
+
 template <bool REL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBinThreads)
 bin_hist_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t, uint32_t *__restrict__ hist,
                 uint32_t G) {
   extern __shared__ uint32_t sh[];
@@ -111,10 +137,18 @@ bin_hist_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileG
   __syncthreads();
   const uint32_t p0 = g * slice;
   const uint32_t p1 = min(n, p0 + slice);
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    int cx, cy, cz;
-    floor_cells<REL>(pos, p, t, cx, cy, cz);
-    atomicAdd(&sh[tile_key(cx, cy, cz, t)], 1u);
+  for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
+    const uint32_t p = b0 + threadIdx.x;
+    const bool active = p < p1;
+    uint32_t key = 0xffffffffu;
+    if (active) {
+      int cx, cy, cz;
+      floor_cells<REL>(pos, p, t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, t);
+    }
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key, active);
+    if (active && ((heads >> (threadIdx.x & 63)) & 1ull)) atomicAdd(&sh[key], run_length(heads, act));
   }
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) hist[size_t(k) * G + g] = sh[k];
@@ -224,7 +258,7 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
       d.tile = k;
       d.start = st + o;
       d.count = min(uint32_t(kChunk), cnt - o);
-      d.pad = 0;
+      d.pad = cnt > uint32_t(kChunk) ? 1u : 0u;  // tile split over several chunks
       desc[c] = d;
     }
   }
@@ -233,23 +267,40 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
 // -------------------------------------------------------------- scatter
 
 template <bool REL>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBinThreads)
 bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t,
-                   const uint32_t *__restrict__ offs, uint32_t G, float *__restrict__ spos, uint32_t *__restrict__ sidx) {
+                   const uint32_t *__restrict__ offs, uint32_t G, float4 *__restrict__ spos) {
   extern __shared__ uint32_t cur[];
   const uint32_t g = blockIdx.x;
   for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) cur[k] = offs[size_t(k) * G + g];
   __syncthreads();
   const uint32_t p0 = g * slice;
   const uint32_t p1 = min(n, p0 + slice);
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x) {
-    int cx, cy, cz;
-    floor_cells<REL>(pos, p, t, cx, cy, cz);
-    const uint32_t dst = atomicAdd(&cur[tile_key(cx, cy, cz, t)], 1u);
-    spos[3 * size_t(dst) + 0] = pos[3 * size_t(p) + 0];
-    spos[3 * size_t(dst) + 1] = pos[3 * size_t(p) + 1];
-    spos[3 * size_t(dst) + 2] = pos[3 * size_t(p) + 2];
-    sidx[dst] = p;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
+    const uint32_t p = b0 + threadIdx.x;
+    const bool active = p < p1;
+    uint32_t key = 0xffffffffu;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (active) {
+      x = pos[3 * size_t(p)];
+      y = pos[3 * size_t(p) + 1];
+      z = pos[3 * size_t(p) + 2];
+      int cx, cy, cz;
+      floor_cells<REL>(pos, p, t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, t);
+    }
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key, active);
+    const bool head = active && ((heads >> lane) & 1ull);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cur[key], run_length(heads, act));
+    // each lane takes the base of the run it belongs to (its nearest head at
+    // or below it) plus its distance from that head
+    const unsigned long long below = heads & ((2ull << lane) - 1ull);
+    const int my_head = below ? 63 - __clzll((long long)below) : 0;
+    base = __shfl(base, my_head);
+    if (active) spos[base + uint32_t(lane - my_head)] = make_float4(x, y, z, __uint_as_float(p));
   }
 }
 
@@ -294,31 +345,29 @@ __device__ __forceinline__ void lcorner_rel(float p, float o, int n, int origin,
   l = (d >= 0 && d < kWin) ? d : -1;
 }
 
-template <bool REL, bool FIXED, int NC>
+template <bool REL>
 __global__ void __launch_bounds__(256)
-bin_accum_kernel(const float *__restrict__ spos, const uint32_t *__restrict__ sidx,
-                 const float *__restrict__ weight, float wscalar, const ChunkDesc *__restrict__ desc,
-                 const uint32_t *__restrict__ nchunks_total, TileGrid t, const float *__restrict__ wmax_dev,
-                 float wmax_host, uint32_t n, float *__restrict__ partial, float *__restrict__ overflow) {
-  extern __shared__ unsigned char smem[];
+bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
+                 const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, TileGrid t,
+                 const float *__restrict__ wmax_dev, float wmax_host, uint32_t n, float *__restrict__ partial,
+                 float *__restrict__ overflow) {
+  __shared__ unsigned long long win[kWinCells];
   const uint32_t c = blockIdx.x;
+  const int ch = int(blockIdx.y);
   if (c >= *nchunks_total) return;
   const ChunkDesc d = desc[c];
-  const uint32_t tz = t.div_tz.div(d.tile);
-  const uint32_t tzr = d.tile - tz * uint32_t(t.tz);
-  const uint32_t ty = t.div_ty.div(tz);
-  const uint32_t tyr = tz - ty * uint32_t(t.ty);
-  const int ox = int(ty) * kTile, oy = int(tyr) * kTile, oz = int(tzr) * kTile;
-  typedef typename std::conditional<FIXED, unsigned long long, float>::type Acc;
-  Acc *win = reinterpret_cast<Acc *>(smem);
-  for (int i = threadIdx.x; i < NC * kWinCells; i += blockDim.x) win[i] = Acc(0);
-  double scale = 1.0;
-  if (FIXED) scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
+  const uint32_t txy = t.div_tz.div(d.tile);
+  const uint32_t itz = d.tile - txy * uint32_t(t.tz);
+  const uint32_t itx = t.div_ty.div(txy);
+  const uint32_t ity = txy - itx * uint32_t(t.ty);
+  const int ox = int(itx) * kTile, oy = int(ity) * kTile, oz = int(itz) * kTile;
+  for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) win[i] = 0ull;
+  const double scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
-    const uint32_t s = d.start + q;
-    const float x = spos[3 * size_t(s)], y = spos[3 * size_t(s) + 1], z = spos[3 * size_t(s) + 2];
-    const uint32_t pidx = sidx[s];
+    const float4 rec = spos[d.start + q];
+    const float x = rec.x, y = rec.y, z = rec.z;
+    const uint32_t pidx = __float_as_uint(rec.w);
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
     float wx[2], wy[2], wz[2];
     if (REL) {
@@ -339,105 +388,99 @@ bin_accum_kernel(const float *__restrict__ spos, const uint32_t *__restrict__ si
       ly[0] = ay.l0; ly[1] = ay.l1; wy[0] = ay.w0; wy[1] = ay.w1;
       lz[0] = az.l0; lz[1] = az.l1; wz[0] = az.w0; wz[1] = az.w1;
     }
-    float wp[NC];
-#pragma unroll
-    for (int ch = 0; ch < NC; ++ch)
-      wp[ch] = weight ? weight[size_t(pidx) * NC + ch] : wscalar;
+    const float wp = weight ? weight[size_t(pidx) * nc + ch] : wscalar;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
       const float wc = wx[bx] * wy[by] * wz[bz];
       const int Lx = lx[bx], Ly = ly[by], Lz = lz[bz];
       if (REL && (Lx == -2 || Ly == -2 || Lz == -2)) continue;   // dropped corner
+      const float v = wp * wc;
       if (!REL || (Lx >= 0 && Ly >= 0 && Lz >= 0)) {
-        const int li = (Lx * kWin + Ly) * kWin + Lz;
-#pragma unroll
-        for (int ch = 0; ch < NC; ++ch) {
-          const float v = wp[ch] * wc;
-          if (FIXED)
-            atomicAdd(reinterpret_cast<unsigned long long *>(&win[ch * kWinCells + li]),
-                      (unsigned long long)(long long)rint(double(v) * scale));
-          else
-            atomicAdd(reinterpret_cast<float *>(&win[ch * kWinCells + li]), v);
-        }
+        atomicAdd(&win[(Lx * kWin + Ly) * kWin + Lz], (unsigned long long)(long long)rint(double(v) * scale));
       } else {
         // rounding put a relative corner outside the window: global path
         const uint32_t cell = (uint32_t(gx[bx]) * uint32_t(t.ny) + uint32_t(gy[by])) * uint32_t(t.nz) + uint32_t(gz[bz]);
-#pragma unroll
-        for (int ch = 0; ch < NC; ++ch)
-          atomicAdd(&overflow[size_t(ch) * t.nx * t.ny * t.nz + cell], wp[ch] * wc);
+        atomicAdd(&overflow[size_t(ch) * t.nx * t.ny * t.nz + cell], v);
       }
     }
   }
   __syncthreads();
-  Acc *out = reinterpret_cast<Acc *>(partial) + size_t(c) * NC * kWinCells;
-  for (int i = threadIdx.x; i < NC * kWinCells; i += blockDim.x) out[i] = win[i];
+  // A tile with a single chunk stores its window rounded once to float.  A
+  // tile split over several chunks keeps the exact int64 windows: which
+  // particle lands in which chunk depends on the scatter's atomic order, so
+  // only exact sums across its chunks are reproducible bit for bit.
+  const size_t slot = (size_t(c) * nc + ch) * kWinCells;
+  if (d.pad == 0) {
+    const double inv = 1.0 / scale;
+    float *out = partial + slot * 2;
+    for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) out[i] = float(double((long long)win[i]) * inv);
+  } else {
+    unsigned long long *out = reinterpret_cast<unsigned long long *>(partial) + slot;
+    for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) out[i] = win[i];
+  }
 }
 
 // --------------------------------------------------------------- reduce
 
-// Tiles (and window offsets) covering cell c along one axis: the tile that
-// owns c at local index c - origin, and, when c starts a tile (or is 0), the
-// tile owning c - 1 (mod n) at local index tile_len (its +1 halo).
-__device__ __forceinline__ int axis_contrib(int c, int n, int ntile, int tid[2], int loc[2]) {
-  const int t0 = c / kTile;
-  tid[0] = t0;
-  loc[0] = c - t0 * kTile;
-  if (loc[0] != 0) return 1;
-  const int pc = (c == 0) ? n - 1 : c - 1;
-  const int t1 = pc / kTile;
-  tid[1] = t1;
-  loc[1] = pc - t1 * kTile + 1;
-  (void)ntile;
-  return 2;
+// One workgroup per tile.  Cell (a, b, c) of tile T (local coordinates)
+// receives the deposits of T's own windows at (a, b, c) and, on T's low
+// faces, the +1 halo of the lower neighbour(s): along an axis where the local
+// coordinate is 0, the neighbour at -1 (periodic) contributes at its own
+// local index = its tile length.  The (up to 8) sources and each tile's
+// chunks are summed in a fixed order.
+__device__ __forceinline__ int tile_len(int it, int ntile, int n) {
+  return (it == ntile - 1) ? n - it * kTile : kTile;
 }
 
-template <bool FIXED, int NC>
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
-                  const uint32_t *__restrict__ nchunk, TileGrid t, const float *__restrict__ overflow, int has_overflow,
-                  const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n, int accumulate,
-                  float *__restrict__ mesh) {
+                  const uint32_t *__restrict__ nchunk, TileGrid t, int nc, const float *__restrict__ overflow,
+                  int has_overflow, const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n,
+                  int accumulate, float *__restrict__ mesh) {
+  const unsigned long long *ipart = reinterpret_cast<const unsigned long long *>(partial);
+  const double inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
+  const uint32_t tile = blockIdx.x;
+  const int ch = int(blockIdx.y);
+  const uint32_t txy = t.div_tz.div(tile);
+  const int itz = int(tile - txy * uint32_t(t.tz));
+  const int itx = int(t.div_ty.div(txy));
+  const int ity = int(txy - uint32_t(itx) * uint32_t(t.ty));
+  const int Lx = tile_len(itx, t.tx, t.nx), Ly = tile_len(ity, t.ty, t.ny), Lz = tile_len(itz, t.tz, t.nz);
+  // lower neighbours (periodic) and their lengths
+  const int px = itx ? itx - 1 : t.tx - 1, py = ity ? ity - 1 : t.ty - 1, pz = itz ? itz - 1 : t.tz - 1;
+  const int PLx = tile_len(px, t.tx, t.nx), PLy = tile_len(py, t.ty, t.ny), PLz = tile_len(pz, t.tz, t.nz);
   const uint32_t M = uint32_t(t.nx) * t.ny * t.nz;
-  typedef typename std::conditional<FIXED, unsigned long long, float>::type Acc;
-  const Acc *part = reinterpret_cast<const Acc *>(partial);
-  double inv = 1.0;
-  if (FIXED) inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
-    uint32_t r = t.div_nz.div(e);
-    const int k = int(e - r * uint32_t(t.nz));
-    const uint32_t i = t.div_ny.div(r);
-    const int j = int(r - i * uint32_t(t.ny));
-    int tx[2], lx[2], ty[2], ly[2], tz[2], lz[2];
-    const int nxc = axis_contrib(int(i), t.nx, t.tx, tx, lx);
-    const int nyc = axis_contrib(j, t.ny, t.ty, ty, ly);
-    const int nzc = axis_contrib(k, t.nz, t.tz, tz, lz);
-    Acc acc[NC];
-#pragma unroll
-    for (int ch = 0; ch < NC; ++ch) acc[ch] = Acc(0);
-    for (int a = 0; a < nxc; ++a)
-      for (int b = 0; b < nyc; ++b)
-        for (int cz = 0; cz < nzc; ++cz) {
-          const uint32_t tile = (uint32_t(tx[a]) * uint32_t(t.ty) + uint32_t(ty[b])) * uint32_t(t.tz) + uint32_t(tz[cz]);
-          const int li = (lx[a] * kWin + ly[b]) * kWin + lz[cz];
-          const uint32_t c0 = chunk_off[tile], nc = nchunk[tile];
-          for (uint32_t q = 0; q < nc; ++q) {
-            const Acc *w = part + size_t(c0 + q) * NC * kWinCells + li;
-#pragma unroll
-            for (int ch = 0; ch < NC; ++ch) acc[ch] += w[ch * kWinCells];
+  const int cells_in_tile = Lx * Ly * Lz;
+  for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
+    const int c = e % Lz;
+    const int ab = e / Lz;
+    const int b = ab % Ly;
+    const int a = ab / Ly;
+    float acc = 0.0f;
+    const int nxs = a == 0 ? 2 : 1, nys = b == 0 ? 2 : 1, nzs = c == 0 ? 2 : 1;
+    for (int sx = 0; sx < nxs; ++sx)
+      for (int sy = 0; sy < nys; ++sy)
+        for (int sz = 0; sz < nzs; ++sz) {
+          const int jx = sx ? px : itx, jy = sy ? py : ity, jz = sz ? pz : itz;
+          const int la = sx ? PLx : a, lb = sy ? PLy : b, lc = sz ? PLz : c;
+          const uint32_t src = (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
+          const int li = (la * kWin + lb) * kWin + lc;
+          const uint32_t c0 = chunk_off[src], cn = nchunk[src];
+          if (cn == 1) {
+            acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + li];
+          } else if (cn > 1) {
+            unsigned long long s = 0ull;
+            for (uint32_t q = 0; q < cn; ++q) s += ipart[(size_t(c0 + q) * nc + ch) * kWinCells + li];
+            acc += float(double((long long)s) * inv);
           }
         }
-#pragma unroll
-    for (int ch = 0; ch < NC; ++ch) {
-      float v;
-      if (FIXED)
-        v = float(double((long long)acc[ch]) * inv);
-      else
-        v = float(acc[ch]);
-      if (has_overflow) v += overflow[size_t(ch) * M + e];
-      float *dst = mesh + size_t(ch) * M + e;
-      *dst = accumulate ? *dst + v : v;
-    }
+    float v = acc;
+    const uint32_t cell = (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) * uint32_t(t.nz) +
+                          uint32_t(itz * kTile + c);
+    if (has_overflow) v += overflow[size_t(ch) * M + cell];
+    float *dst = mesh + size_t(ch) * M + cell;
+    *dst = accumulate ? *dst + v : v;
   }
 }
 
@@ -451,19 +494,18 @@ int zero_f32(hipStream_t s, float *p, int64_t m);
 // Workspace layout of the binned paint (byte offsets).
 struct BinWs {
   uint32_t G, slice, maxchunks;
-  size_t hist, sums, tstart, tcount, nchunk, choff, totals, desc, spos, sidx, partial, overflow, wmax, total;
+  size_t hist, sums, tstart, tcount, nchunk, choff, totals, desc, spos, partial, overflow, wmax, total;
 };
 
 static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
   BinWs w;
   const uint64_t nn = uint64_t(n > 0 ? n : 1);
-  w.G = uint32_t(std::min<uint64_t>(512, (nn + 2047) / 2048));
+  w.G = uint32_t(std::min<uint64_t>(1024, (nn + 4095) / 4096));
   if (w.G < 1) w.G = 1;
   w.slice = uint32_t((nn + w.G - 1) / w.G);
   w.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
   const size_t KG = size_t(t.K) * w.G;
   const size_t nsums = (std::max<size_t>(KG, t.K) + kScanBlock - 1) / kScanBlock + 1;
-  const size_t acc = (flags & JPM_DETERMINISTIC) ? 8 : 4;
   const size_t M = size_t(t.nx) * t.ny * t.nz;
   size_t o = 0;
   w.hist = o;     o += align_up(KG * 4);
@@ -474,9 +516,8 @@ static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
   w.choff = o;    o += align_up(size_t(t.K) * 4);
   w.totals = o;   o += align_up(16);
   w.desc = o;     o += align_up(size_t(w.maxchunks) * sizeof(ChunkDesc));
-  w.spos = o;     o += align_up(size_t(nn) * 12);
-  w.sidx = o;     o += align_up(size_t(nn) * 4);
-  w.partial = o;  o += align_up(size_t(w.maxchunks) * nc * kWinCells * acc);
+  w.spos = o;     o += align_up(size_t(nn) * 16);
+  w.partial = o;  o += align_up(size_t(w.maxchunks) * nc * kWinCells * 8);
   w.overflow = o; o += (flags & JPM_RELATIVE) ? align_up(size_t(nc) * M * 4) : 0;
   w.wmax = o;     o += align_up(4);
   w.total = o;
@@ -500,36 +541,6 @@ static int set_lds(K kernel, size_t bytes) {
   return JPM_OK;
 }
 
-template <bool REL, bool FIXED, int NC>
-static int launch_accum(hipStream_t s, const BinWs &w, char *base, const float *weight, float wscalar,
-                        const TileGrid &t, float wmax_host, uint32_t n) {
-  const size_t lds = size_t(NC) * kWinCells * (FIXED ? 8 : 4);
-  auto kern = bin_accum_kernel<REL, FIXED, NC>;
-  int rc = set_lds(kern, lds);
-  if (rc) return rc;
-  hipLaunchKernelGGL(kern, dim3(w.maxchunks), dim3(256), lds, s, reinterpret_cast<const float *>(base + w.spos),
-                     reinterpret_cast<const uint32_t *>(base + w.sidx), weight, wscalar,
-                     reinterpret_cast<const ChunkDesc *>(base + w.desc),
-                     reinterpret_cast<const uint32_t *>(base + w.totals) + 1, t,
-                     reinterpret_cast<const float *>(base + w.wmax), wmax_host, n,
-                     reinterpret_cast<float *>(base + w.partial), reinterpret_cast<float *>(base + w.overflow));
-  JPM_LAUNCHED("bin_accum_kernel");
-  return JPM_OK;
-}
-
-template <bool FIXED, int NC>
-static int launch_reduce(hipStream_t s, const BinWs &w, char *base, const TileGrid &t, int rel, float wmax_host,
-                         int has_w, uint32_t n, int accumulate, float *mesh) {
-  const int64_t M = int64_t(t.nx) * t.ny * t.nz;
-  hipLaunchKernelGGL((bin_reduce_kernel<FIXED, NC>), dim3(grid_for(M, 256, 256 * 128)), dim3(256), 0, s,
-                     reinterpret_cast<const float *>(base + w.partial), reinterpret_cast<const uint32_t *>(base + w.choff),
-                     reinterpret_cast<const uint32_t *>(base + w.nchunk), t,
-                     reinterpret_cast<const float *>(base + w.overflow), rel,
-                     reinterpret_cast<const float *>(base + w.wmax), wmax_host, has_w, n, accumulate, mesh);
-  JPM_LAUNCHED("bin_reduce_kernel");
-  return JPM_OK;
-}
-
 // nc meshes (stride nx*ny*nz) = [mesh +] paint of n particles, channel c with
 // weight[p*nc + c] (or wscalar when weight is NULL).
 int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
@@ -545,7 +556,6 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
   const int64_t M = cells(nx, ny, nz);
   JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
   const bool rel = flags & JPM_RELATIVE;
-  const bool fixed = flags & JPM_DETERMINISTIC;
   JPM_REQUIRE(!rel || n == M, "relative painting needs one particle per lattice node (n=%lld, mesh %lld)",
               (long long)n, (long long)M);
   const BinWs w = bin_ws(t, n, nc, flags);
@@ -572,7 +582,7 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     auto kern = rel ? bin_hist_kernel<true> : bin_hist_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(256), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G);
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G);
     JPM_LAUNCHED("bin_hist_kernel");
   }
   // 2. scatter offsets, tile table, chunk table
@@ -592,39 +602,33 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     auto kern = rel ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(256), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G,
-                       reinterpret_cast<float *>(base + w.spos), reinterpret_cast<uint32_t *>(base + w.sidx));
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G,
+                       reinterpret_cast<float4 *>(base + w.spos));
     JPM_LAUNCHED("bin_scatter_kernel");
   }
-  // 4. per-chunk LDS accumulation
+  // 4. per-(chunk, channel) LDS accumulation
   if (rel) {
     rc = zero_f32(s, reinterpret_cast<float *>(base + w.overflow), nc * M);
     if (rc) return rc;
   }
-  float wmax_host = fabsf(wscalar);
-  if (fixed && weight) {
+  const float wmax_host = fabsf(wscalar);
+  if (weight) {
     rc = abs_max(s, weight, uint32_t(n * nc), reinterpret_cast<float *>(base + w.wmax));
     if (rc) return rc;
   }
-#define JPM_ACC(R, F, C) launch_accum<R, F, C>(s, w, base, weight, wscalar, t, wmax_host, uint32_t(n))
-  if (nc == 1) {
-    if (rel) rc = fixed ? JPM_ACC(true, true, 1) : JPM_ACC(true, false, 1);
-    else rc = fixed ? JPM_ACC(false, true, 1) : JPM_ACC(false, false, 1);
-  } else {
-    if (rel) rc = fixed ? JPM_ACC(true, true, 3) : JPM_ACC(true, false, 3);
-    else rc = fixed ? JPM_ACC(false, true, 3) : JPM_ACC(false, false, 3);
-  }
-#undef JPM_ACC
-  if (rc) return rc;
-  // 5. gather-reduce of the windows
-  const int hw = weight != nullptr;
-  if (nc == 1)
-    rc = fixed ? launch_reduce<true, 1>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh)
-               : launch_reduce<false, 1>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh);
-  else
-    rc = fixed ? launch_reduce<true, 3>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh)
-               : launch_reduce<false, 3>(s, w, base, t, rel, wmax_host, hw, uint32_t(n), accumulate, mesh);
-  return rc;
+  auto *partial = reinterpret_cast<float *>(base + w.partial);
+  const float *wmax_dev = reinterpret_cast<const float *>(base + w.wmax);
+  float *overflow = reinterpret_cast<float *>(base + w.overflow);
+  hipLaunchKernelGGL(rel ? bin_accum_kernel<true> : bin_accum_kernel<false>, dim3(w.maxchunks, nc), dim3(256), 0, s,
+                     reinterpret_cast<const float4 *>(base + w.spos), weight, nc, wscalar,
+                     reinterpret_cast<const ChunkDesc *>(base + w.desc), totals + 1, t, wmax_dev, wmax_host,
+                     uint32_t(n), partial, overflow);
+  JPM_LAUNCHED("bin_accum_kernel");
+  // 5. per-tile gather-reduce of the windows
+  hipLaunchKernelGGL(bin_reduce_kernel, dim3(t.K, nc), dim3(256), 0, s, partial, choff, nchunk, t, nc, overflow,
+                     int(rel), wmax_dev, wmax_host, int(weight != nullptr), uint32_t(n), int(accumulate), mesh);
+  JPM_LAUNCHED("bin_reduce_kernel");
+  return JPM_OK;
 }
 
 }  // namespace jpm

@@ -186,13 +186,19 @@ paint_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weig
 
 __global__ void __launch_bounds__(256)
 abs_max_kernel(const float *__restrict__ w, uint32_t n, float *__restrict__ out) {
+  __shared__ float sh[4];
   float m = 0.0f;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
     m = fmaxf(m, fabsf(w[p]));
-  // wave reduction, then one atomic per wave (values are >= 0, so the
-  // unsigned bit pattern orders like the float)
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(reinterpret_cast<unsigned int *>(out), __float_as_uint(m));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  // one atomic per workgroup; values are >= 0, so the unsigned bit pattern
+  // orders like the float
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    atomicMax(reinterpret_cast<unsigned int *>(out), __float_as_uint(m));
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -341,7 +347,7 @@ static Grid make_grid(int nx, int ny, int nz) {
 // *out = max |w[i]| over n floats (out is zeroed first).
 int abs_max(hipStream_t s, const float *w, uint32_t n, float *out) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, out, 1u);
-  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 256 * 128)), dim3(256), 0, s, w, n, out);
+  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, w, n, out);
   JPM_LAUNCHED("abs_max_kernel");
   return JPM_OK;
 }
@@ -393,7 +399,7 @@ int cic_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n
   JPM_LAUNCHED("zero_u64_kernel");
   if (hw) {
     hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
-    hipLaunchKernelGGL(abs_max_kernel, dim3(grid), dim3(256), 0, s, weight, uint32_t(n), wmax);
+    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, weight, uint32_t(n), wmax);
     JPM_LAUNCHED("abs_max_kernel");
   }
   const float wh = fabsf(wscalar);
@@ -501,7 +507,7 @@ int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int
   float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(nc) * size_t(m) * 8));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m));
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
-  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 256 * 128)), dim3(256), 0, s, weight,
+  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 2048)), dim3(256), 0, s, weight,
                      uint32_t(n * nc), wmax);
   JPM_LAUNCHED("abs_max_kernel");
 #define JPM_PM(R, C)                                                                                  \
