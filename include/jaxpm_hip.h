@@ -162,6 +162,26 @@ int jpm_kspace_force_f32(void *stream, int nx, int ny, int nz,
                          const void *delta_k, void *out3, float r_split,
                          float scale, uint32_t flags);
 
+/* out = scale * G(k) Lr(k) [C(k)] delta_k: the potential of pm.py:51-52
+ * (one read, one write of the half spectrum; out may not alias delta_k).
+ * With the default 4-point gradient the pipelines inverse-transform this
+ * once and take the gradient in real space (jpm_fd4_gradient_f32). */
+int jpm_kspace_potential_f32(void *stream, int nx, int ny, int nz,
+                             const void *delta_k, void *out, float r_split,
+                             float scale, uint32_t flags);
+
+/* out3[d] = -scale * D_d phi with the periodic 4-point central difference
+ * D f(c) = (8 (f(c+1) - f(c-1)) - (f(c+2) - f(c-2))) / 12, whose Fourier
+ * multiplier is exactly gradient_kernel(order=1) = i (8 sin w - sin 2w) / 6
+ * (kernels.py:63-66); -D_d IFFT[P] == IFFT[-g_d P] (pm.py:54-56). */
+int jpm_fd4_gradient_f32(void *stream, int nx, int ny, int nz,
+                         const float *phi, float scale, float *out3);
+
+/* out = scale * sum_d D_d g3[d]: the transpose of jpm_fd4_gradient_f32
+ * (D^T = -D), used by the force adjoint. */
+int jpm_fd4_divergence_f32(void *stream, int nx, int ny, int nz,
+                           const float *g3, float scale, float *out);
+
 /* Multiply a half spectrum in place by scale * C(k) (cic_compensation,
  * kernels.py:118-136): the k-space step of compensate_cic,
  * painting.py:263-275. */

@@ -47,6 +47,9 @@ SIGNATURES = {
     "jpm_kspace_force_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
     "jpm_kspace_force_adjoint_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
     "jpm_kspace_compensate_f32": (_i, [_vp, _i, _i, _i, _vp, _f]),
+    "jpm_kspace_potential_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
+    "jpm_fd4_gradient_f32": (_i, [_vp, _i, _i, _i, _vp, _f, _vp]),
+    "jpm_fd4_divergence_f32": (_i, [_vp, _i, _i, _i, _vp, _f, _vp]),
     "jpm_kspace_linear_field_f32": (_i, [_vp, _i, _i, _i, _vp, _f, _f, _f, _vp, _vp, _i, _f]),
     "jpm_kspace_shear6_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _u32]),
     "jpm_lpt2_source_f32": (_i, [_vp, _i64, _vp, _vp]),

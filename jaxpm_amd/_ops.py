@@ -205,6 +205,31 @@ def kspace_force(dk, shape, r_split=0.0, scale=1.0, flags=0):
     return out
 
 
+def kspace_potential(dk, shape, r_split=0.0, scale=1.0, flags=0):
+    dk = _c64(dk, "delta_k")
+    nx, ny, nz = _dims(shape)
+    out = torch.empty(half_shape(shape), dtype=torch.complex64, device=dk.device)
+    call("jpm_kspace_potential_f32", stream(), nx, ny, nz, ptr(dk), ptr(out), float(r_split), float(scale), flags)
+    return out
+
+
+def fd4_gradient(phi, scale=1.0):
+    """(nx,ny,nz) -> (3,nx,ny,nz): -scale * 4-point central difference."""
+    phi = _f32(phi, "potential")
+    nx, ny, nz = _dims(phi.shape)
+    out = torch.empty((3, nx, ny, nz), dtype=torch.float32, device=phi.device)
+    call("jpm_fd4_gradient_f32", stream(), nx, ny, nz, ptr(phi), float(scale), ptr(out))
+    return out
+
+
+def fd4_divergence(g3, scale=1.0):
+    g3 = _f32(g3, "meshes")
+    nx, ny, nz = _dims(g3.shape[1:])
+    out = torch.empty((nx, ny, nz), dtype=torch.float32, device=g3.device)
+    call("jpm_fd4_divergence_f32", stream(), nx, ny, nz, ptr(g3), float(scale), ptr(out))
+    return out
+
+
 def kspace_force_adjoint(in3, shape, r_split=0.0, scale=1.0, flags=0):
     in3 = _c64(in3, "spectra")
     nx, ny, nz = _dims(shape)

@@ -6,14 +6,16 @@
 // bitwise reproducible whatever the order of deposits.
 //
 // Passes (all stream-ordered, no host synchronisation):
-//   hist     per-workgroup LDS histogram of tile keys -> hist[key][wg]; lanes
-//            read consecutive particles and each run of equal keys within a
-//            wavefront adds one count (Lagrangian order keeps runs long)
-//   scan     exclusive scan of hist (key-major) -> scatter offsets; tile
-//            start/count; chunk offsets (ceil(count / CH) chunks per tile)
+//   count    per-workgroup LDS histogram of tile keys (lanes read consecutive
+//            particles; each run of equal keys within a wavefront adds one
+//            count -- Lagrangian order keeps runs long), then one global
+//            atomic per non-empty (tile, workgroup) pair
+//   scan     tile start = exclusive scan of the counts; chunk offsets
+//            (ceil(count / CH) chunks per tile)
 //   chunks   chunk descriptors (tile, start, count)
-//   scatter  each workgroup re-reads its slice and writes its particles to
-//            their tile's range as float4 (x, y, z, original index bits)
+//   scatter  each workgroup recounts its slice in LDS, reserves one range per
+//            non-empty tile with a global atomic on the tile's cursor, and
+//            writes its particles there as float4 (x, y, z, index bits)
 //   accum    one workgroup per (chunk, channel): LDS window of int64, the
 //            deposits, window -> partial
 //   reduce   one workgroup per tile: for each of its cells, the tile's own
@@ -105,11 +107,12 @@ __device__ __forceinline__ uint32_t tile_key(int cx, int cy, int cz, const TileG
   return (uint32_t(cx / kTile) * uint32_t(t.ty) + uint32_t(cy / kTile)) * uint32_t(t.tz) + uint32_t(cz / kTile);
 }
 
-// ---------------------------------------------------------------- hist
+// --------------------------------------------------------------- count
 
 // Runs of equal keys among the 64 lanes of a wavefront: `heads` has bit l set
-// when lane l starts a run (its key differs from lane l-1's, or l is the
-// first active lane); a head's run ends where the next head begins.
+// when lane l starts a run (its key differs from lane l-1's, or l is lane 0);
+// a head's run ends where the next head begins or at the last active lane
+// (active lanes are always a prefix of the wavefront).
 __device__ __forceinline__ unsigned long long run_heads(uint32_t key, bool active) {
   const int lane = threadIdx.x & 63;
   const uint32_t prev = __shfl_up(key, 1);
@@ -121,22 +124,14 @@ __device__ __forceinline__ uint32_t run_length(unsigned long long heads, unsigne
   const int lane = threadIdx.x & 63;
   const unsigned long long above = heads & ~((2ull << lane) - 1ull);
   const int next = above ? __ffsll((long long)above) - 1 : 64;
-  // active lanes are a prefix of the wavefront (p < p1 tail), so the run
-  // stops at the first inactive lane too
   const int last_active = 63 - __clzll((long long)act);
   return uint32_t(min(next, last_active + 1) - lane);
 }
 
+// LDS histogram of the tile keys of particles [p0, p1) into sh[K].
 template <bool REL>
-__global__ void __launch_bounds__(kBinThreads)
-bin_hist_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t, uint32_t *__restrict__ hist,
-                uint32_t G) {
-  extern __shared__ uint32_t sh[];
-  const uint32_t g = blockIdx.x;
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) sh[k] = 0;
-  __syncthreads();
-  const uint32_t p0 = g * slice;
-  const uint32_t p1 = min(n, p0 + slice);
+__device__ __forceinline__ void slice_histogram(const float *__restrict__ pos, uint32_t p0, uint32_t p1,
+                                                const TileGrid &t, uint32_t *sh) {
   for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
     const uint32_t p = b0 + threadIdx.x;
     const bool active = p < p1;
@@ -150,8 +145,19 @@ bin_hist_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileG
     const unsigned long long heads = run_heads(key, active);
     if (active && ((heads >> (threadIdx.x & 63)) & 1ull)) atomicAdd(&sh[key], run_length(heads, act));
   }
+}
+
+template <bool REL>
+__global__ void __launch_bounds__(kBinThreads)
+bin_count_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t, uint32_t *__restrict__ count) {
+  extern __shared__ uint32_t sh[];
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) sh[k] = 0;
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) hist[size_t(k) * G + g] = sh[k];
+  const uint32_t p0 = blockIdx.x * slice;
+  slice_histogram<REL>(pos, p0, min(n, p0 + slice), t, sh);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x)
+    if (sh[k]) atomicAdd(&count[k], sh[k]);
 }
 
 // ---------------------------------------------------------------- scan
@@ -211,6 +217,10 @@ __global__ void __launch_bounds__(256) scan_sums_kernel(uint32_t *__restrict__ s
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
+__global__ void __launch_bounds__(256) zero_u32_kernel(uint32_t *__restrict__ a, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0u;
+}
+
 __global__ void __launch_bounds__(256) scan_add_kernel(uint32_t *__restrict__ a, uint32_t n,
                                                        const uint32_t *__restrict__ sums) {
   const size_t base = size_t(blockIdx.x) * kScanBlock + size_t(threadIdx.x) * kScanItems;
@@ -229,18 +239,11 @@ static int exclusive_scan(hipStream_t s, uint32_t *a, uint32_t n, uint32_t *sums
   return JPM_OK;
 }
 
-// tile_start[k] = offsets[k*G] (after the scan), tile_count[k] = next start
-// - start; nchunk[k] = ceil(count / kChunk).
+// nchunk[k] = ceil(count[k] / kChunk).
 __global__ void __launch_bounds__(256)
-tile_table_kernel(const uint32_t *__restrict__ offs, uint32_t G, uint32_t K, uint32_t n,
-                  uint32_t *__restrict__ tile_start, uint32_t *__restrict__ tile_count, uint32_t *__restrict__ nchunk) {
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
-    const uint32_t st = offs[size_t(k) * G];
-    const uint32_t en = (k + 1 < K) ? offs[size_t(k + 1) * G] : n;
-    tile_start[k] = st;
-    tile_count[k] = en - st;
-    nchunk[k] = (en - st + kChunk - 1) / kChunk;
-  }
+tile_chunks_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t *__restrict__ nchunk) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x)
+    nchunk[k] = (count[k] + kChunk - 1) / kChunk;
 }
 
 struct ChunkDesc {
@@ -269,13 +272,18 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
 template <bool REL>
 __global__ void __launch_bounds__(kBinThreads)
 bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t,
-                   const uint32_t *__restrict__ offs, uint32_t G, float4 *__restrict__ spos) {
+                   uint32_t *__restrict__ cursor, float4 *__restrict__ spos) {
   extern __shared__ uint32_t cur[];
-  const uint32_t g = blockIdx.x;
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) cur[k] = offs[size_t(k) * G + g];
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) cur[k] = 0;
   __syncthreads();
-  const uint32_t p0 = g * slice;
+  const uint32_t p0 = blockIdx.x * slice;
   const uint32_t p1 = min(n, p0 + slice);
+  slice_histogram<REL>(pos, p0, p1, t, cur);
+  __syncthreads();
+  // reserve this workgroup's range in every tile it touches
+  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x)
+    if (cur[k]) cur[k] = atomicAdd(&cursor[k], cur[k]);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
     const uint32_t p = b0 + threadIdx.x;
@@ -494,26 +502,24 @@ int zero_f32(hipStream_t s, float *p, int64_t m);
 // Workspace layout of the binned paint (byte offsets).
 struct BinWs {
   uint32_t G, slice, maxchunks;
-  size_t hist, sums, tstart, tcount, nchunk, choff, totals, desc, spos, partial, overflow, wmax, total;
+  size_t count, tstart, cursor, nchunk, choff, sums, totals, desc, spos, partial, overflow, wmax, total;
 };
 
 static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
   BinWs w;
   const uint64_t nn = uint64_t(n > 0 ? n : 1);
-  w.G = uint32_t(std::min<uint64_t>(1024, (nn + 4095) / 4096));
-  if (w.G < 1) w.G = 1;
-  w.slice = uint32_t((nn + w.G - 1) / w.G);
+  w.slice = 16384;
+  w.G = uint32_t((nn + w.slice - 1) / w.slice);
   w.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
-  const size_t KG = size_t(t.K) * w.G;
-  const size_t nsums = (std::max<size_t>(KG, t.K) + kScanBlock - 1) / kScanBlock + 1;
+  const size_t nsums = (size_t(t.K) + kScanBlock - 1) / kScanBlock + 1;
   const size_t M = size_t(t.nx) * t.ny * t.nz;
   size_t o = 0;
-  w.hist = o;     o += align_up(KG * 4);
-  w.sums = o;     o += align_up(nsums * 4);
+  w.count = o;    o += align_up(size_t(t.K) * 4);
   w.tstart = o;   o += align_up(size_t(t.K) * 4);
-  w.tcount = o;   o += align_up(size_t(t.K) * 4);
+  w.cursor = o;   o += align_up(size_t(t.K) * 4);
   w.nchunk = o;   o += align_up(size_t(t.K) * 4);
   w.choff = o;    o += align_up(size_t(t.K) * 4);
+  w.sums = o;     o += align_up(nsums * 4);
   w.totals = o;   o += align_up(16);
   w.desc = o;     o += align_up(size_t(w.maxchunks) * sizeof(ChunkDesc));
   w.spos = o;     o += align_up(size_t(nn) * 16);
@@ -567,34 +573,36 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
   }
   JPM_REQUIRE(pos != nullptr, "positions are null");
   char *base = static_cast<char *>(ws);
-  uint32_t *hist = reinterpret_cast<uint32_t *>(base + w.hist);
-  uint32_t *sums = reinterpret_cast<uint32_t *>(base + w.sums);
+  uint32_t *count = reinterpret_cast<uint32_t *>(base + w.count);
   uint32_t *tstart = reinterpret_cast<uint32_t *>(base + w.tstart);
-  uint32_t *tcount = reinterpret_cast<uint32_t *>(base + w.tcount);
+  uint32_t *cursor = reinterpret_cast<uint32_t *>(base + w.cursor);
   uint32_t *nchunk = reinterpret_cast<uint32_t *>(base + w.nchunk);
   uint32_t *choff = reinterpret_cast<uint32_t *>(base + w.choff);
+  uint32_t *sums = reinterpret_cast<uint32_t *>(base + w.sums);
   uint32_t *totals = reinterpret_cast<uint32_t *>(base + w.totals);  // [0] particles, [1] chunks
-  const uint32_t KG = t.K * w.G;
   const size_t lds_hist = size_t(t.K) * 4;
 
-  // 1. histogram of tile keys per workgroup slice
+  // 1. tile counts
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, count, t.K);
+  JPM_LAUNCHED("zero_u32_kernel");
   {
-    auto kern = rel ? bin_hist_kernel<true> : bin_hist_kernel<false>;
+    auto kern = rel ? bin_count_kernel<true> : bin_count_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G);
-    JPM_LAUNCHED("bin_hist_kernel");
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, count);
+    JPM_LAUNCHED("bin_count_kernel");
   }
-  // 2. scatter offsets, tile table, chunk table
-  rc = exclusive_scan(s, hist, KG, sums, totals);
+  // 2. tile starts, scatter cursors, chunk table
+  JPM_HIP(hipMemcpyAsync(tstart, count, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
+  rc = exclusive_scan(s, tstart, t.K, sums, totals);
   if (rc) return rc;
-  hipLaunchKernelGGL(tile_table_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, hist, w.G, t.K, uint32_t(n),
-                     tstart, tcount, nchunk);
-  JPM_LAUNCHED("tile_table_kernel");
+  JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, count, t.K, nchunk);
+  JPM_LAUNCHED("tile_chunks_kernel");
   JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
   rc = exclusive_scan(s, choff, t.K, sums, totals + 1);
   if (rc) return rc;
-  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, tstart, tcount, choff, t.K,
+  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, tstart, count, choff, t.K,
                      reinterpret_cast<ChunkDesc *>(base + w.desc));
   JPM_LAUNCHED("chunk_desc_kernel");
   // 3. scatter particles into tile order
@@ -602,7 +610,7 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     auto kern = rel ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, hist, w.G,
+    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, cursor,
                        reinterpret_cast<float4 *>(base + w.spos));
     JPM_LAUNCHED("bin_scatter_kernel");
   }

@@ -50,6 +50,31 @@ int lpt2_source_vjp(hipStream_t, int64_t, const float *, const float *, float *)
 int spectrum_half_to_full(hipStream_t, int, int, int, const void *, void *);
 
 int axpy(hipStream_t, int64_t, float, const float *, float *);
+int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
+int fd4_gradient(hipStream_t, int, int, int, const float *, float, float *);
+int fd4_divergence(hipStream_t, int, int, int, const float *, float, float *);
+
+// Force meshes phi3 = scale * -g_d G Lr [C] delta_k, inverse-transformed.
+// With the default 4-point gradient this is one inverse FFT of the potential
+// followed by the real-space stencil (`tmp_mesh`, M floats, holds the
+// potential); with JPM_K_GRAD_EXACT it is three inverse FFTs of the three
+// gradient spectra (`spec3`, 3 half spectra).
+static int force_meshes(hipStream_t s, int nx, int ny, int nz, const void *delta_k, float r_split, float scale,
+                        uint32_t kflags, void *spec3, float *tmp_mesh, float *phi3) {
+  const int64_t m = cells(nx, ny, nz);
+  const float sc = float(double(scale) / double(m));
+  int rc;
+  if (kflags & JPM_K_GRAD_EXACT) {
+    rc = kspace_force(s, nx, ny, nz, delta_k, spec3, r_split, sc, kflags);
+    if (rc) return rc;
+    return fft_c2r(s, nx, ny, nz, 3, spec3, phi3);
+  }
+  rc = kspace_potential(s, nx, ny, nz, delta_k, spec3, r_split, sc, kflags);
+  if (rc) return rc;
+  rc = fft_c2r(s, nx, ny, nz, 1, spec3, tmp_mesh);
+  if (rc) return rc;
+  return fd4_gradient(s, nx, ny, nz, tmp_mesh, 1.0f, phi3);
+}
 int binned_paint(hipStream_t, int, int, int, const float *, int64_t, const float *, float, int, float *, uint32_t,
                  void *, size_t);
 size_t binned_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
@@ -116,9 +141,7 @@ static int force_field(hipStream_t s, int nx, int ny, int nz, const float *pos, 
     if (rc) return rc;
     delta_k = spec;
   }
-  rc = kspace_force(s, nx, ny, nz, delta_k, spec3, r_split, float(double(scale) / double(m)), kflags);
-  if (rc) return rc;
-  return fft_c2r(s, nx, ny, nz, 3, spec3, phi3);
+  return force_meshes(s, nx, ny, nz, delta_k, r_split, scale, kflags, spec3, mesh, phi3);
 }
 
 // Workspace of force_vjp: [3 cotangent meshes][3 half spectra][half
@@ -162,10 +185,21 @@ static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, c
   const uint32_t rel = pflags & JPM_RELATIVE;
   rc = pipeline_paint(s, nx, ny, nz, pos, n, cot, 3, g3, pflags, base + w.paint, w.total - w.paint);
   if (rc) return rc;
-  rc = fft_r2c(s, nx, ny, nz, 3, g3, spec3);
-  if (rc) return rc;
-  rc = kspace_force_adjoint(s, nx, ny, nz, spec3, spec, r_split, float(double(scale) / double(m)), kflags);
-  if (rc) return rc;
+  if (kflags & JPM_K_GRAD_EXACT) {
+    rc = fft_r2c(s, nx, ny, nz, 3, g3, spec3);
+    if (rc) return rc;
+    rc = kspace_force_adjoint(s, nx, ny, nz, spec3, spec, r_split, float(double(scale) / double(m)), kflags);
+    if (rc) return rc;
+  } else {
+    // rho_bar = A (sum_d D_d g_d): the stencil's transpose in real space,
+    // then the (self-adjoint) potential operator A.
+    rc = fd4_divergence(s, nx, ny, nz, g3, 1.0f, rho);
+    if (rc) return rc;
+    rc = fft_r2c(s, nx, ny, nz, 1, rho, spec3);
+    if (rc) return rc;
+    rc = kspace_potential(s, nx, ny, nz, spec3, spec, r_split, float(double(scale) / double(m)), kflags);
+    if (rc) return rc;
+  }
   rc = fft_c2r(s, nx, ny, nz, 1, spec, rho);
   if (rc) return rc;
   rc = cic_grad_pos(s, nx, ny, nz, phi3, 3, m, pos, n, cot, nullptr, scale, gpos, rel, 1);
@@ -177,7 +211,7 @@ static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, c
 // spectra][deterministic-paint scratch].  The painted mesh lives in the first
 // of the three force meshes; it is dead once the R2C has read it.
 struct ForceWs {
-  size_t meshes, spec, spec3, paint, total;
+  size_t meshes, spec, spec3, tmp, paint, total;
 };
 
 static ForceWs force_ws(int nx, int ny, int nz, int64_t n, uint32_t flags) {
@@ -186,7 +220,8 @@ static ForceWs force_ws(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   w.meshes = 0;
   w.spec = w.meshes + align_up(3 * m * sizeof(float));
   w.spec3 = w.spec + align_up(h * 8);
-  w.paint = w.spec3 + align_up(3 * h * 8);
+  w.tmp = w.spec3 + align_up(3 * h * 8);
+  w.paint = w.tmp + align_up(m * sizeof(float));
   w.total = w.paint + pipeline_paint_ws(nx, ny, nz, n, 1, flags);
   return w;
 }
@@ -211,9 +246,8 @@ static int pm_forces(hipStream_t s, int nx, int ny, int nz, const float *pos, in
     if (rc) return rc;
     delta_k = spec;
   }
-  rc = kspace_force(s, nx, ny, nz, delta_k, spec3, r_split, float(double(scale) / double(m)), kflags);
-  if (rc) return rc;
-  rc = fft_c2r(s, nx, ny, nz, 3, spec3, meshes);
+  rc = force_meshes(s, nx, ny, nz, delta_k, r_split, scale, kflags, spec3, reinterpret_cast<float *>(base + w.tmp),
+                    meshes);
   if (rc) return rc;
   return cic_read(s, nx, ny, nz, meshes, 3, m, pos, n, forces, pflags & JPM_RELATIVE);
 }
@@ -288,6 +322,19 @@ int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch, const void 
 int jpm_kspace_force_f32(void *stream, int nx, int ny, int nz, const void *delta_k, void *out3, float r_split,
                          float scale, uint32_t flags) {
   JPM_GUARD(kspace_force(as_stream(stream), nx, ny, nz, delta_k, out3, r_split, scale, flags));
+}
+
+int jpm_kspace_potential_f32(void *stream, int nx, int ny, int nz, const void *delta_k, void *out, float r_split,
+                             float scale, uint32_t flags) {
+  JPM_GUARD(kspace_potential(as_stream(stream), nx, ny, nz, delta_k, out, r_split, scale, flags));
+}
+
+int jpm_fd4_gradient_f32(void *stream, int nx, int ny, int nz, const float *phi, float scale, float *out3) {
+  JPM_GUARD(fd4_gradient(as_stream(stream), nx, ny, nz, phi, scale, out3));
+}
+
+int jpm_fd4_divergence_f32(void *stream, int nx, int ny, int nz, const float *g3, float scale, float *out) {
+  JPM_GUARD(fd4_divergence(as_stream(stream), nx, ny, nz, g3, scale, out));
 }
 
 int jpm_kspace_compensate_f32(void *stream, int nx, int ny, int nz, void *field_k, float scale) {

@@ -146,6 +146,31 @@ force_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid h
   }
 }
 
+// Potential only: out = scale * G Lr [C] delta_k (the real-space 4-point
+// stencil then takes the gradient).
+__global__ void __launch_bounds__(256)
+potential_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid hg, Tabs t, float r2,
+                 float scale, int deconv) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
+    uint32_t i, j, k;
+    hg.coords(e, i, j, k);
+    const uint32_t jy = t.oy + j, kz = t.oz + k;
+    float kk = (t.k2[i] + t.k2[jy]) + t.k2[kz];
+    float f = green(kk);
+    if (r2 != 0.0f) {
+      float kke = (t.k2e[i] + t.k2e[jy]) + t.k2e[kz];
+      f = f * expf(-kke * r2);
+    }
+    if (deconv) {
+      float p = (t.s[i] * t.s[jy]) * t.s[kz];
+      f = f / (p * p);
+    }
+    f = f * scale;
+    const float2 v = dk[e];
+    out[e] = make_float2(v.x * f, v.y * f);
+  }
+}
+
 __global__ void __launch_bounds__(256)
 compensate_kernel(float2 *__restrict__ fk, HalfGrid hg, Tabs t, float scale) {
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
@@ -361,6 +386,22 @@ int kspace_force(hipStream_t s, int nx, int ny, int nz, const void *dk, void *ou
                      static_cast<const float2 *>(dk), static_cast<float2 *>(out3), hg, t, r_split * r_split,
                      scale, int((flags & JPM_K_DECONV) != 0));
   JPM_LAUNCHED("force_kernel");
+  return JPM_OK;
+}
+
+int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void *out, float r_split, float scale,
+                     uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(dk && out, "null spectrum pointer");
+  Tabs t;
+  rc = get_tabs(s, nx, ny, nz, flags, t);
+  if (rc) return rc;
+  HalfGrid hg = make_half(nx, ny, nz);
+  hipLaunchKernelGGL(potential_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                     static_cast<const float2 *>(dk), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
+                     int((flags & JPM_K_DECONV) != 0));
+  JPM_LAUNCHED("potential_kernel");
   return JPM_OK;
 }
 

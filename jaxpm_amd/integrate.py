@@ -61,6 +61,9 @@ class Stepper:
         self.pflags = pflags
         self.flags = pflags | (kflags << 8)
         self.dev = _lib.device()
+        # optional HIP-event trace: name -> list of (start, end) events around
+        # each launch on torch's current stream (set by the benchmark)
+        self.trace = None
         fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         self.ws = torch.empty(max(fb, vb), dtype=torch.uint8, device=self.dev)
@@ -68,31 +71,44 @@ class Stepper:
     def new_phi(self):
         return torch.empty((3, *self.shape), dtype=torch.float32, device=self.dev)
 
+    def _timed(self, name, fn, *args):
+        if self.trace is None:
+            return call(fn, *args)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call(fn, *args)
+        e1.record()
+        self.trace.setdefault(name, []).append((e0, e1))
+
     def field(self, x, phi):
-        call("jpm_force_field_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n, None, ptr(phi),
-             self.r_split, 1.0, self.flags, ptr(self.ws), self.ws.numel())
+        self._timed("field", "jpm_force_field_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n, None,
+                    ptr(phi), self.r_split, 1.0, self.flags, ptr(self.ws), self.ws.numel())
 
     def kick(self, phi, x, v, gamma):
-        call("jpm_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n, float(gamma), ptr(v),
-             self.pflags & _lib.RELATIVE)
+        self._timed("kick", "jpm_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n,
+                    float(gamma), ptr(v), self.pflags & _lib.RELATIVE)
 
     def axpy(self, a, x, y):
-        call("jpm_axpy_f32", stream(), 3 * self.n, float(a), ptr(x), ptr(y))
+        self._timed("axpy", "jpm_axpy_f32", stream(), 3 * self.n, float(a), ptr(x), ptr(y))
 
     def vjp(self, phi, x, cot, scale, gpos):
-        call("jpm_force_vjp_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n, ptr(cot),
-             float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws), self.ws.numel())
+        self._timed("vjp", "jpm_force_vjp_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n,
+                    ptr(cot), float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws), self.ws.numel())
 
 
 def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None):
-    """Advance steps [start, stop) in place; ``keep`` collects (x, phi) after
-    each step's force evaluation."""
+    """Advance steps [start, stop) in place.  With ``keep`` (a list of
+    preallocated (x, phi) buffer pairs, one per step) each step's positions
+    are copied out and its force meshes are computed straight into the
+    step's buffer."""
     for i in range(start, stop):
         stepper.axpy(beta[i], v, x)
+        if keep is not None:
+            xk, phi = keep[i - start]
+            xk.copy_(x)
         stepper.field(x, phi)
         stepper.kick(phi, x, v, gamma[i])
-        if keep is not None:
-            keep.append((x.clone(), phi.clone()))
 
 
 class NBody(torch.autograd.Function):
@@ -125,17 +141,17 @@ class NBody(torch.autograd.Function):
         lx = (gx if gx is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
         lv = (gv if gv is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
         phi = st.new_phi()
+        keep = [(torch.empty_like(x_ref), st.new_phi()) for _ in range(min(ctx.k, S))]
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
             xc, vc = ctx.ckpts[s0]
             x, v = xc.clone(), vc.clone()
-            keep = []
             _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep)
             for i in reversed(range(s0, s1)):
                 xi, phii = keep[i - s0]
                 st.vjp(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
                 st.axpy(beta[i], lx, lv)            # drift i: x += beta v
-            del keep
+        del keep
         # the first half kick used x0 before any drift: recompute its field
         xs = ctx.ckpts[0][0]
         st.field(xs, phi)

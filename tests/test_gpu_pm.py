@@ -167,3 +167,34 @@ def test_pm_forces_gradient_matches_finite_differences(gpu):
         fd = (lp - lm) / (2 * eps)
         an = np.sum(grad * d)
         assert abs(an - fd) < 2e-3 * (abs(fd) + 1e-3), (j, an, fd)
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 32), (8, 6, 5), (1, 2, 4)])
+def test_fd4_stencil_equals_kspace_gradient(gpu, shape):
+    """-D_d IFFT[P] == IFFT[-g_d P] (gradient_kernel order 1), and the
+    divergence kernel is the stencil's transpose."""
+    from jaxpm_amd import _ops
+    from oracle import kernels as ok
+    r = np.random.default_rng(28)
+    phi = r.normal(size=shape).astype(np.float32)
+    got = host(_ops.fd4_gradient(dev(phi)))
+    kv = ok.fftk(shape)
+    for d in range(3):
+        ref = np.fft.ifftn(-ok.gradient_kernel(kv, d) * np.fft.fftn(phi)).real
+        assert np.abs(got[d] - ref).max() < 2e-5 * max(np.abs(phi).max(), 1.0)
+    g3 = r.normal(size=(3, *shape)).astype(np.float32)
+    lhs = float(np.sum(got.astype(np.float64) * g3))
+    rhs = float(np.sum(phi.astype(np.float64) * host(_ops.fd4_divergence(dev(g3)))))
+    assert abs(lhs - rhs) < 1e-4 * (abs(lhs) + 1.0)
+
+
+def test_kspace_potential(gpu):
+    from jaxpm_amd import _ops
+    from oracle import kernels as ok
+    shape = (16, 12, 20)
+    r = np.random.default_rng(29)
+    rho = r.normal(size=shape).astype(np.float32)
+    pot = host(_ops.c2r(_ops.kspace_potential(_ops.r2c(dev(rho)), shape, 1.2, 1.0 / rho.size), shape))
+    kv = ok.fftk(shape)
+    ref = np.fft.ifftn(np.fft.fftn(rho) * ok.invlaplace_kernel(kv) * ok.longrange_kernel(kv, 1.2)).real
+    assert rel_err(pot, ref) < 5e-6
