@@ -61,6 +61,13 @@ int jpm_shutdown(void);                 /* frees cached FFT plans and tables */
 int jpm_abi_version(void);
 const char *jpm_last_error(void);
 
+/* Per-kernel timing for benchmarks: while enabled, the library records a
+ * pair of HIP events on the launch stream around each of its kernels;
+ * jpm_timing_report waits for them and writes "name launches total_ms" lines
+ * (then clears).  No counterpart in the reference. */
+int jpm_timing_enable(int on);
+int jpm_timing_report(char *buf, size_t len);
+
 /* ---------------------------------------------------------------- painting */
 
 /* Bytes of workspace jpm_cic_paint_f32 needs for this call (0 unless
@@ -279,6 +286,45 @@ int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3,
                       const float *pos, int64_t n, const float *cot,
                       float scale, float *gpos, float r_split, uint32_t flags,
                       void *ws, size_t ws_bytes);
+
+/* Workspace bytes for jpm_force_kick_f32 (0 when the mesh has more than
+ * 32768 16^3 tiles or flags ask for the exact gradient / atomic paint, which
+ * the sorted pipeline does not cover). */
+size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n,
+                                     uint32_t flags);
+
+/* One force evaluation of the leapfrog on the tile-sorted particles:
+ * phi (M floats, out) = Re IFFT[G Lr [C] FFT(paint(pos))] / M, then
+ * vel[p] += gamma * read(-D phi, pos[p]) with the 4-point stencil D applied
+ * in LDS (write_forces != 0: vel[p] = gamma * forces instead).  pm.py:12-58
+ * fused with the kick of ode.py:142-143.  phi is what jpm_force_vjp_pot_f32
+ * needs for the adjoint of this evaluation. */
+int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos,
+                       int64_t n, float *vel, float gamma, float *phi,
+                       float r_split, uint32_t flags, int write_forces,
+                       void *ws, size_t ws_bytes, void *keep_sort,
+                       size_t keep_bytes);
+
+/* Bytes of a tile sort of n particles: the `keep_sort` buffer of
+ * jpm_force_kick_f32 (optional; when given the sort is built there and
+ * stays valid for jpm_force_vjp_pot_f32 of the same positions). */
+size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n);
+
+/* Workspace bytes for jpm_force_vjp_pot_f32 (0 when unsupported, as above). */
+size_t jpm_force_vjp_pot_workspace_size(int nx, int ny, int nz, int64_t n,
+                                        uint32_t flags);
+
+/* gpos[n][3] += scale * J(pos)^T cot for the forces of jpm_force_kick_f32,
+ * given that evaluation's potential phi (and optionally its kept tile sort,
+ * `sorted`, so the positions are not sorted again): the 3-channel paint of cot,
+ * the stencil divergence, the potential operator and one LDS-window gather
+ * for the direct and the density terms. */
+int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz,
+                          const float *phi, const float *pos, int64_t n,
+                          const float *cot, float scale, float *gpos,
+                          float r_split, uint32_t flags, void *ws,
+                          size_t ws_bytes, const void *sorted,
+                          size_t sorted_bytes);
 
 /* y += a * x over n floats (16-byte aligned): the drift x += h v / (a^3 E)
  * of the leapfrog built on make_ode_fn (ode.py:139-140) and its adjoint. */

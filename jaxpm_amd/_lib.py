@@ -35,6 +35,8 @@ SIGNATURES = {
     "jpm_shutdown": (_i, []),
     "jpm_abi_version": (_i, []),
     "jpm_last_error": (ctypes.c_char_p, []),
+    "jpm_timing_enable": (_i, [_i]),
+    "jpm_timing_report": (_i, [ctypes.c_char_p, _sz]),
     "jpm_cic_paint_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_cic_paint_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _vp, _u32, _vp, _sz]),
     "jpm_cic_read_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _u32]),
@@ -61,6 +63,11 @@ SIGNATURES = {
     "jpm_force_field_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
     "jpm_force_vjp_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_vjp_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz]),
+    "jpm_force_kick_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
+    "jpm_force_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _vp, _f, _u32, _i, _vp, _sz, _vp, _sz]),
+    "jpm_bin_sort_size": (_sz, [_i, _i, _i, _i64]),
+    "jpm_force_vjp_pot_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
+    "jpm_force_vjp_pot_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
     "jpm_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _f, _vp, _u32]),
     "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
@@ -132,3 +139,20 @@ def stream():
 
 def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def timing(enable):
+    """Turn the library's per-kernel HIP-event timing on or off."""
+    call("jpm_timing_enable", int(bool(enable)))
+
+
+def timing_report():
+    """{kernel: (launches, total_ms)} since the last report (waits for the
+    recorded events)."""
+    buf = ctypes.create_string_buffer(1 << 16)
+    call("jpm_timing_report", buf, len(buf))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, cnt, ms = line.rsplit(" ", 2)
+        out[name] = (int(cnt), float(ms))
+    return out

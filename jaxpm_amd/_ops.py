@@ -302,13 +302,23 @@ def pm_forces_fused(pos, shape, delta_k=None, r_split=0.0, scale=1.0, relative=F
     pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
         (_lib.ATOMIC if atomic else 0)
     flags = pflags | (kf << 8)
+    if out is None:
+        out = torch.empty((n, 3), dtype=torch.float32, device=pos.device)
+    if delta_k is None and n > 0:
+        # tile-sorted pipeline: potential + LDS stencil gather (writes forces)
+        kb = size("jpm_force_kick_workspace_size", nx, ny, nz, n, flags)
+        if kb:
+            if ws is None or ws.numel() < kb:
+                ws = torch.empty(kb, dtype=torch.uint8, device=pos.device)
+            phi = torch.empty((nx, ny, nz), dtype=torch.float32, device=pos.device)
+            call("jpm_force_kick_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(out), float(scale), ptr(phi),
+                 float(r_split), flags, 1, ptr(ws), ws.numel(), None, 0)
+            return out
     wsb = size("jpm_pm_forces_workspace_size", nx, ny, nz, n, flags)
     if ws is None or ws.numel() < wsb:
         ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device)
     if delta_k is not None:
         delta_k = _c64(delta_k, "delta_k")
-    if out is None:
-        out = torch.empty((n, 3), dtype=torch.float32, device=pos.device)
     call("jpm_pm_forces_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(delta_k), ptr(out), float(r_split),
          float(scale), flags, ptr(ws), ws.numel())
     return out

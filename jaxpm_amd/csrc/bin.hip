@@ -23,66 +23,13 @@
 //            neighbour tiles, summed in a fixed order and converted
 #include <type_traits>
 
-#include "common.hpp"
+#include "tiles.hpp"
 
 namespace jpm {
 
-constexpr int kTile = 16;                       // tile edge in cells
-constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
-constexpr int kWinCells = kWin * kWin * kWin;   // 4913
-constexpr int kChunk = 8192;                    // max particles per chunk
-constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
 constexpr int kBinThreads = 256;                // hist / scatter workgroup size
 constexpr int kScanItems = 8;                   // scan: elements per thread
 constexpr int kScanBlock = 256 * kScanItems;
-
-struct TileGrid {
-  int nx, ny, nz;   // mesh
-  int tx, ty, tz;   // tiles per axis
-  uint32_t K;       // tiles
-  FastDiv div_nz, div_ny, div_tz, div_ty;
-};
-
-static TileGrid make_tiles(int nx, int ny, int nz) {
-  TileGrid t;
-  t.nx = nx;
-  t.ny = ny;
-  t.nz = nz;
-  t.tx = (nx + kTile - 1) / kTile;
-  t.ty = (ny + kTile - 1) / kTile;
-  t.tz = (nz + kTile - 1) / kTile;
-  t.K = uint32_t(t.tx) * t.ty * t.tz;
-  t.div_nz = FastDiv(uint32_t(nz));
-  t.div_ny = FastDiv(uint32_t(ny));
-  t.div_tz = FastDiv(uint32_t(t.tz));
-  t.div_ty = FastDiv(uint32_t(t.ty));
-  return t;
-}
-
-__device__ __forceinline__ int wrapi(int c, int n) {
-  int r = c % n;
-  return r < 0 ? r + n : r;
-}
-
-// Floor cell of a particle: absolute -> int32(floor(x)) mod n (painting.py:
-// 23, 35-37); relative -> the low corner of enmesh (painting_utils.py:48-57),
-// or, when that corner is dropped, the high corner minus one.
-__device__ __forceinline__ int floor_cell_abs(float x, int n) { return wrapi(int(floorf(x)), n); }
-
-__device__ __forceinline__ float jmod(float x, float L) {
-  float m = fmodf(x, L);
-  if (m != 0.0f && m < 0.0f) m = m + L;
-  return m;
-}
-
-__device__ __forceinline__ int floor_cell_rel(float p, int n) {
-  const float L = float(n);
-  float q = floorf(jmod(p, L));
-  if (q >= 0.0f && q < L) return int(q);
-  float q1 = floorf(jmod(p + 1.0f, L));
-  if (q1 >= 0.0f && q1 < L) return wrapi(int(q1) - 1, n);
-  return 0;  // NaN: every corner is dropped, any tile will do
-}
 
 template <bool REL>
 __device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, const TileGrid &t, int &cx,
@@ -246,10 +193,6 @@ tile_chunks_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t *__r
     nchunk[k] = (count[k] + kChunk - 1) / kChunk;
 }
 
-struct ChunkDesc {
-  uint32_t tile, start, count, pad;
-};
-
 __global__ void __launch_bounds__(256)
 chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ tile_count,
                   const uint32_t *__restrict__ chunk_off, uint32_t K, ChunkDesc *__restrict__ desc) {
@@ -313,45 +256,6 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, Ti
 }
 
 // ----------------------------------------------------------------- accum
-
-// One corner axis of a particle relative to its tile origin: local window
-// index (or -1 when the corner is dropped / outside the window) and weight.
-struct LAxis {
-  int l0, l1;
-  float w0, w1;
-};
-
-__device__ __forceinline__ LAxis laxis_abs(float x, int origin, int n) {
-  LAxis a;
-  const float f = floorf(x);
-  const float f1 = f + 1.0f;
-  a.w0 = 1.0f - fabsf(x - f);
-  a.w1 = 1.0f - fabsf(x - f1);
-  const int c0 = wrapi(int(f), n);
-  a.l0 = c0 - origin;  // the floor cell lies in the tile by construction
-  a.l1 = a.l0 + 1;
-  return a;
-}
-
-// enmesh corner (painting_utils.py:48-69) mapped into the window; corners
-// that land outside [origin, origin + 16] (rounding edge cases) report -1 and
-// are deposited by the caller through the overflow path.
-__device__ __forceinline__ void lcorner_rel(float p, float o, int n, int origin, int &l, float &w, int &cell) {
-  const float L = float(n);
-  float q = floorf(jmod(p + o, L));
-  float nd = p - q;
-  nd = nd - rintf(nd / L) * L;
-  const bool ok = (q >= 0.0f) && (q < L);
-  cell = ok ? int(q) : -1;
-  w = ok ? 1.0f - fabsf(nd) : 0.0f;
-  if (!ok) {
-    l = -2;  // dropped: no deposit at all
-    return;
-  }
-  int d = int(q) - origin;
-  if (d < 0) d += n;  // window may wrap past the mesh end
-  l = (d >= 0 && d < kWin) ? d : -1;
-}
 
 template <bool REL>
 __global__ void __launch_bounds__(256)
@@ -437,10 +341,6 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
 // coordinate is 0, the neighbour at -1 (periodic) contributes at its own
 // local index = its tile length.  The (up to 8) sources and each tile's
 // chunks are summed in a fixed order.
-__device__ __forceinline__ int tile_len(int it, int ntile, int n) {
-  return (it == ntile - 1) ? n - it * kTile : kTile;
-}
-
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
                   const uint32_t *__restrict__ nchunk, TileGrid t, int nc, const float *__restrict__ overflow,
@@ -499,20 +399,27 @@ namespace jpm {
 int abs_max(hipStream_t s, const float *w, uint32_t n, float *out);
 int zero_f32(hipStream_t s, float *p, int64_t m);
 
-// Workspace layout of the binned paint (byte offsets).
-struct BinWs {
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    JPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                int(bytes)));
+  return JPM_OK;
+}
+
+// Workspace of the tile sort (byte offsets).
+struct SortWs {
   uint32_t G, slice, maxchunks;
-  size_t count, tstart, cursor, nchunk, choff, sums, totals, desc, spos, partial, overflow, wmax, total;
+  size_t count, tstart, cursor, nchunk, choff, sums, totals, desc, spos, total;
 };
 
-static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
-  BinWs w;
+static SortWs sort_ws(const TileGrid &t, int64_t n) {
+  SortWs w;
   const uint64_t nn = uint64_t(n > 0 ? n : 1);
   w.slice = 16384;
   w.G = uint32_t((nn + w.slice - 1) / w.slice);
   w.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
   const size_t nsums = (size_t(t.K) + kScanBlock - 1) / kScanBlock + 1;
-  const size_t M = size_t(t.nx) * t.ny * t.nz;
   size_t o = 0;
   w.count = o;    o += align_up(size_t(t.K) * 4);
   w.tstart = o;   o += align_up(size_t(t.K) * 4);
@@ -523,54 +430,77 @@ static BinWs bin_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags) {
   w.totals = o;   o += align_up(16);
   w.desc = o;     o += align_up(size_t(w.maxchunks) * sizeof(ChunkDesc));
   w.spos = o;     o += align_up(size_t(nn) * 16);
-  w.partial = o;  o += align_up(size_t(w.maxchunks) * nc * kWinCells * 8);
-  w.overflow = o; o += (flags & JPM_RELATIVE) ? align_up(size_t(nc) * M * 4) : 0;
-  w.wmax = o;     o += align_up(4);
   w.total = o;
   return w;
+}
+
+// Workspace of the window accumulation that follows a sort.
+static size_t accum_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags, size_t *partial, size_t *overflow,
+                       size_t *wmax) {
+  const uint64_t nn = uint64_t(n > 0 ? n : 1);
+  const uint32_t maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
+  const size_t M = size_t(t.nx) * t.ny * t.nz;
+  size_t o = 0;
+  *partial = o;  o += align_up(size_t(maxchunks) * nc * kWinCells * 8);
+  *overflow = o; o += (flags & JPM_RELATIVE) ? align_up(size_t(nc) * M * 4) : 0;
+  *wmax = o;     o += align_up(4);
+  return o;
 }
 
 bool binned_paint_supported(int nx, int ny, int nz) {
   return nx > 0 && ny > 0 && nz > 0 && make_tiles(nx, ny, nz).K <= uint32_t(kMaxKeysLds);
 }
 
+size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return sort_ws(make_tiles(nx, ny, nz), n).total;
+}
+
+size_t bins_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
+  size_t a, b, c;
+  return accum_ws(make_tiles(nx, ny, nz), n, nc, flags, &a, &b, &c);
+}
+
 size_t binned_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
-  return bin_ws(make_tiles(nx, ny, nz), n, nc, flags).total;
+  return align_up(bin_sort_workspace_size(nx, ny, nz, n)) + bins_paint_workspace_size(nx, ny, nz, n, nc, flags);
 }
 
-template <typename K>
-static int set_lds(K kernel, size_t bytes) {
-  if (bytes > 65536)
-    JPM_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                int(bytes)));
-  return JPM_OK;
+// The Bins view of a sort already held in `ws` (same shape, count, flags).
+Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws) {
+  const TileGrid t = make_tiles(nx, ny, nz);
+  const SortWs w = sort_ws(t, n);
+  const char *base = static_cast<const char *>(ws);
+  Bins b;
+  b.t = t;
+  b.n = uint32_t(n);
+  b.maxchunks = w.maxchunks;
+  b.rel = flags & JPM_RELATIVE;
+  b.spos = reinterpret_cast<const float4 *>(base + w.spos);
+  b.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
+  b.nchunk = reinterpret_cast<const uint32_t *>(base + w.nchunk);
+  b.choff = reinterpret_cast<const uint32_t *>(base + w.choff);
+  b.total_chunks = reinterpret_cast<const uint32_t *>(base + w.totals) + 1;
+  return b;
 }
 
-// nc meshes (stride nx*ny*nz) = [mesh +] paint of n particles, channel c with
-// weight[p*nc + c] (or wscalar when weight is NULL).
-int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
-                 float wscalar, int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+// Sort n particles into tiles (workspace `ws` of bin_sort_workspace_size).
+int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
+             size_t ws_bytes, Bins &out) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
-  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
-  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  JPM_REQUIRE(n >= 1 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
   const TileGrid t = make_tiles(nx, ny, nz);
   JPM_REQUIRE(t.K <= uint32_t(kMaxKeysLds), "mesh (%d,%d,%d) has %u tiles, above the binned paint's %d",
               nx, ny, nz, t.K, kMaxKeysLds);
   const int64_t M = cells(nx, ny, nz);
-  JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
+  JPM_REQUIRE(3 * M <= (int64_t(1) << 32), "mesh too large");
   const bool rel = flags & JPM_RELATIVE;
-  JPM_REQUIRE(!rel || n == M, "relative painting needs one particle per lattice node (n=%lld, mesh %lld)",
+  JPM_REQUIRE(!rel || n == M, "relative mode needs one particle per lattice node (n=%lld, mesh %lld)",
               (long long)n, (long long)M);
-  const BinWs w = bin_ws(t, n, nc, flags);
-  JPM_REQUIRE(ws && ws_bytes >= w.total, "binned paint needs %zu workspace bytes, got %zu", w.total, ws_bytes);
-  const bool accumulate = !(flags & JPM_ZERO_MESH);
-  if (n == 0) {
-    if (!accumulate) return zero_f32(s, mesh, nc * M);
-    return JPM_OK;
-  }
+  const SortWs w = sort_ws(t, n);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "tile sort needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(pos != nullptr, "positions are null");
   char *base = static_cast<char *>(ws);
   uint32_t *count = reinterpret_cast<uint32_t *>(base + w.count);
@@ -589,6 +519,7 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     auto kern = rel ? bin_count_kernel<true> : bin_count_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
+    JPM_TIMED(s, "bin_count_kernel");
     hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, count);
     JPM_LAUNCHED("bin_count_kernel");
   }
@@ -610,33 +541,86 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     auto kern = rel ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
+    JPM_TIMED(s, "bin_scatter_kernel");
     hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, cursor,
                        reinterpret_cast<float4 *>(base + w.spos));
     JPM_LAUNCHED("bin_scatter_kernel");
   }
-  // 4. per-(chunk, channel) LDS accumulation
-  if (rel) {
-    rc = zero_f32(s, reinterpret_cast<float *>(base + w.overflow), nc * M);
+  out.t = t;
+  out.n = uint32_t(n);
+  out.maxchunks = w.maxchunks;
+  out.rel = rel;
+  out.spos = reinterpret_cast<const float4 *>(base + w.spos);
+  out.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
+  out.nchunk = nchunk;
+  out.choff = choff;
+  out.total_chunks = totals + 1;
+  return JPM_OK;
+}
+
+// nc meshes (stride M) = [mesh +] window sums of the sorted particles, channel
+// c with weight[p*nc + c] (or wscalar when weight is NULL).
+int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
+               void *ws, size_t ws_bytes) {
+  JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
+  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  const TileGrid &t = b.t;
+  const int64_t M = int64_t(t.nx) * t.ny * t.nz;
+  JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
+  size_t opart, oover, owmax;
+  const size_t need = accum_ws(t, b.n, nc, b.rel ? JPM_RELATIVE : 0u, &opart, &oover, &owmax);
+  JPM_REQUIRE(ws && ws_bytes >= need, "binned paint needs %zu workspace bytes, got %zu", need, ws_bytes);
+  const bool accumulate = !(flags & JPM_ZERO_MESH);
+  char *base = static_cast<char *>(ws);
+  int rc;
+  if (b.rel) {
+    rc = zero_f32(s, reinterpret_cast<float *>(base + oover), nc * M);
     if (rc) return rc;
   }
   const float wmax_host = fabsf(wscalar);
   if (weight) {
-    rc = abs_max(s, weight, uint32_t(n * nc), reinterpret_cast<float *>(base + w.wmax));
+    rc = abs_max(s, weight, uint32_t(int64_t(b.n) * nc), reinterpret_cast<float *>(base + owmax));
     if (rc) return rc;
   }
-  auto *partial = reinterpret_cast<float *>(base + w.partial);
-  const float *wmax_dev = reinterpret_cast<const float *>(base + w.wmax);
-  float *overflow = reinterpret_cast<float *>(base + w.overflow);
-  hipLaunchKernelGGL(rel ? bin_accum_kernel<true> : bin_accum_kernel<false>, dim3(w.maxchunks, nc), dim3(256), 0, s,
-                     reinterpret_cast<const float4 *>(base + w.spos), weight, nc, wscalar,
-                     reinterpret_cast<const ChunkDesc *>(base + w.desc), totals + 1, t, wmax_dev, wmax_host,
-                     uint32_t(n), partial, overflow);
+  auto *partial = reinterpret_cast<float *>(base + opart);
+  const float *wmax_dev = reinterpret_cast<const float *>(base + owmax);
+  float *overflow = reinterpret_cast<float *>(base + oover);
+  {
+    JPM_TIMED(s, nc == 1 ? "bin_accum_kernel[nc=1]" : "bin_accum_kernel[nc=3]");
+    hipLaunchKernelGGL(b.rel ? bin_accum_kernel<true> : bin_accum_kernel<false>, dim3(b.maxchunks, nc), dim3(256), 0,
+                       s, b.spos, weight, nc, wscalar, b.desc, b.total_chunks, t, wmax_dev, wmax_host, b.n, partial,
+                       overflow);
+  }
   JPM_LAUNCHED("bin_accum_kernel");
-  // 5. per-tile gather-reduce of the windows
-  hipLaunchKernelGGL(bin_reduce_kernel, dim3(t.K, nc), dim3(256), 0, s, partial, choff, nchunk, t, nc, overflow,
-                     int(rel), wmax_dev, wmax_host, int(weight != nullptr), uint32_t(n), int(accumulate), mesh);
+  {
+    JPM_TIMED(s, nc == 1 ? "bin_reduce_kernel[nc=1]" : "bin_reduce_kernel[nc=3]");
+    hipLaunchKernelGGL(bin_reduce_kernel, dim3(t.K, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, t, nc, overflow,
+                       int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh);
+  }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;
+}
+
+int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
+                 float wscalar, int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  JPM_REQUIRE(mesh != nullptr, "mesh is null");
+  const int64_t M = cells(nx, ny, nz);
+  JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
+  const size_t need = binned_paint_workspace_size(nx, ny, nz, n, nc, flags);
+  JPM_REQUIRE(ws && ws_bytes >= need, "binned paint needs %zu workspace bytes, got %zu", need, ws_bytes);
+  if (n == 0) {
+    if (flags & JPM_ZERO_MESH) return zero_f32(s, mesh, nc * M);
+    return JPM_OK;
+  }
+  const size_t sort_bytes = align_up(bin_sort_workspace_size(nx, ny, nz, n));
+  Bins b;
+  rc = bin_sort(s, nx, ny, nz, pos, n, flags, ws, sort_bytes, b);
+  if (rc) return rc;
+  return bins_paint(s, b, weight, wscalar, nc, mesh, flags, static_cast<char *>(ws) + sort_bytes, ws_bytes - sort_bytes);
 }
 
 }  // namespace jpm

@@ -1,11 +1,16 @@
 // extern "C" entry points of libjaxpm_hip.so (declared in jaxpm_hip.h).
 // Each wrapper clears the thread's error, forwards to the jpm:: routine and
 // turns any C++ exception into JPM_E_NOMEM / JPM_E_INVALID.
+#include <atomic>
+#include <cstring>
 #include <exception>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
-#include "common.hpp"
+#include "tiles.hpp"
 
 namespace jpm {
 
@@ -22,6 +27,22 @@ int fail(int code, const char *fmt, ...) {
 }
 
 void clear_error() { t_err.clear(); }
+
+// ------------------------------------------------------------- timing
+static std::atomic<bool> g_timing{false};
+static std::mutex g_timing_mu;
+struct TimedLaunch {
+  const char *name;
+  hipEvent_t a, b;
+};
+static std::vector<TimedLaunch> g_launches;
+
+bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+
+void timing_record(hipStream_t, const char *name, hipEvent_t begin, hipEvent_t end) {
+  std::lock_guard<std::mutex> lk(g_timing_mu);
+  g_launches.push_back({name, begin, end});
+}
 const char *last_error() { return t_err.c_str(); }
 
 int cic_paint(hipStream_t, int, int, int, const float *, int64_t, const float *, float, float *, uint32_t,
@@ -102,6 +123,116 @@ static int pipeline_paint(hipStream_t s, int nx, int ny, int nz, const float *po
   return binned_paint(s, nx, ny, nz, pos, n, w, 1.0f, nc, mesh, f, ws, ws_bytes);
 }
 int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, float, float *, uint32_t);
+
+int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t);
+size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
+int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int);
+int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *);
+
+// ------------------------------------------------------ sorted pipelines
+//
+// The leapfrog's two hot calls, built on one tile sort of the positions:
+//   force_kick: sort, window paint, R2C, potential, C2R -> phi (kept for the
+//     adjoint), then the window gather applies the 4-point stencil to phi and
+//     kicks v += gamma F.
+//   force_vjp_pot: sort, 3-channel window paint of the cotangent, stencil
+//     divergence, R2C, potential, C2R -> rho_bar, then one window gather for
+//     both the direct term (on phi) and the density term (on rho_bar).
+
+struct SortedWs {
+  size_t sort, paint, mesh, g3, spec, spec2, total;
+};
+
+static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pflags) {
+  const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
+  SortedWs w;
+  size_t o = 0;
+  w.sort = o;  o += align_up(bin_sort_workspace_size(nx, ny, nz, n));
+  w.paint = o; o += align_up(bins_paint_workspace_size(nx, ny, nz, n, nc, pflags & JPM_RELATIVE));
+  w.mesh = o;  o += align_up(m * 4);
+  w.g3 = o;    o += (nc == 3) ? align_up(3 * m * 4) : 0;
+  w.spec = o;  o += align_up(h * 8);
+  w.spec2 = o; o += align_up(h * 8);
+  w.total = o;
+  return w;
+}
+
+static bool sorted_supported(int nx, int ny, int nz, uint32_t flags) {
+  return binned_paint_supported(nx, ny, nz) && !((flags >> 8) & JPM_K_GRAD_EXACT) && !(flags & JPM_ATOMIC);
+}
+
+static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
+                      float *phi, float r_split, uint32_t flags, int write_forces, void *ws, size_t ws_bytes,
+                      void *keep_sort, size_t keep_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs <= 32768 tiles and the 4-point gradient");
+  JPM_REQUIRE(n >= 1, "no particles");
+  JPM_REQUIRE(pos && vel && phi, "null pointer argument");
+  const SortedWs w = sorted_ws(nx, ny, nz, n, 1, pflags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "force_kick needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  char *base = static_cast<char *>(ws);
+  const int64_t m = cells(nx, ny, nz);
+  Bins b;
+  if (keep_sort) {
+    JPM_REQUIRE(keep_bytes >= bin_sort_workspace_size(nx, ny, nz, n), "keep_sort buffer too small (%zu bytes)",
+                keep_bytes);
+    rc = bin_sort(s, nx, ny, nz, pos, n, pflags, keep_sort, keep_bytes, b);
+  } else {
+    rc = bin_sort(s, nx, ny, nz, pos, n, pflags, base + w.sort, w.paint - w.sort, b);
+  }
+  if (rc) return rc;
+  float *mesh = reinterpret_cast<float *>(base + w.mesh);
+  rc = bins_paint(s, b, nullptr, 1.0f, 1, mesh, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint);
+  if (rc) return rc;
+  rc = fft_r2c(s, nx, ny, nz, 1, mesh, base + w.spec);
+  if (rc) return rc;
+  rc = kspace_potential(s, nx, ny, nz, base + w.spec, base + w.spec2, r_split, float(1.0 / double(m)), kflags);
+  if (rc) return rc;
+  rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, phi);
+  if (rc) return rc;
+  return bins_kick(s, b, phi, gamma, vel, write_forces);
+}
+
+static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
+                         const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
+                         size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs <= 32768 tiles and the 4-point gradient");
+  JPM_REQUIRE(n >= 1, "no particles");
+  JPM_REQUIRE(phi && pos && cot && gpos, "null pointer argument");
+  const SortedWs w = sorted_ws(nx, ny, nz, n, 3, pflags);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "force_vjp needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  char *base = static_cast<char *>(ws);
+  const int64_t m = cells(nx, ny, nz);
+  Bins b;
+  if (sorted) {
+    // the sort of these positions kept by jpm_force_kick_f32
+    JPM_REQUIRE(sorted_bytes >= bin_sort_workspace_size(nx, ny, nz, n), "sorted buffer too small (%zu bytes)",
+                sorted_bytes);
+    b = bins_view(nx, ny, nz, n, pflags, sorted);
+  } else {
+    rc = bin_sort(s, nx, ny, nz, pos, n, pflags, base + w.sort, w.paint - w.sort, b);
+    if (rc) return rc;
+  }
+  float *g3 = reinterpret_cast<float *>(base + w.g3);
+  float *rho = reinterpret_cast<float *>(base + w.mesh);
+  rc = bins_paint(s, b, cot, 1.0f, 3, g3, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint);
+  if (rc) return rc;
+  rc = fd4_divergence(s, nx, ny, nz, g3, 1.0f, rho);
+  if (rc) return rc;
+  rc = fft_r2c(s, nx, ny, nz, 1, rho, base + w.spec);
+  if (rc) return rc;
+  rc = kspace_potential(s, nx, ny, nz, base + w.spec, base + w.spec2, r_split, float(double(scale) / double(m)),
+                        kflags);
+  if (rc) return rc;
+  rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, rho);
+  if (rc) return rc;
+  return bins_grad(s, b, phi, rho, cot, scale, gpos);
+}
 
 // Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
 struct FieldWs {
@@ -289,6 +420,47 @@ int jpm_shutdown(void) {
 
 int jpm_abi_version(void) { return JPM_ABI_VERSION; }
 
+int jpm_timing_enable(int on) {
+  clear_error();
+  g_timing.store(on != 0);
+  return JPM_OK;
+}
+
+int jpm_timing_report(char *buf, size_t len) {
+  clear_error();
+  std::vector<TimedLaunch> ls;
+  {
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    ls.swap(g_launches);
+  }
+  std::map<std::string, std::pair<long, double>> agg;
+  int rc = JPM_OK;
+  for (auto &l : ls) {
+    float ms = 0.0f;
+    if (hipEventSynchronize(l.b) == hipSuccess && hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) {
+      auto &e = agg[l.name];
+      e.first += 1;
+      e.second += ms;
+    } else {
+      rc = fail(JPM_E_HIP, "timing event failed for %s", l.name);
+    }
+    (void)hipEventDestroy(l.a);
+    (void)hipEventDestroy(l.b);
+  }
+  std::string out;
+  for (auto &kv : agg) {
+    char line[256];
+    snprintf(line, sizeof line, "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    out += line;
+  }
+  if (buf && len) {
+    strncpy(buf, out.c_str(), len - 1);
+    buf[len - 1] = 0;
+    if (out.size() >= len) return rc ? rc : fail(JPM_E_NOMEM, "timing report truncated (%zu bytes)", out.size());
+  }
+  return rc;
+}
+
 const char *jpm_last_error(void) { return last_error(); }
 
 size_t jpm_cic_paint_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
@@ -427,6 +599,35 @@ int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3, const 
 size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
   return force_ws(nx, ny, nz, n, flags & 0xffu).total;
+}
+
+size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !sorted_supported(nx, ny, nz, flags)) return 0;
+  return sorted_ws(nx, ny, nz, n, 1, flags & 0xffu).total;
+}
+
+int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
+                       float *phi, float r_split, uint32_t flags, int write_forces, void *ws, size_t ws_bytes,
+                       void *keep_sort, size_t keep_bytes) {
+  JPM_GUARD(force_kick(as_stream(stream), nx, ny, nz, pos, n, vel, gamma, phi, r_split, flags, write_forces, ws,
+                       ws_bytes, keep_sort, keep_bytes));
+}
+
+size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
+  return bin_sort_workspace_size(nx, ny, nz, n);
+}
+
+size_t jpm_force_vjp_pot_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !sorted_supported(nx, ny, nz, flags)) return 0;
+  return sorted_ws(nx, ny, nz, n, 3, flags & 0xffu).total;
+}
+
+int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
+                          const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
+                          size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+  JPM_GUARD(force_vjp_pot(as_stream(stream), nx, ny, nz, phi, pos, n, cot, scale, gpos, r_split, flags, ws,
+                          ws_bytes, sorted, sorted_bytes));
 }
 
 size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {

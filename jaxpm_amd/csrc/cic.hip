@@ -347,7 +347,10 @@ static Grid make_grid(int nx, int ny, int nz) {
 // *out = max |w[i]| over n floats (out is zeroed first).
 int abs_max(hipStream_t s, const float *w, uint32_t n, float *out) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, out, 1u);
-  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, w, n, out);
+  {
+    JPM_TIMED(s, "abs_max_kernel");
+    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, w, n, out);
+  }
   JPM_LAUNCHED("abs_max_kernel");
   return JPM_OK;
 }
@@ -399,7 +402,10 @@ int cic_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n
   JPM_LAUNCHED("zero_u64_kernel");
   if (hw) {
     hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
-    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, weight, uint32_t(n), wmax);
+    {
+      JPM_TIMED(s, "abs_max_kernel");
+      hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, weight, uint32_t(n), wmax);
+    }
     JPM_LAUNCHED("abs_max_kernel");
   }
   const float wh = fabsf(wscalar);
@@ -507,8 +513,11 @@ int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int
   float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(nc) * size_t(m) * 8));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m));
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
-  hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 2048)), dim3(256), 0, s, weight,
-                     uint32_t(n * nc), wmax);
+  {
+    JPM_TIMED(s, "abs_max_kernel");
+    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 2048)), dim3(256), 0, s, weight,
+                       uint32_t(n * nc), wmax);
+  }
   JPM_LAUNCHED("abs_max_kernel");
 #define JPM_PM(R, C)                                                                                  \
   hipLaunchKernelGGL((paint_multi_fixed_kernel<R, C>), dim3(grid), dim3(256), 0, s, pos, weight, uint32_t(n), \
@@ -625,8 +634,11 @@ int axpy(hipStream_t s, int64_t n, float a, const float *x, float *y) {
               "axpy operands must be 16-byte aligned");
   const uint32_t n4 = uint32_t(n / 4);
   if (n4) {
-    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n4, 256, 256 * 128)), dim3(256), 0, s, a,
-                       reinterpret_cast<const float4 *>(x), reinterpret_cast<float4 *>(y), n4);
+    {
+      JPM_TIMED(s, "axpy_kernel");
+      hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n4, 256, 256 * 128)), dim3(256), 0, s, a,
+                         reinterpret_cast<const float4 *>(x), reinterpret_cast<float4 *>(y), n4);
+    }
     JPM_LAUNCHED("axpy_kernel");
   }
   if (uint32_t(n) > 4 * n4) {
@@ -650,11 +662,17 @@ int kick(hipStream_t s, int nx, int ny, int nz, const float *phi3, const float *
   Grid g = make_grid(nx, ny, nz);
   const unsigned grid = grid_for(n, 256, 256 * 128);
   if (rel)
-    hipLaunchKernelGGL(kick_kernel<true>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g, gamma,
-                       vel);
+    {
+      JPM_TIMED(s, "kick_kernel");
+      hipLaunchKernelGGL(kick_kernel<true>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g, gamma,
+                         vel);
+    }
   else
-    hipLaunchKernelGGL(kick_kernel<false>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g,
-                       gamma, vel);
+    {
+      JPM_TIMED(s, "kick_kernel");
+      hipLaunchKernelGGL(kick_kernel<false>, dim3(grid), dim3(256), 0, s, phi3, uint32_t(m), pos, uint32_t(n), g,
+                         gamma, vel);
+    }
   JPM_LAUNCHED("kick_kernel");
   return JPM_OK;
 }

@@ -39,6 +39,32 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
                          hipGetErrorString(e_));                               \
   } while (0)
 
+// Optional per-kernel timing: when enabled (jpm_timing_enable), launches
+// wrapped in JPM_TIMED record a pair of HIP events on their stream; the
+// report sums the elapsed times per kernel name.
+bool timing_enabled();
+void timing_record(hipStream_t s, const char *name, hipEvent_t begin, hipEvent_t end);
+
+struct TimedScope {
+  hipStream_t s;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedScope(hipStream_t st, const char *nm) : s(st), name(nm) {
+    if (timing_enabled() && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+      (void)hipEventRecord(a, s);
+    else
+      a = b = nullptr;
+  }
+  ~TimedScope() {
+    if (a && b) {
+      (void)hipEventRecord(b, s);
+      timing_record(s, name, a, b);
+    }
+  }
+};
+
+#define JPM_TIMED(stream, name) ::jpm::TimedScope jpm_timed_scope_(stream, name)
+
 // Largest mesh the 32-bit flat cell index supports (1290^3 cells).
 constexpr int64_t kMaxCells = (int64_t(1) << 31) - 1;
 

@@ -67,6 +67,7 @@ int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, v
   hipfftHandle h;
   rc = plan_locked(nx, ny, nz, HIPFFT_R2C, batch, h);
   if (rc) return rc;
+  JPM_TIMED(s, "hipfft_r2c");
   hipfftResult r = hipfftSetStream(h, s);
   if (r == HIPFFT_SUCCESS)
     r = hipfftExecR2C(h, const_cast<hipfftReal *>(in), static_cast<hipfftComplex *>(out));
@@ -83,6 +84,7 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
   hipfftHandle h;
   rc = plan_locked(nx, ny, nz, HIPFFT_C2R, batch, h);
   if (rc) return rc;
+  JPM_TIMED(s, "hipfft_c2r");
   hipfftResult r = hipfftSetStream(h, s);
   if (r == HIPFFT_SUCCESS) r = hipfftExecC2R(h, static_cast<hipfftComplex *>(in), out);
   if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2R: %s", fft_err(r));

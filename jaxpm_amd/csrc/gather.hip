@@ -1,0 +1,252 @@
+// Window gathers over the tile sort: the force read (fused with the kick or
+// writing the forces) and the position cotangent of the force, both taking
+// the *potential* phi and applying the 4-point gradient stencil in LDS.
+//
+// One workgroup per chunk of a tile's particles.  The workgroup loads the
+// potential over the tile plus a 2-cell stencil halo, [o-2, o+19)^3 (21^3
+// floats, periodic), into LDS; each particle's 8 CIC corners lie in
+// [o, o+17) so every stencil tap -D_d phi(c) = -(8 (phi(c+1) - phi(c-1)) -
+// (phi(c+2) - phi(c-2))) / 12 is an LDS read.  Results go back to the
+// particle's original index.  Relative-mode corners that rounding puts
+// outside the window (see lcorner_rel) read the mesh directly.
+#include "tiles.hpp"
+
+namespace jpm {
+
+constexpr int kHalo = 2;
+constexpr int kPw = kWin + 2 * kHalo;        // 21
+constexpr int kPwCells = kPw * kPw * kPw;    // 9261
+
+struct CornerSet {
+  int l[3][2];      // window-local index per axis and corner bit (-1 outside, -2 dropped)
+  int g[3][2];      // global cell per axis and corner bit (relative mode fallback)
+  float w[3][2];    // CIC weight factors
+  float d[3][2];    // their derivatives
+};
+
+template <bool REL>
+__device__ __forceinline__ void corner_set(float4 rec, const TileGrid &t, int ox, int oy, int oz, CornerSet &cs) {
+  const float x = rec.x, y = rec.y, z = rec.z;
+  if (REL) {
+    const uint32_t pidx = __float_as_uint(rec.w);
+    uint32_t r = t.div_nz.div(pidx);
+    uint32_t k = pidx - r * uint32_t(t.nz);
+    uint32_t i = t.div_ny.div(r);
+    uint32_t j = r - i * uint32_t(t.ny);
+    const float p[3] = {float(i) + x, float(j) + y, float(k) + z};
+    const int n[3] = {t.nx, t.ny, t.nz};
+    const int o[3] = {ox, oy, oz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        lcorner_rel(p[a], float(b), n[a], o[a], cs.l[a][b], cs.w[a][b], cs.g[a][b]);
+        // the same nd the weight came from gives the derivative sign
+        const float L = float(n[a]);
+        float q = floorf(jmod(p[a] + float(b), L));
+        float nd = p[a] - q;
+        nd = nd - rintf(nd / L) * L;
+        cs.d[a][b] = cs.l[a][b] == -2 ? 0.0f : (nd >= 0.0f ? -1.0f : 1.0f);
+      }
+    }
+  } else {
+    const float p[3] = {x, y, z};
+    const int n[3] = {t.nx, t.ny, t.nz};
+    const int o[3] = {ox, oy, oz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      LAxis ax = laxis_abs(p[a], o[a], n[a]);
+      cs.l[a][0] = ax.l0;
+      cs.l[a][1] = ax.l1;
+      cs.w[a][0] = ax.w0;
+      cs.w[a][1] = ax.w1;
+      const float f = floorf(p[a]);
+      cs.d[a][0] = (p[a] - f) >= 0.0f ? -1.0f : 1.0f;
+      cs.d[a][1] = (p[a] - (f + 1.0f)) >= 0.0f ? -1.0f : 1.0f;
+      cs.g[a][0] = cs.g[a][1] = 0;
+    }
+  }
+}
+
+__device__ __forceinline__ int wrapn(int c, int n) {
+  int r = c % n;
+  return r < 0 ? r + n : r;
+}
+
+// Load phi over [o - halo, o - halo + edge)^3 (periodic) into win.
+__device__ __forceinline__ void load_window(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
+                                            int halo, int edge, float *win) {
+  const int cells = edge * edge * edge;
+  for (int e = threadIdx.x; e < cells; e += blockDim.x) {
+    const int c = e % edge;
+    const int ab = e / edge;
+    const int b = ab % edge;
+    const int a = ab / edge;
+    const int gx = wrapn(ox - halo + a, t.nx), gy = wrapn(oy - halo + b, t.ny), gz = wrapn(oz - halo + c, t.nz);
+    win[e] = phi[(uint32_t(gx) * uint32_t(t.ny) + uint32_t(gy)) * uint32_t(t.nz) + uint32_t(gz)];
+  }
+}
+
+constexpr float kC8 = 8.0f / 12.0f;
+constexpr float kC1 = 1.0f / 12.0f;
+
+// -D phi at window cell (a, b, c) (already offset by the halo), all 3 axes.
+__device__ __forceinline__ void stencil_lds(const float *win, int a, int b, int c, float F[3]) {
+  const int sx = kPw * kPw, sy = kPw;
+  const int e = (a * kPw + b) * kPw + c;
+  F[0] = -(kC8 * (win[e + sx] - win[e - sx]) - kC1 * (win[e + 2 * sx] - win[e - 2 * sx]));
+  F[1] = -(kC8 * (win[e + sy] - win[e - sy]) - kC1 * (win[e + 2 * sy] - win[e - 2 * sy]));
+  F[2] = -(kC8 * (win[e + 1] - win[e - 1]) - kC1 * (win[e + 2] - win[e - 2]));
+}
+
+// Same stencil read straight from the mesh (out-of-window fallback).
+__device__ __forceinline__ void stencil_global(const float *__restrict__ phi, const TileGrid &t, int x, int y, int z,
+                                               float F[3]) {
+  auto at = [&](int i, int j, int k) {
+    return phi[(uint32_t(wrapn(i, t.nx)) * uint32_t(t.ny) + uint32_t(wrapn(j, t.ny))) * uint32_t(t.nz) +
+               uint32_t(wrapn(k, t.nz))];
+  };
+  F[0] = -(kC8 * (at(x + 1, y, z) - at(x - 1, y, z)) - kC1 * (at(x + 2, y, z) - at(x - 2, y, z)));
+  F[1] = -(kC8 * (at(x, y + 1, z) - at(x, y - 1, z)) - kC1 * (at(x, y + 2, z) - at(x, y - 2, z)));
+  F[2] = -(kC8 * (at(x, y, z + 1) - at(x, y, z - 1)) - kC1 * (at(x, y, z + 2) - at(x, y, z - 2)));
+}
+
+// MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F.
+template <bool REL, int MODE>
+__global__ void __launch_bounds__(256)
+bin_kick_kernel(Bins bins, const float *__restrict__ phi, float gamma, float *__restrict__ vel) {
+  __shared__ float win[kPwCells];
+  const uint32_t c = blockIdx.x;
+  if (c >= *bins.total_chunks) return;
+  const ChunkDesc d = bins.desc[c];
+  const TileGrid &t = bins.t;
+  int itx, ity, itz;
+  tile_origin(d.tile, t, itx, ity, itz);
+  const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+  load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
+  __syncthreads();
+  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
+    const float4 rec = bins.spos[d.start + q];
+    CornerSet cs;
+    corner_set<REL>(rec, t, ox, oy, oz, cs);
+    float s[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+      const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
+      if (REL && (la == -2 || lb == -2 || lc == -2)) continue;
+      const float wc = cs.w[0][bx] * cs.w[1][by] * cs.w[2][bz];
+      float F[3];
+      if (!REL || (la >= 0 && lb >= 0 && lc >= 0))
+        stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+      else
+        stencil_global(phi, t, cs.g[0][bx], cs.g[1][by], cs.g[2][bz], F);
+      s[0] += F[0] * wc;
+      s[1] += F[1] * wc;
+      s[2] += F[2] * wc;
+    }
+    const uint32_t idx = __float_as_uint(rec.w);
+    float *v = vel + 3 * size_t(idx);
+    if (MODE == 0) {
+      v[0] += gamma * s[0];
+      v[1] += gamma * s[1];
+      v[2] += gamma * s[2];
+    } else {
+      v[0] = gamma * s[0];
+      v[1] = gamma * s[1];
+      v[2] = gamma * s[2];
+    }
+  }
+}
+
+// gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx.
+template <bool REL>
+__global__ void __launch_bounds__(256)
+bin_grad_kernel(Bins bins, const float *__restrict__ phi, const float *__restrict__ rho_bar,
+                const float *__restrict__ cot, float scale, float *__restrict__ gpos) {
+  __shared__ float win[kPwCells];
+  __shared__ float rwin[kWinCells];
+  const uint32_t c = blockIdx.x;
+  if (c >= *bins.total_chunks) return;
+  const ChunkDesc d = bins.desc[c];
+  const TileGrid &t = bins.t;
+  int itx, ity, itz;
+  tile_origin(d.tile, t, itx, ity, itz);
+  const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+  load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
+  load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
+  __syncthreads();
+  const uint32_t M = uint32_t(t.nx) * t.ny * t.nz;
+  (void)M;
+  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
+    const float4 rec = bins.spos[d.start + q];
+    const uint32_t idx = __float_as_uint(rec.w);
+    CornerSet cs;
+    corner_set<REL>(rec, t, ox, oy, oz, cs);
+    const float u0 = cot[3 * size_t(idx)], u1 = cot[3 * size_t(idx) + 1], u2 = cot[3 * size_t(idx) + 2];
+    float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+      const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
+      if (REL && (la == -2 || lb == -2 || lc == -2)) continue;
+      float F[3], rv;
+      if (!REL || (la >= 0 && lb >= 0 && lc >= 0)) {
+        stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+        rv = rwin[(la * kWin + lb) * kWin + lc];
+      } else {
+        const int x = cs.g[0][bx], y = cs.g[1][by], z = cs.g[2][bz];
+        stencil_global(phi, t, x, y, z, F);
+        rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
+      }
+      const float f = u0 * F[0] + u1 * F[1] + u2 * F[2];
+      const float dwx = cs.d[0][bx] * cs.w[1][by] * cs.w[2][bz];
+      const float dwy = cs.w[0][bx] * cs.d[1][by] * cs.w[2][bz];
+      const float dwz = cs.w[0][bx] * cs.w[1][by] * cs.d[2][bz];
+      gd[0] += f * dwx;
+      gd[1] += f * dwy;
+      gd[2] += f * dwz;
+      gr[0] += rv * dwx;
+      gr[1] += rv * dwy;
+      gr[2] += rv * dwz;
+    }
+    float *g = gpos + 3 * size_t(idx);
+    g[0] += scale * gd[0] + gr[0];
+    g[1] += scale * gd[1] + gr[1];
+    g[2] += scale * gd[2] + gr[2];
+  }
+}
+
+int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces) {
+  JPM_REQUIRE(phi && vel, "null pointer argument");
+  const dim3 grid(b.maxchunks), block(256);
+  if (b.rel) {
+    if (write_forces) { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<true, 1>), grid, block, 0, s, b, phi, gamma, vel); }
+    else { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<true, 0>), grid, block, 0, s, b, phi, gamma, vel); }
+  } else {
+    if (write_forces) { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<false, 1>), grid, block, 0, s, b, phi, gamma, vel); }
+    else { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<false, 0>), grid, block, 0, s, b, phi, gamma, vel); }
+  }
+  JPM_LAUNCHED("bin_kick_kernel");
+  return JPM_OK;
+}
+
+int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
+              float *gpos) {
+  JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
+  const dim3 grid(b.maxchunks), block(256);
+  if (b.rel)
+    {
+      JPM_TIMED(s, "bin_grad_kernel");
+      hipLaunchKernelGGL(bin_grad_kernel<true>, grid, block, 0, s, b, phi, rho_bar, cot, scale, gpos);
+    }
+  else
+    {
+      JPM_TIMED(s, "bin_grad_kernel");
+      hipLaunchKernelGGL(bin_grad_kernel<false>, grid, block, 0, s, b, phi, rho_bar, cot, scale, gpos);
+    }
+  JPM_LAUNCHED("bin_grad_kernel");
+  return JPM_OK;
+}
+
+}  // namespace jpm

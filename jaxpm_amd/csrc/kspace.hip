@@ -382,9 +382,12 @@ int kspace_force(hipStream_t s, int nx, int ny, int nz, const void *dk, void *ou
   rc = get_tabs(s, nx, ny, nz, flags, t);
   if (rc) return rc;
   HalfGrid hg = make_half(nx, ny, nz);
-  hipLaunchKernelGGL(force_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                     static_cast<const float2 *>(dk), static_cast<float2 *>(out3), hg, t, r_split * r_split,
-                     scale, int((flags & JPM_K_DECONV) != 0));
+  {
+    JPM_TIMED(s, "force_kernel");
+    hipLaunchKernelGGL(force_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                       static_cast<const float2 *>(dk), static_cast<float2 *>(out3), hg, t, r_split * r_split,
+                       scale, int((flags & JPM_K_DECONV) != 0));
+  }
   JPM_LAUNCHED("force_kernel");
   return JPM_OK;
 }
@@ -398,9 +401,12 @@ int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void
   rc = get_tabs(s, nx, ny, nz, flags, t);
   if (rc) return rc;
   HalfGrid hg = make_half(nx, ny, nz);
-  hipLaunchKernelGGL(potential_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                     static_cast<const float2 *>(dk), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
-                     int((flags & JPM_K_DECONV) != 0));
+  {
+    JPM_TIMED(s, "potential_kernel");
+    hipLaunchKernelGGL(potential_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                       static_cast<const float2 *>(dk), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
+                       int((flags & JPM_K_DECONV) != 0));
+  }
   JPM_LAUNCHED("potential_kernel");
   return JPM_OK;
 }
@@ -434,8 +440,11 @@ int kspace_linear_field(hipStream_t s, int nx, int ny, int nz, void *fk, float L
   const float3 nm = make_float3(float(nx), float(ny), float(nz));
   const float mcells = float(cells(nx, ny, nz));
   const float vol = float(double(Lx) * double(Ly) * double(Lz));
-  hipLaunchKernelGGL(linear_field_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                     static_cast<float2 *>(fk), hg, t, box, nm, mcells, vol, ktab, pktab, ntab, scale);
+  {
+    JPM_TIMED(s, "linear_field_kernel");
+    hipLaunchKernelGGL(linear_field_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                       static_cast<float2 *>(fk), hg, t, box, nm, mcells, vol, ktab, pktab, ntab, scale);
+  }
   JPM_LAUNCHED("linear_field_kernel");
   return JPM_OK;
 }
@@ -450,8 +459,11 @@ int kspace_shear6(hipStream_t s, int nx, int ny, int nz, const void *dk, void *o
   rc = get_tabs(s, nx, ny, nz, flags, t);
   if (rc) return rc;
   HalfGrid hg = make_half(nx, ny, nz);
-  hipLaunchKernelGGL(shear6_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                     static_cast<const float2 *>(dk), static_cast<float2 *>(out6), hg, t, scale);
+  {
+    JPM_TIMED(s, "shear6_kernel");
+    hipLaunchKernelGGL(shear6_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                       static_cast<const float2 *>(dk), static_cast<float2 *>(out6), hg, t, scale);
+  }
   JPM_LAUNCHED("shear6_kernel");
   return JPM_OK;
 }
@@ -475,9 +487,12 @@ int kspace_force_adjoint(hipStream_t s, int nx, int ny, int nz, const void *in3,
   rc = get_tabs(s, nx, ny, nz, flags, t);
   if (rc) return rc;
   HalfGrid hg = make_half(nx, ny, nz);
-  hipLaunchKernelGGL(force_adjoint_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                     static_cast<const float2 *>(in3), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
-                     int((flags & JPM_K_DECONV) != 0));
+  {
+    JPM_TIMED(s, "force_adjoint_kernel");
+    hipLaunchKernelGGL(force_adjoint_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+                       static_cast<const float2 *>(in3), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
+                       int((flags & JPM_K_DECONV) != 0));
+  }
   JPM_LAUNCHED("force_adjoint_kernel");
   return JPM_OK;
 }

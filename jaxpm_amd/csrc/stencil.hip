@@ -101,8 +101,11 @@ int fd4_gradient(hipStream_t s, int nx, int ny, int nz, const float *phi, float 
   JPM_REQUIRE(phi && out3, "null pointer argument");
   const int64_t m = cells(nx, ny, nz);
   JPM_REQUIRE(3 * m <= (int64_t(1) << 32), "mesh too large");
-  hipLaunchKernelGGL(fd4_gradient_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, phi,
-                     make_mesh3(nx, ny, nz), scale, out3);
+  {
+    JPM_TIMED(s, "fd4_gradient_kernel");
+    hipLaunchKernelGGL(fd4_gradient_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, phi,
+                       make_mesh3(nx, ny, nz), scale, out3);
+  }
   JPM_LAUNCHED("fd4_gradient_kernel");
   return JPM_OK;
 }
@@ -113,8 +116,11 @@ int fd4_divergence(hipStream_t s, int nx, int ny, int nz, const float *g3, float
   JPM_REQUIRE(g3 && out, "null pointer argument");
   const int64_t m = cells(nx, ny, nz);
   JPM_REQUIRE(3 * m <= (int64_t(1) << 32), "mesh too large");
-  hipLaunchKernelGGL(fd4_divergence_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, g3,
-                     make_mesh3(nx, ny, nz), scale, out);
+  {
+    JPM_TIMED(s, "fd4_divergence_kernel");
+    hipLaunchKernelGGL(fd4_divergence_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, g3,
+                       make_mesh3(nx, ny, nz), scale, out);
+  }
   JPM_LAUNCHED("fd4_divergence_kernel");
   return JPM_OK;
 }

@@ -1,0 +1,135 @@
+// Tile geometry shared by the tile sort, the window paint and the window
+// gathers (16^3-cell tiles; particles sorted by the tile of their floor cell).
+#pragma once
+
+#include "common.hpp"
+
+namespace jpm {
+
+constexpr int kTile = 16;                       // tile edge in cells
+constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
+constexpr int kWinCells = kWin * kWin * kWin;   // 4913
+constexpr int kChunk = 8192;                    // max particles per chunk
+constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
+
+struct TileGrid {
+  int nx, ny, nz;   // mesh
+  int tx, ty, tz;   // tiles per axis
+  uint32_t K;       // tiles
+  FastDiv div_nz, div_ny, div_tz, div_ty;
+};
+
+inline TileGrid make_tiles(int nx, int ny, int nz) {
+  TileGrid t;
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.tx = (nx + kTile - 1) / kTile;
+  t.ty = (ny + kTile - 1) / kTile;
+  t.tz = (nz + kTile - 1) / kTile;
+  t.K = uint32_t(t.tx) * t.ty * t.tz;
+  t.div_nz = FastDiv(uint32_t(nz));
+  t.div_ny = FastDiv(uint32_t(ny));
+  t.div_tz = FastDiv(uint32_t(t.tz));
+  t.div_ty = FastDiv(uint32_t(t.ty));
+  return t;
+}
+
+__device__ __forceinline__ int wrapi(int c, int n) {
+  int r = c % n;
+  return r < 0 ? r + n : r;
+}
+
+// Floor cell of a particle: absolute -> int32(floor(x)) mod n (painting.py:
+// 23, 35-37); relative -> the low corner of enmesh (painting_utils.py:48-57),
+// or, when that corner is dropped, the high corner minus one.
+__device__ __forceinline__ int floor_cell_abs(float x, int n) { return wrapi(int(floorf(x)), n); }
+
+__device__ __forceinline__ float jmod(float x, float L) {
+  float m = fmodf(x, L);
+  if (m != 0.0f && m < 0.0f) m = m + L;
+  return m;
+}
+
+__device__ __forceinline__ int floor_cell_rel(float p, int n) {
+  const float L = float(n);
+  float q = floorf(jmod(p, L));
+  if (q >= 0.0f && q < L) return int(q);
+  float q1 = floorf(jmod(p + 1.0f, L));
+  if (q1 >= 0.0f && q1 < L) return wrapi(int(q1) - 1, n);
+  return 0;  // NaN: every corner is dropped, any tile will do
+}
+
+struct ChunkDesc {
+  uint32_t tile, start, count, pad;
+};
+
+// The result of a tile sort: particles as float4 (x, y, z, index bits)
+// grouped by tile, chunked into runs of at most kChunk particles of one tile.
+struct Bins {
+  TileGrid t;
+  uint32_t n, maxchunks;
+  bool rel;
+  const float4 *spos;
+  const ChunkDesc *desc;
+  const uint32_t *nchunk, *choff, *total_chunks;
+};
+
+int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
+             size_t ws_bytes, Bins &out);
+size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n);
+Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws);
+
+// One corner axis of a particle relative to its tile origin: local window
+// index (or -1 when the corner is dropped / outside the window) and weight.
+struct LAxis {
+  int l0, l1;
+  float w0, w1;
+};
+
+__device__ __forceinline__ LAxis laxis_abs(float x, int origin, int n) {
+  LAxis a;
+  const float f = floorf(x);
+  const float f1 = f + 1.0f;
+  a.w0 = 1.0f - fabsf(x - f);
+  a.w1 = 1.0f - fabsf(x - f1);
+  const int c0 = wrapi(int(f), n);
+  a.l0 = c0 - origin;  // the floor cell lies in the tile by construction
+  a.l1 = a.l0 + 1;
+  return a;
+}
+
+// enmesh corner (painting_utils.py:48-69) mapped into the window; corners
+// that land outside [origin, origin + 16] (rounding edge cases) report -1 and
+// are deposited by the caller through the overflow path.
+__device__ __forceinline__ void lcorner_rel(float p, float o, int n, int origin, int &l, float &w, int &cell) {
+  const float L = float(n);
+  float q = floorf(jmod(p + o, L));
+  float nd = p - q;
+  nd = nd - rintf(nd / L) * L;
+  const bool ok = (q >= 0.0f) && (q < L);
+  cell = ok ? int(q) : -1;
+  w = ok ? 1.0f - fabsf(nd) : 0.0f;
+  if (!ok) {
+    l = -2;  // dropped: no deposit at all
+    return;
+  }
+  int d = int(q) - origin;
+  if (d < 0) d += n;  // window may wrap past the mesh end
+  l = (d >= 0 && d < kWin) ? d : -1;
+}
+
+__device__ __forceinline__ int tile_len(int it, int ntile, int n) {
+  return (it == ntile - 1) ? n - it * kTile : kTile;
+}
+
+
+// Tile origin of a tile index.
+__device__ __forceinline__ void tile_origin(uint32_t tile, const TileGrid &t, int &itx, int &ity, int &itz) {
+  const uint32_t txy = t.div_tz.div(tile);
+  itz = int(tile - txy * uint32_t(t.tz));
+  itx = int(t.div_ty.div(txy));
+  ity = int(txy - uint32_t(itx) * uint32_t(t.ty));
+}
+
+}  // namespace jpm

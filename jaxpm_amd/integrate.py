@@ -66,10 +66,48 @@ class Stepper:
         self.trace = None
         fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
-        self.ws = torch.empty(max(fb, vb), dtype=torch.uint8, device=self.dev)
+        kb = size("jpm_force_kick_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
+        pb = size("jpm_force_vjp_pot_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
+        # the tile-sorted pipeline (potential kept per evaluation, stencil in
+        # LDS) when the mesh and flags allow it; otherwise three force meshes
+        self.sorted = kb > 0 and pb > 0
+        self.sort_bytes = size("jpm_bin_sort_size", self.nx, self.ny, self.nz, self.n) if self.sorted else 0
+        self.ws = torch.empty(max(fb, vb, kb, pb), dtype=torch.uint8, device=self.dev)
 
     def new_phi(self):
         return torch.empty((3, *self.shape), dtype=torch.float32, device=self.dev)
+
+    def new_field(self, keep_sort=False):
+        """Buffer for what one force evaluation keeps for its adjoint: the
+        potential and (``keep_sort``) the tile sort of the positions on the
+        sorted pipeline, or the three force meshes."""
+        if self.sorted:
+            pot = torch.empty(self.shape, dtype=torch.float32, device=self.dev)
+            srt = torch.empty(self.sort_bytes, dtype=torch.uint8, device=self.dev) if keep_sort else None
+            return (pot, srt)
+        return (self.new_phi(), None)
+
+    def field_kick(self, x, v, gamma, field):
+        """v += gamma F(x); ``field`` (from new_field) receives what the
+        adjoint of this evaluation needs."""
+        buf, srt = field
+        if self.sorted:
+            self._timed("field_kick", "jpm_force_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
+                        ptr(v), float(gamma), ptr(buf), self.r_split, self.flags, 0, ptr(self.ws),
+                        self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+        else:
+            self.field(x, buf)
+            self.kick(buf, x, v, gamma)
+
+    def vjp_field(self, field, x, cot, scale, gpos):
+        """gpos += scale J(x)^T cot given the evaluation's ``field``."""
+        buf, srt = field
+        if self.sorted:
+            self._timed("vjp", "jpm_force_vjp_pot_f32", stream(), self.nx, self.ny, self.nz, ptr(buf), ptr(x),
+                        self.n, ptr(cot), float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws),
+                        self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+        else:
+            self.vjp(buf, x, cot, scale, gpos)
 
     def _timed(self, name, fn, *args):
         if self.trace is None:
@@ -107,8 +145,7 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None):
         if keep is not None:
             xk, phi = keep[i - start]
             xk.copy_(x)
-        stepper.field(x, phi)
-        stepper.kick(phi, x, v, gamma[i])
+        stepper.field_kick(x, v, gamma[i], phi)
 
 
 class NBody(torch.autograd.Function):
@@ -120,9 +157,8 @@ class NBody(torch.autograd.Function):
         S = len(beta)
         x = x0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
-        phi = stepper.new_phi()
-        stepper.field(x, phi)
-        stepper.kick(phi, x, v, k0)
+        phi = stepper.new_field()
+        stepper.field_kick(x, v, k0, phi)
         k = S if checkpoint_every is None else max(1, int(checkpoint_every))
         ckpts = {0: (x.clone(), v.clone())}
         for s0 in range(0, S, k):
@@ -140,8 +176,8 @@ class NBody(torch.autograd.Function):
         x_ref = ctx.ckpts[0][0]
         lx = (gx if gx is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
         lv = (gv if gv is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
-        phi = st.new_phi()
-        keep = [(torch.empty_like(x_ref), st.new_phi()) for _ in range(min(ctx.k, S))]
+        phi = st.new_field()
+        keep = [(torch.empty_like(x_ref), st.new_field(keep_sort=True)) for _ in range(min(ctx.k, S))]
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
             xc, vc = ctx.ckpts[s0]
@@ -149,13 +185,14 @@ class NBody(torch.autograd.Function):
             _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep)
             for i in reversed(range(s0, s1)):
                 xi, phii = keep[i - s0]
-                st.vjp(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
+                st.vjp_field(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
                 st.axpy(beta[i], lx, lv)            # drift i: x += beta v
         del keep
         # the first half kick used x0 before any drift: recompute its field
         xs = ctx.ckpts[0][0]
-        st.field(xs, phi)
-        st.vjp(phi, xs, lv, k0, lx)
+        scratch_v = torch.zeros_like(xs)
+        st.field_kick(xs, scratch_v, 0.0, phi)
+        st.vjp_field(phi, xs, lv, k0, lx)
         ctx.ckpts = None
         return lx, lv, None, None, None
 

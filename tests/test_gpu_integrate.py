@@ -70,3 +70,55 @@ def test_leapfrog_adjoint(gpu, checkpoint_every):
     lx, lv = oa.leapfrog_vjp(x0, v0, steps, oc.Cosmology(), shape, gx, gv)
     assert rel_err(xt.grad.cpu().numpy(), lx) < 2e-4
     assert rel_err(vt.grad.cpu().numpy(), lv) < 2e-4
+
+
+SORTED_SHAPES = [(16, 16, 16), (8, 8, 8), (17, 33, 20), (32, 32, 64)]
+
+
+@pytest.mark.parametrize("shape", SORTED_SHAPES)
+@pytest.mark.parametrize("relative", [False, True])
+def test_sorted_force_kick_and_vjp(gpu, shape, relative):
+    """The tile-sorted pipeline (potential + LDS stencil gathers) against the
+    oracle's pm_forces and force VJP."""
+    from jaxpm_amd import integrate
+    from oracle import pm as opm
+    print(f"sorted {shape} rel={relative}", flush=True)
+    r = np.random.default_rng(35)
+    n = int(np.prod(shape))
+    disp = r.normal(scale=1.2, size=(n, 3)).astype(np.float32)
+    pos = disp if relative else (lattice(shape) + disp).astype(np.float32)
+    st = integrate.Stepper(shape, n, relative=relative)
+    assert st.sorted
+    xt = torch.as_tensor(pos, device="cuda")
+    v = torch.full((n, 3), 0.25, device="cuda")
+    phi = st.new_field()
+    st.field_kick(xt, v, 0.5, phi)
+    ref_f = opm.pm_forces(pos.reshape(*shape, 3) if relative else pos, mesh_shape=shape,
+                          paint_absolute_pos=not relative).reshape(-1, 3)
+    assert rel_err(v.cpu().numpy() - 0.25, 0.5 * ref_f) < 3e-5
+    if relative:
+        return  # the oracle adjoint restates the absolute painter
+    u = r.normal(size=(n, 3)).astype(np.float32)
+    g = torch.zeros_like(xt)
+    st.vjp_field(phi, xt, torch.as_tensor(u, device="cuda"), 0.7, g)
+    ref = 0.7 * oa.force_vjp(pos, u, shape)
+    assert rel_err(g.cpu().numpy(), ref) < 5e-5
+    # the same evaluation keeping its tile sort for the adjoint
+    keep = st.new_field(keep_sort=True)
+    st.field_kick(xt, torch.zeros_like(xt), 0.5, keep)
+    g2 = torch.zeros_like(xt)
+    st.vjp_field(keep, xt, torch.as_tensor(u, device="cuda"), 0.7, g2)
+    assert rel_err(g2.cpu().numpy(), ref) < 5e-5
+
+
+def test_library_kernel_timing(gpu):
+    from jaxpm_amd import _lib, _ops
+    _lib.timing_report()
+    _lib.timing(True)
+    m = torch.zeros((32, 32, 32), device="cuda")
+    _ops.paint(m, torch.rand((4096, 3), device="cuda") * 32)
+    torch.cuda.synchronize()
+    _lib.timing(False)
+    rep = _lib.timing_report()
+    assert rep["bin_count_kernel"][0] == 1 and rep["bin_accum_kernel[nc=1]"][1] > 0
+    assert _lib.timing_report() == {}
