@@ -27,6 +27,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch from hardware counters (FETCH_SIZE and WRITE_SIZE
+# passes of `python -m jaxpm_amd.traffic_probe 256`, each scaled by the same
+# counter's reading for a plain copy of a known 64 MiB: x1.999 for reads,
+# x1.000 for writes), by kernel and mesh side.  The commands and the counter
+# values are in profiles/bin_kick_kernel_256_traffic.md.
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel", 256): 1151.5e6}
 
 
 def parse():
@@ -199,7 +205,8 @@ def main():
                                f"2LPT IC at a=0.1, {args.steps} steps to a=1, checkpoint every {args.checkpoint}",
                    "mesh": n, "particles": N, "checkpoint_every": args.checkpoint},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": PMC_TRAFFIC_BYTES.get((dom, n)),
                      "algorithmic_bytes": int(algo[dom]), "ms_per_launch": round(t_dom, 4)},
         "kernels": kinfo,
         "cpu_baseline": cpu,

@@ -1,0 +1,53 @@
+"""Launch a fixed set of kernels for hardware-counter passes.
+
+    python -m jaxpm_amd.traffic_probe [mesh]
+
+Runs, on a 256^3 (default) workload: a plain device copy of a known size
+(the calibration launch), then one tile-sorted force evaluation with kick
+(`jpm_force_kick_f32`) at the clustered state the benchmark ends in.  Meant
+to be run under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`; the
+per-dispatch counters of `bin_kick_kernel` divided by those of the copy give
+its HBM traffic in units of the copy's known byte count.
+"""
+import sys
+
+import numpy as np
+import torch
+
+
+def main():
+    from jaxpm_amd import _lib, integrate, pm
+    from jaxpm_amd import cosmology as pc
+    n = int(sys.argv[1]) if len(sys.argv) > 1 else 256
+    dev = _lib.device()
+    shape = (n, n, n)
+    N = n ** 3
+    # calibration: copy of 4N bytes (read 4N, write 4N)
+    a = torch.randn(N, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    cosmo = pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    k, pk = pc.pk_table(cosmo)
+    gen = torch.Generator(device=dev).manual_seed(2)
+    with torch.no_grad():
+        ic = pm.linear_field(shape, (4.0 * n,) * 3, pm.TabulatedPk(k, pk), torch.randn(shape, generator=gen, device=dev))
+        lattice = torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
+                                             indexing="ij"), -1)
+        dx, p, _ = pm.lpt(cosmo, ic, lattice, a=0.1, order=2)
+        x = (lattice + dx).reshape(-1, 3).contiguous()
+        v = p.reshape(-1, 3).contiguous()
+        st = integrate.Stepper(shape, N)
+        steps = np.linspace(0.1, 1.0, 11)
+        k0, beta, gamma = integrate.kdk_schedule(cosmo, steps)
+        field = st.new_field()
+        st.field_kick(x, v, k0, field)
+        for i in range(len(beta)):
+            st.axpy(beta[i], v, x)
+            st.field_kick(x, v, gamma[i], field)
+    torch.cuda.synchronize()
+    print(f"traffic probe done: copy of {4 * N} bytes, {len(beta) + 1} force evaluations at {n}^3", flush=True)
+
+
+if __name__ == "__main__":
+    main()
