@@ -204,6 +204,22 @@ int jpm_kspace_linear_field_f32(void *stream, int nx, int ny, int nz,
                                 const float *k_tab, const float *pk_tab,
                                 int ntab, float scale);
 
+/* Workspace bytes for jpm_kspace_linear_field_table_vjp_f32. */
+size_t jpm_linear_field_table_vjp_workspace_size(int ntab);
+
+/* Cotangent of pk_tab for jpm_kspace_linear_field_f32 followed by C2R: given
+ * the noise half spectrum noise_k and the half spectrum cot_k of the real
+ * output's cotangent (both unnormalised R2C), out[ntab] (device, float64) =
+ * sum over the full spectrum of Re(W conj G) dc/dpk_tab, c the forward
+ * multiplier.  The adjoint of pm.py:135-143 with respect to the pk table
+ * (jax.grad of linear_field through jnp.interp).  Deterministic. */
+int jpm_kspace_linear_field_table_vjp_f32(void *stream, int nx, int ny, int nz,
+                                          const void *noise_k, const void *cot_k,
+                                          float Lx, float Ly, float Lz,
+                                          const float *k_tab, const float *pk_tab,
+                                          int ntab, float scale, double *out,
+                                          void *ws, size_t ws_bytes);
+
 /* Six half spectra scale * g_i g_j G(k) delta_k for (i,j) in
  * (0,0),(1,1),(2,2),(0,1),(0,2),(1,2): the 2LPT shear terms of pm.py:89-109. */
 int jpm_kspace_shear6_f32(void *stream, int nx, int ny, int nz,
@@ -245,6 +261,32 @@ int jpm_spectrum_full_to_half_f32(void *stream, int nx, int ny, int nz,
  * distributed.py:37-38 for real input). */
 int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz,
                                   const void *half, void *full);
+
+/* ------------------------------------------------------------ analysis */
+
+/* Workspace bytes for jpm_power_spectrum_f32 (per-workgroup partial sums). */
+size_t jpm_power_spectrum_workspace_size(int nx, int ny, int nz, int nedges,
+                                         int npoles, int cross);
+
+/* Mode sums of the binned power spectrum, utils.py:76-128 (power_spectrum,
+ * bins of _initialize_pk utils.py:14-73), over the R2C half spectra spec1 and
+ * (cross spectrum, nullable) spec2 of real [nx][ny][nz] meshes, counting the
+ * Hermitian mirror of every mode the half spectrum does not store.
+ * |k| = float32 sqrt of the float32-rounded float64 sum of (fac_d *
+ * fftfreq)^2, fac_d = 2 pi n_d / L_d; bin = np.digitize(|k|, edges) over the
+ * nedges increasing float64 device edges (nedges + 1 bins).  out (device,
+ * float64) is [slot][bin]: slot 0 mode count, 1 sum of |k|, 2+q the real
+ * part of sum (2 l_q + 1) P_l_q(mu) X1 conj(X2) for pole poles[q] (host
+ * array, npoles <= 8), and for cross spectra 2+npoles+q its imaginary part;
+ * mu = k.los / |k| (0 at k = 0) when any pole is nonzero.  The FFTs are
+ * unnormalised: divide the power sums by nx*ny*nz for norm='ortho'.
+ * Deterministic. */
+int jpm_power_spectrum_f32(void *stream, int nx, int ny, int nz,
+                           const void *spec1, const void *spec2, double fac_x,
+                           double fac_y, double fac_z, const double *edges,
+                           int nedges, const int *poles, int npoles,
+                           double los_x, double los_y, double los_z,
+                           double *out, void *ws, size_t ws_bytes);
 
 /* ------------------------------------------------------------- pipeline */
 
@@ -325,6 +367,15 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz,
                           float r_split, uint32_t flags, void *ws,
                           size_t ws_bytes, const void *sorted,
                           size_t sorted_bytes);
+
+/* Workspace bytes for jpm_dot_f32. */
+size_t jpm_dot_workspace_size(void);
+
+/* *out (device float64) = sum_i x[i] y[i], accumulated in float64 in a fixed
+ * order: the scalar cotangents of the leapfrog coefficients and growth factors
+ * (what jax.grad forms for the scalars of ode.py:139-143 and pm.py:84-124). */
+int jpm_dot_f32(void *stream, int64_t n, const float *x, const float *y,
+                double *out, void *ws, size_t ws_bytes);
 
 /* y += a * x over n floats (16-byte aligned): the drift x += h v / (a^3 E)
  * of the leapfrog built on make_ode_fn (ode.py:139-140) and its adjoint. */

@@ -28,6 +28,7 @@ _i64 = ctypes.c_int64
 _u32 = ctypes.c_uint32
 _f = ctypes.c_float
 _sz = ctypes.c_size_t
+_d = ctypes.c_double
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -59,6 +60,14 @@ SIGNATURES = {
     "jpm_lpt2_source_vjp_f32": (_i, [_vp, _i64, _vp, _vp, _vp]),
     "jpm_spectrum_full_to_half_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
     "jpm_spectrum_half_to_full_f32": (_i, [_vp, _i, _i, _i, _vp, _vp]),
+    "jpm_linear_field_table_vjp_workspace_size": (_sz, [_i]),
+    "jpm_kspace_linear_field_table_vjp_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _f, _vp, _vp, _i, _f, _vp, _vp,
+                                                   _sz]),
+    "jpm_dot_workspace_size": (_sz, []),
+    "jpm_dot_f32": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_power_spectrum_workspace_size": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "jpm_power_spectrum_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _d, _d, _d, _vp, _i, _vp, _i, _d, _d, _d, _vp, _vp,
+                                    _sz]),
     "jpm_force_field_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_field_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
     "jpm_force_vjp_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),

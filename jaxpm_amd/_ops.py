@@ -322,3 +322,34 @@ def pm_forces_fused(pos, shape, delta_k=None, r_split=0.0, scale=1.0, relative=F
     call("jpm_pm_forces_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(delta_k), ptr(out), float(r_split),
          float(scale), flags, ptr(ws), ws.numel())
     return out
+
+
+# ---------------------------------------------------------------- analysis
+
+def power_spectrum_sums(spec1, spec2, shape, fac, edges, poles, los):
+    """Per-bin mode sums over half spectra (jpm_power_spectrum_f32).
+
+    Returns a float64 device tensor [slot][bin]: count, sum |k|, Re sums per
+    pole and, for a cross spectrum, Im sums per pole."""
+    import ctypes
+    spec1 = _c64(spec1, "spectrum")
+    nx, ny, nz = _dims(shape)
+    if tuple(spec1.shape) != half_shape((nx, ny, nz)):
+        raise ValueError(f"spectrum shape {tuple(spec1.shape)} does not match mesh {tuple(shape)}")
+    if spec2 is not None:
+        spec2 = _c64(spec2, "spectrum2")
+        if spec2.shape != spec1.shape:
+            raise ValueError("the two spectra differ in shape")
+    edges = torch.as_tensor(edges, dtype=torch.float64).to(spec1.device).contiguous()
+    ne = int(edges.numel())
+    npoles = len(poles)
+    cross = int(spec2 is not None)
+    slots = 2 + npoles * (2 if cross else 1)
+    out = torch.empty((slots, ne + 1), dtype=torch.float64, device=spec1.device)
+    ws = torch.empty(size("jpm_power_spectrum_workspace_size", nx, ny, nz, ne, npoles, cross),
+                     dtype=torch.uint8, device=spec1.device)
+    parr = (ctypes.c_int * max(npoles, 1))(*[int(p) for p in poles])
+    call("jpm_power_spectrum_f32", stream(), nx, ny, nz, ptr(spec1), ptr(spec2) if cross else None,
+         float(fac[0]), float(fac[1]), float(fac[2]), ptr(edges) if ne else None, ne, parr, npoles,
+         float(los[0]), float(los[1]), float(los[2]), ptr(out), ptr(ws), ws.numel())
+    return out

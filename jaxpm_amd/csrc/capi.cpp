@@ -69,6 +69,14 @@ int spectrum_full_to_half(hipStream_t, int, int, int, const void *, void *);
 int kspace_shear6_adjoint(hipStream_t, int, int, int, const void *, void *, float, uint32_t);
 int lpt2_source_vjp(hipStream_t, int64_t, const float *, const float *, float *);
 int spectrum_half_to_full(hipStream_t, int, int, int, const void *, void *);
+size_t power_spectrum_ws(int, int, int, int, int, int);
+size_t linear_field_table_vjp_ws(int);
+int kspace_linear_field_table_vjp(hipStream_t, int, int, int, const void *, const void *, float, float, float,
+                                  const float *, const float *, int, float, double *, void *, size_t);
+size_t dot_ws();
+int dot(hipStream_t, int64_t, const float *, const float *, double *, void *, size_t);
+int power_spectrum(hipStream_t, int, int, int, const void *, const void *, double, double, double, const double *,
+                   int, const int *, int, double, double, double, double *, void *, size_t);
 
 int axpy(hipStream_t, int64_t, float, const float *, float *);
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
@@ -564,6 +572,32 @@ int jpm_spectrum_full_to_half_f32(void *stream, int nx, int ny, int nz, const vo
 
 int jpm_spectrum_half_to_full_f32(void *stream, int nx, int ny, int nz, const void *half, void *full) {
   JPM_GUARD(spectrum_half_to_full(as_stream(stream), nx, ny, nz, half, full));
+}
+
+size_t jpm_linear_field_table_vjp_workspace_size(int ntab) { return linear_field_table_vjp_ws(ntab); }
+
+int jpm_kspace_linear_field_table_vjp_f32(void *stream, int nx, int ny, int nz, const void *noise_k, const void *cot_k,
+                                          float Lx, float Ly, float Lz, const float *k_tab, const float *pk_tab,
+                                          int ntab, float scale, double *out, void *ws, size_t ws_bytes) {
+  JPM_GUARD(kspace_linear_field_table_vjp(as_stream(stream), nx, ny, nz, noise_k, cot_k, Lx, Ly, Lz, k_tab, pk_tab,
+                                          ntab, scale, out, ws, ws_bytes));
+}
+
+size_t jpm_dot_workspace_size(void) { return dot_ws(); }
+
+int jpm_dot_f32(void *stream, int64_t n, const float *x, const float *y, double *out, void *ws, size_t ws_bytes) {
+  JPM_GUARD(dot(as_stream(stream), n, x, y, out, ws, ws_bytes));
+}
+
+size_t jpm_power_spectrum_workspace_size(int nx, int ny, int nz, int nedges, int npoles, int cross) {
+  return power_spectrum_ws(nx, ny, nz, nedges, npoles, cross);
+}
+
+int jpm_power_spectrum_f32(void *stream, int nx, int ny, int nz, const void *spec1, const void *spec2, double fac_x,
+                           double fac_y, double fac_z, const double *edges, int nedges, const int *poles, int npoles,
+                           double los_x, double los_y, double los_z, double *out, void *ws, size_t ws_bytes) {
+  JPM_GUARD(power_spectrum(as_stream(stream), nx, ny, nz, spec1, spec2, fac_x, fac_y, fac_z, edges, nedges, poles,
+                           npoles, los_x, los_y, los_z, out, ws, ws_bytes));
 }
 
 size_t jpm_force_field_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {

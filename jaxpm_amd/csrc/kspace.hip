@@ -219,6 +219,94 @@ linear_field_kernel(float2 *__restrict__ fk, HalfGrid hg, Tabs t, float3 box, fl
   }
 }
 
+// Cotangent of the linear field's power-spectrum table.  With c(k) =
+// sqrt(pk(|k|) M / V) * scale the forward output is C2R(W c) for the noise
+// spectrum W, so for a real-space cotangent with spectrum G,
+//   dL/dpk(k) = mult(k) Re(W conj G) c / (2 pk)   (mult 2 for the modes whose
+// Hermitian mirror is not stored), and pk(k) = (1 - t) pktab[i-1] + t pktab[i]
+// (jnp.interp; end values outside the table).  Lanes of one table interval
+// are summed across the wave, then added to per-workgroup float64 sums in LDS.
+__global__ void __launch_bounds__(256)
+linear_field_table_vjp_kernel(const float2 *__restrict__ wk, const float2 *__restrict__ gk, HalfGrid hg, Tabs t,
+                              float3 box, float3 nm, float mcells, float vol, const float *__restrict__ ktab,
+                              const float *__restrict__ pktab, int ntab, float scale, int nz,
+                              double *__restrict__ partial) {
+  extern __shared__ double tab_acc[];
+  for (int e = threadIdx.x; e < ntab; e += blockDim.x) tab_acc[e] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // every lane of a wave runs the same number of iterations (the bound is
+  // rounded up to whole grid strides) so the cross-lane sums see full waves
+  const uint32_t iters = (hg.H + stride - 1) / stride;
+  for (uint32_t it = 0; it < iters; ++it) {
+    const uint32_t e = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = e < hg.H;
+    int seg = 1;
+    double lo = 0.0, hi = 0.0;
+    if (valid) {
+      uint32_t i, j, k;
+      hg.coords(e, i, j, k);
+      float ax = t.k[i] / box.x * nm.x, ay = t.k[t.oy + j] / box.y * nm.y, az = t.k[t.oz + k] / box.z * nm.z;
+      float kmesh = sqrtf((ax * ax + ay * ay) + az * az);
+      float pk = interp(kmesh, ktab, pktab, ntab);
+      if (pk > 0.0f) {
+        float c = sqrtf(pk * mcells / vol) * scale;
+        const float2 w = wk[e], g = gk[e];
+        const double mult = (k > 0 && 2 * int(k) != nz) ? 2.0 : 1.0;
+        const double val = mult * (double(w.x) * double(g.x) + double(w.y) * double(g.y)) * double(c) /
+                           (2.0 * double(pk));
+        int lo_i = 0, hi_i = ntab;
+        while (lo_i < hi_i) {
+          int mid = (lo_i + hi_i) >> 1;
+          if (ktab[mid] <= kmesh) lo_i = mid + 1; else hi_i = mid;
+        }
+        seg = lo_i < 1 ? 1 : (lo_i > ntab - 1 ? ntab - 1 : lo_i);
+        const float dx = ktab[seg] - ktab[seg - 1];
+        if (kmesh < ktab[0]) {
+          lo = val;                     // f = fp[0]
+        } else if (kmesh > ktab[ntab - 1]) {
+          hi = val;                     // f = fp[n-1]
+        } else if (fabsf(dx) <= 0.0f) {
+          lo = val;
+        } else {
+          const double tt = double((kmesh - ktab[seg - 1]) / dx);
+          lo = val * (1.0 - tt);
+          hi = val * tt;
+        }
+      }
+    }
+    uint64_t todo = __ballot(valid);
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int ls = __shfl(seg, leader);
+      const bool mine = valid && seg == ls;
+      todo &= ~uint64_t(__ballot(mine));
+      double a = mine ? lo : 0.0, b = mine ? hi : 0.0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+      }
+      if (lane == leader) {
+        atomicAdd(&tab_acc[ls - 1], a);
+        atomicAdd(&tab_acc[ls], b);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ntab; e += blockDim.x) partial[size_t(blockIdx.x) * ntab + e] = tab_acc[e];
+}
+
+__global__ void table_reduce_kernel(const double *__restrict__ partial, int blocks, int ntab,
+                                    double *__restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ntab) return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) s += partial[size_t(b) * ntab + e];
+  out[e] = s;
+}
+
 __global__ void __launch_bounds__(256)
 shear6_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid hg, Tabs t, float scale) {
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
@@ -446,6 +534,46 @@ int kspace_linear_field(hipStream_t s, int nx, int ny, int nz, void *fk, float L
                        static_cast<float2 *>(fk), hg, t, box, nm, mcells, vol, ktab, pktab, ntab, scale);
   }
   JPM_LAUNCHED("linear_field_kernel");
+  return JPM_OK;
+}
+
+constexpr int kTableVjpBlocks = 256;
+constexpr int kTableVjpMax = 8192;   // float64 LDS sums per workgroup (64 KiB)
+
+size_t linear_field_table_vjp_ws(int ntab) {
+  return ntab < 2 ? 0 : size_t(kTableVjpBlocks) * size_t(ntab) * sizeof(double);
+}
+
+int kspace_linear_field_table_vjp(hipStream_t s, int nx, int ny, int nz, const void *wk, const void *gk, float Lx,
+                                  float Ly, float Lz, const float *ktab, const float *pktab, int ntab, float scale,
+                                  double *out, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(wk && gk && ktab && pktab && out, "null pointer argument");
+  JPM_REQUIRE(ntab >= 2 && ntab <= kTableVjpMax, "power-spectrum table of %d entries (2..%d)", ntab,
+              kTableVjpMax);
+  JPM_REQUIRE(Lx > 0 && Ly > 0 && Lz > 0, "box size must be positive");
+  JPM_REQUIRE(ws && ws_bytes >= linear_field_table_vjp_ws(ntab), "workspace of %zu bytes, need %zu", ws_bytes,
+              linear_field_table_vjp_ws(ntab));
+  Tabs t;
+  rc = get_tabs(s, nx, ny, nz, 0, t);
+  if (rc) return rc;
+  HalfGrid hg = make_half(nx, ny, nz);
+  const float3 box = make_float3(Lx, Ly, Lz);
+  const float3 nm = make_float3(float(nx), float(ny), float(nz));
+  const float mcells = float(cells(nx, ny, nz));
+  const float vol = float(double(Lx) * double(Ly) * double(Lz));
+  double *partial = static_cast<double *>(ws);
+  {
+    JPM_TIMED(s, "linear_field_table_vjp_kernel");
+    hipLaunchKernelGGL(linear_field_table_vjp_kernel, dim3(kTableVjpBlocks), dim3(256), size_t(ntab) * sizeof(double),
+                       s, static_cast<const float2 *>(wk), static_cast<const float2 *>(gk), hg, t, box, nm, mcells,
+                       vol, ktab, pktab, ntab, scale, nz, partial);
+    JPM_LAUNCHED("linear_field_table_vjp_kernel");
+  }
+  hipLaunchKernelGGL(table_reduce_kernel, dim3((ntab + 255) / 256), dim3(256), 0, s, partial, kTableVjpBlocks, ntab,
+                     out);
+  JPM_LAUNCHED("table_reduce_kernel");
   return JPM_OK;
 }
 

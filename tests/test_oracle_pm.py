@@ -128,36 +128,106 @@ def test_kdk_time_reversible():
     np.testing.assert_allclose(v2, v0, atol=1e-9)
 
 
-def brute_pk(mesh, box):
+def brute_pk(mesh, box, mesh2=None, poles=(0,), los=(0., 0., 1.), dk=None):
+    """Direct DFT by its definition, one mode at a time, binned by a scan
+    over the edges; Legendre polynomials in closed form."""
     n = np.array(mesh.shape)
-    L = np.array(box)
-    fk = np.fft.fftn(mesh, norm='ortho')
-    dk = 4 * np.pi / L.min()
+    L = np.array(box, np.float64)
+    grid = np.stack(np.meshgrid(*[np.arange(s) for s in n], indexing='ij'), -1).reshape(-1, 3)
+    m1 = mesh.reshape(-1)
+    m2 = m1 if mesh2 is None else mesh2.reshape(-1)
+    dk = 4 * np.pi / L.min() if dk is None else dk
     kmax = np.pi * (n / L).min()
     edges = np.arange(dk, kmax, dk) + dk / 2
-    sums = np.zeros(len(edges) + 1)
+    leg = {0: lambda u: 1.0, 1: lambda u: u, 2: lambda u: (3 * u * u - 1) / 2,
+           4: lambda u: (35 * u ** 4 - 30 * u * u + 3) / 8}
+    re = np.zeros((len(poles), len(edges) + 1))
+    im = np.zeros_like(re)
     cnt = np.zeros(len(edges) + 1)
     ksum = np.zeros(len(edges) + 1)
-    for i in range(n[0]):
-        for j in range(n[1]):
-            for l in range(n[2]):
-                kv = [2 * np.pi * np.fft.fftfreq(n[d])[idx] * n[d] / L[d] for d, idx in enumerate((i, j, l))]
-                kk = np.sqrt(sum(q * q for q in kv))
-                b = np.searchsorted(edges, kk, side='right')
-                sums[b] += abs(fk[i, j, l]) ** 2
-                cnt[b] += 1
-                ksum[b] += kk
-    P = (sums / cnt)[1:-1] * np.prod(L / n)
-    return (ksum / cnt)[1:-1], P
+    for idx in np.ndindex(*n):
+        ph = np.exp(-2j * np.pi * (grid * (np.array(idx) / n)).sum(-1))
+        f1 = (m1 * ph).sum() / np.sqrt(n.prod())
+        f2 = (m2 * ph).sum() / np.sqrt(n.prod())
+        kv = [2 * np.pi * (i if i <= (s - 1) // 2 else i - s) / l for i, s, l in zip(idx, n, L)]
+        kk = float(np.sqrt(np.float32(sum(q * q for q in kv))))
+        b = 0
+        while b < len(edges) and edges[b] <= kk:
+            b += 1
+        mu = 0.0 if kk == 0 else sum(q * l for q, l in zip(kv, los)) / kk
+        cnt[b] += 1
+        ksum[b] += kk
+        for i, ell in enumerate(poles):
+            w = f1 * np.conj(f2) * (2 * ell + 1) * leg[ell](mu)
+            re[i, b] += w.real
+            im[i, b] += w.imag
+    ps = re if mesh2 is None else np.hypot(re, im)
+    return (ksum / cnt)[1:-1], (ps / cnt)[:, 1:-1] * np.prod(L / n)
 
 
-def test_power_spectrum_brute_force():
-    mesh = np.random.default_rng(2).normal(size=(8, 8, 16))
-    box = (32., 32., 64.)
-    k0, p0 = ou.power_spectrum(mesh, box_shape=box)
-    k1, p1 = brute_pk(mesh, box)
+@pytest.mark.parametrize("shape,box,dk", [((8, 8, 16), (32., 32., 64.), 0.15), ((6, 5, 7), (30., 25., 35.), 0.12),
+                                          ((12, 12, 12), (12., 12., 12.), None)])
+def test_power_spectrum_brute_force(shape, box, dk):
+    """The default edges of a 12^3 unit-cell box sit at 3 and 5 times the
+    fundamental, exactly on the |k| of modes (3,0,0), (3,4,0), (5,0,0)."""
+    r = np.random.default_rng(2)
+    mesh = r.normal(size=shape)
+    k0, p0 = ou.power_spectrum(mesh, box_shape=box, kedges=dk)
+    k1, p1 = brute_pk(mesh, box, dk=dk)
+    assert len(k0) >= 1
     np.testing.assert_allclose(k0, k1, rtol=1e-12)
-    np.testing.assert_allclose(p0, p1, rtol=1e-12)
+    np.testing.assert_allclose(p0, p1[0], rtol=1e-10)
+    # cross spectra and multipoles, the line of sight normalised inside
+    mesh2 = r.normal(size=shape)
+    los = np.array([1., 2., 2.])
+    k0, p0 = ou.power_spectrum(mesh, mesh2, box_shape=box, kedges=dk, multipoles=[0, 1, 2, 4], los=los)
+    _, p1 = brute_pk(mesh, box, mesh2, (0, 1, 2, 4), los / 3.0, dk=dk)
+    np.testing.assert_allclose(p0, p1, rtol=1e-10, atol=1e-12 * np.abs(p1).max())
+    _, p0 = ou.power_spectrum(mesh, box_shape=box, kedges=dk, multipoles=[0, 2], los=los)
+    _, p1 = brute_pk(mesh, box, None, (0, 2), los / 3.0, dk=dk)
+    np.testing.assert_allclose(p0, p1, rtol=1e-10, atol=1e-12 * np.abs(p1).max())
+
+
+def test_power_spectrum_multipoles_of_plane_waves():
+    """A wave along the line of sight has mu = +-1 (P_ell = 1), one across
+    it mu = 0 (P_2 = -1/2, P_4 = 3/8); mu divides by the float32 |k|, hence
+    the 1e-6."""
+    n, L = 16, 64.
+    x = np.arange(n)
+    wave = np.cos(2 * np.pi * 2 * x / n)
+    along = np.broadcast_to(wave[None, None, :], (n, n, n))
+    across = np.broadcast_to(wave[:, None, None], (n, n, n))
+    _, pa = ou.power_spectrum(along, box_shape=(L,) * 3, multipoles=[0, 2, 4])
+    _, pc = ou.power_spectrum(across, box_shape=(L,) * 3, multipoles=[0, 2, 4])
+    b = np.argmax(pa[0])
+    np.testing.assert_allclose(pa[:, b] / pa[0, b], [1, 5, 9], rtol=1e-6)
+    np.testing.assert_allclose(pc[:, b] / pc[0, b], [1, -2.5, 9 * 3 / 8], rtol=1e-6)
+
+
+def test_power_spectrum_cross_identities():
+    r = np.random.default_rng(3)
+    m = r.normal(size=(8, 8, 8))
+    box = (40., 40., 40.)
+    _, pa = ou.power_spectrum(m, box_shape=box)
+    _, pc = ou.power_spectrum(m, m, box_shape=box)
+    _, pn = ou.power_spectrum(m, -m, box_shape=box)     # |sum| of the complex bins
+    np.testing.assert_allclose(pc, pa, rtol=1e-12)
+    np.testing.assert_allclose(pn, pa, rtol=1e-12)
+    _, t = ou.transfer(m, 3 * m, box)
+    np.testing.assert_allclose(t, 3.0, rtol=1e-12)
+    _, c = ou.coherence(m, m, box)
+    np.testing.assert_allclose(c, 1.0, rtol=1e-12)
+
+
+def test_power_spectrum_kedges_forms():
+    m = np.random.default_rng(4).normal(size=(16, 16, 16))
+    k, p = ou.power_spectrum(m, kedges=5)          # dk = kmax / 6
+    assert len(k) == len(p) == len(np.arange(np.pi / 6, np.pi, np.pi / 6)) - 1
+    k, p = ou.power_spectrum(m, kedges=0.3)
+    assert len(k) == len(np.arange(0.3, np.pi, 0.3)) - 1
+    edges = [0.2, 0.5, 1.0, 2.0]
+    k, p = ou.power_spectrum(m, kedges=edges)
+    assert len(k) == 3 and np.all((k > 0.2) & (k < 2.0))
 
 
 def test_power_spectrum_plane_wave():
