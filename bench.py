@@ -115,11 +115,11 @@ def main():
 
     def fwd_adj(S, trace=None):
         steps = np.linspace(0.1, 1.0, S + 1)
-        sched = integrate.kdk_schedule(cosmo, steps)
+        sched = integrate.schedule_tensor(integrate.kdk_schedule(cosmo, steps))
         stepper.trace = trace
         xt = x0.clone().requires_grad_(True)
         vt = v0.clone().requires_grad_(True)
-        x, v = integrate.NBody.apply(xt, vt, stepper, sched, args.checkpoint)
+        x, v = integrate.NBody.apply(xt, vt, sched, stepper, args.checkpoint)
         rho = painting.cic_paint(torch.zeros(shape, device=dev), x)
         loss = ((rho - target) ** 2).sum() / sigma ** 2
         loss.backward()

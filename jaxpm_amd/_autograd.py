@@ -111,23 +111,71 @@ class Lpt2Source(torch.autograd.Function):
 
 class LinearField(torch.autograd.Function):
     """noise -> Re IFFT[FFT(noise) sqrt(pk(|k|) M / V)] (pm.py:129-144) with a
-    tabulated pk.  The multiplier is real and even, so the map is symmetric."""
+    tabulated pk.  The multiplier is real and even, so the map is symmetric in
+    the noise; the cotangent of the pk table comes from
+    jpm_kspace_linear_field_table_vjp_f32."""
 
     @staticmethod
     def forward(ctx, noise, box, ktab, pktab):
         shape = tuple(noise.shape)
-        fk = _ops.kspace_linear_field(_ops.r2c(noise.contiguous()), shape, box, ktab, pktab,
-                                      1.0 / noise.numel())
-        ctx.save_for_backward(ktab, pktab)
+        noise = noise.contiguous()
+        fk = _ops.kspace_linear_field(_ops.r2c(noise), shape, box, ktab, pktab, 1.0 / noise.numel())
+        ctx.save_for_backward(noise, ktab, pktab)
         ctx.box, ctx.shape = box, shape
         return _ops.c2r(fk, shape, clobber=True)
 
     @staticmethod
     def backward(ctx, g):
-        ktab, pktab = ctx.saved_tensors
-        fk = _ops.kspace_linear_field(_ops.r2c(g.contiguous()), ctx.shape, ctx.box, ktab, pktab,
-                                      1.0 / g.numel())
-        return _ops.c2r(fk, ctx.shape, clobber=True), None, None, None
+        noise, ktab, pktab = ctx.saved_tensors
+        gk = _ops.r2c(g.contiguous())
+        gn = gt = None
+        if ctx.needs_input_grad[3]:
+            gt = _ops.linear_field_table_vjp(_ops.r2c(noise), gk, ctx.shape, ctx.box, ktab, pktab,
+                                             1.0 / g.numel()).to(pktab.dtype)
+        if ctx.needs_input_grad[0]:
+            fk = _ops.kspace_linear_field(gk, ctx.shape, ctx.box, ktab, pktab, 1.0 / g.numel())
+            gn = _ops.c2r(fk, ctx.shape, clobber=True)
+        return gn, None, None, gt
+
+
+class LinComb(torch.autograd.Function):
+    """y = sum_j c[j] x_j for float32 device tensors x_j of one shape and a
+    host float64 coefficient vector c: the growth-factor products of lpt
+    (pm.py:84-124).  Each term is added by jpm_axpy_f32 in order, so
+    y = fl(fl(c0 x0) + fl(c1 x1)) as the reference's ``dx + dx2``; the
+    cotangent of c[j] is <g, x_j> (jpm_dot_f32, float64)."""
+
+    @staticmethod
+    def forward(ctx, c, *xs):
+        cv = [float(v) for v in c.detach().to(torch.float64).reshape(-1).tolist()]
+        if len(cv) != len(xs):
+            raise ValueError(f"{len(cv)} coefficients for {len(xs)} terms")
+        xs = [x.contiguous() for x in xs]
+        y = torch.zeros_like(xs[0])
+        for cj, xj in zip(cv, xs):
+            _ops.axpy(cj, xj, y)
+        ctx.cv = cv
+        ctx.c_meta = (c.dtype, c.device)
+        ctx.save_for_backward(*xs)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        xs = ctx.saved_tensors
+        g = g.contiguous()
+        gc = None
+        if ctx.needs_input_grad[0]:
+            d = torch.empty(len(xs), dtype=torch.float64, device=g.device)
+            for j, xj in enumerate(xs):
+                _ops.dot_into(d, j, g, xj)
+            gc = d.to(device=ctx.c_meta[1], dtype=ctx.c_meta[0])
+        gxs = []
+        for j, cj in enumerate(ctx.cv):
+            if ctx.needs_input_grad[1 + j]:
+                gxs.append(_ops.axpy(cj, g, torch.zeros_like(g)))
+            else:
+                gxs.append(None)
+        return (gc, *gxs)
 
 
 class Compensate(torch.autograd.Function):

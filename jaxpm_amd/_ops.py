@@ -4,6 +4,8 @@ Every wrapper takes contiguous float32 / complex64 device tensors, allocates
 its outputs with torch on the same device and launches on torch's current
 stream.  Shapes are validated here and again by the library.
 """
+import ctypes
+
 import torch
 
 from . import _lib
@@ -331,7 +333,6 @@ def power_spectrum_sums(spec1, spec2, shape, fac, edges, poles, los):
 
     Returns a float64 device tensor [slot][bin]: count, sum |k|, Re sums per
     pole and, for a cross spectrum, Im sums per pole."""
-    import ctypes
     spec1 = _c64(spec1, "spectrum")
     nx, ny, nz = _dims(shape)
     if tuple(spec1.shape) != half_shape((nx, ny, nz)):
@@ -353,3 +354,50 @@ def power_spectrum_sums(spec1, spec2, shape, fac, edges, poles, los):
          float(fac[0]), float(fac[1]), float(fac[2]), ptr(edges) if ne else None, ne, parr, npoles,
          float(los[0]), float(los[1]), float(los[2]), ptr(out), ptr(ws), ws.numel())
     return out
+
+
+def linear_field_table_vjp(noise_k, cot_k, shape, box, ktab, pktab, scale=1.0):
+    """float64 (ntab,) cotangent of the pk table of kspace_linear_field + C2R
+    (jpm_kspace_linear_field_table_vjp_f32)."""
+    noise_k = _c64(noise_k, "noise spectrum")
+    cot_k = _c64(cot_k, "cotangent spectrum")
+    nx, ny, nz = _dims(shape)
+    ktab = _f32(ktab, "k table")
+    pktab = _f32(pktab, "pk table")
+    nt = ktab.numel()
+    if pktab.numel() != nt:
+        raise ValueError("k and P(k) tables differ in length")
+    out = torch.empty(nt, dtype=torch.float64, device=noise_k.device)
+    wsb = size("jpm_linear_field_table_vjp_workspace_size", nt)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=noise_k.device)
+    call("jpm_kspace_linear_field_table_vjp_f32", stream(), nx, ny, nz, ptr(noise_k), ptr(cot_k), float(box[0]),
+         float(box[1]), float(box[2]), ptr(ktab), ptr(pktab), nt, float(scale), ptr(out), ptr(ws), wsb)
+    return out
+
+
+_dot_ws = {}
+
+
+def dot_into(out, i, x, y):
+    """out[i] (a float64 device tensor) = <x, y> over float32 tensors of one
+    size, accumulated in float64 (jpm_dot_f32)."""
+    x = _f32(x, "x")
+    y = _f32(y, "y")
+    if x.numel() != y.numel():
+        raise ValueError(f"dot of {x.numel()} and {y.numel()} elements")
+    if out.dtype != torch.float64 or not out.is_contiguous():
+        raise TypeError("dot output must be a contiguous float64 tensor")
+    ws = _dot_ws.get(x.device)
+    if ws is None:
+        ws = _dot_ws[x.device] = torch.empty(size("jpm_dot_workspace_size"), dtype=torch.uint8, device=x.device)
+    call("jpm_dot_f32", stream(), x.numel(), ptr(x), ptr(y), ctypes.c_void_p(out.data_ptr() + 8 * int(i)), ptr(ws),
+         ws.numel())
+
+
+def axpy(a, x, y):
+    """y += a x in place (jpm_axpy_f32); float32 tensors of one size."""
+    x = _f32(x, "x")
+    if not y.is_contiguous() or y.dtype != torch.float32 or y.numel() != x.numel():
+        raise ValueError("axpy target must be a contiguous float32 tensor of the same size")
+    call("jpm_axpy_f32", stream(), x.numel(), float(a), ptr(x), ptr(y))
+    return y

@@ -19,7 +19,7 @@ the segment's positions and force meshes, and walks it backwards with
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, _ops
 from . import growth as gr
 from ._convert import to_device
 from ._lib import call, ptr, size, stream
@@ -99,6 +99,19 @@ class Stepper:
             self.field(x, buf)
             self.kick(buf, x, v, gamma)
 
+    def forces(self, x, out, field):
+        """out = F(x) (the force field_kick would kick with); ``field``
+        receives the evaluation's adjoint data as in field_kick."""
+        buf, srt = field
+        if self.sorted:
+            self._timed("forces", "jpm_force_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
+                        ptr(out), 1.0, ptr(buf), self.r_split, self.flags, 1, ptr(self.ws), self.ws.numel(),
+                        ptr(srt), 0 if srt is None else srt.numel())
+        else:
+            self.field(x, buf)
+            out.zero_()
+            self.kick(buf, x, out, 1.0)
+
     def vjp_field(self, field, x, cot, scale, gpos):
         """gpos += scale J(x)^T cot given the evaluation's ``field``."""
         buf, srt = field
@@ -135,37 +148,68 @@ class Stepper:
                     ptr(cot), float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws), self.ws.numel())
 
 
-def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None):
+def schedule_tensor(sched):
+    """(k0, beta, gamma) -> the float64 coefficient vector [k0, beta, gamma]
+    that :class:`NBody` takes."""
+    k0, beta, gamma = sched
+    return torch.as_tensor(np.concatenate([[k0], np.asarray(beta, np.float64), np.asarray(gamma, np.float64)]),
+                           dtype=torch.float64)
+
+
+def _split(coeffs):
+    c = coeffs.detach().to(torch.float64).cpu().numpy()
+    if c.ndim != 1 or c.size < 3 or c.size % 2 == 0:
+        raise ValueError("leapfrog coefficients must be [k0, beta(S), gamma(S)]")
+    S = (c.size - 1) // 2
+    return float(c[0]), c[1:1 + S], c[1 + S:]
+
+
+def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=False):
     """Advance steps [start, stop) in place.  With ``keep`` (a list of
-    preallocated (x, phi) buffer pairs, one per step) each step's positions
-    are copied out and its force meshes are computed straight into the
-    step's buffer."""
+    preallocated per-step buffers) each step's positions are copied out and
+    its force evaluation writes into the step's field buffer; with
+    ``forces`` the step also keeps the velocity it drifted with and the
+    forces it kicked with (v += gamma F as forces-then-axpy, the same float
+    operations as the fused kick)."""
     for i in range(start, stop):
         stepper.axpy(beta[i], v, x)
         if keep is not None:
-            xk, phi = keep[i - start]
-            xk.copy_(x)
+            kp = keep[i - start]
+            kp[0].copy_(x)
+            phi = kp[1]
+            if forces:
+                kp[2].copy_(v)
+                stepper.forces(x, kp[3], phi)
+                stepper.axpy(gamma[i], kp[3], v)
+                continue
         stepper.field_kick(x, v, gamma[i], phi)
 
 
 class NBody(torch.autograd.Function):
-    """(x0, v0) -> (xS, vS) by the leapfrog, with a hand-written adjoint."""
+    """(x0, v0, c) -> (xS, vS) by the leapfrog, with a hand-written adjoint.
+
+    ``c`` is the float64 host vector [k0, beta(S), gamma(S)] of
+    :func:`kdk_schedule`.  Its cotangent, when asked for, is
+    dL/dk0 = <lambda_v, F(x0)>, dL/dbeta_i = <lambda_x, v_i>,
+    dL/dgamma_i = <lambda_v, F(x_{i+1})>, each a float64 device dot product
+    of the adjoint state with the recomputed trajectory."""
 
     @staticmethod
-    def forward(ctx, x0, v0, stepper, sched, checkpoint_every):
-        k0, beta, gamma = sched
+    def forward(ctx, x0, v0, coeffs, stepper, checkpoint_every):
+        k0, beta, gamma = _split(coeffs)
         S = len(beta)
         x = x0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
         phi = stepper.new_field()
         stepper.field_kick(x, v, k0, phi)
         k = S if checkpoint_every is None else max(1, int(checkpoint_every))
-        ckpts = {0: (x.clone(), v.clone())}
+        ckpts = {0: (x.clone(), v.clone())}   # x0 and v after the first half kick
         for s0 in range(0, S, k):
             _run(stepper, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi)
             if s0 + k < S:
                 ckpts[s0 + k] = (x.clone(), v.clone())
-        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = stepper, sched, k, ckpts
+        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = stepper, (k0, beta, gamma), k, ckpts
+        ctx.c_meta = (coeffs.dtype, coeffs.device)
         return x, v
 
     @staticmethod
@@ -173,28 +217,45 @@ class NBody(torch.autograd.Function):
         st = ctx.stepper
         k0, beta, gamma = ctx.sched
         S = len(beta)
-        x_ref = ctx.ckpts[0][0]
-        lx = (gx if gx is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
-        lv = (gv if gv is not None else torch.zeros_like(x_ref)).detach().clone().contiguous()
+        want_c = ctx.needs_input_grad[2]
+        x0, v0 = ctx.ckpts[0]
+        lx = (gx if gx is not None else torch.zeros_like(x0)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(x0)).detach().clone().contiguous()
+        dots = torch.zeros(1 + 2 * S, dtype=torch.float64, device=x0.device) if want_c else None
         phi = st.new_field()
-        keep = [(torch.empty_like(x_ref), st.new_field(keep_sort=True)) for _ in range(min(ctx.k, S))]
+
+        def buffers():
+            b = [torch.empty_like(x0), st.new_field(keep_sort=True)]
+            if want_c:
+                b += [torch.empty_like(x0), torch.empty_like(x0)]
+            return b
+
+        keep = [buffers() for _ in range(min(ctx.k, S))]
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
             xc, vc = ctx.ckpts[s0]
             x, v = xc.clone(), vc.clone()
-            _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep)
+            _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep, forces=want_c)
             for i in reversed(range(s0, s1)):
-                xi, phii = keep[i - s0]
-                st.vjp_field(phii, xi, lv, gamma[i], lx)   # kick i: v += gamma F(x_{i+1})
-                st.axpy(beta[i], lx, lv)            # drift i: x += beta v
+                kp = keep[i - s0]
+                if want_c:
+                    _ops.dot_into(dots, 1 + S + i, lv, kp[3])     # gamma_i
+                st.vjp_field(kp[1], kp[0], lv, gamma[i], lx)      # kick i: v += gamma F(x_{i+1})
+                if want_c:
+                    _ops.dot_into(dots, 1 + i, lx, kp[2])         # beta_i
+                st.axpy(beta[i], lx, lv)                          # drift i: x += beta v
         del keep
         # the first half kick used x0 before any drift: recompute its field
-        xs = ctx.ckpts[0][0]
-        scratch_v = torch.zeros_like(xs)
-        st.field_kick(xs, scratch_v, 0.0, phi)
-        st.vjp_field(phi, xs, lv, k0, lx)
+        if want_c:
+            f0 = torch.empty_like(x0)
+            st.forces(x0, f0, phi)
+            _ops.dot_into(dots, 0, lv, f0)
+        else:
+            st.field_kick(x0, torch.zeros_like(x0), 0.0, phi)
+        st.vjp_field(phi, x0, lv, k0, lx)
         ctx.ckpts = None
-        return lx, lv, None, None, None
+        gc = dots.to(device=ctx.c_meta[1], dtype=ctx.c_meta[0]) if want_c else None
+        return lx, lv, gc, None, None
 
 
 def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, deterministic=False, r_split=0.0,
@@ -206,7 +267,7 @@ def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, dete
     v0 = to_device(vel).reshape(-1, 3)
     st = Stepper(mesh_shape, x0.shape[0], relative=not paint_absolute_pos, deterministic=deterministic,
                  r_split=r_split)
-    sched = kdk_schedule(cosmo, a_steps)
-    x, v = NBody.apply(x0, v0, st, sched, checkpoint_every)
+    coeffs = schedule_tensor(kdk_schedule(cosmo, a_steps))
+    x, v = NBody.apply(x0, v0, coeffs, st, checkpoint_every)
     shape = to_device(pos).shape
     return x.reshape(shape), v.reshape(shape)

@@ -13,7 +13,7 @@ from . import growth as gr
 from ._convert import single_device, to_device
 from .distributed import normal_field
 
-__all__ = ["pm_forces", "lpt", "linear_field", "TabulatedPk"]
+__all__ = ["pm_forces", "lpt", "linear_field", "TabulatedPk", "lpt_coefficients", "lpt_fields", "lpt_combine"]
 
 
 def _needs_grad(*ts):
@@ -109,6 +109,56 @@ def lpt(cosmo, initial_conditions, particles=None, a=0.1, halo_size=0, sharding=
         p = p + p2
         f = f + ff2
     return dx, p, f
+
+
+def lpt_coefficients(cosmo, a=0.1, order=1):
+    """Host float64 growth coefficients of :func:`lpt` (pm.py:77-124):
+    [D1, a^2 f1 E, a^2 E dGfa] and, for order 2, [3/7 D2, a^2 f2 E,
+    a^2 E dGf2a] -- dx1 = c0 Psi1, p1 = c1 dx1, f1 = c2 Psi1 and likewise for
+    the second-order field Psi2."""
+    a = _coef(a)
+    E = _coef(np.sqrt(gr.Esqr(cosmo, a)))
+    c = [_coef(gr.growth_factor(cosmo, a)), a ** 2 * _coef(gr.growth_rate(cosmo, a)) * E,
+         a ** 2 * E * _coef(gr.dGfa(cosmo, a))]
+    if order == 2:
+        c += [3 / 7 * _coef(gr.growth_factor_second(cosmo, a)), a ** 2 * _coef(gr.growth_rate_second(cosmo, a)) * E,
+              a ** 2 * E * _coef(gr.dGf2a(cosmo, a))]
+    return np.asarray(c, np.float64)
+
+
+def lpt_fields(initial_conditions, particles=None, order=1):
+    """The force fields lpt scales: Psi1 = pm_forces(delta) read at the
+    particles (pm.py:79-82) and, for order 2, Psi2 from the 2LPT source
+    (pm.py:88-112).  Differentiable in the initial conditions."""
+    ic = to_device(initial_conditions)
+    shape = tuple(ic.shape)
+    rel = particles is None
+    parts = torch.zeros((*shape, 3), dtype=torch.float32, device=ic.device) if rel else to_device(particles)
+    psi1 = _forces_from_field(ic, parts.reshape(-1, 3), rel, 0.0, 0).reshape(parts.shape)
+    if order == 1:
+        return psi1, None
+    phi6 = _autograd.ShearField.apply(ic, 0) if _needs_grad(ic) else \
+        _ops.c2r(_ops.kspace_shear6(_ops.r2c(ic), shape, 1.0 / ic.numel()), shape, clobber=True)
+    delta2 = _autograd.Lpt2Source.apply(phi6) if _needs_grad(phi6) else _ops.lpt2_source(phi6)
+    psi2 = _forces_from_field(delta2, parts.reshape(-1, 3), rel, 0.0, 0).reshape(parts.shape)
+    return psi1, psi2
+
+
+def lpt_combine(coeffs, psi1, psi2=None):
+    """(dx, p, f) from the fields of :func:`lpt_fields` and a float64 host
+    tensor of :func:`lpt_coefficients`, differentiable in both (the
+    cotangents of the coefficients are float64 dot products)."""
+    lc = _autograd.LinComb.apply
+    one = torch.ones(2, dtype=torch.float64)
+    dx = lc(coeffs[0:1], psi1)
+    p = lc(coeffs[1:2], dx)
+    f = lc(coeffs[2:3], psi1)
+    if psi2 is None:
+        return dx, p, f
+    dx2 = lc(coeffs[3:4], psi2)
+    p2 = lc(coeffs[4:5], dx2)
+    f2 = lc(coeffs[5:6], psi2)
+    return lc(one, dx, dx2), lc(one, p, p2), lc(one, f, f2)
 
 
 class TabulatedPk:
