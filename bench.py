@@ -45,6 +45,11 @@ def parse():
     ap.add_argument("--checkpoint", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--workload", choices=["pm", "diffsim"], default="pm",
+                    help="pm: the default forward+adjoint step benchmark; diffsim: the differentiable "
+                         "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8)")
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="leapfrog steps of the plain stored-trajectory autodiff comparison (0: skip)")
     return ap.parse_args()
 
 
@@ -80,12 +85,65 @@ def cpu_baseline(n, box_cells_per_side, x, v):
                       f"VJP, drift adjoint) of oracle/cpu_step.py, float32 NumPy/SciPy c2c FFT, {dt:.1f} s"}
 
 
+def diffsim(args):
+    """Differentiable single simulation: 128^3 particles and mesh, box
+    512 Mpc/h, 2LPT at a=0.1 + 40 leapfrog steps to a=1, checkpoints every 8
+    steps; loss sum((delta - delta_t)^2)/0.1^2 against the final density of
+    seed-4 noise; gradients w.r.t. the seed-3 white noise and (Omega_c,
+    sigma8).  One timed unit is one forward + gradient simulation."""
+    import torch
+
+    from jaxpm_amd import _lib, diffsim
+    from jaxpm_amd import cosmology as pc
+    dev = _lib.device()
+    n = 128 if args.mesh == 256 else args.mesh
+    shape = (n, n, n)
+    S = 40
+    model = diffsim.LptNBody(shape, (4.0 * n,) * 3, np.linspace(0.1, 1.0, S + 1),
+                             base_cosmo=pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8),
+                             order=2, checkpoint_every=8)
+    noise = torch.as_tensor(np.random.default_rng(3).standard_normal(shape, dtype=np.float32), device=dev)
+    with torch.no_grad():
+        tnoise = torch.as_tensor(np.random.default_rng(4).standard_normal(shape, dtype=np.float32), device=dev)
+        target = model(tnoise, model.parameter_vector()) - 1.0
+
+    def run():
+        nt = noise.clone().requires_grad_(True)
+        params = model.parameter_vector().requires_grad_(True)
+        rho = model(nt, params)
+        loss = ((rho - 1.0 - target) ** 2).sum() / 0.1 ** 2
+        loss.backward()
+        return float(loss.detach()), nt.grad, params.grad
+
+    for _ in range(max(args.warmup, 1) if args.warmup > 0 else 0):
+        run()
+    torch.cuda.synchronize()
+    reps = max(1, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss, gn, gp = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    out = {"metric": "forward+gradient simulations/s, 128^3 particles and mesh, 2LPT + 40 leapfrog steps",
+           "value": round(1.0 / dt, 4), "unit": "sims/s", "n_gpus": 1, "steps": reps, "warmup": args.warmup,
+           "ms_per_step": round(1000 * dt, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"differentiable simulation {n}^3, box {4 * n} Mpc/h, 2LPT a=0.1, {S} steps, "
+                                  "checkpoint every 8, gradients w.r.t. noise and (Omega_c, sigma8)"},
+           "loss": loss, "grad_noise_rms": float(gn.pow(2).mean().sqrt()),
+           "grad_params": {"Omega_c": float(gp[0]), "sigma8": float(gp[1])},
+           "peak_mem_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the reference's sharded "
               "halo exchange and pencil FFTs are out of scope); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "diffsim":
+        return diffsim(args)
     import torch
 
     from jaxpm_amd import _lib, _ops, integrate, painting, pm
@@ -184,6 +242,43 @@ def main():
         "ms": round(t_paint, 4), "GB_s": round((12 * N + 4 * M) / (t_paint * 1e-3) / 1e9, 1),
         "frac_hbm": round((12 * N + 4 * M) / (t_paint * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
+    # the plain per-simulation form: the same leapfrog as differentiable
+    # tensor expressions, autograd over the stored trajectory
+    plain = None
+    if args.plain_steps > 0:
+        P = args.plain_steps
+
+        def plain_fwd_adj():
+            sched = integrate.schedule_tensor(integrate.kdk_schedule(cosmo, np.linspace(0.1, 1.0, P + 1)))
+            xt = x0.clone().requires_grad_(True)
+            vt = v0.clone().requires_grad_(True)
+            x, v = integrate.leapfrog_autodiff(xt, vt, sched, shape)
+            rho = painting.cic_paint(torch.zeros(shape, device=dev), x)
+            (((rho - target) ** 2).sum() / sigma ** 2).backward()
+            return xt.grad
+
+        plain_fwd_adj()
+        torch.cuda.synchronize()
+        base_mem = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        t0 = time.perf_counter()
+        gp = plain_fwd_adj()
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        mem_plain = torch.cuda.max_memory_allocated() - base_mem
+        torch.cuda.reset_peak_memory_stats()
+        _, gc = fwd_adj(P)
+        torch.cuda.synchronize()
+        mem_custom = torch.cuda.max_memory_allocated() - base_mem
+        rel = float((gp - gc).abs().max() / gc.abs().max())
+        rms = float((gp - gc).pow(2).mean().sqrt() / gc.pow(2).mean().sqrt())
+        plain = {"steps_per_s": round(P / t_plain, 3), "steps": P,
+                 "speedup_of_custom_adjoint": round(steps_per_s / (P / t_plain), 3),
+                 "grad_max_rel_diff": rel, "grad_rms_rel_diff": rms,
+                 "extra_mem_gb_plain": round(mem_plain / 1e9, 2), "extra_mem_gb_custom": round(mem_custom / 1e9, 2),
+                 "what": "same leapfrog as tensor expressions over pm_forces (per-op adjoints), autograd over "
+                         "the stored trajectory, no checkpointing; grad w.r.t. x0 compared with the custom adjoint"}
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, n, x0.cpu().numpy(), v0.cpu().numpy())
@@ -209,6 +304,7 @@ def main():
                      "traffic": PMC_TRAFFIC_BYTES.get((dom, n)),
                      "algorithmic_bytes": int(algo[dom]), "ms_per_launch": round(t_dom, 4)},
         "kernels": kinfo,
+        "plain_autodiff": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -96,42 +96,79 @@ def Omega_de_a(cosmo, a):
 
 # ------------------------------------------------------------------ growth
 
-def _deriv(cosmo, a, y):
-    """d/d(ln a) of y = (D1, D2, dD1/da, dD2/da)."""
-    d1, d2, v1, v2 = y
+def _qr(cosmo, a):
+    """Coefficients of the growth ODE at scale factor(s) a:
+    d2D/da2 = -q dD/da + r D (+ the 2LPT source -r D1^2 for D2)."""
     om = Omega_m_a(cosmo, a)
     q = (2.0 - 0.5 * (om + (1.0 + 3.0 * w(cosmo, a)) * Omega_de_a(cosmo, a))) / a
     r = 1.5 * om / (a * a)
-    acc1 = -q * v1 + r * d1
-    acc2 = -q * v2 + r * d2 - r * d1 * d1
-    return np.array([v1, v2, acc1, acc2]) * a, (acc1, acc2)
+    return q, r
 
 
 def growth_tables(cosmo):
     """(atab, gtab, ftab, htab, g2tab, f2tab, h2tab) -- the 7-tuple that
-    growth.py:218-221 and 255-258 index."""
+    growth.py:218-221 and 255-258 index.
+
+    Classical RK4 in ln a on 256 log-spaced nodes from a = 1e-3 with 32 fixed
+    substeps per interval, for y = (D1, D2, dD1/da, dD2/da) with growing-mode
+    initial values (D1 = a, D2 = -3/7 a^2).  The ODE coefficients are
+    evaluated for every RK4 stage at once (vectorised over the fixed step
+    grid); the stages themselves run on scalars.  A fixed step makes the
+    tables smooth in the cosmological parameters."""
     t = cosmo._cache.get("growth")
     if t is not None:
         return t
     atab = np.logspace(_LOG10_AMIN, 0.0, _STEPS)
     lna = np.log(atab)
-    y = np.array([atab[0], -3.0 / 7.0 * atab[0] ** 2, 1.0, -6.0 / 7.0 * atab[0]])
-    ys = np.empty((_STEPS, 4))
-    acc = np.empty((_STEPS, 2))
-    ys[0] = y
-    acc[0] = _deriv(cosmo, atab[0], y)[1]
+    # ln a of every substep start, by the same repeated addition as the steps
+    xs = np.empty((_STEPS - 1, _SUBSTEPS))
+    hs = np.empty(_STEPS - 1)
     for i in range(_STEPS - 1):
         h = (lna[i + 1] - lna[i]) / _SUBSTEPS
+        hs[i] = h
         x = lna[i]
-        for _ in range(_SUBSTEPS):
-            k1 = _deriv(cosmo, np.exp(x), y)[0]
-            k2 = _deriv(cosmo, np.exp(x + 0.5 * h), y + 0.5 * h * k1)[0]
-            k3 = _deriv(cosmo, np.exp(x + 0.5 * h), y + 0.5 * h * k2)[0]
-            k4 = _deriv(cosmo, np.exp(x + h), y + h * k3)[0]
-            y = y + h / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+        for j in range(_SUBSTEPS):
+            xs[i, j] = x
             x = x + h
-        ys[i + 1] = y
-        acc[i + 1] = _deriv(cosmo, atab[i + 1], y)[1]
+    hh = hs[:, None]
+    a0, am, a1 = np.exp(xs), np.exp(xs + 0.5 * hh), np.exp(xs + hh)
+    q0, r0 = _qr(cosmo, a0)
+    qm, rm = _qr(cosmo, am)
+    q1, r1 = _qr(cosmo, a1)
+    qn, rn = _qr(cosmo, atab)
+    a0, am, a1 = a0.tolist(), am.tolist(), a1.tolist()
+    q0, r0, qm, rm, q1, r1 = q0.tolist(), r0.tolist(), qm.tolist(), rm.tolist(), q1.tolist(), r1.tolist()
+
+    def f(A, Q, R, d1, d2, v1, v2):
+        acc1 = -Q * v1 + R * d1
+        acc2 = -Q * v2 + R * d2 - R * d1 * d1
+        return v1 * A, v2 * A, acc1 * A, acc2 * A
+
+    def accel(Q, R, d1, d2, v1, v2):
+        return -Q * v1 + R * d1, -Q * v2 + R * d2 - R * d1 * d1
+
+    d1, d2, v1, v2 = float(atab[0]), -3.0 / 7.0 * float(atab[0]) ** 2, 1.0, -6.0 / 7.0 * float(atab[0])
+    ys = np.empty((_STEPS, 4))
+    acc = np.empty((_STEPS, 2))
+    ys[0] = (d1, d2, v1, v2)
+    acc[0] = accel(float(qn[0]), float(rn[0]), d1, d2, v1, v2)
+    for i in range(_STEPS - 1):
+        h = float(hs[i])
+        hh2 = 0.5 * h
+        h6 = h / 6.0
+        A0, AM, A1 = a0[i], am[i], a1[i]
+        Q0, R0, QM, RM, Q1, R1 = q0[i], r0[i], qm[i], rm[i], q1[i], r1[i]
+        for j in range(_SUBSTEPS):
+            k1 = f(A0[j], Q0[j], R0[j], d1, d2, v1, v2)
+            k2 = f(AM[j], QM[j], RM[j], d1 + hh2 * k1[0], d2 + hh2 * k1[1], v1 + hh2 * k1[2], v2 + hh2 * k1[3])
+            k3 = f(AM[j], QM[j], RM[j], d1 + hh2 * k2[0], d2 + hh2 * k2[1], v1 + hh2 * k2[2], v2 + hh2 * k2[3])
+            k4 = f(A1[j], Q1[j], R1[j], d1 + h * k3[0], d2 + h * k3[1], v1 + h * k3[2], v2 + h * k3[3])
+            d1 = d1 + h6 * (k1[0] + 2 * k2[0] + 2 * k3[0] + k4[0])
+            d2 = d2 + h6 * (k1[1] + 2 * k2[1] + 2 * k3[1] + k4[1])
+            v1 = v1 + h6 * (k1[2] + 2 * k2[2] + 2 * k3[2] + k4[2])
+            v2 = v2 + h6 * (k1[3] + 2 * k2[3] + 2 * k3[3] + k4[3])
+        ys[i + 1] = (d1, d2, v1, v2)
+        acc[i + 1] = accel(float(qn[i + 1]), float(rn[i + 1]), d1, d2, v1, v2)
     d1, d2, v1, v2 = ys.T
     gtab = d1 / d1[-1]
     g2tab = d2 / d2[-1]

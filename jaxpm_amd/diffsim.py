@@ -26,14 +26,27 @@ from . import _autograd, integrate, pm
 from . import cosmology as cs
 from ._convert import to_device
 
-__all__ = ["CosmoCoefficients", "LptNBody"]
+__all__ = ["CosmoCoefficients", "LptNBody", "central_jacobian"]
+
+
+def central_jacobian(fn, x, rel_step):
+    """(len(fn(x)), len(x)) central-difference Jacobian of a host function,
+    step ``rel_step * max(|x_j|, 1e-3)`` per parameter."""
+    cols = []
+    for j in range(len(x)):
+        h = rel_step * max(abs(x[j]), 1e-3)
+        xp, xm = x.copy(), x.copy()
+        xp[j] += h
+        xm[j] -= h
+        cols.append((np.asarray(fn(xp), np.float64) - np.asarray(fn(xm), np.float64)) / (2 * h))
+    return np.stack(cols, 1)
 
 
 class CosmoCoefficients(torch.autograd.Function):
     """params (float64 host vector) -> fn(params) (float64 host vector).
 
-    Backward: J^T g with J the central-difference Jacobian of ``fn``, step
-    ``rel_step * max(|p|, 1e-3)`` per parameter."""
+    Backward: J^T g with J = ``fn.jacobian(x)`` when ``fn`` provides one (a
+    cached Jacobian), else the central-difference Jacobian of ``fn``."""
 
     @staticmethod
     def forward(ctx, params, fn, rel_step):
@@ -44,17 +57,35 @@ class CosmoCoefficients(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        x, fn = ctx.x, ctx.fn
+        jac = getattr(ctx.fn, "jacobian", None)
+        J = jac(ctx.x) if jac is not None else central_jacobian(ctx.fn, ctx.x, ctx.rel_step)
         gv = g.detach().to(torch.float64).cpu().numpy()
-        out = np.empty(len(x))
-        for j in range(len(x)):
-            h = ctx.rel_step * max(abs(x[j]), 1e-3)
-            xp, xm = x.copy(), x.copy()
-            xp[j] += h
-            xm[j] -= h
-            col = (np.asarray(fn(xp), np.float64) - np.asarray(fn(xm), np.float64)) / (2 * h)
-            out[j] = float(col @ gv)
-        return torch.as_tensor(out, dtype=ctx.meta[0], device=ctx.meta[1]), None, None
+        return torch.as_tensor(J.T @ gv, dtype=ctx.meta[0], device=ctx.meta[1]), None, None
+
+
+class _Quantities:
+    """A host function of the parameter vector with its value and Jacobian
+    memoised per parameter point (a noise-only optimisation reuses them)."""
+
+    def __init__(self, fn, rel_step, size=8):
+        self.fn, self.rel_step, self.size = fn, rel_step, size
+        self._val, self._jac = {}, {}
+
+    def _put(self, d, k, v):
+        if len(d) >= self.size:
+            d.pop(next(iter(d)))
+        d[k] = v
+        return v
+
+    def __call__(self, x):
+        k = tuple(float(v) for v in x)
+        return self._val[k] if k in self._val else self._put(self._val, k, self.fn(np.asarray(x, np.float64)))
+
+    def jacobian(self, x):
+        k = tuple(float(v) for v in x)
+        if k in self._jac:
+            return self._jac[k]
+        return self._put(self._jac, k, central_jacobian(self.fn, np.asarray(x, np.float64), self.rel_step))
 
 
 class LptNBody:
@@ -99,6 +130,7 @@ class LptNBody:
         self.stepper = integrate.Stepper(self.shape, n, relative=not self.absolute, deterministic=deterministic)
         self._ktab_dev = None
         self._lattice = None
+        self._q = _Quantities(self.quantities, rel_step)
 
     def cosmology(self, values):
         return self.base.replace(**{nm: float(v) for nm, v in zip(self.names, values)})
@@ -121,7 +153,7 @@ class LptNBody:
         if tuple(noise.shape) != self.shape:
             raise ValueError(f"noise shape {tuple(noise.shape)} does not match {self.shape}")
         params = torch.as_tensor(params, dtype=torch.float64) if not isinstance(params, torch.Tensor) else params
-        q = CosmoCoefficients.apply(params, self.quantities, self.rel_step)
+        q = CosmoCoefficients.apply(params, self._q, self.rel_step)
         dev = noise.device
         if self._ktab_dev is None:
             self._ktab_dev = torch.as_tensor(self.ktab, dtype=torch.float32, device=dev)

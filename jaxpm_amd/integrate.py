@@ -258,6 +258,28 @@ class NBody(torch.autograd.Function):
         return lx, lv, gc, None, None
 
 
+def leapfrog_autodiff(x0, v0, coeffs, mesh_shape, paint_absolute_pos=True):
+    """The plain form of the same leapfrog, for comparison with
+    :class:`NBody`: every step is a differentiable tensor expression
+    (``pm_forces`` through the per-operation adjoints of ``_autograd``, kicks
+    and drifts as tensor arithmetic) and autograd differentiates it over the
+    stored trajectory -- the analogue of jax.grad through the reference's
+    integrator with all intermediates kept (no checkpointing, no fused
+    kernels)."""
+    from .pm import pm_forces
+    k0, beta, gamma = _split(coeffs)
+
+    def force(x):
+        return pm_forces(x, mesh_shape=mesh_shape, paint_absolute_pos=paint_absolute_pos)
+
+    x = x0
+    v = v0 + k0 * force(x0)
+    for i in range(len(beta)):
+        x = x + float(beta[i]) * v
+        v = v + float(gamma[i]) * force(x)
+    return x, v
+
+
 def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, deterministic=False, r_split=0.0,
              checkpoint_every=None):
     """Integrate ``(pos, vel)`` over ``a_steps`` (absolute grid-unit positions,
