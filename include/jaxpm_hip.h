@@ -387,6 +387,64 @@ int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3,
                  const float *pos, int64_t n, float gamma, float *vel,
                  uint32_t flags);
 
+/* -------------------------------------------------------------- batches
+ *
+ * Many simulations of different mesh sizes and cosmologies stepped together:
+ * each stage of a force evaluation is one launch over the whole batch (the
+ * FFTs and k-space passes one per mesh shape).  The reference runs
+ * simulations one at a time (a Python loop over pm_forces / the integrator,
+ * or jax.vmap over equal shapes); these entry points have no single
+ * counterpart in it and serve the batched leapfrog of jaxpm_amd.integrate.
+ * A batch concatenates its simulations: positions, velocities and
+ * cotangents [N][3] (N = sum n_b), meshes [M] (M = sum nx ny nz), in the
+ * order given; simulations of one mesh shape must be contiguous. */
+
+/* Describe a batch: dims[b*3 .. b*3+2] the mesh of simulation b, counts[b]
+ * its particles; flags JPM_RELATIVE for displacements (counts[b] = cells).
+ * Writes an opaque handle to *batch (a device table is uploaded; this call
+ * synchronises).  Each mesh <= 32768 tiles of 16^3. */
+int jpm_batch_create(int nsims, const int *dims, const int64_t *counts,
+                     uint32_t flags, void **batch);
+int jpm_batch_destroy(void *batch);
+
+/* Bytes of a kept tile sort / of the force_kick, force_vjp and dot
+ * workspaces of this batch. */
+size_t jpm_batch_sort_size(const void *batch);
+size_t jpm_batch_force_kick_workspace_size(const void *batch);
+size_t jpm_batch_force_vjp_workspace_size(const void *batch);
+size_t jpm_batch_dot_workspace_size(const void *batch);
+
+/* jpm_force_kick_f32 for every simulation at once: phi[M] receives each
+ * simulation's potential, vel_b += gamma[b] F_b (or = gamma[b] F_b with
+ * write_forces); gamma is a device array [nsims].  keep_sort (nullable)
+ * receives the tile sort for jpm_batch_force_vjp_f32.  Replaces the
+ * pm_forces (pm.py:12-58) + kick (ode.py:143) of each simulation. */
+int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos,
+                             float *vel, const float *gamma, float *phi,
+                             float r_split, uint32_t kflags, int write_forces,
+                             void *ws, size_t ws_bytes, void *keep_sort,
+                             size_t keep_bytes);
+
+/* jpm_force_vjp_pot_f32 for every simulation: gpos_b += scale[b] J_b^T cot_b
+ * given the evaluation's phi (and kept sort).  pot_scale[b] must be
+ * float(scale[b] / (nx ny nz)_b), the factor the potential pass applies. */
+int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
+                            const float *pos, const float *cot,
+                            const float *scale, const float *pot_scale,
+                            float *gpos, float r_split, uint32_t kflags,
+                            void *ws, size_t ws_bytes, const void *sorted,
+                            size_t sorted_bytes);
+
+/* y_b += a[b] x_b over each simulation's 3 n_b floats (the drift of
+ * ode.py:140 with per-simulation coefficients). */
+int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a,
+                       const float *x, float *y);
+
+/* out[b] (device float64) = <x_b, y_b> over each simulation's 3 n_b floats,
+ * fixed-order float64 sums (the per-simulation coefficient cotangents). */
+int jpm_batch_dot_f32(void *stream, const void *batch, const float *x,
+                      const float *y, double *out, void *ws, size_t ws_bytes);
+
 #ifdef __cplusplus
 }
 #endif

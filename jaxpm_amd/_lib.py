@@ -65,6 +65,16 @@ SIGNATURES = {
                                                    _sz]),
     "jpm_dot_workspace_size": (_sz, []),
     "jpm_dot_f32": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_batch_create": (_i, [_i, _vp, _vp, _u32, ctypes.POINTER(_vp)]),
+    "jpm_batch_destroy": (_i, [_vp]),
+    "jpm_batch_sort_size": (_sz, [_vp]),
+    "jpm_batch_force_kick_workspace_size": (_sz, [_vp]),
+    "jpm_batch_force_vjp_workspace_size": (_sz, [_vp]),
+    "jpm_batch_dot_workspace_size": (_sz, [_vp]),
+    "jpm_batch_force_kick_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _i, _vp, _sz, _vp, _sz]),
+    "jpm_batch_force_vjp_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
+    "jpm_batch_axpy_f32": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "jpm_batch_dot_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "jpm_power_spectrum_workspace_size": (_sz, [_i, _i, _i, _i, _i, _i]),
     "jpm_power_spectrum_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _d, _d, _d, _vp, _i, _vp, _i, _d, _d, _d, _vp, _vp,
                                     _sz]),

@@ -1,0 +1,289 @@
+// Batches of simulations of different mesh sizes and cosmologies, stepped
+// together: every stage of a force evaluation (tile sort, window paint,
+// potential, gather) is one launch over all simulations of the batch --
+// kernels look their simulation up in a device table by tile or workgroup
+// index -- except the FFTs and the per-shape k-space passes, which run once
+// per mesh shape as strided batches.  Per-simulation leapfrog coefficients
+// are device arrays, so a captured time loop replays without host work.
+//
+// Layout: the batch concatenates its simulations in order.  Particles
+// [N][3], meshes [M], half spectra [H]; simulations of one shape are
+// contiguous.
+#include <algorithm>
+#include <new>
+
+#include "tiles.hpp"
+
+namespace jpm {
+
+int fft_r2c(hipStream_t, int, int, int, int, const float *, void *);
+int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
+int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *, float, float, const float *,
+                           uint32_t);
+int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
+int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int);
+int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *);
+
+// ------------------------------------------------------------ description
+
+int batch_create(int B, const int *dims, const int64_t *counts, uint32_t flags, SimBatch **out) {
+  JPM_REQUIRE(out, "null output handle");
+  *out = nullptr;
+  JPM_REQUIRE(B >= 1 && B <= 65535, "batch of %d simulations", B);
+  JPM_REQUIRE(dims && counts, "null dims or counts");
+  const bool rel = flags & JPM_RELATIVE;
+  SimBatch *bt = new (std::nothrow) SimBatch;
+  JPM_REQUIRE(bt, "out of host memory");
+  bt->B = B;
+  bt->rel = rel;
+  uint64_t K = 0, N = 0, M = 0, H = 0, G = 0;
+  uint32_t maxK = 0;
+  for (int b = 0; b < B; ++b) {
+    const int nx = dims[3 * b], ny = dims[3 * b + 1], nz = dims[3 * b + 2];
+    const int64_t n = counts[b];
+    if (int rc = check_dims(nx, ny, nz)) {
+      delete bt;
+      return rc;
+    }
+    const TileGrid t = make_tiles(nx, ny, nz);
+    const int64_t m = cells(nx, ny, nz);
+    if (n < 1 || (rel && n != m) || t.K > uint32_t(kMaxKeysLds)) {
+      delete bt;
+      return fail(JPM_E_INVALID, "simulation %d: mesh (%d,%d,%d) with %lld particles (%s)", b, nx, ny, nz,
+                  (long long)n, t.K > uint32_t(kMaxKeysLds) ? "more than 32768 tiles" :
+                  rel ? "relative mode needs one particle per cell" : "no particles");
+    }
+    SimRef r;
+    r.t = t;
+    r.tile0 = uint32_t(K);
+    r.part0 = uint32_t(N);
+    r.n = uint32_t(n);
+    r.cell0 = uint32_t(M);
+    r.blk0 = uint32_t(G);
+    r.idx = uint32_t(b);
+    bt->host.push_back(r);
+    SimClass *last = bt->classes.empty() ? nullptr : &bt->classes.back();
+    if (last && last->nx == nx && last->ny == ny && last->nz == nz) {
+      last->count++;
+    } else {
+      for (const SimClass &c : bt->classes)
+        if (c.nx == nx && c.ny == ny && c.nz == nz) {
+          delete bt;
+          return fail(JPM_E_INVALID, "simulations of mesh (%d,%d,%d) must be contiguous in the batch", nx, ny, nz);
+        }
+      SimClass c;
+      c.nx = nx;
+      c.ny = ny;
+      c.nz = nz;
+      c.first = b;
+      c.count = 1;
+      c.cell0 = uint32_t(M);
+      c.half0 = uint32_t(H);
+      bt->classes.push_back(c);
+    }
+    K += t.K;
+    N += uint64_t(n);
+    M += uint64_t(m);
+    H += uint64_t(half_cells(nx, ny, nz));
+    G += (uint64_t(n) + kSlice - 1) / kSlice;
+    maxK = std::max(maxK, t.K);
+  }
+  if (N > uint64_t(kMaxCells) || 3 * M > (uint64_t(1) << 32) || 3 * H > (uint64_t(1) << 32)) {
+    delete bt;
+    return fail(JPM_E_INVALID, "batch too large: %llu particles, %llu cells", (unsigned long long)N,
+                (unsigned long long)M);
+  }
+  bt->K = uint32_t(K);
+  bt->N = uint32_t(N);
+  bt->M = uint32_t(M);
+  bt->H = uint32_t(H);
+  bt->G = uint32_t(G);
+  bt->maxK = maxK;
+  hipError_t e = hipMalloc(&bt->dev, sizeof(SimRef) * size_t(B));
+  if (e == hipSuccess) e = hipMemcpy(bt->dev, bt->host.data(), sizeof(SimRef) * size_t(B), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (bt->dev) (void)hipFree(bt->dev);
+    delete bt;
+    return fail(JPM_E_HIP, "uploading the simulation table failed: %s", hipGetErrorString(e));
+  }
+  *out = bt;
+  return JPM_OK;
+}
+
+int batch_destroy(SimBatch *bt) {
+  if (!bt) return JPM_OK;
+  hipError_t e = bt->dev ? hipFree(bt->dev) : hipSuccess;
+  delete bt;
+  if (e != hipSuccess) return fail(JPM_E_HIP, "freeing the simulation table failed: %s", hipGetErrorString(e));
+  return JPM_OK;
+}
+
+// --------------------------------------------------------------- workspace
+
+struct BatchWs {
+  size_t sort, paint, mesh, g3, spec, spec2, total;
+};
+
+static BatchWs batch_ws(const SimBatch &bt, int nc) {
+  BatchWs w;
+  size_t o = 0;
+  w.sort = o;  o += align_up(batch_sort_workspace_size(bt));
+  w.paint = o; o += align_up(batch_paint_workspace_size(bt, nc));
+  w.mesh = o;  o += align_up(size_t(bt.M) * 4);
+  w.g3 = o;    o += nc == 3 ? align_up(3 * size_t(bt.M) * 4) : 0;
+  w.spec = o;  o += align_up(size_t(bt.H) * 8);
+  w.spec2 = o; o += align_up(size_t(bt.H) * 8);
+  w.total = o;
+  return w;
+}
+
+size_t batch_kick_ws(const SimBatch &bt) { return batch_ws(bt, 1).total; }
+size_t batch_vjp_ws(const SimBatch &bt) { return batch_ws(bt, 3).total; }
+
+// ---------------------------------------------------------------- pipelines
+
+static int check_kflags(uint32_t kflags) {
+  JPM_REQUIRE(!(kflags & JPM_K_GRAD_EXACT), "batched pipelines use the 4-point gradient (no JPM_K_GRAD_EXACT)");
+  return JPM_OK;
+}
+
+// For every simulation b: phi_b = potential of the CIC density of its
+// particles; vel_b += gamma[b] F_b(pos_b) (or = F_b with write_forces).
+int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float *vel, const float *gamma, float *phi,
+                     float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes, void *keep,
+                     size_t keep_bytes) {
+  if (int rc = check_kflags(kflags)) return rc;
+  JPM_REQUIRE(pos && vel && phi && gamma, "null pointer argument");
+  const BatchWs w = batch_ws(bt, 1);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched force_kick needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  char *base = static_cast<char *>(ws);
+  BatchBins b;
+  int rc;
+  if (keep) {
+    JPM_REQUIRE(keep_bytes >= batch_sort_workspace_size(bt), "keep_sort buffer too small (%zu bytes)", keep_bytes);
+    rc = batch_sort(s, bt, pos, keep, keep_bytes, b);
+  } else {
+    rc = batch_sort(s, bt, pos, base + w.sort, w.paint - w.sort, b);
+  }
+  if (rc) return rc;
+  float *mesh = reinterpret_cast<float *>(base + w.mesh);
+  float2 *spec = reinterpret_cast<float2 *>(base + w.spec), *spec2 = reinterpret_cast<float2 *>(base + w.spec2);
+  rc = batch_paint(s, bt, b, nullptr, 1, mesh, base + w.paint, w.mesh - w.paint);
+  if (rc) return rc;
+  for (const SimClass &c : bt.classes) {
+    const float inv_m = float(1.0 / double(cells(c.nx, c.ny, c.nz)));
+    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, mesh + c.cell0, spec + c.half0))) return rc;
+    if ((rc = kspace_potential_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, spec2 + c.half0, r_split, inv_m,
+                                     nullptr, kflags)))
+      return rc;
+    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, phi + c.cell0))) return rc;
+  }
+  return batch_kick(s, b, phi, gamma, vel, write_forces);
+}
+
+// For every simulation b: gpos_b += scale[b] J_b^T cot_b, with pot_scale[b] =
+// scale[b] / M_b applied inside the potential.
+int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const float *pos, const float *cot,
+                    const float *scale, const float *pot_scale, float *gpos, float r_split, uint32_t kflags, void *ws,
+                    size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+  if (int rc = check_kflags(kflags)) return rc;
+  JPM_REQUIRE(phi && pos && cot && scale && pot_scale && gpos, "null pointer argument");
+  const BatchWs w = batch_ws(bt, 3);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched force_vjp needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  char *base = static_cast<char *>(ws);
+  BatchBins b;
+  int rc;
+  if (sorted) {
+    JPM_REQUIRE(sorted_bytes >= batch_sort_workspace_size(bt), "sorted buffer too small (%zu bytes)", sorted_bytes);
+    b = batch_view(bt, sorted);
+  } else {
+    rc = batch_sort(s, bt, pos, base + w.sort, w.paint - w.sort, b);
+    if (rc) return rc;
+  }
+  float *g3 = reinterpret_cast<float *>(base + w.g3);
+  float *rho = reinterpret_cast<float *>(base + w.mesh);
+  float2 *spec = reinterpret_cast<float2 *>(base + w.spec), *spec2 = reinterpret_cast<float2 *>(base + w.spec2);
+  rc = batch_paint(s, bt, b, cot, 3, g3, base + w.paint, w.mesh - w.paint);
+  if (rc) return rc;
+  for (const SimClass &c : bt.classes) {
+    if ((rc = fd4_divergence_batch(s, c.nx, c.ny, c.nz, c.count, g3 + c.cell0, size_t(bt.M), 1.0f, rho + c.cell0)))
+      return rc;
+    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, rho + c.cell0, spec + c.half0))) return rc;
+    if ((rc = kspace_potential_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, spec2 + c.half0, r_split, 0.0f,
+                                     pot_scale + c.first, kflags)))
+      return rc;
+    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, rho + c.cell0))) return rc;
+  }
+  return batch_grad(s, b, phi, rho, cot, scale, gpos);
+}
+
+// ------------------------------------------------- per-simulation vectors
+
+constexpr int kVecBlocks = 64;   // workgroups per simulation
+
+// y_b += a[b] x_b over each simulation's 3 n_b floats.
+__global__ void __launch_bounds__(256)
+batch_axpy_kernel(const SimRef *__restrict__ sims, const float *__restrict__ a, const float *__restrict__ x,
+                  float *__restrict__ y) {
+  const SimRef &sim = sims[blockIdx.y];
+  const float c = a[blockIdx.y];
+  const size_t lo = 3 * size_t(sim.part0), hi = lo + 3 * size_t(sim.n);
+  for (size_t i = lo + size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < hi; i += size_t(gridDim.x) * blockDim.x)
+    y[i] += c * x[i];
+}
+
+__device__ __forceinline__ double bsum_d(double v) {
+  __shared__ double part[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) s += part[w];
+  return s;
+}
+
+__global__ void __launch_bounds__(256)
+batch_dot_partial_kernel(const SimRef *__restrict__ sims, const float *__restrict__ x, const float *__restrict__ y,
+                         double *__restrict__ partial) {
+  const SimRef &sim = sims[blockIdx.y];
+  const size_t lo = 3 * size_t(sim.part0), hi = lo + 3 * size_t(sim.n);
+  double s = 0.0;
+  for (size_t i = lo + size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < hi; i += size_t(gridDim.x) * blockDim.x)
+    s += double(x[i]) * double(y[i]);
+  s = bsum_d(s);
+  if (threadIdx.x == 0) partial[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(256)
+batch_dot_final_kernel(const double *__restrict__ partial, int blocks, double *__restrict__ out) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) s += partial[size_t(blockIdx.x) * blocks + b];
+  s = bsum_d(s);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+int batch_axpy(hipStream_t s, const SimBatch &bt, const float *a, const float *x, float *y) {
+  JPM_REQUIRE(a && x && y, "null pointer argument");
+  JPM_TIMED(s, "batch_axpy_kernel");
+  hipLaunchKernelGGL(batch_axpy_kernel, dim3(kVecBlocks * 4, bt.B), dim3(256), 0, s, bt.dev, a, x, y);
+  JPM_LAUNCHED("batch_axpy_kernel");
+  return JPM_OK;
+}
+
+size_t batch_dot_ws(const SimBatch &bt) { return size_t(bt.B) * kVecBlocks * sizeof(double); }
+
+int batch_dot(hipStream_t s, const SimBatch &bt, const float *x, const float *y, double *out, void *ws,
+              size_t ws_bytes) {
+  JPM_REQUIRE(x && y && out, "null pointer argument");
+  JPM_REQUIRE(ws && ws_bytes >= batch_dot_ws(bt), "batched dot needs %zu workspace bytes", batch_dot_ws(bt));
+  double *partial = static_cast<double *>(ws);
+  hipLaunchKernelGGL(batch_dot_partial_kernel, dim3(kVecBlocks, bt.B), dim3(256), 0, s, bt.dev, x, y, partial);
+  JPM_LAUNCHED("batch_dot_partial_kernel");
+  hipLaunchKernelGGL(batch_dot_final_kernel, dim3(bt.B), dim3(256), 0, s, partial, kVecBlocks, out);
+  JPM_LAUNCHED("batch_dot_final_kernel");
+  return JPM_OK;
+}
+
+}  // namespace jpm

@@ -31,13 +31,15 @@ constexpr int kBinThreads = 256;                // hist / scatter workgroup size
 constexpr int kScanItems = 8;                   // scan: elements per thread
 constexpr int kScanBlock = 256 * kScanItems;
 
+// Floor cell of particle p (global index; `local` its index within its
+// simulation, which relative mode decodes into the lattice node).
 template <bool REL>
-__device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, const TileGrid &t, int &cx,
-                                            int &cy, int &cz) {
+__device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, uint32_t local,
+                                            const TileGrid &t, int &cx, int &cy, int &cz) {
   const float x = pos[3 * size_t(p)], y = pos[3 * size_t(p) + 1], z = pos[3 * size_t(p) + 2];
   if (REL) {
-    uint32_t r = t.div_nz.div(p);
-    uint32_t k = p - r * uint32_t(t.nz);
+    uint32_t r = t.div_nz.div(local);
+    uint32_t k = local - r * uint32_t(t.nz);
     uint32_t i = t.div_ny.div(r);
     uint32_t j = r - i * uint32_t(t.ny);
     cx = floor_cell_rel(float(i) + x, t.nx);
@@ -75,18 +77,25 @@ __device__ __forceinline__ uint32_t run_length(unsigned long long heads, unsigne
   return uint32_t(min(next, last_active + 1) - lane);
 }
 
-// LDS histogram of the tile keys of particles [p0, p1) into sh[K].
+// The particle range [p0, p1) of count/scatter workgroup `blk` of simulation s.
+__device__ __forceinline__ void block_range(const SimRef &s, uint32_t blk, uint32_t slice, uint32_t &p0,
+                                            uint32_t &p1) {
+  p0 = s.part0 + (blk - s.blk0) * slice;
+  p1 = min(s.part0 + s.n, p0 + slice);
+}
+
+// LDS histogram of the (simulation-local) tile keys of particles [p0, p1).
 template <bool REL>
 __device__ __forceinline__ void slice_histogram(const float *__restrict__ pos, uint32_t p0, uint32_t p1,
-                                                const TileGrid &t, uint32_t *sh) {
+                                                const SimRef &s, uint32_t *sh) {
   for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
     const uint32_t p = b0 + threadIdx.x;
     const bool active = p < p1;
     uint32_t key = 0xffffffffu;
     if (active) {
       int cx, cy, cz;
-      floor_cells<REL>(pos, p, t, cx, cy, cz);
-      key = tile_key(cx, cy, cz, t);
+      floor_cells<REL>(pos, p, p - s.part0, s.t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, s.t);
     }
     const unsigned long long act = __ballot(active);
     const unsigned long long heads = run_heads(key, active);
@@ -94,17 +103,20 @@ __device__ __forceinline__ void slice_histogram(const float *__restrict__ pos, u
   }
 }
 
-template <bool REL>
+template <bool REL, class G>
 __global__ void __launch_bounds__(kBinThreads)
-bin_count_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t, uint32_t *__restrict__ count) {
+bin_count_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ count) {
   extern __shared__ uint32_t sh[];
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) sh[k] = 0;
+  const SimRef &s = g.block_sim(blockIdx.x);
+  const uint32_t K = s.t.K, tile0 = s.tile0;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) sh[k] = 0;
   __syncthreads();
-  const uint32_t p0 = blockIdx.x * slice;
-  slice_histogram<REL>(pos, p0, min(n, p0 + slice), t, sh);
+  uint32_t p0, p1;
+  block_range(s, blockIdx.x, slice, p0, p1);
+  slice_histogram<REL>(pos, p0, p1, s, sh);
   __syncthreads();
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x)
-    if (sh[k]) atomicAdd(&count[k], sh[k]);
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    if (sh[k]) atomicAdd(&count[tile0 + k], sh[k]);
 }
 
 // ---------------------------------------------------------------- scan
@@ -212,20 +224,22 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
 
 // -------------------------------------------------------------- scatter
 
-template <bool REL>
+template <bool REL, class G>
 __global__ void __launch_bounds__(kBinThreads)
-bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, TileGrid t,
-                   uint32_t *__restrict__ cursor, float4 *__restrict__ spos) {
+bin_scatter_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ cursor,
+                   float4 *__restrict__ spos) {
   extern __shared__ uint32_t cur[];
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x) cur[k] = 0;
+  const SimRef &s = g.block_sim(blockIdx.x);
+  const uint32_t K = s.t.K, tile0 = s.tile0;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) cur[k] = 0;
   __syncthreads();
-  const uint32_t p0 = blockIdx.x * slice;
-  const uint32_t p1 = min(n, p0 + slice);
-  slice_histogram<REL>(pos, p0, p1, t, cur);
+  uint32_t p0, p1;
+  block_range(s, blockIdx.x, slice, p0, p1);
+  slice_histogram<REL>(pos, p0, p1, s, cur);
   __syncthreads();
   // reserve this workgroup's range in every tile it touches
-  for (uint32_t k = threadIdx.x; k < t.K; k += blockDim.x)
-    if (cur[k]) cur[k] = atomicAdd(&cursor[k], cur[k]);
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    if (cur[k]) cur[k] = atomicAdd(&cursor[tile0 + k], cur[k]);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
@@ -238,8 +252,8 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, Ti
       y = pos[3 * size_t(p) + 1];
       z = pos[3 * size_t(p) + 2];
       int cx, cy, cz;
-      floor_cells<REL>(pos, p, t, cx, cy, cz);
-      key = tile_key(cx, cy, cz, t);
+      floor_cells<REL>(pos, p, p - s.part0, s.t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, s.t);
     }
     const unsigned long long act = __ballot(active);
     const unsigned long long heads = run_heads(key, active);
@@ -257,10 +271,10 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t n, uint32_t slice, Ti
 
 // ----------------------------------------------------------------- accum
 
-template <bool REL>
+template <bool REL, class G>
 __global__ void __launch_bounds__(256)
 bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
-                 const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, TileGrid t,
+                 const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, G g, uint32_t M,
                  const float *__restrict__ wmax_dev, float wmax_host, uint32_t n, float *__restrict__ partial,
                  float *__restrict__ overflow) {
   __shared__ unsigned long long win[kWinCells];
@@ -268,11 +282,12 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
   const int ch = int(blockIdx.y);
   if (c >= *nchunks_total) return;
   const ChunkDesc d = desc[c];
-  const uint32_t txy = t.div_tz.div(d.tile);
-  const uint32_t itz = d.tile - txy * uint32_t(t.tz);
-  const uint32_t itx = t.div_ty.div(txy);
-  const uint32_t ity = txy - itx * uint32_t(t.ty);
-  const int ox = int(itx) * kTile, oy = int(ity) * kTile, oz = int(itz) * kTile;
+  const SimRef &sim = g.tile_sim(d.tile);
+  const TileGrid &t = sim.t;
+  const uint32_t part0 = sim.part0, cell0 = sim.cell0;
+  int itx, ity, itz;
+  tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
+  const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
   for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) win[i] = 0ull;
   const double scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
   __syncthreads();
@@ -283,8 +298,9 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
     float wx[2], wy[2], wz[2];
     if (REL) {
-      uint32_t r = t.div_nz.div(pidx);
-      uint32_t k = pidx - r * uint32_t(t.nz);
+      const uint32_t local = pidx - part0;
+      uint32_t r = t.div_nz.div(local);
+      uint32_t k = local - r * uint32_t(t.nz);
       uint32_t i = t.div_ny.div(r);
       uint32_t j = r - i * uint32_t(t.ny);
       const float px = float(i) + x, py = float(j) + y, pz = float(k) + z;
@@ -313,7 +329,7 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
       } else {
         // rounding put a relative corner outside the window: global path
         const uint32_t cell = (uint32_t(gx[bx]) * uint32_t(t.ny) + uint32_t(gy[by])) * uint32_t(t.nz) + uint32_t(gz[bz]);
-        atomicAdd(&overflow[size_t(ch) * t.nx * t.ny * t.nz + cell], v);
+        atomicAdd(&overflow[size_t(ch) * M + cell0 + cell], v);
       }
     }
   }
@@ -341,14 +357,18 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
 // coordinate is 0, the neighbour at -1 (periodic) contributes at its own
 // local index = its tile length.  The (up to 8) sources and each tile's
 // chunks are summed in a fixed order.
+template <class G>
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
-                  const uint32_t *__restrict__ nchunk, TileGrid t, int nc, const float *__restrict__ overflow,
+                  const uint32_t *__restrict__ nchunk, G g, uint32_t M, int nc, const float *__restrict__ overflow,
                   int has_overflow, const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n,
                   int accumulate, float *__restrict__ mesh) {
   const unsigned long long *ipart = reinterpret_cast<const unsigned long long *>(partial);
   const double inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
-  const uint32_t tile = blockIdx.x;
+  const SimRef &sim = g.tile_sim(blockIdx.x);
+  const TileGrid &t = sim.t;
+  const uint32_t tile0 = sim.tile0, cell0 = sim.cell0;
+  const uint32_t tile = blockIdx.x - tile0;
   const int ch = int(blockIdx.y);
   const uint32_t txy = t.div_tz.div(tile);
   const int itz = int(tile - txy * uint32_t(t.tz));
@@ -358,7 +378,6 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
   // lower neighbours (periodic) and their lengths
   const int px = itx ? itx - 1 : t.tx - 1, py = ity ? ity - 1 : t.ty - 1, pz = itz ? itz - 1 : t.tz - 1;
   const int PLx = tile_len(px, t.tx, t.nx), PLy = tile_len(py, t.ty, t.ny), PLz = tile_len(pz, t.tz, t.nz);
-  const uint32_t M = uint32_t(t.nx) * t.ny * t.nz;
   const int cells_in_tile = Lx * Ly * Lz;
   for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
     const int c = e % Lz;
@@ -372,7 +391,7 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
         for (int sz = 0; sz < nzs; ++sz) {
           const int jx = sx ? px : itx, jy = sy ? py : ity, jz = sz ? pz : itz;
           const int la = sx ? PLx : a, lb = sy ? PLy : b, lc = sz ? PLz : c;
-          const uint32_t src = (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
+          const uint32_t src = tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
           const int li = (la * kWin + lb) * kWin + lc;
           const uint32_t c0 = chunk_off[src], cn = nchunk[src];
           if (cn == 1) {
@@ -384,8 +403,8 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
           }
         }
     float v = acc;
-    const uint32_t cell = (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) * uint32_t(t.nz) +
-                          uint32_t(itz * kTile + c);
+    const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
+                                      uint32_t(t.nz) + uint32_t(itz * kTile + c);
     if (has_overflow) v += overflow[size_t(ch) * M + cell];
     float *dst = mesh + size_t(ch) * M + cell;
     *dst = accumulate ? *dst + v : v;
@@ -407,42 +426,55 @@ static int set_lds(K kernel, size_t bytes) {
   return JPM_OK;
 }
 
+// Sizes of a tile sort (batch totals): tiles, particles, count/scatter
+// workgroups of `slice` particles, chunk-table capacity, and the largest
+// per-simulation tile count (the LDS histogram size).
+struct SortDims {
+  uint32_t K, n, G, slice, maxchunks, maxK, B;
+};
+
+static SortDims one_dims(const TileGrid &t, int64_t n) {
+  SortDims d;
+  const uint64_t nn = uint64_t(n > 0 ? n : 1);
+  d.K = t.K;
+  d.n = uint32_t(n);
+  d.slice = kSlice;
+  d.G = uint32_t((nn + d.slice - 1) / d.slice);
+  d.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
+  d.maxK = t.K;
+  d.B = 1;
+  return d;
+}
+
 // Workspace of the tile sort (byte offsets).
 struct SortWs {
-  uint32_t G, slice, maxchunks;
   size_t count, tstart, cursor, nchunk, choff, sums, totals, desc, spos, total;
 };
 
-static SortWs sort_ws(const TileGrid &t, int64_t n) {
+static SortWs sort_ws(const SortDims &d) {
   SortWs w;
-  const uint64_t nn = uint64_t(n > 0 ? n : 1);
-  w.slice = 16384;
-  w.G = uint32_t((nn + w.slice - 1) / w.slice);
-  w.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
-  const size_t nsums = (size_t(t.K) + kScanBlock - 1) / kScanBlock + 1;
+  const size_t nn = size_t(d.n > 0 ? d.n : 1);
+  const size_t nsums = (size_t(d.K) + kScanBlock - 1) / kScanBlock + 1;
   size_t o = 0;
-  w.count = o;    o += align_up(size_t(t.K) * 4);
-  w.tstart = o;   o += align_up(size_t(t.K) * 4);
-  w.cursor = o;   o += align_up(size_t(t.K) * 4);
-  w.nchunk = o;   o += align_up(size_t(t.K) * 4);
-  w.choff = o;    o += align_up(size_t(t.K) * 4);
+  w.count = o;    o += align_up(size_t(d.K) * 4);
+  w.tstart = o;   o += align_up(size_t(d.K) * 4);
+  w.cursor = o;   o += align_up(size_t(d.K) * 4);
+  w.nchunk = o;   o += align_up(size_t(d.K) * 4);
+  w.choff = o;    o += align_up(size_t(d.K) * 4);
   w.sums = o;     o += align_up(nsums * 4);
   w.totals = o;   o += align_up(16);
-  w.desc = o;     o += align_up(size_t(w.maxchunks) * sizeof(ChunkDesc));
-  w.spos = o;     o += align_up(size_t(nn) * 16);
+  w.desc = o;     o += align_up(size_t(d.maxchunks) * sizeof(ChunkDesc));
+  w.spos = o;     o += align_up(nn * 16);
   w.total = o;
   return w;
 }
 
 // Workspace of the window accumulation that follows a sort.
-static size_t accum_ws(const TileGrid &t, int64_t n, int nc, uint32_t flags, size_t *partial, size_t *overflow,
+static size_t accum_ws(const SortDims &d, size_t M, int nc, bool rel, size_t *partial, size_t *overflow,
                        size_t *wmax) {
-  const uint64_t nn = uint64_t(n > 0 ? n : 1);
-  const uint32_t maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
-  const size_t M = size_t(t.nx) * t.ny * t.nz;
   size_t o = 0;
-  *partial = o;  o += align_up(size_t(maxchunks) * nc * kWinCells * 8);
-  *overflow = o; o += (flags & JPM_RELATIVE) ? align_up(size_t(nc) * M * 4) : 0;
+  *partial = o;  o += align_up(size_t(d.maxchunks) * nc * kWinCells * 8);
+  *overflow = o; o += rel ? align_up(size_t(nc) * M * 4) : 0;
   *wmax = o;     o += align_up(4);
   return o;
 }
@@ -453,13 +485,14 @@ bool binned_paint_supported(int nx, int ny, int nz) {
 
 size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return sort_ws(make_tiles(nx, ny, nz), n).total;
+  return sort_ws(one_dims(make_tiles(nx, ny, nz), n)).total;
 }
 
 size_t bins_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
   size_t a, b, c;
-  return accum_ws(make_tiles(nx, ny, nz), n, nc, flags, &a, &b, &c);
+  return accum_ws(one_dims(make_tiles(nx, ny, nz), n), size_t(cells(nx, ny, nz)), nc, flags & JPM_RELATIVE, &a, &b,
+                  &c);
 }
 
 size_t binned_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
@@ -467,22 +500,89 @@ size_t binned_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, ui
   return align_up(bin_sort_workspace_size(nx, ny, nz, n)) + bins_paint_workspace_size(nx, ny, nz, n, nc, flags);
 }
 
-// The Bins view of a sort already held in `ws` (same shape, count, flags).
-Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws) {
-  const TileGrid t = make_tiles(nx, ny, nz);
-  const SortWs w = sort_ws(t, n);
+template <class G>
+static BinsT<G> view_of(const G &g, const SortDims &d, uint32_t M, bool rel, const void *ws) {
+  const SortWs w = sort_ws(d);
   const char *base = static_cast<const char *>(ws);
-  Bins b;
-  b.t = t;
-  b.n = uint32_t(n);
-  b.maxchunks = w.maxchunks;
-  b.rel = flags & JPM_RELATIVE;
+  BinsT<G> b;
+  b.g = g;
+  b.K = d.K;
+  b.n = d.n;
+  b.M = M;
+  b.maxchunks = d.maxchunks;
+  b.rel = rel;
   b.spos = reinterpret_cast<const float4 *>(base + w.spos);
   b.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
   b.nchunk = reinterpret_cast<const uint32_t *>(base + w.nchunk);
   b.choff = reinterpret_cast<const uint32_t *>(base + w.choff);
   b.total_chunks = reinterpret_cast<const uint32_t *>(base + w.totals) + 1;
   return b;
+}
+
+// The Bins view of a sort already held in `ws` (same shape, count, flags).
+Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws) {
+  const TileGrid t = make_tiles(nx, ny, nz);
+  OneSim g{one_sim(nx, ny, nz, n)};
+  return view_of(g, one_dims(t, n), uint32_t(cells(nx, ny, nz)), flags & JPM_RELATIVE, ws);
+}
+
+// Sort the particles of every simulation into its tiles (workspace of
+// sort_ws(d).total bytes).
+template <class G>
+static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, bool rel, const float *pos, void *ws,
+                     size_t ws_bytes, BinsT<G> &out) {
+  const SortWs w = sort_ws(d);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "tile sort needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  JPM_REQUIRE(pos != nullptr, "positions are null");
+  JPM_REQUIRE(d.maxK <= uint32_t(kMaxKeysLds), "%u tiles in one mesh, above the binned paint's %d", d.maxK,
+              kMaxKeysLds);
+  char *base = static_cast<char *>(ws);
+  uint32_t *count = reinterpret_cast<uint32_t *>(base + w.count);
+  uint32_t *tstart = reinterpret_cast<uint32_t *>(base + w.tstart);
+  uint32_t *cursor = reinterpret_cast<uint32_t *>(base + w.cursor);
+  uint32_t *nchunk = reinterpret_cast<uint32_t *>(base + w.nchunk);
+  uint32_t *choff = reinterpret_cast<uint32_t *>(base + w.choff);
+  uint32_t *sums = reinterpret_cast<uint32_t *>(base + w.sums);
+  uint32_t *totals = reinterpret_cast<uint32_t *>(base + w.totals);  // [0] particles, [1] chunks
+  const size_t lds_hist = size_t(d.maxK) * 4;
+  int rc;
+
+  // 1. tile counts
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K);
+  JPM_LAUNCHED("zero_u32_kernel");
+  {
+    auto kern = rel ? bin_count_kernel<true, G> : bin_count_kernel<false, G>;
+    rc = set_lds(kern, lds_hist);
+    if (rc) return rc;
+    JPM_TIMED(s, "bin_count_kernel");
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, count);
+    JPM_LAUNCHED("bin_count_kernel");
+  }
+  // 2. tile starts, scatter cursors, chunk table
+  JPM_HIP(hipMemcpyAsync(tstart, count, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+  rc = exclusive_scan(s, tstart, d.K, sums, totals);
+  if (rc) return rc;
+  JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K, nchunk);
+  JPM_LAUNCHED("tile_chunks_kernel");
+  JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+  rc = exclusive_scan(s, choff, d.K, sums, totals + 1);
+  if (rc) return rc;
+  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, tstart, count, choff, d.K,
+                     reinterpret_cast<ChunkDesc *>(base + w.desc));
+  JPM_LAUNCHED("chunk_desc_kernel");
+  // 3. scatter particles into tile order
+  {
+    auto kern = rel ? bin_scatter_kernel<true, G> : bin_scatter_kernel<false, G>;
+    rc = set_lds(kern, lds_hist);
+    if (rc) return rc;
+    JPM_TIMED(s, "bin_scatter_kernel");
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, cursor,
+                       reinterpret_cast<float4 *>(base + w.spos));
+    JPM_LAUNCHED("bin_scatter_kernel");
+  }
+  out = view_of(g, d, M, rel, ws);
+  return JPM_OK;
 }
 
 // Sort n particles into tiles (workspace `ws` of bin_sort_workspace_size).
@@ -499,76 +599,21 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
   const bool rel = flags & JPM_RELATIVE;
   JPM_REQUIRE(!rel || n == M, "relative mode needs one particle per lattice node (n=%lld, mesh %lld)",
               (long long)n, (long long)M);
-  const SortWs w = sort_ws(t, n);
-  JPM_REQUIRE(ws && ws_bytes >= w.total, "tile sort needs %zu workspace bytes, got %zu", w.total, ws_bytes);
-  JPM_REQUIRE(pos != nullptr, "positions are null");
-  char *base = static_cast<char *>(ws);
-  uint32_t *count = reinterpret_cast<uint32_t *>(base + w.count);
-  uint32_t *tstart = reinterpret_cast<uint32_t *>(base + w.tstart);
-  uint32_t *cursor = reinterpret_cast<uint32_t *>(base + w.cursor);
-  uint32_t *nchunk = reinterpret_cast<uint32_t *>(base + w.nchunk);
-  uint32_t *choff = reinterpret_cast<uint32_t *>(base + w.choff);
-  uint32_t *sums = reinterpret_cast<uint32_t *>(base + w.sums);
-  uint32_t *totals = reinterpret_cast<uint32_t *>(base + w.totals);  // [0] particles, [1] chunks
-  const size_t lds_hist = size_t(t.K) * 4;
-
-  // 1. tile counts
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, count, t.K);
-  JPM_LAUNCHED("zero_u32_kernel");
-  {
-    auto kern = rel ? bin_count_kernel<true> : bin_count_kernel<false>;
-    rc = set_lds(kern, lds_hist);
-    if (rc) return rc;
-    JPM_TIMED(s, "bin_count_kernel");
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, count);
-    JPM_LAUNCHED("bin_count_kernel");
-  }
-  // 2. tile starts, scatter cursors, chunk table
-  JPM_HIP(hipMemcpyAsync(tstart, count, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
-  rc = exclusive_scan(s, tstart, t.K, sums, totals);
-  if (rc) return rc;
-  JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, count, t.K, nchunk);
-  JPM_LAUNCHED("tile_chunks_kernel");
-  JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(t.K) * 4, hipMemcpyDeviceToDevice, s));
-  rc = exclusive_scan(s, choff, t.K, sums, totals + 1);
-  if (rc) return rc;
-  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(t.K, 256)), dim3(256), 0, s, tstart, count, choff, t.K,
-                     reinterpret_cast<ChunkDesc *>(base + w.desc));
-  JPM_LAUNCHED("chunk_desc_kernel");
-  // 3. scatter particles into tile order
-  {
-    auto kern = rel ? bin_scatter_kernel<true> : bin_scatter_kernel<false>;
-    rc = set_lds(kern, lds_hist);
-    if (rc) return rc;
-    JPM_TIMED(s, "bin_scatter_kernel");
-    hipLaunchKernelGGL(kern, dim3(w.G), dim3(kBinThreads), lds_hist, s, pos, uint32_t(n), w.slice, t, cursor,
-                       reinterpret_cast<float4 *>(base + w.spos));
-    JPM_LAUNCHED("bin_scatter_kernel");
-  }
-  out.t = t;
-  out.n = uint32_t(n);
-  out.maxchunks = w.maxchunks;
-  out.rel = rel;
-  out.spos = reinterpret_cast<const float4 *>(base + w.spos);
-  out.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
-  out.nchunk = nchunk;
-  out.choff = choff;
-  out.total_chunks = totals + 1;
-  return JPM_OK;
+  OneSim g{one_sim(nx, ny, nz, n)};
+  return sort_into(s, g, one_dims(t, n), uint32_t(M), rel, pos, ws, ws_bytes, out);
 }
 
-// nc meshes (stride M) = [mesh +] window sums of the sorted particles, channel
-// c with weight[p*nc + c] (or wscalar when weight is NULL).
-int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
-               void *ws, size_t ws_bytes) {
+// nc meshes (stride b.M) = [mesh +] window sums of the sorted particles,
+// channel c with weight[p*nc + c] (or wscalar when weight is NULL).
+template <class G>
+static int paint_sorted(hipStream_t s, const BinsT<G> &b, const SortDims &d, const float *weight, float wscalar,
+                        int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
   JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
   JPM_REQUIRE(mesh != nullptr, "mesh is null");
-  const TileGrid &t = b.t;
-  const int64_t M = int64_t(t.nx) * t.ny * t.nz;
+  const int64_t M = b.M;
   JPM_REQUIRE(int64_t(nc) * M <= (int64_t(1) << 32), "mesh too large");
   size_t opart, oover, owmax;
-  const size_t need = accum_ws(t, b.n, nc, b.rel ? JPM_RELATIVE : 0u, &opart, &oover, &owmax);
+  const size_t need = accum_ws(d, size_t(M), nc, b.rel, &opart, &oover, &owmax);
   JPM_REQUIRE(ws && ws_bytes >= need, "binned paint needs %zu workspace bytes, got %zu", need, ws_bytes);
   const bool accumulate = !(flags & JPM_ZERO_MESH);
   char *base = static_cast<char *>(ws);
@@ -587,18 +632,24 @@ int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar,
   float *overflow = reinterpret_cast<float *>(base + oover);
   {
     JPM_TIMED(s, nc == 1 ? "bin_accum_kernel[nc=1]" : "bin_accum_kernel[nc=3]");
-    hipLaunchKernelGGL(b.rel ? bin_accum_kernel<true> : bin_accum_kernel<false>, dim3(b.maxchunks, nc), dim3(256), 0,
-                       s, b.spos, weight, nc, wscalar, b.desc, b.total_chunks, t, wmax_dev, wmax_host, b.n, partial,
-                       overflow);
+    auto kern = b.rel ? bin_accum_kernel<true, G> : bin_accum_kernel<false, G>;
+    hipLaunchKernelGGL(kern, dim3(b.maxchunks, nc), dim3(256), 0, s, b.spos, weight, nc, wscalar, b.desc,
+                       b.total_chunks, b.g, b.M, wmax_dev, wmax_host, b.n, partial, overflow);
   }
   JPM_LAUNCHED("bin_accum_kernel");
   {
     JPM_TIMED(s, nc == 1 ? "bin_reduce_kernel[nc=1]" : "bin_reduce_kernel[nc=3]");
-    hipLaunchKernelGGL(bin_reduce_kernel, dim3(t.K, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, t, nc, overflow,
+    auto kern = bin_reduce_kernel<G>;
+    hipLaunchKernelGGL(kern, dim3(b.K, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
                        int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh);
   }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;
+}
+
+int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
+               void *ws, size_t ws_bytes) {
+  return paint_sorted(s, b, one_dims(b.g.s.t, b.n), weight, wscalar, nc, mesh, flags, ws, ws_bytes);
 }
 
 int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
@@ -621,6 +672,42 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
   rc = bin_sort(s, nx, ny, nz, pos, n, flags, ws, sort_bytes, b);
   if (rc) return rc;
   return bins_paint(s, b, weight, wscalar, nc, mesh, flags, static_cast<char *>(ws) + sort_bytes, ws_bytes - sort_bytes);
+}
+
+// ------------------------------------------------------------- batches
+
+static SortDims batch_dims(const SimBatch &bt) {
+  SortDims d;
+  d.K = bt.K;
+  d.n = bt.N;
+  d.slice = kSlice;
+  d.G = bt.G;
+  d.maxchunks = uint32_t((uint64_t(bt.N) + kChunk - 1) / kChunk + bt.K + uint32_t(bt.B));
+  d.maxK = bt.maxK;
+  d.B = uint32_t(bt.B);
+  return d;
+}
+
+size_t batch_sort_workspace_size(const SimBatch &bt) { return sort_ws(batch_dims(bt)).total; }
+
+size_t batch_paint_workspace_size(const SimBatch &bt, int nc) {
+  size_t a, b, c;
+  return accum_ws(batch_dims(bt), bt.M, nc, bt.rel, &a, &b, &c);
+}
+
+int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out) {
+  SimTable g{bt.dev, bt.B};
+  return sort_into(s, g, batch_dims(bt), bt.M, bt.rel, pos, ws, ws_bytes, out);
+}
+
+BatchBins batch_view(const SimBatch &bt, const void *ws) {
+  SimTable g{bt.dev, bt.B};
+  return view_of(g, batch_dims(bt), bt.M, bt.rel, ws);
+}
+
+int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
+                void *ws, size_t ws_bytes) {
+  return paint_sorted(s, b, batch_dims(bt), weight, 1.0f, nc, mesh, JPM_ZERO_MESH, ws, ws_bytes);
 }
 
 }  // namespace jpm

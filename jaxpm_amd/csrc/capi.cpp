@@ -74,6 +74,17 @@ size_t linear_field_table_vjp_ws(int);
 int kspace_linear_field_table_vjp(hipStream_t, int, int, int, const void *, const void *, float, float, float,
                                   const float *, const float *, int, float, double *, void *, size_t);
 size_t dot_ws();
+int batch_create(int, const int *, const int64_t *, uint32_t, SimBatch **);
+int batch_destroy(SimBatch *);
+size_t batch_kick_ws(const SimBatch &);
+size_t batch_vjp_ws(const SimBatch &);
+int batch_force_kick(hipStream_t, const SimBatch &, const float *, float *, const float *, float *, float, uint32_t,
+                     int, void *, size_t, void *, size_t);
+int batch_force_vjp(hipStream_t, const SimBatch &, const float *, const float *, const float *, const float *,
+                    const float *, float *, float, uint32_t, void *, size_t, const void *, size_t);
+int batch_axpy(hipStream_t, const SimBatch &, const float *, const float *, float *);
+size_t batch_dot_ws(const SimBatch &);
+int batch_dot(hipStream_t, const SimBatch &, const float *, const float *, double *, void *, size_t);
 int dot(hipStream_t, int64_t, const float *, const float *, double *, void *, size_t);
 int power_spectrum(hipStream_t, int, int, int, const void *, const void *, double, double, double, const double *,
                    int, const int *, int, double, double, double, double *, void *, size_t);
@@ -584,6 +595,46 @@ int jpm_kspace_linear_field_table_vjp_f32(void *stream, int nx, int ny, int nz, 
 }
 
 size_t jpm_dot_workspace_size(void) { return dot_ws(); }
+
+int jpm_batch_create(int nsims, const int *dims, const int64_t *counts, uint32_t flags, void **batch) {
+  JPM_GUARD(batch_create(nsims, dims, counts, flags, reinterpret_cast<SimBatch **>(batch)));
+}
+
+int jpm_batch_destroy(void *batch) { JPM_GUARD(batch_destroy(static_cast<SimBatch *>(batch))); }
+
+static const SimBatch *as_batch(const void *b) { return static_cast<const SimBatch *>(b); }
+
+size_t jpm_batch_sort_size(const void *batch) { return batch ? batch_sort_workspace_size(*as_batch(batch)) : 0; }
+size_t jpm_batch_force_kick_workspace_size(const void *batch) { return batch ? batch_kick_ws(*as_batch(batch)) : 0; }
+size_t jpm_batch_force_vjp_workspace_size(const void *batch) { return batch ? batch_vjp_ws(*as_batch(batch)) : 0; }
+size_t jpm_batch_dot_workspace_size(const void *batch) { return batch ? batch_dot_ws(*as_batch(batch)) : 0; }
+
+int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos, float *vel, const float *gamma,
+                             float *phi, float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes,
+                             void *keep_sort, size_t keep_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, write_forces,
+                             ws, ws_bytes, keep_sort, keep_bytes));
+}
+
+int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi, const float *pos, const float *cot,
+                            const float *scale, const float *pot_scale, float *gpos, float r_split, uint32_t kflags,
+                            void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_force_vjp(as_stream(stream), *as_batch(batch), phi, pos, cot, scale, pot_scale, gpos, r_split,
+                            kflags, ws, ws_bytes, sorted, sorted_bytes));
+}
+
+int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a, const float *x, float *y) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_axpy(as_stream(stream), *as_batch(batch), a, x, y));
+}
+
+int jpm_batch_dot_f32(void *stream, const void *batch, const float *x, const float *y, double *out, void *ws,
+                      size_t ws_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_dot(as_stream(stream), *as_batch(batch), x, y, out, ws, ws_bytes));
+}
 
 int jpm_dot_f32(void *stream, int64_t n, const float *x, const float *y, double *out, void *ws, size_t ws_bytes) {
   JPM_GUARD(dot(as_stream(stream), n, x, y, out, ws, ws_bytes));

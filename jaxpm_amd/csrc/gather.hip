@@ -25,10 +25,11 @@ struct CornerSet {
 };
 
 template <bool REL>
-__device__ __forceinline__ void corner_set(float4 rec, const TileGrid &t, int ox, int oy, int oz, CornerSet &cs) {
+__device__ __forceinline__ void corner_set(float4 rec, const TileGrid &t, uint32_t part0, int ox, int oy, int oz,
+                                           CornerSet &cs) {
   const float x = rec.x, y = rec.y, z = rec.z;
   if (REL) {
-    const uint32_t pidx = __float_as_uint(rec.w);
+    const uint32_t pidx = __float_as_uint(rec.w) - part0;
     uint32_t r = t.div_nz.div(pidx);
     uint32_t k = pidx - r * uint32_t(t.nz);
     uint32_t i = t.div_ny.div(r);
@@ -111,24 +112,30 @@ __device__ __forceinline__ void stencil_global(const float *__restrict__ phi, co
   F[2] = -(kC8 * (at(x, y, z + 1) - at(x, y, z - 1)) - kC1 * (at(x, y, z + 2) - at(x, y, z - 2)));
 }
 
-// MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F.
-template <bool REL, int MODE>
+// MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F.  gamma is
+// per simulation (gamma_dev[sim]) when gamma_dev is given.
+template <bool REL, int MODE, class G>
 __global__ void __launch_bounds__(256)
-bin_kick_kernel(Bins bins, const float *__restrict__ phi, float gamma, float *__restrict__ vel) {
+bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
+                float *__restrict__ vel) {
   __shared__ float win[kPwCells];
   const uint32_t c = blockIdx.x;
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
-  const TileGrid &t = bins.t;
+  const SimRef &sim = bins.g.tile_sim(d.tile);
+  const TileGrid &t = sim.t;
+  const uint32_t part0 = sim.part0;
+  const float *phi = phi_all + sim.cell0;
+  const float gamma = gamma_dev ? gamma_dev[sim.idx] : gamma_s;
   int itx, ity, itz;
-  tile_origin(d.tile, t, itx, ity, itz);
+  tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
     const float4 rec = bins.spos[d.start + q];
     CornerSet cs;
-    corner_set<REL>(rec, t, ox, oy, oz, cs);
+    corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     float s[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
@@ -159,30 +166,35 @@ bin_kick_kernel(Bins bins, const float *__restrict__ phi, float gamma, float *__
   }
 }
 
-// gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx.
-template <bool REL>
+// gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx, scale
+// per simulation (scale_dev[sim]) when scale_dev is given.
+template <bool REL, class G>
 __global__ void __launch_bounds__(256)
-bin_grad_kernel(Bins bins, const float *__restrict__ phi, const float *__restrict__ rho_bar,
-                const float *__restrict__ cot, float scale, float *__restrict__ gpos) {
+bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
+                const float *__restrict__ cot, float scale_s, const float *__restrict__ scale_dev,
+                float *__restrict__ gpos) {
   __shared__ float win[kPwCells];
   __shared__ float rwin[kWinCells];
   const uint32_t c = blockIdx.x;
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
-  const TileGrid &t = bins.t;
+  const SimRef &sim = bins.g.tile_sim(d.tile);
+  const TileGrid &t = sim.t;
+  const uint32_t part0 = sim.part0;
+  const float *phi = phi_all + sim.cell0;
+  const float *rho_bar = rho_all + sim.cell0;
+  const float scale = scale_dev ? scale_dev[sim.idx] : scale_s;
   int itx, ity, itz;
-  tile_origin(d.tile, t, itx, ity, itz);
+  tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
   __syncthreads();
-  const uint32_t M = uint32_t(t.nx) * t.ny * t.nz;
-  (void)M;
   for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
     const float4 rec = bins.spos[d.start + q];
     const uint32_t idx = __float_as_uint(rec.w);
     CornerSet cs;
-    corner_set<REL>(rec, t, ox, oy, oz, cs);
+    corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     const float u0 = cot[3 * size_t(idx)], u1 = cot[3 * size_t(idx) + 1], u2 = cot[3 * size_t(idx) + 2];
     float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -217,36 +229,58 @@ bin_grad_kernel(Bins bins, const float *__restrict__ phi, const float *__restric
   }
 }
 
-int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces) {
+template <class G>
+static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float gamma, const float *gamma_dev,
+                       float *vel, int write_forces) {
   JPM_REQUIRE(phi && vel, "null pointer argument");
   const dim3 grid(b.maxchunks), block(256);
+  JPM_TIMED(s, "bin_kick_kernel");
   if (b.rel) {
-    if (write_forces) { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<true, 1>), grid, block, 0, s, b, phi, gamma, vel); }
-    else { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<true, 0>), grid, block, 0, s, b, phi, gamma, vel); }
+    if (write_forces)
+      hipLaunchKernelGGL((bin_kick_kernel<true, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+    else
+      hipLaunchKernelGGL((bin_kick_kernel<true, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
   } else {
-    if (write_forces) { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<false, 1>), grid, block, 0, s, b, phi, gamma, vel); }
-    else { JPM_TIMED(s, "bin_kick_kernel"); hipLaunchKernelGGL((bin_kick_kernel<false, 0>), grid, block, 0, s, b, phi, gamma, vel); }
+    if (write_forces)
+      hipLaunchKernelGGL((bin_kick_kernel<false, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+    else
+      hipLaunchKernelGGL((bin_kick_kernel<false, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
   }
   JPM_LAUNCHED("bin_kick_kernel");
   return JPM_OK;
 }
 
-int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
-              float *gpos) {
+template <class G>
+static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const float *rho_bar, const float *cot,
+                       float scale, const float *scale_dev, float *gpos) {
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
   const dim3 grid(b.maxchunks), block(256);
+  JPM_TIMED(s, "bin_grad_kernel");
   if (b.rel)
-    {
-      JPM_TIMED(s, "bin_grad_kernel");
-      hipLaunchKernelGGL(bin_grad_kernel<true>, grid, block, 0, s, b, phi, rho_bar, cot, scale, gpos);
-    }
+    hipLaunchKernelGGL((bin_grad_kernel<true, G>), grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
   else
-    {
-      JPM_TIMED(s, "bin_grad_kernel");
-      hipLaunchKernelGGL(bin_grad_kernel<false>, grid, block, 0, s, b, phi, rho_bar, cot, scale, gpos);
-    }
+    hipLaunchKernelGGL((bin_grad_kernel<false, G>), grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
   JPM_LAUNCHED("bin_grad_kernel");
   return JPM_OK;
+}
+
+int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float *gamma_dev, float *vel,
+               int write_forces) {
+  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, write_forces);
+}
+
+int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
+               const float *scale_dev, float *gpos) {
+  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos);
+}
+
+int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces) {
+  return kick_sorted(s, b, phi, gamma, nullptr, vel, write_forces);
+}
+
+int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
+              float *gpos) {
+  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos);
 }
 
 }  // namespace jpm

@@ -150,7 +150,12 @@ force_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid h
 // stencil then takes the gradient).
 __global__ void __launch_bounds__(256)
 potential_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid hg, Tabs t, float r2,
-                 float scale, int deconv) {
+                 float scale_s, const float *__restrict__ scale_dev, int deconv) {
+  // blockIdx.y: spectrum of a batch of equal shapes (stride H), scaled by
+  // scale_dev[y] when given
+  dk += size_t(blockIdx.y) * hg.H;
+  out += size_t(blockIdx.y) * hg.H;
+  const float scale = scale_dev ? scale_dev[blockIdx.y] : scale_s;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
     uint32_t i, j, k;
     hg.coords(e, i, j, k);
@@ -480,23 +485,29 @@ int kspace_force(hipStream_t s, int nx, int ny, int nz, const void *dk, void *ou
   return JPM_OK;
 }
 
-int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void *out, float r_split, float scale,
-                     uint32_t flags) {
+int kspace_potential_batch(hipStream_t s, int nx, int ny, int nz, int batch, const void *dk, void *out,
+                           float r_split, float scale, const float *scale_dev, uint32_t flags) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(dk && out, "null spectrum pointer");
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d spectra", batch);
   Tabs t;
   rc = get_tabs(s, nx, ny, nz, flags, t);
   if (rc) return rc;
   HalfGrid hg = make_half(nx, ny, nz);
   {
     JPM_TIMED(s, "potential_kernel");
-    hipLaunchKernelGGL(potential_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
+    hipLaunchKernelGGL(potential_kernel, dim3(grid_for(hg.H, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
                        static_cast<const float2 *>(dk), static_cast<float2 *>(out), hg, t, r_split * r_split, scale,
-                       int((flags & JPM_K_DECONV) != 0));
+                       scale_dev, int((flags & JPM_K_DECONV) != 0));
   }
   JPM_LAUNCHED("potential_kernel");
   return JPM_OK;
+}
+
+int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void *out, float r_split, float scale,
+                     uint32_t flags) {
+  return kspace_potential_batch(s, nx, ny, nz, 1, dk, out, r_split, scale, nullptr, flags);
 }
 
 int kspace_compensate(hipStream_t s, int nx, int ny, int nz, void *fk, float scale) {

@@ -63,10 +63,14 @@ fd4_gradient_kernel(const float *__restrict__ phi, Mesh3 g, float scale, float *
 // Adjoint of fd4_gradient_kernel: out = scale * sum_d (D_d g_d), since
 // (-D_d)^T = D_d.
 __global__ void __launch_bounds__(256)
-fd4_divergence_kernel(const float *__restrict__ g3, Mesh3 g, float scale, float *__restrict__ out) {
+fd4_divergence_kernel(const float *__restrict__ g3, size_t chan_stride, Mesh3 g, float scale, float *__restrict__ out) {
+  // blockIdx.y: mesh of a batch of equal shapes (stride M); the three input
+  // channels lie chan_stride floats apart
   const uint32_t M = g.nx * g.ny * g.nz;
+  g3 += size_t(blockIdx.y) * M;
+  out += size_t(blockIdx.y) * M;
   const float c8 = scale * (8.0f / 12.0f), c1 = scale * (1.0f / 12.0f);
-  const float *gx = g3, *gy = g3 + M, *gz = g3 + 2 * size_t(M);
+  const float *gx = g3, *gy = g3 + chan_stride, *gz = g3 + 2 * chan_stride;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
     const uint32_t r = g.div_nz.div(e);
     const uint32_t k = e - r * g.nz;
@@ -110,19 +114,25 @@ int fd4_gradient(hipStream_t s, int nx, int ny, int nz, const float *phi, float 
   return JPM_OK;
 }
 
-int fd4_divergence(hipStream_t s, int nx, int ny, int nz, const float *g3, float scale, float *out) {
+int fd4_divergence_batch(hipStream_t s, int nx, int ny, int nz, int batch, const float *g3, size_t chan_stride,
+                         float scale, float *out) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(g3 && out, "null pointer argument");
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d meshes", batch);
   const int64_t m = cells(nx, ny, nz);
   JPM_REQUIRE(3 * m <= (int64_t(1) << 32), "mesh too large");
   {
     JPM_TIMED(s, "fd4_divergence_kernel");
-    hipLaunchKernelGGL(fd4_divergence_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, g3,
-                       make_mesh3(nx, ny, nz), scale, out);
+    hipLaunchKernelGGL(fd4_divergence_kernel, dim3(grid_for(m, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
+                       g3, chan_stride, make_mesh3(nx, ny, nz), scale, out);
   }
   JPM_LAUNCHED("fd4_divergence_kernel");
   return JPM_OK;
+}
+
+int fd4_divergence(hipStream_t s, int nx, int ny, int nz, const float *g3, float scale, float *out) {
+  return fd4_divergence_batch(s, nx, ny, nz, 1, g3, size_t(cells(nx, ny, nz)), scale, out);
 }
 
 }  // namespace jpm

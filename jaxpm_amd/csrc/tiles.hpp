@@ -2,6 +2,8 @@
 // gathers (16^3-cell tiles; particles sorted by the tile of their floor cell).
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 namespace jpm {
@@ -11,6 +13,7 @@ constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
 constexpr int kWinCells = kWin * kWin * kWin;   // 4913
 constexpr int kChunk = 8192;                    // max particles per chunk
 constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
+constexpr uint32_t kSlice = 16384;              // particles per count / scatter workgroup
 
 struct TileGrid {
   int nx, ny, nz;   // mesh
@@ -64,21 +67,95 @@ struct ChunkDesc {
   uint32_t tile, start, count, pad;
 };
 
-// The result of a tile sort: particles as float4 (x, y, z, index bits)
-// grouped by tile, chunked into runs of at most kChunk particles of one tile.
-struct Bins {
+// One simulation as the tile-sort kernels see it.  A batch concatenates its
+// simulations: particles [part0, part0 + n) of the positions, tiles
+// [tile0, tile0 + t.K) of the tile arrays, cells [cell0, cell0 + M) of the
+// meshes, count/scatter workgroups from blk0.
+struct SimRef {
   TileGrid t;
-  uint32_t n, maxchunks;
+  uint32_t tile0, part0, n, cell0, blk0, idx;
+};
+
+// A single simulation: the table is one entry held by value.
+struct OneSim {
+  SimRef s;
+  __device__ __forceinline__ const SimRef &tile_sim(uint32_t) const { return s; }
+  __device__ __forceinline__ const SimRef &block_sim(uint32_t) const { return s; }
+};
+
+// A batch: a device table of B entries, ordered by tile0 and blk0, looked up
+// by binary search (the lookups are uniform across a workgroup).
+struct SimTable {
+  const SimRef *sims;
+  int B;
+  __device__ __forceinline__ const SimRef &tile_sim(uint32_t tile) const {
+    int lo = 0, hi = B - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sims[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+    }
+    return sims[lo];
+  }
+  __device__ __forceinline__ const SimRef &block_sim(uint32_t blk) const {
+    int lo = 0, hi = B - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sims[mid].blk0 <= blk) lo = mid; else hi = mid - 1;
+    }
+    return sims[lo];
+  }
+};
+
+// The result of a tile sort: particles as float4 (x, y, z, global index
+// bits) grouped by tile, chunked into runs of at most kChunk particles of one
+// tile.  K, n and M are the batch totals.
+template <class G>
+struct BinsT {
+  G g;
+  uint32_t K, n, M, maxchunks;
   bool rel;
   const float4 *spos;
   const ChunkDesc *desc;
   const uint32_t *nchunk, *choff, *total_chunks;
 };
+using Bins = BinsT<OneSim>;
+using BatchBins = BinsT<SimTable>;
+
+// A batch of simulations: the host description and the device table.
+// Simulations of one mesh shape are contiguous (a "class"), so each class's
+// meshes and half spectra form one strided batch for hipFFT.
+struct SimClass {
+  int nx, ny, nz, first, count;
+  uint32_t cell0, half0;
+};
+
+struct SimBatch {
+  int B = 0;
+  bool rel = false;
+  uint32_t K = 0, N = 0, M = 0, H = 0, G = 0, maxK = 0;
+  std::vector<SimRef> host;
+  std::vector<SimClass> classes;
+  SimRef *dev = nullptr;
+};
+
+inline SimRef one_sim(int nx, int ny, int nz, int64_t n) {
+  SimRef r;
+  r.t = make_tiles(nx, ny, nz);
+  r.tile0 = r.part0 = r.cell0 = r.blk0 = r.idx = 0;
+  r.n = uint32_t(n);
+  return r;
+}
 
 int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
              size_t ws_bytes, Bins &out);
 size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n);
 Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws);
+size_t batch_sort_workspace_size(const SimBatch &bt);
+size_t batch_paint_workspace_size(const SimBatch &bt, int nc);
+int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out);
+BatchBins batch_view(const SimBatch &bt, const void *ws);
+int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
+                void *ws, size_t ws_bytes);
 
 // One corner axis of a particle relative to its tile origin: local window
 // index (or -1 when the corner is dropped / outside the window) and weight.

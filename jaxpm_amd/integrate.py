@@ -293,3 +293,190 @@ def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, dete
     x, v = NBody.apply(x0, v0, coeffs, st, checkpoint_every)
     shape = to_device(pos).shape
     return x.reshape(shape), v.reshape(shape)
+
+
+# ------------------------------------------------------------------ batches
+
+class BatchStepper:
+    """A batch of simulations of different meshes and particle counts, stepped
+    by one launch per stage (``jpm_batch_*``).  Simulations of one mesh shape
+    must be contiguous; arrays are the concatenation of the simulations in
+    order: particles [N, 3], meshes [M]."""
+
+    def __init__(self, mesh_shapes, counts, relative=False, r_split=0.0, kflags=0):
+        import ctypes
+        self.shapes = [tuple(int(s) for s in sh) for sh in mesh_shapes]
+        self.counts = [int(c) for c in counts]
+        if len(self.shapes) != len(self.counts) or not self.shapes:
+            raise ValueError("one particle count per mesh shape")
+        self.B = len(self.shapes)
+        self.relative = bool(relative)
+        self.r_split = float(r_split)
+        self.kflags = int(kflags)
+        self.cells = [a * b * c for a, b, c in self.shapes]
+        self.N = sum(self.counts)
+        self.M = sum(self.cells)
+        self.part0 = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
+        self.cell0 = np.concatenate([[0], np.cumsum(self.cells)[:-1]]).astype(np.int64)
+        self.dev = _lib.device()
+        dims = (ctypes.c_int * (3 * self.B))(*[d for sh in self.shapes for d in sh])
+        cnts = (ctypes.c_int64 * self.B)(*self.counts)
+        h = ctypes.c_void_p()
+        _lib.load()
+        call("jpm_batch_create", self.B, dims, cnts, _lib.RELATIVE if relative else 0, ctypes.byref(h))
+        self.handle = h
+        self._destroy = _lib.load().jpm_batch_destroy
+        self.sort_bytes = size("jpm_batch_sort_size", h)
+        wb = max(size("jpm_batch_force_kick_workspace_size", h), size("jpm_batch_force_vjp_workspace_size", h))
+        self.ws = torch.empty(wb, dtype=torch.uint8, device=self.dev)
+        self.dot_ws = torch.empty(size("jpm_batch_dot_workspace_size", h), dtype=torch.uint8, device=self.dev)
+        self.trace = None
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self._destroy(h)
+            self.handle = None
+
+    def new_field(self, keep_sort=False):
+        pot = torch.empty(self.M, dtype=torch.float32, device=self.dev)
+        srt = torch.empty(self.sort_bytes, dtype=torch.uint8, device=self.dev) if keep_sort else None
+        return (pot, srt)
+
+    def _kick(self, x, v, gamma, field, write):
+        buf, srt = field
+        call("jpm_batch_force_kick_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(buf), self.r_split,
+             self.kflags, int(write), ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+
+    def field_kick(self, x, v, gamma, field):
+        """v_b += gamma[b] F_b(x_b) for every simulation; gamma a float32
+        device array [B]."""
+        self._kick(x, v, gamma, field, False)
+
+    def forces(self, x, out, field, ones):
+        """out = F(x) (``ones`` a float32 device array of 1.0, [B])."""
+        self._kick(x, out, ones, field, True)
+
+    def vjp_field(self, field, x, cot, scale, pot_scale, gpos):
+        buf, srt = field
+        call("jpm_batch_force_vjp_f32", stream(), self.handle, ptr(buf), ptr(x), ptr(cot), ptr(scale),
+             ptr(pot_scale), ptr(gpos), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(), ptr(srt),
+             0 if srt is None else srt.numel())
+
+    def axpy(self, a, x, y):
+        call("jpm_batch_axpy_f32", stream(), self.handle, ptr(a), ptr(x), ptr(y))
+
+    def dot_into(self, out, x, y):
+        """out[:] (float64 device, [B]) = per-simulation <x_b, y_b>."""
+        call("jpm_batch_dot_f32", stream(), self.handle, ptr(x), ptr(y), ptr(out), ptr(self.dot_ws),
+             self.dot_ws.numel())
+
+
+class BatchSchedule:
+    """Per-simulation leapfrog coefficients as the float32 device arrays the
+    batched kernels read: k0 [B], beta [S, B], gamma [S, B], and for the
+    adjoint pot_scale [S, B] = float32(gamma / M_b) and k0 / M_b."""
+
+    def __init__(self, coeffs, cells, dev):
+        c = coeffs.detach().to(torch.float64).cpu().numpy()
+        if c.ndim != 2 or c.shape[1] < 3 or c.shape[1] % 2 == 0:
+            raise ValueError("batched leapfrog coefficients must be [B, 1 + 2 S]")
+        B = c.shape[0]
+        S = (c.shape[1] - 1) // 2
+        self.S = S
+        f32 = c.astype(np.float32)
+        m = np.asarray(cells, np.float64)
+        self.k0 = torch.as_tensor(f32[:, 0].copy(), device=dev)
+        self.beta = torch.as_tensor(np.ascontiguousarray(f32[:, 1:1 + S].T), device=dev)
+        self.gamma = torch.as_tensor(np.ascontiguousarray(f32[:, 1 + S:].T), device=dev)
+        self.gamma_pot = torch.as_tensor(
+            np.ascontiguousarray((f32[:, 1 + S:].astype(np.float64) / m[:, None]).astype(np.float32).T), device=dev)
+        self.k0_pot = torch.as_tensor((f32[:, 0].astype(np.float64) / m).astype(np.float32), device=dev)
+        self.zeros = torch.zeros(B, dtype=torch.float32, device=dev)
+        self.ones = torch.ones(B, dtype=torch.float32, device=dev)
+
+
+def _run_batch(st, sc, x, v, start, stop, phi, keep=None, forces=False):
+    for i in range(start, stop):
+        st.axpy(sc.beta[i], v, x)
+        if keep is not None:
+            kp = keep[i - start]
+            kp[0].copy_(x)
+            phi = kp[1]
+            if forces:
+                kp[2].copy_(v)
+                st.forces(x, kp[3], phi, sc.ones)
+                st.axpy(sc.gamma[i], kp[3], v)
+                continue
+        st.field_kick(x, v, sc.gamma[i], phi)
+
+
+class BatchNBody(torch.autograd.Function):
+    """(x0, v0, c) -> (xS, vS) for a batch of simulations by the leapfrog,
+    with the checkpointed hand-written adjoint of :class:`NBody`; ``c`` is
+    the float64 host array [B, 1 + 2 S] of each simulation's [k0, beta,
+    gamma] and receives per-simulation cotangents."""
+
+    @staticmethod
+    def forward(ctx, x0, v0, coeffs, stepper, checkpoint_every):
+        st = stepper
+        sc = BatchSchedule(coeffs, st.cells, st.dev)
+        S = sc.S
+        x = x0.detach().clone().contiguous()
+        v = v0.detach().clone().contiguous()
+        phi = st.new_field()
+        st.field_kick(x, v, sc.k0, phi)
+        k = S if checkpoint_every is None else max(1, int(checkpoint_every))
+        ckpts = {0: (x.clone(), v.clone())}
+        for s0 in range(0, S, k):
+            _run_batch(st, sc, x, v, s0, min(S, s0 + k), phi)
+            if s0 + k < S:
+                ckpts[s0 + k] = (x.clone(), v.clone())
+        ctx.st, ctx.sc, ctx.k, ctx.ckpts = st, sc, k, ckpts
+        ctx.c_meta = (coeffs.dtype, coeffs.device, tuple(coeffs.shape))
+        return x, v
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        st, sc = ctx.st, ctx.sc
+        S = sc.S
+        want_c = ctx.needs_input_grad[2]
+        x0 = ctx.ckpts[0][0]
+        lx = (gx if gx is not None else torch.zeros_like(x0)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(x0)).detach().clone().contiguous()
+        dots = torch.zeros((1 + 2 * S, st.B), dtype=torch.float64, device=x0.device) if want_c else None
+        phi = st.new_field()
+
+        def buffers():
+            b = [torch.empty_like(x0), st.new_field(keep_sort=True)]
+            if want_c:
+                b += [torch.empty_like(x0), torch.empty_like(x0)]
+            return b
+
+        keep = [buffers() for _ in range(min(ctx.k, S))]
+        for s0 in reversed(range(0, S, ctx.k)):
+            s1 = min(S, s0 + ctx.k)
+            xc, vc = ctx.ckpts[s0]
+            x, v = xc.clone(), vc.clone()
+            _run_batch(st, sc, x, v, s0, s1, phi, keep, forces=want_c)
+            for i in reversed(range(s0, s1)):
+                kp = keep[i - s0]
+                if want_c:
+                    st.dot_into(dots[1 + S + i], lv, kp[3])
+                st.vjp_field(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], lx)
+                if want_c:
+                    st.dot_into(dots[1 + i], lx, kp[2])
+                st.axpy(sc.beta[i], lx, lv)
+        del keep
+        if want_c:
+            f0 = torch.empty_like(x0)
+            st.forces(x0, f0, phi, sc.ones)
+            st.dot_into(dots[0], lv, f0)
+        else:
+            st.field_kick(x0, torch.zeros_like(x0), sc.zeros, phi)
+        st.vjp_field(phi, x0, lv, sc.k0, sc.k0_pot, lx)
+        ctx.ckpts = None
+        gc = None
+        if want_c:
+            gc = dots.t().contiguous().to(device=ctx.c_meta[1], dtype=ctx.c_meta[0])
+        return lx, lv, gc, None, None

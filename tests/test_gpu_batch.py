@@ -1,0 +1,98 @@
+"""Batched leapfrog (one launch per stage over simulations of different
+meshes) against the single-simulation path, simulation by simulation."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (16, 16, 16), (24, 16, 32), (8, 12, 8)]
+
+
+def lattice(shape):
+    return np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1).reshape(-1, 3)
+
+
+def states(shapes, relative, seed):
+    r = np.random.default_rng(seed)
+    xs, vs = [], []
+    for sh in shapes:
+        n = int(np.prod(sh))
+        d = r.normal(scale=0.8, size=(n, 3))
+        xs.append((d if relative else lattice(sh) + d).astype(np.float32))
+        vs.append(r.normal(scale=0.3, size=(n, 3)).astype(np.float32))
+    return xs, vs
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+@pytest.mark.parametrize("relative", [False, True])
+def test_batch_matches_single(gpu, relative):
+    from jaxpm_amd import integrate
+    print(f"batch relative={relative}", flush=True)
+    xs, vs = states(SHAPES, relative, 50)
+    r = np.random.default_rng(51)
+    S = 4
+    coeffs = np.concatenate([r.uniform(0.02, 0.06, (len(SHAPES), 1)), r.uniform(0.1, 0.3, (len(SHAPES), S)),
+                             r.uniform(0.03, 0.08, (len(SHAPES), S))], 1)
+    gxs = [r.normal(size=x.shape) for x in xs]
+    gvs = [r.normal(size=x.shape) for x in xs]
+    # batched
+    st = integrate.BatchStepper(SHAPES, [x.shape[0] for x in xs], relative=relative)
+    xt = torch.as_tensor(np.concatenate(xs), device="cuda").requires_grad_(True)
+    vt = torch.as_tensor(np.concatenate(vs), device="cuda").requires_grad_(True)
+    ct = torch.tensor(coeffs, dtype=torch.float64, requires_grad=True)
+    xb, vb = integrate.BatchNBody.apply(xt, vt, ct, st, 2)
+    gx = torch.as_tensor(np.concatenate(gxs), device="cuda", dtype=torch.float32)
+    gv = torch.as_tensor(np.concatenate(gvs), device="cuda", dtype=torch.float32)
+    ((xb * gx).sum() + (vb * gv).sum()).backward()
+    xb, vb = xb.detach().cpu().numpy(), vb.detach().cpu().numpy()
+    gxb, gvb, gcb = xt.grad.cpu().numpy(), vt.grad.cpu().numpy(), ct.grad.numpy()
+    # one simulation at a time
+    off = 0
+    for b, sh in enumerate(SHAPES):
+        n = xs[b].shape[0]
+        s1 = integrate.Stepper(sh, n, relative=relative)
+        x1 = torch.as_tensor(xs[b], device="cuda").requires_grad_(True)
+        v1 = torch.as_tensor(vs[b], device="cuda").requires_grad_(True)
+        c1 = torch.tensor(coeffs[b], dtype=torch.float64, requires_grad=True)
+        xo, vo = integrate.NBody.apply(x1, v1, c1, s1, 2)
+        ((xo * gx[off:off + n]).sum() + (vo * gv[off:off + n]).sum()).backward()
+        sl = slice(off, off + n)
+        assert rel_err(xb[sl] - xs[b], xo.detach().cpu().numpy() - xs[b]) < 1e-5, b
+        assert rel_err(vb[sl], vo.detach().cpu().numpy()) < 1e-5, b
+        assert rel_err(gxb[sl], x1.grad.cpu().numpy()) < 1e-4, b
+        assert rel_err(gvb[sl], v1.grad.cpu().numpy()) < 1e-4, b
+        np.testing.assert_allclose(gcb[b], c1.grad.numpy(), rtol=1e-4, atol=1e-4 * np.abs(c1.grad.numpy()).max())
+        off += n
+
+
+def test_batch_vectors(gpu):
+    from jaxpm_amd import integrate
+    shapes = [(8, 8, 8), (8, 8, 8), (12, 8, 8)]
+    counts = [100, 512, 37]
+    st = integrate.BatchStepper(shapes, counts)
+    r = np.random.default_rng(52)
+    x = r.normal(size=(sum(counts), 3)).astype(np.float32)
+    y = r.normal(size=(sum(counts), 3)).astype(np.float32)
+    a = np.array([0.5, -1.5, 2.0], np.float32)
+    yt = torch.as_tensor(y, device="cuda")
+    st.axpy(torch.as_tensor(a, device="cuda"), torch.as_tensor(x, device="cuda"), yt)
+    ref = y.copy()
+    out = torch.zeros(3, dtype=torch.float64, device="cuda")
+    st.dot_into(out, torch.as_tensor(x, device="cuda"), torch.as_tensor(y, device="cuda"))
+    off = 0
+    for b, n in enumerate(counts):
+        ref[off:off + n] += a[b] * x[off:off + n]
+        d = np.sum(x[off:off + n].astype(np.float64) * y[off:off + n].astype(np.float64))
+        assert abs(out[b].item() - d) < 1e-12 * np.sum(np.abs(x[off:off + n] * y[off:off + n]))
+        off += n
+    np.testing.assert_array_equal(yt.cpu().numpy(), ref)
+
+
+def test_batch_rejects_interleaved_shapes(gpu):
+    from jaxpm_amd import integrate
+    with pytest.raises(ValueError, match="contiguous"):
+        integrate.BatchStepper([(8, 8, 8), (16, 16, 16), (8, 8, 8)], [512, 4096, 512])
