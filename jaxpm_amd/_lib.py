@@ -160,9 +160,18 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+_timing_on = False
+
+
 def timing(enable):
     """Turn the library's per-kernel HIP-event timing on or off."""
+    global _timing_on
     call("jpm_timing_enable", int(bool(enable)))
+    _timing_on = bool(enable)
+
+
+def timing_enabled():
+    return _timing_on
 
 
 def timing_report():

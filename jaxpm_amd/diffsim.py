@@ -26,7 +26,7 @@ from . import _autograd, integrate, pm
 from . import cosmology as cs
 from ._convert import to_device
 
-__all__ = ["CosmoCoefficients", "LptNBody", "central_jacobian"]
+__all__ = ["CosmoCoefficients", "LptNBody", "BatchLptNBody", "SimSpec", "central_jacobian"]
 
 
 def central_jacobian(fn, x, rel_step):
@@ -88,22 +88,13 @@ class _Quantities:
         return self._put(self._jac, k, central_jacobian(self.fn, np.asarray(x, np.float64), self.rel_step))
 
 
-class LptNBody:
-    """noise [nx,ny,nz], cosmological parameters -> final CIC density.
-
-    ``params`` names the fields of :class:`~jaxpm_amd.cosmology.Cosmology`
-    that vary (default Omega_c and sigma8); the others come from
-    ``base_cosmo``.  The model is linear_field with the Eisenstein-Hu table on
-    ``k_table`` (jnp.interp, as conftest.py:90-94), ``order``-LPT at ``a0``
-    onto the particle lattice, the leapfrog over ``a_steps`` (starting at
-    ``a0``) with the adjoint checkpointed every ``checkpoint_every`` steps,
-    and the CIC paint of the final positions.  Calling it returns the density
-    mesh; torch autograd gives its cotangent with respect to the noise and
-    the parameter vector.
-    """
+class SimSpec:
+    """The host side of one simulation of the forward model: its mesh, box,
+    time steps, cosmology and the float64 quantities that carry the
+    cosmological parameters into the device computation."""
 
     def __init__(self, mesh_shape, box_size, a_steps, base_cosmo=None, params=("Omega_c", "sigma8"), order=2,
-                 paint_absolute_pos=True, checkpoint_every=None, k_table=None, deterministic=False, rel_step=1e-4):
+                 k_table=None, rel_step=1e-4):
         self.shape = tuple(int(s) for s in mesh_shape)
         self.box = tuple(float(b) for b in box_size)
         self.a_steps = np.asarray(a_steps, np.float64)
@@ -117,20 +108,15 @@ class LptNBody:
         if order not in (1, 2):
             raise ValueError(f"order must be 1 or 2, got {order}")
         self.order = order
-        self.absolute = bool(paint_absolute_pos)
-        self.checkpoint_every = checkpoint_every
         self.ktab = np.logspace(-4, 1, 128) if k_table is None else np.asarray(k_table, np.float64)
-        self.deterministic = deterministic
         self.rel_step = rel_step
         S = len(self.a_steps) - 1
         nl = 3 * order
         nt = len(self.ktab)
+        self.S = S
         self.slices = {"pk": slice(0, nt), "lpt": slice(nt, nt + nl), "kdk": slice(nt + nl, nt + nl + 1 + 2 * S)}
-        n = int(np.prod(self.shape))
-        self.stepper = integrate.Stepper(self.shape, n, relative=not self.absolute, deterministic=deterministic)
-        self._ktab_dev = None
-        self._lattice = None
-        self._q = _Quantities(self.quantities, rel_step)
+        self.n = int(np.prod(self.shape))
+        self.q = _Quantities(self.quantities, rel_step)
 
     def cosmology(self, values):
         return self.base.replace(**{nm: float(v) for nm, v in zip(self.names, values)})
@@ -148,28 +134,131 @@ class LptNBody:
         c = self.base if cosmo is None else cosmo
         return torch.tensor([float(getattr(c, nm)) for nm in self.names], dtype=torch.float64)
 
+    def initial_state(self, noise, params, absolute, lattice, ktab_dev):
+        """(q, x0, v0): the host quantities (differentiable in ``params``) and
+        the LPT positions and momenta (differentiable in the noise and q)."""
+        params = torch.as_tensor(params, dtype=torch.float64) if not isinstance(params, torch.Tensor) else params
+        q = CosmoCoefficients.apply(params, self.q, self.rel_step)
+        pktab = q[self.slices["pk"]].to(device=noise.device, dtype=torch.float32)
+        ic = _autograd.LinearField.apply(noise, self.box, ktab_dev, pktab)
+        psi1, psi2 = pm.lpt_fields(ic, lattice if absolute else None, self.order)
+        dx, p, _ = pm.lpt_combine(q[self.slices["lpt"]], psi1, psi2)
+        x0 = (lattice + dx) if absolute else dx
+        return q, x0.reshape(-1, 3), p.reshape(-1, 3)
+
+
+def _lattice(shape, dev):
+    return torch.stack(torch.meshgrid(*[torch.arange(s, device=dev, dtype=torch.float32) for s in shape],
+                                      indexing="ij"), -1)
+
+
+class LptNBody:
+    """noise [nx,ny,nz], cosmological parameters -> final CIC density.
+
+    ``params`` names the fields of :class:`~jaxpm_amd.cosmology.Cosmology`
+    that vary (default Omega_c and sigma8); the others come from
+    ``base_cosmo``.  The model is linear_field with the Eisenstein-Hu table on
+    ``k_table`` (jnp.interp, as conftest.py:90-94), ``order``-LPT at ``a0``
+    onto the particle lattice, the leapfrog over ``a_steps`` (starting at
+    ``a0``) with the adjoint checkpointed every ``checkpoint_every`` steps,
+    and the CIC paint of the final positions.  Calling it returns the density
+    mesh; torch autograd gives its cotangent with respect to the noise and
+    the parameter vector.  ``method="plain"`` runs the same model with the
+    leapfrog differentiated by autograd over the stored trajectory
+    (:func:`~jaxpm_amd.integrate.leapfrog_autodiff`, no parameter cotangent
+    for the leapfrog coefficients), for comparison.
+    """
+
+    def __init__(self, mesh_shape, box_size, a_steps, base_cosmo=None, params=("Omega_c", "sigma8"), order=2,
+                 paint_absolute_pos=True, checkpoint_every=None, k_table=None, deterministic=False, rel_step=1e-4,
+                 method="custom"):
+        self.spec = SimSpec(mesh_shape, box_size, a_steps, base_cosmo, params, order, k_table, rel_step)
+        self.shape, self.box, self.a_steps = self.spec.shape, self.spec.box, self.spec.a_steps
+        self.absolute = bool(paint_absolute_pos)
+        self.checkpoint_every = checkpoint_every
+        self.deterministic = deterministic
+        if method not in ("custom", "plain"):
+            raise ValueError(f"unknown method {method!r}")
+        self.method = method
+        self.stepper = integrate.Stepper(self.shape, self.spec.n, relative=not self.absolute,
+                                         deterministic=deterministic) if method == "custom" else None
+        self._ktab_dev = None
+        self._lattice = None
+
+    def quantities(self, values):
+        return self.spec.quantities(values)
+
+    def parameter_vector(self, cosmo=None):
+        return self.spec.parameter_vector(cosmo)
+
     def __call__(self, noise, params):
         noise = to_device(noise)
         if tuple(noise.shape) != self.shape:
             raise ValueError(f"noise shape {tuple(noise.shape)} does not match {self.shape}")
-        params = torch.as_tensor(params, dtype=torch.float64) if not isinstance(params, torch.Tensor) else params
-        q = CosmoCoefficients.apply(params, self._q, self.rel_step)
         dev = noise.device
         if self._ktab_dev is None:
-            self._ktab_dev = torch.as_tensor(self.ktab, dtype=torch.float32, device=dev)
-        pktab = q[self.slices["pk"]].to(device=dev, dtype=torch.float32)
-        ic = _autograd.LinearField.apply(noise, self.box, self._ktab_dev, pktab)
-        if self.absolute:
-            if self._lattice is None:
-                self._lattice = torch.stack(torch.meshgrid(
-                    *[torch.arange(s, device=dev, dtype=torch.float32) for s in self.shape], indexing="ij"), -1)
-            particles = self._lattice
+            self._ktab_dev = torch.as_tensor(self.spec.ktab, dtype=torch.float32, device=dev)
+            self._lattice = _lattice(self.shape, dev)
+        q, x0, v0 = self.spec.initial_state(noise, params, self.absolute, self._lattice, self._ktab_dev)
+        kdk = q[self.spec.slices["kdk"]]
+        if self.method == "custom":
+            xS, _ = integrate.NBody.apply(x0, v0, kdk, self.stepper, self.checkpoint_every)
         else:
-            particles = None
-        psi1, psi2 = pm.lpt_fields(ic, particles, self.order)
-        dx, p, _ = pm.lpt_combine(q[self.slices["lpt"]], psi1, psi2)
-        x0 = (particles + dx) if self.absolute else dx
-        xS, _ = integrate.NBody.apply(x0.reshape(-1, 3), p.reshape(-1, 3), q[self.slices["kdk"]], self.stepper,
-                                      self.checkpoint_every)
+            xS, _ = integrate.leapfrog_autodiff(x0, v0, kdk, self.shape, self.absolute)
         zeros = torch.zeros(self.shape, dtype=torch.float32, device=dev)
         return _autograd.CicPaint.apply(zeros, xS, None, 1.0, not self.absolute, self.deterministic)
+
+
+class BatchLptNBody:
+    """The forward model of :class:`LptNBody` for a batch of simulations with
+    their own meshes, boxes and cosmologies, the N-body stage stepped by the
+    batched kernels (one launch per stage over all simulations) and, with
+    ``graph=True``, replayed from captured HIP graphs.  Simulations of one
+    mesh shape must be contiguous.  Calling it with a list of noise meshes
+    and a [B, P] parameter tensor returns the list of final densities."""
+
+    def __init__(self, mesh_shapes, box_sizes, a_steps, cosmologies, params=("Omega_c", "sigma8"), order=1,
+                 paint_absolute_pos=True, checkpoint_every=None, k_table=None, graph=True, coeff_grads=False,
+                 rel_step=1e-4):
+        self.specs = [SimSpec(sh, bx, a_steps, c, params, order, k_table, rel_step)
+                      for sh, bx, c in zip(mesh_shapes, box_sizes, cosmologies)]
+        if not (len(self.specs) == len(mesh_shapes) == len(box_sizes) == len(cosmologies)):
+            raise ValueError("one box and one cosmology per mesh shape")
+        self.B = len(self.specs)
+        self.absolute = bool(paint_absolute_pos)
+        self.stepper = integrate.BatchStepper([s.shape for s in self.specs], [s.n for s in self.specs],
+                                              relative=not self.absolute)
+        c0 = np.stack([s.q(s.parameter_vector().numpy())[s.slices["kdk"]] for s in self.specs])
+        self.engine = integrate.BatchLeapfrog(self.stepper, c0, checkpoint_every, coeff_grads=coeff_grads,
+                                              graph=graph)
+        self.checkpoint_every = checkpoint_every
+        self._dev_tabs = None
+
+    def parameter_matrix(self):
+        return torch.stack([s.parameter_vector() for s in self.specs])
+
+    def __call__(self, noises, params):
+        if len(noises) != self.B:
+            raise ValueError(f"{len(noises)} noise meshes for {self.B} simulations")
+        noises = [to_device(nz) for nz in noises]
+        dev = noises[0].device
+        if self._dev_tabs is None:
+            self._dev_tabs = [(torch.as_tensor(s.ktab, dtype=torch.float32, device=dev), _lattice(s.shape, dev))
+                              for s in self.specs]
+        params = torch.as_tensor(params, dtype=torch.float64) if not isinstance(params, torch.Tensor) else params
+        xs, vs, kdk = [], [], []
+        for b, s in enumerate(self.specs):
+            kt, lat = self._dev_tabs[b]
+            q, x0, v0 = s.initial_state(noises[b], params[b], self.absolute, lat, kt)
+            xs.append(x0)
+            vs.append(v0)
+            kdk.append(q[s.slices["kdk"]])
+        x0 = torch.cat(xs)
+        v0 = torch.cat(vs)
+        xS, _ = integrate.BatchNBody.apply(x0, v0, torch.stack(kdk), self.engine, self.checkpoint_every)
+        out, off = [], 0
+        for s in self.specs:
+            zeros = torch.zeros(s.shape, dtype=torch.float32, device=dev)
+            out.append(_autograd.CicPaint.apply(zeros, xS[off:off + s.n], None, 1.0, not self.absolute, False))
+            off += s.n
+        return out

@@ -185,6 +185,19 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
         stepper.field_kick(x, v, gamma[i], phi)
 
 
+def _keep_buffers(st, like, count, forces):
+    """Per-step adjoint data of a segment: positions, the force evaluation's
+    field (potential + kept tile sort) and, for coefficient cotangents, the
+    drift velocity and the forces."""
+    out = []
+    for _ in range(count):
+        b = [torch.empty_like(like), st.new_field(keep_sort=True)]
+        if forces:
+            b += [torch.empty_like(like), torch.empty_like(like)]
+        out.append(b)
+    return out
+
+
 class NBody(torch.autograd.Function):
     """(x0, v0, c) -> (xS, vS) by the leapfrog, with a hand-written adjoint.
 
@@ -192,23 +205,44 @@ class NBody(torch.autograd.Function):
     :func:`kdk_schedule`.  Its cotangent, when asked for, is
     dL/dk0 = <lambda_v, F(x0)>, dL/dbeta_i = <lambda_x, v_i>,
     dL/dgamma_i = <lambda_v, F(x_{i+1})>, each a float64 device dot product
-    of the adjoint state with the recomputed trajectory."""
+    of the adjoint state with the trajectory.
+
+    The forward keeps (x, v) at the start of every segment of
+    ``checkpoint_every`` steps, and the adjoint data of the first half kick
+    and of the last segment, which the backward would otherwise recompute
+    first; every other segment is recomputed from its checkpoint."""
 
     @staticmethod
     def forward(ctx, x0, v0, coeffs, stepper, checkpoint_every):
+        st = stepper
         k0, beta, gamma = _split(coeffs)
         S = len(beta)
+        grad = any(ctx.needs_input_grad[:3])
+        want_c = ctx.needs_input_grad[2]
         x = x0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
-        phi = stepper.new_field()
-        stepper.field_kick(x, v, k0, phi)
+        phi0 = st.new_field(keep_sort=grad)
+        f0 = None
+        if want_c:
+            f0 = torch.empty_like(x)
+            st.forces(x, f0, phi0)
+            st.axpy(k0, f0, v)
+        else:
+            st.field_kick(x, v, k0, phi0)
         k = S if checkpoint_every is None else max(1, int(checkpoint_every))
+        last = ((S - 1) // k) * k
         ckpts = {0: (x.clone(), v.clone())}   # x0 and v after the first half kick
+        keep = _keep_buffers(st, x, min(k, S), want_c) if grad else None
+        phi = st.new_field()
         for s0 in range(0, S, k):
-            _run(stepper, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi)
+            if grad and s0 == last:
+                _run(st, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi, keep, forces=want_c)
+            else:
+                _run(st, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi)
             if s0 + k < S:
                 ckpts[s0 + k] = (x.clone(), v.clone())
-        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = stepper, (k0, beta, gamma), k, ckpts
+        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = st, (k0, beta, gamma), k, ckpts
+        ctx.kept = (keep, phi0, f0, last)
         ctx.c_meta = (coeffs.dtype, coeffs.device)
         return x, v
 
@@ -218,24 +252,18 @@ class NBody(torch.autograd.Function):
         k0, beta, gamma = ctx.sched
         S = len(beta)
         want_c = ctx.needs_input_grad[2]
-        x0, v0 = ctx.ckpts[0]
+        keep, phi0, f0, last = ctx.kept
+        x0 = ctx.ckpts[0][0]
         lx = (gx if gx is not None else torch.zeros_like(x0)).detach().clone().contiguous()
         lv = (gv if gv is not None else torch.zeros_like(x0)).detach().clone().contiguous()
         dots = torch.zeros(1 + 2 * S, dtype=torch.float64, device=x0.device) if want_c else None
         phi = st.new_field()
-
-        def buffers():
-            b = [torch.empty_like(x0), st.new_field(keep_sort=True)]
-            if want_c:
-                b += [torch.empty_like(x0), torch.empty_like(x0)]
-            return b
-
-        keep = [buffers() for _ in range(min(ctx.k, S))]
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
-            xc, vc = ctx.ckpts[s0]
-            x, v = xc.clone(), vc.clone()
-            _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep, forces=want_c)
+            if s0 != last:
+                xc, vc = ctx.ckpts[s0]
+                x, v = xc.clone(), vc.clone()
+                _run(st, x, v, k0, beta, gamma, s0, s1, phi, keep, forces=want_c)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
                 if want_c:
@@ -244,16 +272,11 @@ class NBody(torch.autograd.Function):
                 if want_c:
                     _ops.dot_into(dots, 1 + i, lx, kp[2])         # beta_i
                 st.axpy(beta[i], lx, lv)                          # drift i: x += beta v
-        del keep
-        # the first half kick used x0 before any drift: recompute its field
+        # the first half kick, v += k0 F(x0), from its kept field
         if want_c:
-            f0 = torch.empty_like(x0)
-            st.forces(x0, f0, phi)
             _ops.dot_into(dots, 0, lv, f0)
-        else:
-            st.field_kick(x0, torch.zeros_like(x0), 0.0, phi)
-        st.vjp_field(phi, x0, lv, k0, lx)
-        ctx.ckpts = None
+        st.vjp_field(phi0, x0, lv, k0, lx)
+        ctx.ckpts = ctx.kept = None
         gc = dots.to(device=ctx.c_meta[1], dtype=ctx.c_meta[0]) if want_c else None
         return lx, lv, gc, None, None
 
@@ -375,25 +398,41 @@ class BatchStepper:
 class BatchSchedule:
     """Per-simulation leapfrog coefficients as the float32 device arrays the
     batched kernels read: k0 [B], beta [S, B], gamma [S, B], and for the
-    adjoint pot_scale [S, B] = float32(gamma / M_b) and k0 / M_b."""
+    adjoint the factors the potential pass applies, float32(gamma / M_b)
+    and float32(k0 / M_b).  ``update`` rewrites them in place, so captured
+    graphs see new values."""
 
     def __init__(self, coeffs, cells, dev):
-        c = coeffs.detach().to(torch.float64).cpu().numpy()
+        c = self._check(coeffs)
+        B, S = c.shape[0], (c.shape[1] - 1) // 2
+        self.B, self.S, self.cells, self.dev = B, S, np.asarray(cells, np.float64), dev
+        z = lambda *sh: torch.zeros(sh, dtype=torch.float32, device=dev)  # noqa: E731
+        self.k0, self.beta, self.gamma = z(B), z(S, B), z(S, B)
+        self.gamma_pot, self.k0_pot = z(S, B), z(B)
+        self.zeros = z(B)
+        self.ones = torch.ones(B, dtype=torch.float32, device=dev)
+        self.update(coeffs)
+
+    @staticmethod
+    def _check(coeffs):
+        c = coeffs.detach().to(torch.float64).cpu().numpy() if isinstance(coeffs, torch.Tensor) else \
+            np.asarray(coeffs, np.float64)
         if c.ndim != 2 or c.shape[1] < 3 or c.shape[1] % 2 == 0:
             raise ValueError("batched leapfrog coefficients must be [B, 1 + 2 S]")
-        B = c.shape[0]
-        S = (c.shape[1] - 1) // 2
-        self.S = S
+        return c
+
+    def update(self, coeffs):
+        c = self._check(coeffs)
+        if c.shape != (self.B, 1 + 2 * self.S):
+            raise ValueError(f"coefficients of shape {c.shape}, expected {(self.B, 1 + 2 * self.S)}")
+        S, m = self.S, self.cells
         f32 = c.astype(np.float32)
-        m = np.asarray(cells, np.float64)
-        self.k0 = torch.as_tensor(f32[:, 0].copy(), device=dev)
-        self.beta = torch.as_tensor(np.ascontiguousarray(f32[:, 1:1 + S].T), device=dev)
-        self.gamma = torch.as_tensor(np.ascontiguousarray(f32[:, 1 + S:].T), device=dev)
-        self.gamma_pot = torch.as_tensor(
-            np.ascontiguousarray((f32[:, 1 + S:].astype(np.float64) / m[:, None]).astype(np.float32).T), device=dev)
-        self.k0_pot = torch.as_tensor((f32[:, 0].astype(np.float64) / m).astype(np.float32), device=dev)
-        self.zeros = torch.zeros(B, dtype=torch.float32, device=dev)
-        self.ones = torch.ones(B, dtype=torch.float32, device=dev)
+        host = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32))  # noqa: E731
+        self.k0.copy_(host(f32[:, 0]))
+        self.beta.copy_(host(f32[:, 1:1 + S].T))
+        self.gamma.copy_(host(f32[:, 1 + S:].T))
+        self.gamma_pot.copy_(host((f32[:, 1 + S:].astype(np.float64) / m[:, None]).astype(np.float32).T))
+        self.k0_pot.copy_(host((f32[:, 0].astype(np.float64) / m).astype(np.float32)))
 
 
 def _run_batch(st, sc, x, v, start, stop, phi, keep=None, forces=False):
@@ -411,72 +450,157 @@ def _run_batch(st, sc, x, v, start, stop, phi, keep=None, forces=False):
         st.field_kick(x, v, sc.gamma[i], phi)
 
 
+class BatchLeapfrog:
+    """The batched leapfrog and its checkpointed adjoint over static device
+    buffers.  ``forward`` / ``backward`` run the sweeps eagerly or, with
+    ``graph=True``, replay two HIP graphs captured once (the time loop is a
+    fixed sequence of stream-ordered launches: no host synchronisation, no
+    allocation, no memset).  The adjoint is that of :class:`NBody`, per
+    simulation; with ``coeff_grads`` the backward also forms the float64
+    cotangents of every simulation's [k0, beta, gamma]."""
+
+    def __init__(self, stepper, coeffs, checkpoint_every=None, coeff_grads=False, graph=False):
+        st = self.st = stepper
+        self.sc = BatchSchedule(coeffs, st.cells, st.dev)
+        S = self.S = self.sc.S
+        self.k = S if checkpoint_every is None else max(1, int(checkpoint_every))
+        self.want_c = bool(coeff_grads)
+        e = lambda: torch.empty((st.N, 3), dtype=torch.float32, device=st.dev)  # noqa: E731
+        self.x0, self.v0, self.x, self.v = e(), e(), e(), e()
+        self.ckpts = {s0: (e(), e()) for s0 in range(0, S, self.k)}
+        self.last = ((S - 1) // self.k) * self.k
+        self.phi = st.new_field()
+        self.phi0 = st.new_field(keep_sort=True)
+        self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c)
+        self.gx, self.gv, self.lx, self.lv, self.xs, self.vs, self.f0 = e(), e(), e(), e(), e(), e(), e()
+        self.dots = torch.zeros((1 + 2 * S, st.B), dtype=torch.float64, device=st.dev)
+        self.graphs = None
+        if graph:
+            self._capture()
+
+    def _fwd(self):
+        st, sc = self.st, self.sc
+        x, v = self.x, self.v
+        x.copy_(self.x0)
+        v.copy_(self.v0)
+        # the first half kick and the last segment keep their adjoint data
+        if self.want_c:
+            st.forces(x, self.f0, self.phi0, sc.ones)
+            st.axpy(sc.k0, self.f0, v)
+        else:
+            st.field_kick(x, v, sc.k0, self.phi0)
+        for s0 in range(0, self.S, self.k):
+            if s0 == 0:
+                self.ckpts[0][0].copy_(x)
+                self.ckpts[0][1].copy_(v)
+            if s0 == self.last:
+                _run_batch(st, sc, x, v, s0, min(self.S, s0 + self.k), self.phi, self.keep, forces=self.want_c)
+            else:
+                _run_batch(st, sc, x, v, s0, min(self.S, s0 + self.k), self.phi)
+            if s0 + self.k < self.S:
+                self.ckpts[s0 + self.k][0].copy_(x)
+                self.ckpts[s0 + self.k][1].copy_(v)
+
+    def _bwd(self):
+        st, sc, S = self.st, self.sc, self.S
+        lx, lv, dots = self.lx, self.lv, self.dots
+        lx.copy_(self.gx)
+        lv.copy_(self.gv)
+        if self.want_c:
+            dots.zero_()
+        for s0 in reversed(range(0, S, self.k)):
+            s1 = min(S, s0 + self.k)
+            if s0 != self.last:
+                self.xs.copy_(self.ckpts[s0][0])
+                self.vs.copy_(self.ckpts[s0][1])
+                _run_batch(st, sc, self.xs, self.vs, s0, s1, self.phi, self.keep, forces=self.want_c)
+            for i in reversed(range(s0, s1)):
+                kp = self.keep[i - s0]
+                if self.want_c:
+                    st.dot_into(dots[1 + S + i], lv, kp[3])
+                st.vjp_field(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], lx)
+                if self.want_c:
+                    st.dot_into(dots[1 + i], lx, kp[2])
+                st.axpy(sc.beta[i], lx, lv)
+        # the first half kick, v += k0 F(x0), from its kept field
+        if self.want_c:
+            st.dot_into(dots[0], lv, self.f0)
+        st.vjp_field(self.phi0, self.ckpts[0][0], lv, sc.k0, sc.k0_pot, lx)
+
+    def _capture(self):
+        was = _lib.timing_enabled()
+        _lib.timing(False)
+        try:
+            # one eager pass creates the FFT plans, wave-number tables and
+            # kernel attributes outside the capture
+            self._fwd()
+            self._bwd()
+            torch.cuda.synchronize()
+            gf, gb = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gf):
+                self._fwd()
+            with torch.cuda.graph(gb):
+                self._bwd()
+            torch.cuda.synchronize()
+            self.graphs = (gf, gb)
+        finally:
+            _lib.timing(was)
+
+    def forward(self, x0, v0):
+        """Run the forward sweep from (x0, v0); returns the static (x, v)
+        buffers (valid until the next call)."""
+        self.x0.copy_(x0)
+        self.v0.copy_(v0)
+        if self.graphs:
+            self.graphs[0].replay()
+        else:
+            self._fwd()
+        return self.x, self.v
+
+    def backward(self, gx, gv):
+        """The adjoint of the last forward for output cotangents (gx, gv);
+        returns the static (lx, lv, dots) buffers."""
+        self.gx.copy_(gx)
+        self.gv.copy_(gv)
+        if self.graphs:
+            self.graphs[1].replay()
+        else:
+            self._bwd()
+        return self.lx, self.lv, self.dots
+
+
 class BatchNBody(torch.autograd.Function):
     """(x0, v0, c) -> (xS, vS) for a batch of simulations by the leapfrog,
     with the checkpointed hand-written adjoint of :class:`NBody`; ``c`` is
     the float64 host array [B, 1 + 2 S] of each simulation's [k0, beta,
-    gamma] and receives per-simulation cotangents."""
+    gamma] and receives per-simulation cotangents.  ``engine`` is a
+    :class:`BatchStepper` (an eager :class:`BatchLeapfrog` is built per
+    call) or a :class:`BatchLeapfrog`, possibly graph-captured, whose
+    coefficients are set to ``c``."""
 
     @staticmethod
-    def forward(ctx, x0, v0, coeffs, stepper, checkpoint_every):
-        st = stepper
-        sc = BatchSchedule(coeffs, st.cells, st.dev)
-        S = sc.S
-        x = x0.detach().clone().contiguous()
-        v = v0.detach().clone().contiguous()
-        phi = st.new_field()
-        st.field_kick(x, v, sc.k0, phi)
-        k = S if checkpoint_every is None else max(1, int(checkpoint_every))
-        ckpts = {0: (x.clone(), v.clone())}
-        for s0 in range(0, S, k):
-            _run_batch(st, sc, x, v, s0, min(S, s0 + k), phi)
-            if s0 + k < S:
-                ckpts[s0 + k] = (x.clone(), v.clone())
-        ctx.st, ctx.sc, ctx.k, ctx.ckpts = st, sc, k, ckpts
-        ctx.c_meta = (coeffs.dtype, coeffs.device, tuple(coeffs.shape))
-        return x, v
+    def forward(ctx, x0, v0, coeffs, engine, checkpoint_every):
+        if isinstance(engine, BatchStepper):
+            engine = BatchLeapfrog(engine, coeffs, checkpoint_every, coeff_grads=ctx.needs_input_grad[2])
+        else:
+            engine.sc.update(coeffs)
+            if ctx.needs_input_grad[2] and not engine.want_c:
+                raise ValueError("this BatchLeapfrog was built without coefficient cotangents")
+        x, v = engine.forward(x0.detach().contiguous(), v0.detach().contiguous())
+        ctx.engine = engine
+        ctx.c_meta = (coeffs.dtype, coeffs.device)
+        return x.clone(), v.clone()
 
     @staticmethod
     def backward(ctx, gx, gv):
-        st, sc = ctx.st, ctx.sc
-        S = sc.S
-        want_c = ctx.needs_input_grad[2]
-        x0 = ctx.ckpts[0][0]
-        lx = (gx if gx is not None else torch.zeros_like(x0)).detach().clone().contiguous()
-        lv = (gv if gv is not None else torch.zeros_like(x0)).detach().clone().contiguous()
-        dots = torch.zeros((1 + 2 * S, st.B), dtype=torch.float64, device=x0.device) if want_c else None
-        phi = st.new_field()
-
-        def buffers():
-            b = [torch.empty_like(x0), st.new_field(keep_sort=True)]
-            if want_c:
-                b += [torch.empty_like(x0), torch.empty_like(x0)]
-            return b
-
-        keep = [buffers() for _ in range(min(ctx.k, S))]
-        for s0 in reversed(range(0, S, ctx.k)):
-            s1 = min(S, s0 + ctx.k)
-            xc, vc = ctx.ckpts[s0]
-            x, v = xc.clone(), vc.clone()
-            _run_batch(st, sc, x, v, s0, s1, phi, keep, forces=want_c)
-            for i in reversed(range(s0, s1)):
-                kp = keep[i - s0]
-                if want_c:
-                    st.dot_into(dots[1 + S + i], lv, kp[3])
-                st.vjp_field(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], lx)
-                if want_c:
-                    st.dot_into(dots[1 + i], lx, kp[2])
-                st.axpy(sc.beta[i], lx, lv)
-        del keep
-        if want_c:
-            f0 = torch.empty_like(x0)
-            st.forces(x0, f0, phi, sc.ones)
-            st.dot_into(dots[0], lv, f0)
-        else:
-            st.field_kick(x0, torch.zeros_like(x0), sc.zeros, phi)
-        st.vjp_field(phi, x0, lv, sc.k0, sc.k0_pot, lx)
-        ctx.ckpts = None
+        eng = ctx.engine
+        zeros = None
+        if gx is None or gv is None:
+            zeros = torch.zeros_like(eng.x)
+        lx, lv, dots = eng.backward(gx.contiguous() if gx is not None else zeros,
+                                    gv.contiguous() if gv is not None else zeros)
         gc = None
-        if want_c:
+        if ctx.needs_input_grad[2]:
             gc = dots.t().contiguous().to(device=ctx.c_meta[1], dtype=ctx.c_meta[0])
-        return lx, lv, gc, None, None
+        ctx.engine = None
+        return lx.clone(), lv.clone(), gc, None, None

@@ -96,3 +96,42 @@ def test_batch_rejects_interleaved_shapes(gpu):
     from jaxpm_amd import integrate
     with pytest.raises(ValueError, match="contiguous"):
         integrate.BatchStepper([(8, 8, 8), (16, 16, 16), (8, 8, 8)], [512, 4096, 512])
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    """The captured time loop, replayed several times with new inputs and
+    coefficients and eager work in between, against eager runs."""
+    from jaxpm_amd import integrate
+    print("batch graph replays", flush=True)
+    xs, vs = states(SHAPES, False, 53)
+    r = np.random.default_rng(54)
+    S = 3
+    n = [x.shape[0] for x in xs]
+
+    def coeffs():
+        return np.concatenate([r.uniform(0.02, 0.06, (len(SHAPES), 1)), r.uniform(0.1, 0.3, (len(SHAPES), S)),
+                               r.uniform(0.03, 0.08, (len(SHAPES), S))], 1)
+
+    st = integrate.BatchStepper(SHAPES, n)
+    c0 = coeffs()
+    eager = integrate.BatchLeapfrog(st, c0, 2, coeff_grads=True, graph=False)
+    graphed = integrate.BatchLeapfrog(st, c0, 2, coeff_grads=True, graph=True)
+    x0 = torch.as_tensor(np.concatenate(xs), device="cuda")
+    v0 = torch.as_tensor(np.concatenate(vs), device="cuda")
+    for rep in range(3):
+        c = coeffs()
+        eager.sc.update(c)
+        graphed.sc.update(c)
+        dx = torch.as_tensor(r.normal(scale=0.05, size=x0.shape).astype(np.float32), device="cuda")
+        gx = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+        gv = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+        xg, vg = [t.clone() for t in graphed.forward(x0 + dx, v0)]
+        lg = [t.clone() for t in graphed.backward(gx, gv)]
+        torch.cuda.synchronize()
+        xe, ve = [t.clone() for t in eager.forward(x0 + dx, v0)]
+        le = [t.clone() for t in eager.backward(gx, gv)]
+        torch.cuda.synchronize()
+        print(f"replay {rep}: bitwise x={torch.equal(xg, xe)} lx={torch.equal(lg[0], le[0])}", flush=True)
+        for a, b in zip([xg, vg] + lg, [xe, ve] + le):
+            scale = float(b.abs().max())
+            assert float((a - b).abs().max()) <= 1e-6 * scale, rep
