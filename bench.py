@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # passes of `python -m jaxpm_amd.traffic_probe 256`, each scaled by the same
 # counter's reading for a plain copy of a known 64 MiB: x1.999 for reads,
 # x1.000 for writes), by kernel and mesh side.  The commands and the counter
-# values are in profiles/bin_kick_kernel_256_traffic.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel", 256): 1151.5e6}
+# values are in profiles/sorted_gathers_256_traffic.md.
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel", 256): 1153.3e6, ("bin_grad_kernel", 256): 1710.6e6}
 
 
 def parse():

@@ -120,7 +120,8 @@ size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n,
  * 64-bit fixed point (bitwise reproducible).  nc = 1 (weights per particle or
  * wscalar) or 3 (weights[p*3 + c], the adjoint of the 3-component read).
  * Replaces the scatter_add of painting.py:39-44 / painting_utils.py:110;
- * meshes up to 32768 tiles (544^3). */
+ * meshes of up to 32768 tiles count and scatter through per-workgroup LDS
+ * histograms, larger ones through wavefront-aggregated global atomics. */
 int jpm_cic_paint_binned_f32(void *stream, int nx, int ny, int nz,
                              const float *pos, int64_t n, const float *weights,
                              float wscalar, int nc, float *mesh,
@@ -329,9 +330,9 @@ int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3,
                       float scale, float *gpos, float r_split, uint32_t flags,
                       void *ws, size_t ws_bytes);
 
-/* Workspace bytes for jpm_force_kick_f32 (0 when the mesh has more than
- * 32768 16^3 tiles or flags ask for the exact gradient / atomic paint, which
- * the sorted pipeline does not cover). */
+/* Workspace bytes for jpm_force_kick_f32 (0 when three meshes exceed 2^32
+ * cells or flags ask for the exact gradient / atomic paint, which the sorted
+ * pipeline does not cover). */
 size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n,
                                      uint32_t flags);
 
@@ -402,7 +403,7 @@ int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3,
 /* Describe a batch: dims[b*3 .. b*3+2] the mesh of simulation b, counts[b]
  * its particles; flags JPM_RELATIVE for displacements (counts[b] = cells).
  * Writes an opaque handle to *batch (a device table is uploaded; this call
- * synchronises).  Each mesh <= 32768 tiles of 16^3. */
+ * synchronises). */
 int jpm_batch_create(int nsims, const int *dims, const int64_t *counts,
                      uint32_t flags, void **batch);
 int jpm_batch_destroy(void *batch);

@@ -41,14 +41,12 @@ def _dims(shape):
 
 # ---------------------------------------------------------------- painting
 
-MAX_BINNED_TILES = 32768
-
-
 def binned_ok(shape):
-    t = 1
+    """The binned paint indexes three meshes with 32-bit offsets."""
+    m = 1
     for s in shape:
-        t *= -(-int(s) // 16)
-    return t <= MAX_BINNED_TILES
+        m *= int(s)
+    return 3 * m <= 2 ** 32
 
 
 def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=False, zero=False, method="binned"):

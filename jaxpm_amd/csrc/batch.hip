@@ -47,11 +47,10 @@ int batch_create(int B, const int *dims, const int64_t *counts, uint32_t flags, 
     }
     const TileGrid t = make_tiles(nx, ny, nz);
     const int64_t m = cells(nx, ny, nz);
-    if (n < 1 || (rel && n != m) || t.K > uint32_t(kMaxKeysLds)) {
+    if (n < 1 || (rel && n != m)) {
       delete bt;
       return fail(JPM_E_INVALID, "simulation %d: mesh (%d,%d,%d) with %lld particles (%s)", b, nx, ny, nz,
-                  (long long)n, t.K > uint32_t(kMaxKeysLds) ? "more than 32768 tiles" :
-                  rel ? "relative mode needs one particle per cell" : "no particles");
+                  (long long)n, rel ? "relative mode needs one particle per cell" : "no particles");
     }
     SimRef r;
     r.t = t;

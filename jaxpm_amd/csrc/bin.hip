@@ -119,6 +119,30 @@ bin_count_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *_
     if (sh[k]) atomicAdd(&count[tile0 + k], sh[k]);
 }
 
+// Tile counts for meshes of more than kMaxKeysLds tiles, where no LDS
+// histogram fits: each run of equal keys in a wavefront adds its length with
+// one global atomic (runs along the particle order keep these few).
+template <bool REL, class G>
+__global__ void __launch_bounds__(kBinThreads)
+bin_count_global_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ count) {
+  const SimRef &s = g.block_sim(blockIdx.x);
+  uint32_t p0, p1;
+  block_range(s, blockIdx.x, slice, p0, p1);
+  for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
+    const uint32_t p = b0 + threadIdx.x;
+    const bool active = p < p1;
+    uint32_t key = 0xffffffffu;
+    if (active) {
+      int cx, cy, cz;
+      floor_cells<REL>(pos, p, p - s.part0, s.t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, s.t);
+    }
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key, active);
+    if (active && ((heads >> (threadIdx.x & 63)) & 1ull)) atomicAdd(&count[s.tile0 + key], run_length(heads, act));
+  }
+}
+
 // ---------------------------------------------------------------- scan
 
 // In-place exclusive scan of blocks of kScanBlock elements; block totals to
@@ -262,6 +286,41 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t 
     if (head) base = atomicAdd(&cur[key], run_length(heads, act));
     // each lane takes the base of the run it belongs to (its nearest head at
     // or below it) plus its distance from that head
+    const unsigned long long below = heads & ((2ull << lane) - 1ull);
+    const int my_head = below ? 63 - __clzll((long long)below) : 0;
+    base = __shfl(base, my_head);
+    if (active) spos[base + uint32_t(lane - my_head)] = make_float4(x, y, z, __uint_as_float(p));
+  }
+}
+
+// Scatter for meshes of more than kMaxKeysLds tiles: each wavefront run
+// reserves its range with one global atomic on the tile's cursor.
+template <bool REL, class G>
+__global__ void __launch_bounds__(kBinThreads)
+bin_scatter_global_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ cursor,
+                          float4 *__restrict__ spos) {
+  const SimRef &s = g.block_sim(blockIdx.x);
+  uint32_t p0, p1;
+  block_range(s, blockIdx.x, slice, p0, p1);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t b0 = p0; b0 < p1; b0 += blockDim.x) {
+    const uint32_t p = b0 + threadIdx.x;
+    const bool active = p < p1;
+    uint32_t key = 0xffffffffu;
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    if (active) {
+      x = pos[3 * size_t(p)];
+      y = pos[3 * size_t(p) + 1];
+      z = pos[3 * size_t(p) + 2];
+      int cx, cy, cz;
+      floor_cells<REL>(pos, p, p - s.part0, s.t, cx, cy, cz);
+      key = tile_key(cx, cy, cz, s.t);
+    }
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key, active);
+    const bool head = active && ((heads >> lane) & 1ull);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cursor[s.tile0 + key], run_length(heads, act));
     const unsigned long long below = heads & ((2ull << lane) - 1ull);
     const int my_head = below ? 63 - __clzll((long long)below) : 0;
     base = __shfl(base, my_head);
@@ -480,7 +539,7 @@ static size_t accum_ws(const SortDims &d, size_t M, int nc, bool rel, size_t *pa
 }
 
 bool binned_paint_supported(int nx, int ny, int nz) {
-  return nx > 0 && ny > 0 && nz > 0 && make_tiles(nx, ny, nz).K <= uint32_t(kMaxKeysLds);
+  return nx > 0 && ny > 0 && nz > 0 && 3 * cells(nx, ny, nz) <= (int64_t(1) << 32);
 }
 
 size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n) {
@@ -534,8 +593,7 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
   const SortWs w = sort_ws(d);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "tile sort needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(pos != nullptr, "positions are null");
-  JPM_REQUIRE(d.maxK <= uint32_t(kMaxKeysLds), "%u tiles in one mesh, above the binned paint's %d", d.maxK,
-              kMaxKeysLds);
+  const bool lds = d.maxK <= uint32_t(kMaxKeysLds);   // per-workgroup LDS histograms fit
   char *base = static_cast<char *>(ws);
   uint32_t *count = reinterpret_cast<uint32_t *>(base + w.count);
   uint32_t *tstart = reinterpret_cast<uint32_t *>(base + w.tstart);
@@ -550,13 +608,18 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
   // 1. tile counts
   hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K);
   JPM_LAUNCHED("zero_u32_kernel");
-  {
+  if (lds) {
     auto kern = rel ? bin_count_kernel<true, G> : bin_count_kernel<false, G>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
     JPM_TIMED(s, "bin_count_kernel");
     hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, count);
     JPM_LAUNCHED("bin_count_kernel");
+  } else {
+    auto kern = rel ? bin_count_global_kernel<true, G> : bin_count_global_kernel<false, G>;
+    JPM_TIMED(s, "bin_count_kernel");
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), 0, s, pos, d.slice, g, count);
+    JPM_LAUNCHED("bin_count_global_kernel");
   }
   // 2. tile starts, scatter cursors, chunk table
   JPM_HIP(hipMemcpyAsync(tstart, count, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
@@ -572,7 +635,7 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
                      reinterpret_cast<ChunkDesc *>(base + w.desc));
   JPM_LAUNCHED("chunk_desc_kernel");
   // 3. scatter particles into tile order
-  {
+  if (lds) {
     auto kern = rel ? bin_scatter_kernel<true, G> : bin_scatter_kernel<false, G>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
@@ -580,6 +643,12 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
     hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, cursor,
                        reinterpret_cast<float4 *>(base + w.spos));
     JPM_LAUNCHED("bin_scatter_kernel");
+  } else {
+    auto kern = rel ? bin_scatter_global_kernel<true, G> : bin_scatter_global_kernel<false, G>;
+    JPM_TIMED(s, "bin_scatter_kernel");
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), 0, s, pos, d.slice, g, cursor,
+                       reinterpret_cast<float4 *>(base + w.spos));
+    JPM_LAUNCHED("bin_scatter_global_kernel");
   }
   out = view_of(g, d, M, rel, ws);
   return JPM_OK;
@@ -592,8 +661,6 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
   if (rc) return rc;
   JPM_REQUIRE(n >= 1 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
   const TileGrid t = make_tiles(nx, ny, nz);
-  JPM_REQUIRE(t.K <= uint32_t(kMaxKeysLds), "mesh (%d,%d,%d) has %u tiles, above the binned paint's %d",
-              nx, ny, nz, t.K, kMaxKeysLds);
   const int64_t M = cells(nx, ny, nz);
   JPM_REQUIRE(3 * M <= (int64_t(1) << 32), "mesh too large");
   const bool rel = flags & JPM_RELATIVE;

@@ -186,7 +186,7 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
-  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs <= 32768 tiles and the 4-point gradient");
+  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
   JPM_REQUIRE(pos && vel && phi, "null pointer argument");
   const SortedWs w = sorted_ws(nx, ny, nz, n, 1, pflags);
@@ -220,7 +220,7 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
-  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs <= 32768 tiles and the 4-point gradient");
+  JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
   JPM_REQUIRE(phi && pos && cot && gpos, "null pointer argument");
   const SortedWs w = sorted_ws(nx, ny, nz, n, 3, pflags);

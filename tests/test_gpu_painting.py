@@ -256,3 +256,27 @@ def test_binned_matches_atomic_at_256(gpu):
         scale = max(1.0, float(b.abs().max()))
         assert float((a - b).abs().max()) < 2e-5 * scale
         assert abs(a.double().sum().item() - n) < 1e-5 * n
+
+
+# Over 32768 tiles the tile sort counts and scatters with wavefront-run
+# global atomics instead of LDS histograms; a thin mesh reaches that regime
+# with few cells (257 x 128 x 1 tiles).
+MANY_TILES = (4112, 2048, 1)
+
+
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_binned_paint_over_32768_tiles(gpu, deterministic):
+    from jaxpm_amd import _ops
+    print(f"binned many tiles det={deterministic}", flush=True)
+    shape = MANY_TILES
+    r = np.random.default_rng(44)
+    n = 400000
+    pos = (r.uniform(-0.2, 1.2, size=(n, 3)) * np.array(shape)).astype(np.float32)
+    pos[:50000] = (r.normal(size=(50000, 3)) * 3 + np.array([100.0, 300.0, 0.5])).astype(np.float32)
+    m = dev(np.zeros(shape))
+    _ops.paint(m, dev(pos), zero=True, deterministic=deterministic)
+    close(host(m), op.cic_paint(np.zeros(shape), pos), rel=2e-6)
+    w3 = r.normal(size=(n, 3)).astype(np.float32)
+    got = host(_ops.paint_multi(shape, dev(pos), dev(w3), deterministic=deterministic))
+    for c in range(3):
+        close(got[c], op.cic_paint(np.zeros(shape), pos, w3[:, c]), rel=2e-6)
