@@ -45,9 +45,10 @@ def parse():
     ap.add_argument("--checkpoint", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48"], default="pm",
+    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48", "large"], default="pm",
                     help="pm: the default forward+adjoint step benchmark; diffsim: the differentiable "
                          "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the sort autotuner (default widths)")
     ap.add_argument("--plain-steps", type=int, default=5,
                     help="leapfrog steps of the plain stored-trajectory autodiff comparison (0: skip)")
     return ap.parse_args()
@@ -161,8 +162,16 @@ def batch48(args):
     nr = np.random.default_rng(7)
     noises = [torch.as_tensor(nr.standard_normal(sh, dtype=np.float32), device=dev) for sh in shapes]
     tnoise = [torch.as_tensor(nr.standard_normal(sh, dtype=np.float32), device=dev) for sh in shapes]
+    tuning = (0, 0)
+    if not args.no_tune:
+        from jaxpm_amd import autotune
+        lat = torch.cat([torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
+                                                     indexing="ij"), -1).reshape(-1, 3) for n in sizes])
+        tuning, _ = autotune.tune_batch(shapes, [n ** 3 for n in sizes], lat + 0.3)
+        del lat
     t0 = time.perf_counter()
-    model = diffsim.BatchLptNBody(shapes, boxes, a_steps, cosmos, order=1, checkpoint_every=5, graph=True)
+    model = diffsim.BatchLptNBody(shapes, boxes, a_steps, cosmos, order=1, checkpoint_every=5, graph=True,
+                                  sort_tuning=tuning)
     t_build = time.perf_counter() - t0
     P = model.parameter_matrix()
     with torch.no_grad():
@@ -194,7 +203,7 @@ def batch48(args):
                                   "Mpc/h, Omega_c~U[0.2,0.35], sigma8~U[0.7,0.9], 1LPT a=0.1 + 20 steps, "
                                   "checkpoint every 5, N-body stage graph-captured",
                       "particles": int(sum(n ** 3 for n in sizes))},
-           "graph_capture_s": round(t_build, 2)}
+           "graph_capture_s": round(t_build, 2), "sort_tuning": list(tuning)}
     if args.plain_steps > 0:
         for method in ("custom", "plain"):
             singles = [diffsim.LptNBody(sh, bx, a_steps, base_cosmo=c, order=1, checkpoint_every=5, method=method)
@@ -220,6 +229,63 @@ def batch48(args):
     print(json.dumps(out))
 
 
+def large(args):
+    """Large single-GPU run: 512^3 particles (a lattice of spacing 2 cells)
+    on a 1024^3 mesh, box 1000 Mpc/h, 2LPT at a=0.1 and 30 leapfrog steps to
+    a=1, checkpoints every 5 steps, forward + adjoint with the gradient of
+    sum(delta^2)/0.1^2 w.r.t. the 1024^3 white-noise field.  Reports steps/s
+    and peak device memory (torch's peak allocation and the device's used
+    memory, which includes the hipFFT plans' work areas)."""
+    import torch
+
+    from jaxpm_amd import _autograd, _lib, diffsim, integrate
+    from jaxpm_amd import cosmology as pc
+    dev = _lib.device()
+    nm, npart, S = 1024, 512, 30
+    shape = (nm, nm, nm)
+    cosmo = pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    spec = diffsim.SimSpec(shape, (1000.0,) * 3, np.linspace(0.1, 1.0, S + 1), base_cosmo=cosmo, order=2)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    noise = torch.randn(shape, generator=gen, device=dev)
+    ax = torch.arange(npart, device=dev, dtype=torch.float32) * 2.0
+    lattice = torch.stack(torch.meshgrid(ax, ax, ax, indexing="ij"), -1)
+    ktab = torch.as_tensor(spec.ktab, dtype=torch.float32, device=dev)
+    stepper = integrate.Stepper(shape, npart ** 3)
+    params = spec.parameter_vector()
+    mean = npart ** 3 / nm ** 3
+
+    def run():
+        nt = noise.clone().requires_grad_(True)
+        q, x0, v0 = spec.initial_state(nt, params, True, lattice, ktab)
+        xS, _ = integrate.NBody.apply(x0, v0, q[spec.slices["kdk"]], stepper, 5)
+        rho = _autograd.CicPaint.apply(torch.zeros(shape, device=dev), xS, None, 1.0, False, False)
+        loss = ((rho / mean - 1.0) ** 2).sum() / 0.1 ** 2
+        loss.backward()
+        return float(loss.detach()), nt.grad
+
+    t0 = time.perf_counter()
+    run()                      # plans, tables
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    loss, g = run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    free, total = torch.cuda.mem_get_info()
+    out = {"metric": "fwd+adjoint leapfrog steps/s, 512^3 particles on a 1024^3 mesh (incl. 2LPT and its adjoint)",
+           "value": round(S / dt, 3), "unit": "steps/s", "n_gpus": 1, "steps": S, "warmup": 1,
+           "ms_per_step": round(1000 * dt / S, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "512^3 particles on 1024^3 mesh, box 1000 Mpc/h, 2LPT a=0.1 + 30 steps, checkpoint "
+                                  "every 5, gradient w.r.t. the 1024^3 noise", "particles": npart ** 3, "mesh": nm},
+           "seconds_per_simulation": round(dt, 3), "first_run_s": round(t_first, 2), "loss": loss,
+           "grad_finite": bool(torch.isfinite(g).all()),
+           "peak_torch_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2),
+           "device_used_gb_after": round((total - free) / 1e9, 2), "device_total_gb": round(total / 1e9, 1)}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -230,6 +296,8 @@ def main():
         return diffsim(args)
     if args.workload == "batch48":
         return batch48(args)
+    if args.workload == "large":
+        return large(args)
     import torch
 
     from jaxpm_amd import _lib, _ops, integrate, painting, pm
@@ -255,7 +323,11 @@ def main():
                                     lattice.reshape(-1, 3) + 0.5 * torch.randn((N, 3), generator=gen, device=dev))
     sigma = 0.1
 
-    stepper = integrate.Stepper(shape, N)
+    tuning = (0, 0)
+    if not args.no_tune:
+        from jaxpm_amd import autotune
+        tuning, _ = autotune.tune_stepper(shape, x0)
+    stepper = integrate.Stepper(shape, N, sort_tuning=tuning)
 
     def fwd_adj(S, trace=None):
         steps = np.linspace(0.1, 1.0, S + 1)
@@ -384,7 +456,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"leapfrog forward+adjoint, {n}^3 particles on {n}^3 mesh, box {args.box:g} Mpc/h, "
                                f"2LPT IC at a=0.1, {args.steps} steps to a=1, checkpoint every {args.checkpoint}",
-                   "mesh": n, "particles": N, "checkpoint_every": args.checkpoint},
+                   "mesh": n, "particles": N, "checkpoint_every": args.checkpoint,
+                   "sort_tuning": {"chunk": 2 ** (9 + tuning[0]) if tuning[0] else 8192,
+                                   "slice": 2 ** (11 + tuning[1]) if tuning[1] else 16384,
+                                   "vjp_density_window": "caches" if len(tuning) > 2 and tuning[2] else "lds"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": PMC_TRAFFIC_BYTES.get((dom, n)),

@@ -49,6 +49,18 @@ extern "C" {
 #define JPM_ATOMIC 8u        /* pipelines: paint with global atomics instead
                                 of the binned LDS paint */
 
+/* Pipeline flags (jpm_pm_forces_f32, jpm_force_*, jpm_batch_create): paint
+ * flags in bits 0..7, k-space flags shifted left by 8, and the tile sort's
+ * tuning in bits 16..24 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
+ * accumulation/gather chunk (c = 1..6, default 8192), JPM_TUNE_SLICE(s):
+ * 2^(11+s) particles per counting/scatter workgroup (s = 1..7, default
+ * 16384).  The choice changes speed, never results beyond float rounding;
+ * a kept sort must be read with the flags it was built with. */
+#define JPM_TUNE_CHUNK(c) ((uint32_t)(c) << 16)
+#define JPM_TUNE_SLICE(s) ((uint32_t)(s) << 20)
+#define JPM_TUNE_RHO_GLOBAL (1u << 24)  /* force-VJP gather: density adjoint
+                                           read through the caches, not LDS */
+
 /* k-space flags */
 #define JPM_K_GRAD_EXACT 1u  /* gradient_kernel order=0 (i k, len//2 zeroed)
                                 instead of the default 4-point order=1 */
@@ -348,10 +360,11 @@ int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos,
                        void *ws, size_t ws_bytes, void *keep_sort,
                        size_t keep_bytes);
 
-/* Bytes of a tile sort of n particles: the `keep_sort` buffer of
- * jpm_force_kick_f32 (optional; when given the sort is built there and
- * stays valid for jpm_force_vjp_pot_f32 of the same positions). */
-size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n);
+/* Bytes of a tile sort of n particles under the pipeline `flags` (their
+ * sort tuning bits): the `keep_sort` buffer of jpm_force_kick_f32 (optional;
+ * when given the sort is built there and stays valid for
+ * jpm_force_vjp_pot_f32 of the same positions and flags). */
+size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n, uint32_t flags);
 
 /* Workspace bytes for jpm_force_vjp_pot_f32 (0 when unsupported, as above). */
 size_t jpm_force_vjp_pot_workspace_size(int nx, int ny, int nz, int64_t n,

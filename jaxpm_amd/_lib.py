@@ -20,6 +20,14 @@ K_GRAD_EXACT = 1
 K_LAPL_FD = 2
 K_DECONV = 4
 
+
+def tune_flags(chunk_code=0, slice_code=0, rho_global=0):
+    """Sort / gather tuning bits of the pipeline flags (JPM_TUNE_CHUNK,
+    JPM_TUNE_SLICE, JPM_TUNE_RHO_GLOBAL): chunk = 2^(9 + chunk_code), slice =
+    2^(11 + slice_code), 0 = default; rho_global reads the force-VJP density
+    adjoint through the caches instead of LDS."""
+    return (int(chunk_code) << 16) | (int(slice_code) << 20) | (int(bool(rho_global)) << 24)
+
 E_INVALID = -1
 
 _vp = ctypes.c_void_p
@@ -84,7 +92,7 @@ SIGNATURES = {
     "jpm_force_vjp_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz]),
     "jpm_force_kick_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _vp, _f, _u32, _i, _vp, _sz, _vp, _sz]),
-    "jpm_bin_sort_size": (_sz, [_i, _i, _i, _i64]),
+    "jpm_bin_sort_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_vjp_pot_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_vjp_pot_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),

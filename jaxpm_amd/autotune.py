@@ -1,0 +1,147 @@
+"""Autotuner for the tile sort and the gathers: picks the chunk width
+(particles per accumulation / gather workgroup), the slice width (particles
+per counting / scatter workgroup) and where the force-VJP gather reads the
+density adjoint (an LDS window or the caches) by timing one force evaluation
+with kick and one force VJP on the caller's own particle state, for every
+candidate.
+
+Results are cached per (device, mesh, particle count, mode) in memory and in
+``$XDG_CACHE_HOME/jaxpm_amd/sort_tuning.json`` (``~/.cache`` by default), so
+the search runs once per configuration and machine.  The choice changes speed
+only: every candidate computes the same forces up to float rounding.
+"""
+import json
+import os
+
+import torch
+
+from . import _lib, integrate
+
+CHUNK_CODES = (2, 3, 4, 5)      # 2048, 4096, 8192, 16384 particles
+SLICE_CODES = (2, 3, 4, 5)      # 8192, 16384, 32768, 65536 particles
+RHO_MODES = (0, 1)              # density adjoint in LDS / through the caches
+
+_memo = {}
+
+
+def _cache_path():
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "jaxpm_amd", "sort_tuning.json")
+
+
+def _load():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, value):
+    data = _load()
+    data[key] = value
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + f".{os.getpid()}"
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def _time_ms(fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _key(kind, shapes, counts, relative):
+    dev = torch.cuda.get_device_name(_lib.device())
+    return f"{dev}|{kind}|{shapes}|{counts}|rel={int(bool(relative))}"
+
+
+def _candidates():
+    return [(c, s, rg) for c in CHUNK_CODES for s in SLICE_CODES for rg in RHO_MODES]
+
+
+def _cached(key, use_cache):
+    if not use_cache:
+        return None
+    hit = _memo.get(key) or _load().get(key)
+    if hit and len(hit.get("best", ())) == 3:
+        _memo[key] = hit
+        return tuple(hit["best"])
+    return None
+
+
+def _finish(key, timings):
+    best = min(timings, key=timings.get)
+    _memo[key] = {"best": list(best), "ms": timings[best]}
+    _store(key, _memo[key])
+    return best, timings
+
+
+def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
+    """Best (chunk_code, slice_code, rho_global) for a single simulation at
+    positions ``pos`` ([n, 3] device tensor); returns (codes, {codes: ms})."""
+    shape = tuple(int(s) for s in mesh_shape)
+    n = pos.reshape(-1, 3).shape[0]
+    key = _key("single", shape, n, relative)
+    hit = _cached(key, use_cache)
+    if hit:
+        return hit, {}
+    x = pos.reshape(-1, 3).contiguous()
+    cot = torch.randn_like(x)
+    timings = {}
+    for tune in _candidates():
+        st = integrate.Stepper(shape, n, relative=relative, sort_tuning=tune)
+        if not st.sorted:
+            return (0, 0, 0), {}
+        field = st.new_field(keep_sort=True)
+        v = torch.zeros_like(x)
+        g = torch.zeros_like(x)
+
+        def step():
+            st.field_kick(x, v, 0.0, field)
+            st.vjp_field(field, x, cot, 1.0, g)
+
+        timings[tune] = _time_ms(step, reps)
+        del st, field
+    return _finish(key, timings)
+
+
+def tune_batch(mesh_shapes, counts, pos, relative=False, reps=3, use_cache=True):
+    """Best (chunk_code, slice_code, rho_global) for a batch (BatchStepper
+    layout)."""
+    shapes = [tuple(int(s) for s in sh) for sh in mesh_shapes]
+    counts = [int(c) for c in counts]
+    key = _key("batch", shapes, counts, relative)
+    hit = _cached(key, use_cache)
+    if hit:
+        return hit, {}
+    x = pos.reshape(-1, 3).contiguous()
+    cot = torch.randn_like(x)
+    timings = {}
+    for tune in _candidates():
+        st = integrate.BatchStepper(shapes, counts, relative=relative, sort_tuning=tune)
+        field = st.new_field(keep_sort=True)
+        zero = torch.zeros(st.B, dtype=torch.float32, device=x.device)
+        one = torch.ones(st.B, dtype=torch.float32, device=x.device)
+        v = torch.zeros_like(x)
+        g = torch.zeros_like(x)
+
+        def step():
+            st.field_kick(x, v, zero, field)
+            st.vjp_field(field, x, cot, one, one, g)
+
+        timings[tune] = _time_ms(step, reps)
+        del st, field
+    return _finish(key, timings)

@@ -22,7 +22,8 @@ int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *
                            uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int);
-int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *);
+int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *,
+               bool);
 
 // ------------------------------------------------------------ description
 
@@ -36,6 +37,9 @@ int batch_create(int B, const int *dims, const int64_t *counts, uint32_t flags, 
   JPM_REQUIRE(bt, "out of host memory");
   bt->B = B;
   bt->rel = rel;
+  bt->chunk = tuned_chunk(flags);
+  bt->slice = tuned_slice(flags);
+  bt->rho_lds = tuned_rho_lds(flags);
   uint64_t K = 0, N = 0, M = 0, H = 0, G = 0;
   uint32_t maxK = 0;
   for (int b = 0; b < B; ++b) {
@@ -84,7 +88,7 @@ int batch_create(int B, const int *dims, const int64_t *counts, uint32_t flags, 
     N += uint64_t(n);
     M += uint64_t(m);
     H += uint64_t(half_cells(nx, ny, nz));
-    G += (uint64_t(n) + kSlice - 1) / kSlice;
+    G += (uint64_t(n) + bt->slice - 1) / bt->slice;
     maxK = std::max(maxK, t.K);
   }
   if (N > uint64_t(kMaxCells) || 3 * M > (uint64_t(1) << 32) || 3 * H > (uint64_t(1) << 32)) {
@@ -213,7 +217,7 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
       return rc;
     if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, rho + c.cell0))) return rc;
   }
-  return batch_grad(s, b, phi, rho, cot, scale, gpos);
+  return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds);
 }
 
 // ------------------------------------------------- per-simulation vectors

@@ -224,23 +224,23 @@ static int exclusive_scan(hipStream_t s, uint32_t *a, uint32_t n, uint32_t *sums
 
 // nchunk[k] = ceil(count[k] / kChunk).
 __global__ void __launch_bounds__(256)
-tile_chunks_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t *__restrict__ nchunk) {
+tile_chunks_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk, uint32_t *__restrict__ nchunk) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x)
-    nchunk[k] = (count[k] + kChunk - 1) / kChunk;
+    nchunk[k] = (count[k] + chunk - 1) / chunk;
 }
 
 __global__ void __launch_bounds__(256)
 chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__restrict__ tile_count,
-                  const uint32_t *__restrict__ chunk_off, uint32_t K, ChunkDesc *__restrict__ desc) {
+                  const uint32_t *__restrict__ chunk_off, uint32_t K, uint32_t chunk, ChunkDesc *__restrict__ desc) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < K; k += gridDim.x * blockDim.x) {
     const uint32_t st = tile_start[k], cnt = tile_count[k];
     uint32_t c = chunk_off[k];
-    for (uint32_t o = 0; o < cnt; o += kChunk, ++c) {
+    for (uint32_t o = 0; o < cnt; o += chunk, ++c) {
       ChunkDesc d;
       d.tile = k;
       d.start = st + o;
-      d.count = min(uint32_t(kChunk), cnt - o);
-      d.pad = cnt > uint32_t(kChunk) ? 1u : 0u;  // tile split over several chunks
+      d.count = min(chunk, cnt - o);
+      d.pad = cnt > chunk ? 1u : 0u;  // tile split over several chunks
       desc[c] = d;
     }
   }
@@ -489,17 +489,18 @@ static int set_lds(K kernel, size_t bytes) {
 // workgroups of `slice` particles, chunk-table capacity, and the largest
 // per-simulation tile count (the LDS histogram size).
 struct SortDims {
-  uint32_t K, n, G, slice, maxchunks, maxK, B;
+  uint32_t K, n, G, slice, chunk, maxchunks, maxK, B;
 };
 
-static SortDims one_dims(const TileGrid &t, int64_t n) {
+static SortDims one_dims(const TileGrid &t, int64_t n, uint32_t flags) {
   SortDims d;
   const uint64_t nn = uint64_t(n > 0 ? n : 1);
   d.K = t.K;
   d.n = uint32_t(n);
-  d.slice = kSlice;
+  d.slice = tuned_slice(flags);
+  d.chunk = tuned_chunk(flags);
   d.G = uint32_t((nn + d.slice - 1) / d.slice);
-  d.maxchunks = uint32_t((nn + kChunk - 1) / kChunk + t.K);
+  d.maxchunks = uint32_t((nn + d.chunk - 1) / d.chunk + t.K);
   d.maxK = t.K;
   d.B = 1;
   return d;
@@ -542,21 +543,21 @@ bool binned_paint_supported(int nx, int ny, int nz) {
   return nx > 0 && ny > 0 && nz > 0 && 3 * cells(nx, ny, nz) <= (int64_t(1) << 32);
 }
 
-size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n) {
+size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return sort_ws(one_dims(make_tiles(nx, ny, nz), n)).total;
+  return sort_ws(one_dims(make_tiles(nx, ny, nz), n, flags)).total;
 }
 
 size_t bins_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
   size_t a, b, c;
-  return accum_ws(one_dims(make_tiles(nx, ny, nz), n), size_t(cells(nx, ny, nz)), nc, flags & JPM_RELATIVE, &a, &b,
+  return accum_ws(one_dims(make_tiles(nx, ny, nz), n, flags), size_t(cells(nx, ny, nz)), nc, flags & JPM_RELATIVE, &a, &b,
                   &c);
 }
 
 size_t binned_paint_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || nc < 1) return 0;
-  return align_up(bin_sort_workspace_size(nx, ny, nz, n)) + bins_paint_workspace_size(nx, ny, nz, n, nc, flags);
+  return align_up(bin_sort_workspace_size(nx, ny, nz, n, flags)) + bins_paint_workspace_size(nx, ny, nz, n, nc, flags);
 }
 
 template <class G>
@@ -569,6 +570,7 @@ static BinsT<G> view_of(const G &g, const SortDims &d, uint32_t M, bool rel, con
   b.n = d.n;
   b.M = M;
   b.maxchunks = d.maxchunks;
+  b.chunk = d.chunk;
   b.rel = rel;
   b.spos = reinterpret_cast<const float4 *>(base + w.spos);
   b.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
@@ -582,7 +584,7 @@ static BinsT<G> view_of(const G &g, const SortDims &d, uint32_t M, bool rel, con
 Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws) {
   const TileGrid t = make_tiles(nx, ny, nz);
   OneSim g{one_sim(nx, ny, nz, n)};
-  return view_of(g, one_dims(t, n), uint32_t(cells(nx, ny, nz)), flags & JPM_RELATIVE, ws);
+  return view_of(g, one_dims(t, n, flags), uint32_t(cells(nx, ny, nz)), flags & JPM_RELATIVE, ws);
 }
 
 // Sort the particles of every simulation into its tiles (workspace of
@@ -626,13 +628,13 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
   rc = exclusive_scan(s, tstart, d.K, sums, totals);
   if (rc) return rc;
   JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K, nchunk);
+  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K, d.chunk, nchunk);
   JPM_LAUNCHED("tile_chunks_kernel");
   JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
   rc = exclusive_scan(s, choff, d.K, sums, totals + 1);
   if (rc) return rc;
   hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, tstart, count, choff, d.K,
-                     reinterpret_cast<ChunkDesc *>(base + w.desc));
+                     d.chunk, reinterpret_cast<ChunkDesc *>(base + w.desc));
   JPM_LAUNCHED("chunk_desc_kernel");
   // 3. scatter particles into tile order
   if (lds) {
@@ -667,14 +669,16 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
   JPM_REQUIRE(!rel || n == M, "relative mode needs one particle per lattice node (n=%lld, mesh %lld)",
               (long long)n, (long long)M);
   OneSim g{one_sim(nx, ny, nz, n)};
-  return sort_into(s, g, one_dims(t, n), uint32_t(M), rel, pos, ws, ws_bytes, out);
+  return sort_into(s, g, one_dims(t, n, flags), uint32_t(M), rel, pos, ws, ws_bytes, out);
 }
 
 // nc meshes (stride b.M) = [mesh +] window sums of the sorted particles,
 // channel c with weight[p*nc + c] (or wscalar when weight is NULL).
 template <class G>
-static int paint_sorted(hipStream_t s, const BinsT<G> &b, const SortDims &d, const float *weight, float wscalar,
-                        int nc, float *mesh, uint32_t flags, void *ws, size_t ws_bytes) {
+static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, float wscalar, int nc, float *mesh,
+                        uint32_t flags, void *ws, size_t ws_bytes) {
+  SortDims d{};
+  d.maxchunks = b.maxchunks;
   JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
   JPM_REQUIRE(mesh != nullptr, "mesh is null");
   const int64_t M = b.M;
@@ -716,7 +720,7 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const SortDims &d, con
 
 int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
                void *ws, size_t ws_bytes) {
-  return paint_sorted(s, b, one_dims(b.g.s.t, b.n), weight, wscalar, nc, mesh, flags, ws, ws_bytes);
+  return paint_sorted(s, b, weight, wscalar, nc, mesh, flags, ws, ws_bytes);
 }
 
 int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
@@ -734,7 +738,7 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
     if (flags & JPM_ZERO_MESH) return zero_f32(s, mesh, nc * M);
     return JPM_OK;
   }
-  const size_t sort_bytes = align_up(bin_sort_workspace_size(nx, ny, nz, n));
+  const size_t sort_bytes = align_up(bin_sort_workspace_size(nx, ny, nz, n, flags));
   Bins b;
   rc = bin_sort(s, nx, ny, nz, pos, n, flags, ws, sort_bytes, b);
   if (rc) return rc;
@@ -747,9 +751,10 @@ static SortDims batch_dims(const SimBatch &bt) {
   SortDims d;
   d.K = bt.K;
   d.n = bt.N;
-  d.slice = kSlice;
+  d.slice = bt.slice;
+  d.chunk = bt.chunk;
   d.G = bt.G;
-  d.maxchunks = uint32_t((uint64_t(bt.N) + kChunk - 1) / kChunk + bt.K + uint32_t(bt.B));
+  d.maxchunks = uint32_t((uint64_t(bt.N) + bt.chunk - 1) / bt.chunk + bt.K + uint32_t(bt.B));
   d.maxK = bt.maxK;
   d.B = uint32_t(bt.B);
   return d;
@@ -774,7 +779,7 @@ BatchBins batch_view(const SimBatch &bt, const void *ws) {
 
 int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
                 void *ws, size_t ws_bytes) {
-  return paint_sorted(s, b, batch_dims(bt), weight, 1.0f, nc, mesh, JPM_ZERO_MESH, ws, ws_bytes);
+  return paint_sorted(s, b, weight, 1.0f, nc, mesh, JPM_ZERO_MESH, ws, ws_bytes);
 }
 
 }  // namespace jpm

@@ -122,6 +122,11 @@ bool binned_paint_supported(int, int, int);
 
 // Paint used inside the pipelines: the binned LDS paint unless JPM_ATOMIC or
 // the mesh has more tiles than the binned paint's LDS histogram holds.
+// Pipeline flags: paint flags in bits 0..7, k-space flags in bits 8..15,
+// sort and gather tuning in bits 16..24 (which travel with the paint flags).
+static uint32_t paint_flags(uint32_t flags) { return flags & 0x01ff00ffu; }
+static uint32_t kspace_flags(uint32_t flags) { return (flags >> 8) & 0xffu; }
+
 static bool use_atomic(int nx, int ny, int nz, uint32_t pflags) {
   return (pflags & JPM_ATOMIC) || !binned_paint_supported(nx, ny, nz);
 }
@@ -146,7 +151,7 @@ int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, floa
 int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t);
 size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
 int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int);
-int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *);
+int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool);
 
 // ------------------------------------------------------ sorted pipelines
 //
@@ -166,7 +171,7 @@ static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pf
   const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
   SortedWs w;
   size_t o = 0;
-  w.sort = o;  o += align_up(bin_sort_workspace_size(nx, ny, nz, n));
+  w.sort = o;  o += align_up(bin_sort_workspace_size(nx, ny, nz, n, pflags));
   w.paint = o; o += align_up(bins_paint_workspace_size(nx, ny, nz, n, nc, pflags & JPM_RELATIVE));
   w.mesh = o;  o += align_up(m * 4);
   w.g3 = o;    o += (nc == 3) ? align_up(3 * m * 4) : 0;
@@ -177,7 +182,7 @@ static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pf
 }
 
 static bool sorted_supported(int nx, int ny, int nz, uint32_t flags) {
-  return binned_paint_supported(nx, ny, nz) && !((flags >> 8) & JPM_K_GRAD_EXACT) && !(flags & JPM_ATOMIC);
+  return binned_paint_supported(nx, ny, nz) && !(kspace_flags(flags) & JPM_K_GRAD_EXACT) && !(flags & JPM_ATOMIC);
 }
 
 static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
@@ -185,7 +190,7 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
                       void *keep_sort, size_t keep_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
   JPM_REQUIRE(pos && vel && phi, "null pointer argument");
@@ -195,7 +200,7 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   const int64_t m = cells(nx, ny, nz);
   Bins b;
   if (keep_sort) {
-    JPM_REQUIRE(keep_bytes >= bin_sort_workspace_size(nx, ny, nz, n), "keep_sort buffer too small (%zu bytes)",
+    JPM_REQUIRE(keep_bytes >= bin_sort_workspace_size(nx, ny, nz, n, pflags), "keep_sort buffer too small (%zu bytes)",
                 keep_bytes);
     rc = bin_sort(s, nx, ny, nz, pos, n, pflags, keep_sort, keep_bytes, b);
   } else {
@@ -219,7 +224,7 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
                          size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
   JPM_REQUIRE(phi && pos && cot && gpos, "null pointer argument");
@@ -230,7 +235,7 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   Bins b;
   if (sorted) {
     // the sort of these positions kept by jpm_force_kick_f32
-    JPM_REQUIRE(sorted_bytes >= bin_sort_workspace_size(nx, ny, nz, n), "sorted buffer too small (%zu bytes)",
+    JPM_REQUIRE(sorted_bytes >= bin_sort_workspace_size(nx, ny, nz, n, pflags), "sorted buffer too small (%zu bytes)",
                 sorted_bytes);
     b = bins_view(nx, ny, nz, n, pflags, sorted);
   } else {
@@ -250,7 +255,7 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   if (rc) return rc;
   rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, rho);
   if (rc) return rc;
-  return bins_grad(s, b, phi, rho, cot, scale, gpos);
+  return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags));
 }
 
 // Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
@@ -275,7 +280,7 @@ static int force_field(hipStream_t s, int nx, int ny, int nz, const float *pos, 
                        float *phi3, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   const FieldWs w = field_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "force_field needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(phi3 != nullptr, "phi3 is null");
@@ -322,7 +327,7 @@ static int force_vjp(hipStream_t s, int nx, int ny, int nz, const float *phi3, c
                      size_t ws_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   const VjpWs w = vjp_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "force_vjp needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(phi3 && pos && cot && gpos, "null pointer argument");
@@ -380,7 +385,7 @@ static int pm_forces(hipStream_t s, int nx, int ny, int nz, const float *pos, in
                      float *forces, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  const uint32_t pflags = flags & 0xffu, kflags = flags >> 8;
+  const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   const ForceWs w = force_ws(nx, ny, nz, n, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "pm_forces needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(forces != nullptr, "forces is null");
@@ -653,7 +658,7 @@ int jpm_power_spectrum_f32(void *stream, int nx, int ny, int nz, const void *spe
 
 size_t jpm_force_field_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return field_ws(nx, ny, nz, n, flags & 0xffu).total;
+  return field_ws(nx, ny, nz, n, paint_flags(flags)).total;
 }
 
 int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
@@ -663,7 +668,7 @@ int jpm_force_field_f32(void *stream, int nx, int ny, int nz, const float *pos, 
 
 size_t jpm_force_vjp_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return vjp_ws(nx, ny, nz, n, flags & 0xffu).total;
+  return vjp_ws(nx, ny, nz, n, paint_flags(flags)).total;
 }
 
 int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n,
@@ -683,12 +688,12 @@ int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3, const 
 
 size_t jpm_pm_forces_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return force_ws(nx, ny, nz, n, flags & 0xffu).total;
+  return force_ws(nx, ny, nz, n, paint_flags(flags)).total;
 }
 
 size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || !sorted_supported(nx, ny, nz, flags)) return 0;
-  return sorted_ws(nx, ny, nz, n, 1, flags & 0xffu).total;
+  return sorted_ws(nx, ny, nz, n, 1, paint_flags(flags)).total;
 }
 
 int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
@@ -698,14 +703,14 @@ int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos, i
                        ws_bytes, keep_sort, keep_bytes));
 }
 
-size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n) {
+size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0) return 0;
-  return bin_sort_workspace_size(nx, ny, nz, n);
+  return bin_sort_workspace_size(nx, ny, nz, n, paint_flags(flags));
 }
 
 size_t jpm_force_vjp_pot_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || !sorted_supported(nx, ny, nz, flags)) return 0;
-  return sorted_ws(nx, ny, nz, n, 3, flags & 0xffu).total;
+  return sorted_ws(nx, ny, nz, n, 3, paint_flags(flags)).total;
 }
 
 int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,

@@ -1,4 +1,4 @@
-// hipFFT plan cache: one R2C / C2R plan per (shape, direction, batch),
+// hipFFT plan cache: one R2C / C2R / C2C plan per (shape, type, batch),
 // created on first use and kept until jpm_shutdown.  Plans are bound to the
 // caller's stream under the cache mutex right before execution, so two
 // threads never race on one plan's stream.
@@ -58,54 +58,83 @@ int plan_locked(int nx, int ny, int nz, hipfftType type, int batch, hipfftHandle
 }
 }  // namespace
 
+// Batches execute as sub-batches of at most kMaxFftElems points each, so no
+// plan's work area or offsets grow past what one 1024^3 transform needs.
+constexpr long long kMaxFftElems = 1LL << 30;
+
+static int sub_batch(int nx, int ny, int nz, int batch) {
+  const long long m = cells(nx, ny, nz);
+  long long sub = kMaxFftElems / m;
+  if (sub < 1) sub = 1;
+  return int(sub < batch ? sub : batch);
+}
+
 int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
+  const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
+  const int sub = sub_batch(nx, ny, nz, batch);
   std::lock_guard<std::mutex> lk(g_mu);
-  hipfftHandle h;
-  rc = plan_locked(nx, ny, nz, HIPFFT_R2C, batch, h);
-  if (rc) return rc;
   JPM_TIMED(s, "hipfft_r2c");
-  hipfftResult r = hipfftSetStream(h, s);
-  if (r == HIPFFT_SUCCESS)
-    r = hipfftExecR2C(h, const_cast<hipfftReal *>(in), static_cast<hipfftComplex *>(out));
-  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecR2C: %s", fft_err(r));
+  for (int b0 = 0; b0 < batch; b0 += sub) {
+    const int nb = batch - b0 < sub ? batch - b0 : sub;
+    hipfftHandle h;
+    rc = plan_locked(nx, ny, nz, HIPFFT_R2C, nb, h);
+    if (rc) return rc;
+    hipfftResult r = hipfftSetStream(h, s);
+    if (r == HIPFFT_SUCCESS)
+      r = hipfftExecR2C(h, const_cast<hipfftReal *>(in) + size_t(b0) * M,
+                        static_cast<hipfftComplex *>(out) + size_t(b0) * H);
+    if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecR2C: %s", fft_err(r));
+  }
   return JPM_OK;
 }
 
 int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
+  const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
+  const int sub = sub_batch(nx, ny, nz, batch);
   std::lock_guard<std::mutex> lk(g_mu);
-  hipfftHandle h;
-  rc = plan_locked(nx, ny, nz, HIPFFT_C2R, batch, h);
-  if (rc) return rc;
   JPM_TIMED(s, "hipfft_c2r");
-  hipfftResult r = hipfftSetStream(h, s);
-  if (r == HIPFFT_SUCCESS) r = hipfftExecC2R(h, static_cast<hipfftComplex *>(in), out);
-  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2R: %s", fft_err(r));
+  for (int b0 = 0; b0 < batch; b0 += sub) {
+    const int nb = batch - b0 < sub ? batch - b0 : sub;
+    hipfftHandle h;
+    rc = plan_locked(nx, ny, nz, HIPFFT_C2R, nb, h);
+    if (rc) return rc;
+    hipfftResult r = hipfftSetStream(h, s);
+    if (r == HIPFFT_SUCCESS)
+      r = hipfftExecC2R(h, static_cast<hipfftComplex *>(in) + size_t(b0) * H, out + size_t(b0) * M);
+    if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2R: %s", fft_err(r));
+  }
   return JPM_OK;
 }
 
 int fft_c2c(hipStream_t s, int nx, int ny, int nz, int batch, const void *in, void *out, int direction) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  JPM_REQUIRE(batch >= 1 && batch <= 64, "batch %d out of range", batch);
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(direction == -1 || direction == 1, "direction must be -1 (forward) or +1 (inverse)");
   JPM_REQUIRE(in && out, "null FFT buffer");
+  const size_t M = size_t(cells(nx, ny, nz));
+  const int sub = sub_batch(nx, ny, nz, batch);
   std::lock_guard<std::mutex> lk(g_mu);
-  hipfftHandle h;
-  rc = plan_locked(nx, ny, nz, HIPFFT_C2C, batch, h);
-  if (rc) return rc;
-  hipfftResult r = hipfftSetStream(h, s);
-  if (r == HIPFFT_SUCCESS)
-    r = hipfftExecC2C(h, static_cast<hipfftComplex *>(const_cast<void *>(in)), static_cast<hipfftComplex *>(out),
-                      direction == -1 ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
-  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2C: %s", fft_err(r));
+  for (int b0 = 0; b0 < batch; b0 += sub) {
+    const int nb = batch - b0 < sub ? batch - b0 : sub;
+    hipfftHandle h;
+    rc = plan_locked(nx, ny, nz, HIPFFT_C2C, nb, h);
+    if (rc) return rc;
+    hipfftResult r = hipfftSetStream(h, s);
+    if (r == HIPFFT_SUCCESS)
+      r = hipfftExecC2C(h, static_cast<hipfftComplex *>(const_cast<void *>(in)) + size_t(b0) * M,
+                        static_cast<hipfftComplex *>(out) + size_t(b0) * M,
+                        direction == -1 ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
+    if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2C: %s", fft_err(r));
+  }
   return JPM_OK;
 }
 

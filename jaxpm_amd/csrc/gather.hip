@@ -100,6 +100,58 @@ __device__ __forceinline__ void stencil_lds(const float *win, int a, int b, int 
   F[2] = -(kC8 * (win[e + 1] - win[e - 1]) - kC1 * (win[e + 2] - win[e - 2]));
 }
 
+// -D phi at all 8 CIC corners (a..a+1, b..b+1, c..c+1) of one particle
+// (window indices already offset by the halo), F[d][o] with o = bx | by << 1
+// | bz << 2.  The 24 stencils touch 56 distinct cells: the 2^3 corner cells
+// and, per axis and corner column, the cells at -2, -1, +2, +3.  Each F is
+// the same expression as stencil_lds, so the results agree bit for bit.
+__device__ __forceinline__ void stencil8_lds(const float *win, int a, int b, int c, float F[3][8]) {
+  const int sx = kPw * kPw, sy = kPw;
+  const int e0 = (a * kPw + b) * kPw + c;
+  float C[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) C[o] = win[e0 + (o & 1) * sx + ((o >> 1) & 1) * sy + ((o >> 2) & 1)];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // x: column (b + j, c + k); corners o = 0|q<<1 and 1|q<<1
+    const int j = q & 1, k = q >> 1;
+    const int ex = e0 + j * sy + k;
+    const float m2 = win[ex - 2 * sx], m1 = win[ex - sx], p2 = win[ex + 2 * sx], p3 = win[ex + 3 * sx];
+    const float c0 = C[(j << 1) | (k << 2)], c1 = C[1 | (j << 1) | (k << 2)];
+    F[0][(j << 1) | (k << 2)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    F[0][1 | (j << 1) | (k << 2)] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // y: column (a + i, c + k)
+    const int i = q & 1, k = q >> 1;
+    const int ey = e0 + i * sx + k;
+    const float m2 = win[ey - 2 * sy], m1 = win[ey - sy], p2 = win[ey + 2 * sy], p3 = win[ey + 3 * sy];
+    const float c0 = C[i | (k << 2)], c1 = C[i | 2 | (k << 2)];
+    F[1][i | (k << 2)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    F[1][i | 2 | (k << 2)] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    // z: column (a + i, b + j)
+    const int i = q & 1, j = q >> 1;
+    const int ez = e0 + i * sx + j * sy;
+    const float m2 = win[ez - 2], m1 = win[ez - 1], p2 = win[ez + 2], p3 = win[ez + 3];
+    const float c0 = C[i | (j << 1)], c1 = C[i | (j << 1) | 4];
+    F[2][i | (j << 1)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    F[2][i | (j << 1) | 4] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+}
+
+// The particle's 8 corners form one 2^3 block inside the window (always for
+// absolute positions; relative ones can wrap or round outside).
+__device__ __forceinline__ bool corners_in_window(const CornerSet &cs) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) ok = ok && cs.l[d][0] >= 0 && cs.l[d][1] == cs.l[d][0] + 1;
+  return ok;
+}
+
 // Same stencil read straight from the mesh (out-of-window fallback).
 __device__ __forceinline__ void stencil_global(const float *__restrict__ phi, const TileGrid &t, int x, int y, int z,
                                                float F[3]) {
@@ -137,20 +189,33 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
     CornerSet cs;
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     float s[3] = {0.0f, 0.0f, 0.0f};
+    if (!REL || corners_in_window(cs)) {
+      float F8[3][8];
+      stencil8_lds(win, cs.l[0][0] + kHalo, cs.l[1][0] + kHalo, cs.l[2][0] + kHalo, F8);
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
-      const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
-      if (REL && (la == -2 || lb == -2 || lc == -2)) continue;
-      const float wc = cs.w[0][bx] * cs.w[1][by] * cs.w[2][bz];
-      float F[3];
-      if (!REL || (la >= 0 && lb >= 0 && lc >= 0))
-        stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
-      else
-        stencil_global(phi, t, cs.g[0][bx], cs.g[1][by], cs.g[2][bz], F);
-      s[0] += F[0] * wc;
-      s[1] += F[1] * wc;
-      s[2] += F[2] * wc;
+      for (int o = 0; o < 8; ++o) {
+        const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+        const float wc = cs.w[0][bx] * cs.w[1][by] * cs.w[2][bz];
+        s[0] += F8[0][o] * wc;
+        s[1] += F8[1][o] * wc;
+        s[2] += F8[2][o] * wc;
+      }
+    } else {
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+        const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
+        if (la == -2 || lb == -2 || lc == -2) continue;
+        const float wc = cs.w[0][bx] * cs.w[1][by] * cs.w[2][bz];
+        float F[3];
+        if (la >= 0 && lb >= 0 && lc >= 0)
+          stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+        else
+          stencil_global(phi, t, cs.g[0][bx], cs.g[1][by], cs.g[2][bz], F);
+        s[0] += F[0] * wc;
+        s[1] += F[1] * wc;
+        s[2] += F[2] * wc;
+      }
     }
     const uint32_t idx = __float_as_uint(rec.w);
     float *v = vel + 3 * size_t(idx);
@@ -168,13 +233,16 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
 
 // gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx, scale
 // per simulation (scale_dev[sim]) when scale_dev is given.
-template <bool REL, class G>
+// RLDS: the density-adjoint window in LDS (true) or read from the mesh
+// through the caches (false: 37 KiB of LDS per workgroup instead of 57 KiB,
+// twice the resident workgroups).
+template <bool REL, class G, bool RLDS>
 __global__ void __launch_bounds__(256)
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
                 const float *__restrict__ cot, float scale_s, const float *__restrict__ scale_dev,
                 float *__restrict__ gpos) {
   __shared__ float win[kPwCells];
-  __shared__ float rwin[kWinCells];
+  __shared__ float rwin[RLDS ? kWinCells : 1];
   const uint32_t c = blockIdx.x;
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
@@ -188,7 +256,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
-  load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
+  if (RLDS) load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
     const float4 rec = bins.spos[d.start + q];
@@ -197,30 +265,62 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     const float u0 = cot[3 * size_t(idx)], u1 = cot[3 * size_t(idx) + 1], u2 = cot[3 * size_t(idx) + 2];
     float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
+    if (!REL || corners_in_window(cs)) {
+      float F8[3][8];
+      const int la0 = cs.l[0][0], lb0 = cs.l[1][0], lc0 = cs.l[2][0];
+      stencil8_lds(win, la0 + kHalo, lb0 + kHalo, lc0 + kHalo, F8);
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
-      const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
-      if (REL && (la == -2 || lb == -2 || lc == -2)) continue;
-      float F[3], rv;
-      if (!REL || (la >= 0 && lb >= 0 && lc >= 0)) {
-        stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
-        rv = rwin[(la * kWin + lb) * kWin + lc];
-      } else {
-        const int x = cs.g[0][bx], y = cs.g[1][by], z = cs.g[2][bz];
-        stencil_global(phi, t, x, y, z, F);
-        rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
+      for (int o = 0; o < 8; ++o) {
+        const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+        float rv;
+        if (RLDS) {
+          rv = rwin[((la0 + bx) * kWin + lb0 + by) * kWin + lc0 + bz];
+        } else {
+          const int x = wrapn(ox + la0 + bx, t.nx), y = wrapn(oy + lb0 + by, t.ny), z = wrapn(oz + lc0 + bz, t.nz);
+          rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
+        }
+        const float f = u0 * F8[0][o] + u1 * F8[1][o] + u2 * F8[2][o];
+        const float dwx = cs.d[0][bx] * cs.w[1][by] * cs.w[2][bz];
+        const float dwy = cs.w[0][bx] * cs.d[1][by] * cs.w[2][bz];
+        const float dwz = cs.w[0][bx] * cs.w[1][by] * cs.d[2][bz];
+        gd[0] += f * dwx;
+        gd[1] += f * dwy;
+        gd[2] += f * dwz;
+        gr[0] += rv * dwx;
+        gr[1] += rv * dwy;
+        gr[2] += rv * dwz;
       }
-      const float f = u0 * F[0] + u1 * F[1] + u2 * F[2];
-      const float dwx = cs.d[0][bx] * cs.w[1][by] * cs.w[2][bz];
-      const float dwy = cs.w[0][bx] * cs.d[1][by] * cs.w[2][bz];
-      const float dwz = cs.w[0][bx] * cs.w[1][by] * cs.d[2][bz];
-      gd[0] += f * dwx;
-      gd[1] += f * dwy;
-      gd[2] += f * dwz;
-      gr[0] += rv * dwx;
-      gr[1] += rv * dwy;
-      gr[2] += rv * dwz;
+    } else {
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+        const int la = cs.l[0][bx], lb = cs.l[1][by], lc = cs.l[2][bz];
+        if (la == -2 || lb == -2 || lc == -2) continue;
+        float F[3], rv;
+        if (la >= 0 && lb >= 0 && lc >= 0) {
+          stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+          if (RLDS) {
+            rv = rwin[(la * kWin + lb) * kWin + lc];
+          } else {
+            const int x = wrapn(ox + la, t.nx), y = wrapn(oy + lb, t.ny), z = wrapn(oz + lc, t.nz);
+            rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
+          }
+        } else {
+          const int x = cs.g[0][bx], y = cs.g[1][by], z = cs.g[2][bz];
+          stencil_global(phi, t, x, y, z, F);
+          rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
+        }
+        const float f = u0 * F[0] + u1 * F[1] + u2 * F[2];
+        const float dwx = cs.d[0][bx] * cs.w[1][by] * cs.w[2][bz];
+        const float dwy = cs.w[0][bx] * cs.d[1][by] * cs.w[2][bz];
+        const float dwz = cs.w[0][bx] * cs.w[1][by] * cs.d[2][bz];
+        gd[0] += f * dwx;
+        gd[1] += f * dwy;
+        gd[2] += f * dwz;
+        gr[0] += rv * dwx;
+        gr[1] += rv * dwy;
+        gr[2] += rv * dwz;
+      }
     }
     float *g = gpos + 3 * size_t(idx);
     g[0] += scale * gd[0] + gr[0];
@@ -252,14 +352,13 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
 
 template <class G>
 static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const float *rho_bar, const float *cot,
-                       float scale, const float *scale_dev, float *gpos) {
+                       float scale, const float *scale_dev, float *gpos, bool rho_lds) {
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
   const dim3 grid(b.maxchunks), block(256);
   JPM_TIMED(s, "bin_grad_kernel");
-  if (b.rel)
-    hipLaunchKernelGGL((bin_grad_kernel<true, G>), grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
-  else
-    hipLaunchKernelGGL((bin_grad_kernel<false, G>), grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
+  auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
+                    : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
+  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
   JPM_LAUNCHED("bin_grad_kernel");
   return JPM_OK;
 }
@@ -270,8 +369,8 @@ int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float 
 }
 
 int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
-               const float *scale_dev, float *gpos) {
-  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos);
+               const float *scale_dev, float *gpos, bool rho_lds) {
+  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos, rho_lds);
 }
 
 int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces) {
@@ -279,8 +378,8 @@ int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float
 }
 
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
-              float *gpos) {
-  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos);
+              float *gpos, bool rho_lds) {
+  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds);
 }
 
 }  // namespace jpm

@@ -608,7 +608,7 @@ int kspace_shear6(hipStream_t s, int nx, int ny, int nz, const void *dk, void *o
 }
 
 int lpt2_source(hipStream_t s, int64_t m, const float *phi6, float *delta2) {
-  JPM_REQUIRE(m > 0 && 6 * m <= (int64_t(1) << 32), "mesh size %lld out of range", (long long)m);
+  JPM_REQUIRE(m > 0 && m < (int64_t(1) << 32), "mesh size %lld out of range", (long long)m);  // size_t channel offsets
   JPM_REQUIRE(phi6 && delta2, "null pointer argument");
   hipLaunchKernelGGL(lpt2_source_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, phi6,
                      uint32_t(m), delta2);
@@ -653,7 +653,7 @@ int kspace_shear6_adjoint(hipStream_t s, int nx, int ny, int nz, const void *in6
 }
 
 int lpt2_source_vjp(hipStream_t s, int64_t m, const float *phi6, const float *gbar, float *out6) {
-  JPM_REQUIRE(m > 0 && 6 * m <= (int64_t(1) << 32), "mesh size %lld out of range", (long long)m);
+  JPM_REQUIRE(m > 0 && m < (int64_t(1) << 32), "mesh size %lld out of range", (long long)m);  // size_t channel offsets
   JPM_REQUIRE(phi6 && gbar && out6, "null pointer argument");
   hipLaunchKernelGGL(lpt2_source_vjp_kernel, dim3(grid_for(m, 256, 256 * 128)), dim3(256), 0, s, phi6, gbar,
                      uint32_t(m), out6);

@@ -11,9 +11,25 @@ namespace jpm {
 constexpr int kTile = 16;                       // tile edge in cells
 constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
 constexpr int kWinCells = kWin * kWin * kWin;   // 4913
-constexpr int kChunk = 8192;                    // max particles per chunk
+constexpr int kChunk = 8192;                    // default max particles per chunk
 constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
-constexpr uint32_t kSlice = 16384;              // particles per count / scatter workgroup
+constexpr uint32_t kSlice = 16384;              // default particles per count / scatter workgroup
+
+// Sort tuning carried in bits 16..24 of the pipeline flags (0 = default):
+// chunk = 2^(9 + c) particles for c = bits 16..19 in 1..6 (1024..32768),
+// slice = 2^(11 + s) particles for s = bits 20..23 in 1..7 (4096..262144).
+inline uint32_t tuned_chunk(uint32_t flags) {
+  const uint32_t c = (flags >> 16) & 0xfu;
+  return (c >= 1 && c <= 6) ? (1u << (9 + c)) : uint32_t(kChunk);
+}
+// Bit 24: the force-VJP gather reads the density adjoint through the caches
+// instead of an LDS window.
+constexpr uint32_t kTuneRhoGlobal = 1u << 24;
+inline bool tuned_rho_lds(uint32_t flags) { return !(flags & kTuneRhoGlobal); }
+inline uint32_t tuned_slice(uint32_t flags) {
+  const uint32_t v = (flags >> 20) & 0xfu;
+  return (v >= 1 && v <= 7) ? (1u << (11 + v)) : kSlice;
+}
 
 struct TileGrid {
   int nx, ny, nz;   // mesh
@@ -112,7 +128,7 @@ struct SimTable {
 template <class G>
 struct BinsT {
   G g;
-  uint32_t K, n, M, maxchunks;
+  uint32_t K, n, M, maxchunks, chunk;
   bool rel;
   const float4 *spos;
   const ChunkDesc *desc;
@@ -132,7 +148,8 @@ struct SimClass {
 struct SimBatch {
   int B = 0;
   bool rel = false;
-  uint32_t K = 0, N = 0, M = 0, H = 0, G = 0, maxK = 0;
+  uint32_t K = 0, N = 0, M = 0, H = 0, G = 0, maxK = 0, chunk = kChunk, slice = kSlice;
+  bool rho_lds = true;
   std::vector<SimRef> host;
   std::vector<SimClass> classes;
   SimRef *dev = nullptr;
@@ -148,7 +165,7 @@ inline SimRef one_sim(int nx, int ny, int nz, int64_t n) {
 
 int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
              size_t ws_bytes, Bins &out);
-size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n);
+size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags);
 Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws);
 size_t batch_sort_workspace_size(const SimBatch &bt);
 size_t batch_paint_workspace_size(const SimBatch &bt, int nc);

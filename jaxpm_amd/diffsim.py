@@ -219,7 +219,7 @@ class BatchLptNBody:
 
     def __init__(self, mesh_shapes, box_sizes, a_steps, cosmologies, params=("Omega_c", "sigma8"), order=1,
                  paint_absolute_pos=True, checkpoint_every=None, k_table=None, graph=True, coeff_grads=False,
-                 rel_step=1e-4):
+                 rel_step=1e-4, sort_tuning=None):
         self.specs = [SimSpec(sh, bx, a_steps, c, params, order, k_table, rel_step)
                       for sh, bx, c in zip(mesh_shapes, box_sizes, cosmologies)]
         if not (len(self.specs) == len(mesh_shapes) == len(box_sizes) == len(cosmologies)):
@@ -227,7 +227,7 @@ class BatchLptNBody:
         self.B = len(self.specs)
         self.absolute = bool(paint_absolute_pos)
         self.stepper = integrate.BatchStepper([s.shape for s in self.specs], [s.n for s in self.specs],
-                                              relative=not self.absolute)
+                                              relative=not self.absolute, sort_tuning=sort_tuning)
         c0 = np.stack([s.q(s.parameter_vector().numpy())[s.slices["kdk"]] for s in self.specs])
         self.engine = integrate.BatchLeapfrog(self.stepper, c0, checkpoint_every, coeff_grads=coeff_grads,
                                               graph=graph)

@@ -48,7 +48,8 @@ def kdk_schedule(cosmo, a_steps):
 class Stepper:
     """Device buffers and C-ABI calls of one mesh / particle configuration."""
 
-    def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0, atomic=False):
+    def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0, atomic=False,
+                 sort_tuning=None):
         self.shape = tuple(int(s) for s in mesh_shape)
         self.nx, self.ny, self.nz = self.shape
         self.n = int(n)
@@ -59,20 +60,26 @@ class Stepper:
         pflags = (_lib.RELATIVE if relative else 0) | (_lib.DETERMINISTIC if deterministic else 0) | \
             (_lib.ATOMIC if atomic else 0)
         self.pflags = pflags
-        self.flags = pflags | (kflags << 8)
+        # (chunk_code, slice_code[, rho_global]) of the tile sort and the
+        # gathers, see jaxpm_amd.autotune
+        self.sort_tuning = tuple(sort_tuning) if sort_tuning else (0, 0)
+        self.flags = pflags | (kflags << 8) | _lib.tune_flags(*self.sort_tuning)
         self.dev = _lib.device()
         # optional HIP-event trace: name -> list of (start, end) events around
         # each launch on torch's current stream (set by the benchmark)
         self.trace = None
-        fb = size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
-        vb = size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         kb = size("jpm_force_kick_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         pb = size("jpm_force_vjp_pot_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags)
         # the tile-sorted pipeline (potential kept per evaluation, stencil in
         # LDS) when the mesh and flags allow it; otherwise three force meshes
         self.sorted = kb > 0 and pb > 0
-        self.sort_bytes = size("jpm_bin_sort_size", self.nx, self.ny, self.nz, self.n) if self.sorted else 0
-        self.ws = torch.empty(max(fb, vb, kb, pb), dtype=torch.uint8, device=self.dev)
+        self.sort_bytes = size("jpm_bin_sort_size", self.nx, self.ny, self.nz, self.n, self.flags) if self.sorted else 0
+        if self.sorted:
+            wb = max(kb, pb)
+        else:
+            wb = max(size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags),
+                     size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags))
+        self.ws = torch.empty(wb, dtype=torch.uint8, device=self.dev)
 
     def new_phi(self):
         return torch.empty((3, *self.shape), dtype=torch.float32, device=self.dev)
@@ -132,7 +139,13 @@ class Stepper:
         e1.record()
         self.trace.setdefault(name, []).append((e0, e1))
 
+    def _ensure_ws(self, name):
+        need = size(name, self.nx, self.ny, self.nz, self.n, self.flags)
+        if self.ws.numel() < need:
+            self.ws = torch.empty(need, dtype=torch.uint8, device=self.dev)
+
     def field(self, x, phi):
+        self._ensure_ws("jpm_force_field_workspace_size")
         self._timed("field", "jpm_force_field_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n, None,
                     ptr(phi), self.r_split, 1.0, self.flags, ptr(self.ws), self.ws.numel())
 
@@ -144,6 +157,7 @@ class Stepper:
         self._timed("axpy", "jpm_axpy_f32", stream(), 3 * self.n, float(a), ptr(x), ptr(y))
 
     def vjp(self, phi, x, cot, scale, gpos):
+        self._ensure_ws("jpm_force_vjp_workspace_size")
         self._timed("vjp", "jpm_force_vjp_f32", stream(), self.nx, self.ny, self.nz, ptr(phi), ptr(x), self.n,
                     ptr(cot), float(scale), ptr(gpos), self.r_split, self.flags, ptr(self.ws), self.ws.numel())
 
@@ -326,7 +340,7 @@ class BatchStepper:
     must be contiguous; arrays are the concatenation of the simulations in
     order: particles [N, 3], meshes [M]."""
 
-    def __init__(self, mesh_shapes, counts, relative=False, r_split=0.0, kflags=0):
+    def __init__(self, mesh_shapes, counts, relative=False, r_split=0.0, kflags=0, sort_tuning=None):
         import ctypes
         self.shapes = [tuple(int(s) for s in sh) for sh in mesh_shapes]
         self.counts = [int(c) for c in counts]
@@ -346,7 +360,9 @@ class BatchStepper:
         cnts = (ctypes.c_int64 * self.B)(*self.counts)
         h = ctypes.c_void_p()
         _lib.load()
-        call("jpm_batch_create", self.B, dims, cnts, _lib.RELATIVE if relative else 0, ctypes.byref(h))
+        self.sort_tuning = tuple(sort_tuning) if sort_tuning else (0, 0)
+        call("jpm_batch_create", self.B, dims, cnts,
+             (_lib.RELATIVE if relative else 0) | _lib.tune_flags(*self.sort_tuning), ctypes.byref(h))
         self.handle = h
         self._destroy = _lib.load().jpm_batch_destroy
         self.sort_bytes = size("jpm_batch_sort_size", h)

@@ -150,3 +150,34 @@ def test_sorted_force_kick_and_vjp_over_32768_tiles(gpu):
     st.vjp_field(field, xt, torch.as_tensor(u, device="cuda"), 0.7, g)
     ref = 0.7 * oa.force_vjp(pos, u, shape)
     assert rel_err(g.cpu().numpy(), ref) < 2e-4
+
+
+def test_sort_tuning_changes_speed_not_results(gpu, tmp_path, monkeypatch):
+    """Every chunk / slice width computes the same forces and VJPs (float
+    rounding), and the autotuner returns one of its candidates and caches it."""
+    from jaxpm_amd import autotune, integrate
+    monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
+    shape = (32, 32, 32)
+    r = np.random.default_rng(46)
+    n = int(np.prod(shape))
+    pos = (lattice(shape) + r.normal(scale=2.0, size=(n, 3))).astype(np.float32)
+    pos[:5000] = 10 + r.normal(scale=0.5, size=(5000, 3))      # a heavy tile: several chunks
+    xt = torch.as_tensor(pos, device="cuda")
+    u = torch.as_tensor(r.normal(size=(n, 3)).astype(np.float32), device="cuda")
+    out = {}
+    for tune in [(0, 0), (1, 1), (6, 7), (2, 5, 1), (0, 0, 1)]:
+        print(f"tuning {tune}", flush=True)
+        st = integrate.Stepper(shape, n, sort_tuning=tune)
+        f = st.new_field(keep_sort=True)
+        v = torch.zeros_like(xt)
+        st.field_kick(xt, v, 1.0, f)
+        g = torch.zeros_like(xt)
+        st.vjp_field(f, xt, u, 0.5, g)
+        out[tune] = (v.cpu().numpy(), g.cpu().numpy())
+    for tune, (v, g) in out.items():
+        assert rel_err(v, out[(0, 0)][0]) < 1e-6, tune
+        assert rel_err(g, out[(0, 0)][1]) < 1e-6, tune
+    best, timings = autotune.tune_stepper(shape, xt, reps=1)
+    assert best in timings and len(timings) == len(autotune._candidates())
+    again, t2 = autotune.tune_stepper(shape, xt)
+    assert again == best and t2 == {}
