@@ -454,6 +454,38 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
 int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a,
                        const float *x, float *y);
 
+/* Workspace bytes of the batched initial-condition entry points below. */
+size_t jpm_batch_lpt_workspace_size(const void *batch);
+
+/* linear_field (pm.py:129-144) for every simulation: out_b = C2R[R2C(noise_b)
+ * sqrt(pk_b(|k|) M_b / V_b)] / M_b with boxes[b*3..b*3+2] (device) and
+ * pk_b = linear interpolation of (k_tab[ntab], pk_tabs[b*ntab..]) (device,
+ * one k table for all).  Its own transpose: the noise cotangent is the same
+ * map of the output cotangent. */
+int jpm_batch_linear_field_f32(void *stream, const void *batch, const float *noise,
+                               float *out, const float *boxes, const float *k_tab,
+                               const float *pk_tabs, int ntab, void *ws,
+                               size_t ws_bytes);
+
+/* The first-order LPT force of lpt (pm.py:79-82) at the particle lattice:
+ * psi[p][d] = Re IFFT[-g_d G FFT(field_b)] at lattice node p of simulation
+ * b (one particle per cell in lattice order, the CIC read at a node being
+ * the node value).  _vjp: gfield = the transpose applied to gpsi. */
+int jpm_batch_lattice_forces_f32(void *stream, const void *batch, const float *field,
+                                 float *psi, void *ws, size_t ws_bytes);
+int jpm_batch_lattice_forces_vjp_f32(void *stream, const void *batch,
+                                     const float *gpsi, float *gfield, void *ws,
+                                     size_t ws_bytes);
+
+/* cic_paint of every simulation's particles into zeroed meshes [M]
+ * (workspace of jpm_batch_force_kick_workspace_size), and the position
+ * cotangent gpos_p = sum_o gmesh[idx_o] dW_o/dx (painting.py:15-45 under
+ * jax.grad). */
+int jpm_batch_paint_f32(void *stream, const void *batch, const float *pos,
+                        float *mesh, void *ws, size_t ws_bytes);
+int jpm_batch_paint_vjp_f32(void *stream, const void *batch, const float *pos,
+                            const float *gmesh, float *gpos);
+
 /* out[b] (device float64) = <x_b, y_b> over each simulation's 3 n_b floats,
  * fixed-order float64 sums (the per-simulation coefficient cotangents). */
 int jpm_batch_dot_f32(void *stream, const void *batch, const float *x,

@@ -83,6 +83,12 @@ SIGNATURES = {
     "jpm_batch_force_vjp_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
     "jpm_batch_axpy_f32": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "jpm_batch_dot_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_batch_lpt_workspace_size": (_sz, [_vp]),
+    "jpm_batch_linear_field_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _sz]),
+    "jpm_batch_lattice_forces_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_batch_lattice_forces_vjp_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_batch_paint_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _sz]),
+    "jpm_batch_paint_vjp_f32": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "jpm_power_spectrum_workspace_size": (_sz, [_i, _i, _i, _i, _i, _i]),
     "jpm_power_spectrum_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _d, _d, _d, _vp, _i, _vp, _i, _d, _d, _d, _vp, _vp,
                                     _sz]),

@@ -22,6 +22,11 @@ int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *
                            uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int);
+int kspace_force_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
+int kspace_force_adjoint_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
+int kspace_linear_field_batch(hipStream_t, int, int, int, int, void *, const float *, const float *, const float *, int,
+                              float);
+int grad_pos_batch(hipStream_t, const SimRef *, int, bool, const float *, const float *, float *);
 int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *,
                bool);
 
@@ -218,6 +223,143 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
     if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, rho + c.cell0))) return rc;
   }
   return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds);
+}
+
+// ---------------------------------------- initial conditions and final paint
+//
+// The stages of the forward model around the leapfrog, batched the same way:
+// the Gaussian field of each simulation from its white noise and P(k) table
+// (pm.py:129-144), the first-order LPT forces read at the particle lattice
+// (pm.py:79-82; a CIC read at lattice nodes returns the node values, so the
+// read is a transpose of the three force meshes) with their adjoints, and
+// the CIC paint of the final positions with its position cotangent.
+
+struct LptWs {
+  size_t spec, spec3, f3, total;
+};
+
+static LptWs lpt_ws(const SimBatch &bt) {
+  size_t maxH = 0, maxM = 0;
+  for (const SimClass &c : bt.classes) {
+    maxH = std::max(maxH, size_t(c.count) * size_t(half_cells(c.nx, c.ny, c.nz)));
+    maxM = std::max(maxM, size_t(c.count) * size_t(cells(c.nx, c.ny, c.nz)));
+  }
+  LptWs w;
+  size_t o = 0;
+  w.spec = o;  o += align_up(size_t(bt.H) * 8);
+  w.spec3 = o; o += align_up(3 * maxH * 8);
+  w.f3 = o;    o += align_up(3 * maxM * 4);
+  w.total = o;
+  return w;
+}
+
+size_t batch_lpt_ws(const SimBatch &bt) { return lpt_ws(bt).total; }
+
+int batch_linear_field(hipStream_t s, const SimBatch &bt, const float *noise, float *out, const float *boxes,
+                       const float *ktab, const float *pktabs, int ntab, void *ws, size_t ws_bytes) {
+  JPM_REQUIRE(noise && out && boxes && ktab && pktabs, "null pointer argument");
+  const LptWs w = lpt_ws(bt);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched linear field needs %zu workspace bytes", w.total);
+  float2 *spec = reinterpret_cast<float2 *>(static_cast<char *>(ws) + w.spec);
+  int rc;
+  for (const SimClass &c : bt.classes) {
+    const float inv_m = float(1.0 / double(cells(c.nx, c.ny, c.nz)));
+    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, noise + c.cell0, spec + c.half0))) return rc;
+    if ((rc = kspace_linear_field_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, boxes + 3 * size_t(c.first), ktab,
+                                        pktabs + size_t(c.first) * ntab, ntab, inv_m)))
+      return rc;
+    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, out + c.cell0))) return rc;
+  }
+  return JPM_OK;
+}
+
+// psi[(cell0_b + c) * 3 + d] <-> f3[b][d][c] for the simulations of one class.
+__global__ void __launch_bounds__(256)
+interleave3_kernel(float *__restrict__ psi, float *__restrict__ f3, uint32_t M, uint32_t count, int to_psi) {
+  const size_t total = size_t(count) * M;
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += size_t(gridDim.x) * blockDim.x) {
+    const size_t b = e / M, c = e - b * M;
+    float *f = f3 + 3 * b * M + c;
+    float *p = psi + 3 * e;
+    if (to_psi) {
+      p[0] = f[0];
+      p[1] = f[M];
+      p[2] = f[2 * size_t(M)];
+    } else {
+      f[0] = p[0];
+      f[M] = p[1];
+      f[2 * size_t(M)] = p[2];
+    }
+  }
+}
+
+static int check_lattice(const SimBatch &bt) {
+  for (const SimRef &r : bt.host)
+    JPM_REQUIRE(int64_t(r.n) == int64_t(r.t.nx) * r.t.ny * r.t.nz,
+                "lattice forces need one particle per cell in lattice order (simulation %u)", r.idx);
+  return JPM_OK;
+}
+
+int batch_lattice_forces(hipStream_t s, const SimBatch &bt, const float *field, float *psi, void *ws,
+                         size_t ws_bytes) {
+  JPM_REQUIRE(field && psi, "null pointer argument");
+  if (int rc = check_lattice(bt)) return rc;
+  const LptWs w = lpt_ws(bt);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched lattice forces need %zu workspace bytes", w.total);
+  char *base = static_cast<char *>(ws);
+  float2 *spec = reinterpret_cast<float2 *>(base + w.spec), *spec3 = reinterpret_cast<float2 *>(base + w.spec3);
+  float *f3 = reinterpret_cast<float *>(base + w.f3);
+  int rc;
+  for (const SimClass &c : bt.classes) {
+    const int64_t m = cells(c.nx, c.ny, c.nz);
+    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, field + c.cell0, spec + c.half0))) return rc;
+    if ((rc = kspace_force_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, spec3, float(1.0 / double(m)), 0)))
+      return rc;
+    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, 3 * c.count, spec3, f3))) return rc;
+    hipLaunchKernelGGL(interleave3_kernel, dim3(grid_for(int64_t(c.count) * m, 256)), dim3(256), 0, s,
+                       psi + 3 * size_t(c.cell0), f3, uint32_t(m), uint32_t(c.count), 1);
+    JPM_LAUNCHED("interleave3_kernel");
+  }
+  return JPM_OK;
+}
+
+int batch_lattice_forces_vjp(hipStream_t s, const SimBatch &bt, const float *gpsi, float *gfield, void *ws,
+                             size_t ws_bytes) {
+  JPM_REQUIRE(gpsi && gfield, "null pointer argument");
+  if (int rc = check_lattice(bt)) return rc;
+  const LptWs w = lpt_ws(bt);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched lattice forces need %zu workspace bytes", w.total);
+  char *base = static_cast<char *>(ws);
+  float2 *spec = reinterpret_cast<float2 *>(base + w.spec), *spec3 = reinterpret_cast<float2 *>(base + w.spec3);
+  float *f3 = reinterpret_cast<float *>(base + w.f3);
+  int rc;
+  for (const SimClass &c : bt.classes) {
+    const int64_t m = cells(c.nx, c.ny, c.nz);
+    hipLaunchKernelGGL(interleave3_kernel, dim3(grid_for(int64_t(c.count) * m, 256)), dim3(256), 0, s,
+                       const_cast<float *>(gpsi) + 3 * size_t(c.cell0), f3, uint32_t(m), uint32_t(c.count), 0);
+    JPM_LAUNCHED("interleave3_kernel");
+    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, 3 * c.count, f3, spec3))) return rc;
+    if ((rc = kspace_force_adjoint_batch(s, c.nx, c.ny, c.nz, c.count, spec3, spec + c.half0,
+                                         float(1.0 / double(m)), 0)))
+      return rc;
+    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, gfield + c.cell0))) return rc;
+  }
+  return JPM_OK;
+}
+
+int batch_paint_mesh(hipStream_t s, const SimBatch &bt, const float *pos, float *mesh, void *ws, size_t ws_bytes) {
+  JPM_REQUIRE(pos && mesh, "null pointer argument");
+  const BatchWs w = batch_ws(bt, 1);
+  JPM_REQUIRE(ws && ws_bytes >= w.total, "batched paint needs %zu workspace bytes, got %zu", w.total, ws_bytes);
+  char *base = static_cast<char *>(ws);
+  BatchBins b;
+  int rc = batch_sort(s, bt, pos, base + w.sort, w.paint - w.sort, b);
+  if (rc) return rc;
+  return batch_paint(s, bt, b, nullptr, 1, mesh, base + w.paint, w.mesh - w.paint);
+}
+
+int batch_paint_vjp(hipStream_t s, const SimBatch &bt, const float *pos, const float *gmesh, float *gpos) {
+  return grad_pos_batch(s, bt.dev, bt.B, bt.rel, gmesh, pos, gpos);
 }
 
 // ------------------------------------------------- per-simulation vectors

@@ -85,6 +85,13 @@ int batch_force_vjp(hipStream_t, const SimBatch &, const float *, const float *,
 int batch_axpy(hipStream_t, const SimBatch &, const float *, const float *, float *);
 size_t batch_dot_ws(const SimBatch &);
 int batch_dot(hipStream_t, const SimBatch &, const float *, const float *, double *, void *, size_t);
+size_t batch_lpt_ws(const SimBatch &);
+int batch_linear_field(hipStream_t, const SimBatch &, const float *, float *, const float *, const float *,
+                       const float *, int, void *, size_t);
+int batch_lattice_forces(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
+int batch_lattice_forces_vjp(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
+int batch_paint_mesh(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
+int batch_paint_vjp(hipStream_t, const SimBatch &, const float *, const float *, float *);
 int dot(hipStream_t, int64_t, const float *, const float *, double *, void *, size_t);
 int power_spectrum(hipStream_t, int, int, int, const void *, const void *, double, double, double, const double *,
                    int, const int *, int, double, double, double, double *, void *, size_t);
@@ -172,7 +179,7 @@ static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pf
   SortedWs w;
   size_t o = 0;
   w.sort = o;  o += align_up(bin_sort_workspace_size(nx, ny, nz, n, pflags));
-  w.paint = o; o += align_up(bins_paint_workspace_size(nx, ny, nz, n, nc, pflags & JPM_RELATIVE));
+  w.paint = o; o += align_up(bins_paint_workspace_size(nx, ny, nz, n, nc, pflags));
   w.mesh = o;  o += align_up(m * 4);
   w.g3 = o;    o += (nc == 3) ? align_up(3 * m * 4) : 0;
   w.spec = o;  o += align_up(h * 8);
@@ -633,6 +640,37 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi, c
 int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a, const float *x, float *y) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_axpy(as_stream(stream), *as_batch(batch), a, x, y));
+}
+
+size_t jpm_batch_lpt_workspace_size(const void *batch) { return batch ? batch_lpt_ws(*as_batch(batch)) : 0; }
+
+int jpm_batch_linear_field_f32(void *stream, const void *batch, const float *noise, float *out, const float *boxes,
+                               const float *k_tab, const float *pk_tabs, int ntab, void *ws, size_t ws_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_linear_field(as_stream(stream), *as_batch(batch), noise, out, boxes, k_tab, pk_tabs, ntab, ws,
+                               ws_bytes));
+}
+
+int jpm_batch_lattice_forces_f32(void *stream, const void *batch, const float *field, float *psi, void *ws,
+                                 size_t ws_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_lattice_forces(as_stream(stream), *as_batch(batch), field, psi, ws, ws_bytes));
+}
+
+int jpm_batch_lattice_forces_vjp_f32(void *stream, const void *batch, const float *gpsi, float *gfield, void *ws,
+                                     size_t ws_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_lattice_forces_vjp(as_stream(stream), *as_batch(batch), gpsi, gfield, ws, ws_bytes));
+}
+
+int jpm_batch_paint_f32(void *stream, const void *batch, const float *pos, float *mesh, void *ws, size_t ws_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_paint_mesh(as_stream(stream), *as_batch(batch), pos, mesh, ws, ws_bytes));
+}
+
+int jpm_batch_paint_vjp_f32(void *stream, const void *batch, const float *pos, const float *gmesh, float *gpos) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  JPM_GUARD(batch_paint_vjp(as_stream(stream), *as_batch(batch), pos, gmesh, gpos));
 }
 
 int jpm_batch_dot_f32(void *stream, const void *batch, const float *x, const float *y, double *out, void *ws,

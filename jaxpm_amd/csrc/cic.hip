@@ -9,7 +9,7 @@
 //     0 <= q < L (scatter drops, gather mode='drop' reads 0).
 // The library is built with -ffp-contract=off so these sequences round as
 // written.
-#include "common.hpp"
+#include "tiles.hpp"
 
 namespace jpm {
 
@@ -330,6 +330,58 @@ grad_pos_kernel(const float *__restrict__ mesh, uint32_t cstride, const float *_
     o[1] = gy;
     o[2] = gz;
   }
+}
+
+// Position cotangent of the batched paint: gpos_p = sum_o g[idx_o] dW_o/dx
+// over each simulation's particles and mesh (blockIdx.y: simulation).
+template <bool REL>
+__global__ void __launch_bounds__(256)
+grad_pos_batch_kernel(const SimRef *__restrict__ sims, const float *__restrict__ mesh_all,
+                      const float *__restrict__ pos_all, float *__restrict__ gpos_all) {
+  const SimRef &s = sims[blockIdx.y];
+  Grid g;
+  g.nx = s.t.nx;
+  g.ny = s.t.ny;
+  g.nz = s.t.nz;
+  g.div_nz = s.t.div_nz;
+  g.div_ny = s.t.div_ny;
+  const float *mesh = mesh_all + s.cell0;
+  const float *pos = pos_all + 3 * size_t(s.part0);
+  float *gpos = gpos_all + 3 * size_t(s.part0);
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < s.n; p += gridDim.x * blockDim.x) {
+    Axis ax, ay, az;
+    particle_axes<REL>(pos, p, g, ax, ay, az);
+    const uint32_t ixs[2] = {uint32_t(ax.i0), uint32_t(ax.i1)};
+    const uint32_t iys[2] = {uint32_t(ay.i0), uint32_t(ay.i1)};
+    const uint32_t izs[2] = {uint32_t(az.i0), uint32_t(az.i1)};
+    const float wx[2] = {ax.w0, ax.w1}, wy[2] = {ay.w0, ay.w1}, wz[2] = {az.w0, az.w1};
+    const float dx[2] = {ax.d0, ax.d1}, dy[2] = {ay.d0, ay.d1}, dz[2] = {az.d0, az.d1};
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+      const float v = mesh[(ixs[bx] * uint32_t(g.ny) + iys[by]) * uint32_t(g.nz) + izs[bz]];
+      sx += v * (dx[bx] * wy[by] * wz[bz]);
+      sy += v * (wx[bx] * dy[by] * wz[bz]);
+      sz += v * (wx[bx] * wy[by] * dz[bz]);
+    }
+    float *o = gpos + 3 * size_t(p);
+    o[0] = 1.0f * sx;
+    o[1] = 1.0f * sy;
+    o[2] = 1.0f * sz;
+  }
+}
+
+int grad_pos_batch(hipStream_t s, const SimRef *sims, int B, bool rel, const float *mesh, const float *pos,
+                   float *gpos) {
+  JPM_REQUIRE(sims && mesh && pos && gpos && B >= 1, "null pointer argument");
+  JPM_TIMED(s, "grad_pos_kernel");
+  if (rel)
+    hipLaunchKernelGGL(grad_pos_batch_kernel<true>, dim3(256, B), dim3(256), 0, s, sims, mesh, pos, gpos);
+  else
+    hipLaunchKernelGGL(grad_pos_batch_kernel<false>, dim3(256, B), dim3(256), 0, s, sims, mesh, pos, gpos);
+  JPM_LAUNCHED("grad_pos_batch_kernel");
+  return JPM_OK;
 }
 
 // ------------------------------------------------------------ host side

@@ -52,6 +52,14 @@ int plan_locked(int nx, int ny, int nz, hipfftType type, int batch, hipfftHandle
     hipfftDestroy(h);
     return fail(JPM_E_FFT, "hipfftMakePlanMany(%d,%d,%d, batch %d): %s", nx, ny, nz, batch, fft_err(r));
   }
+  // Plan creation may initialise device data (twiddle tables) on streams of
+  // its own; finish that before the plan's first execution on the caller's
+  // stream.  Plans are created once per shape, outside any captured graph.
+  const hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    hipfftDestroy(h);
+    return fail(JPM_E_HIP, "synchronising after FFT plan creation: %s", hipGetErrorString(e));
+  }
   g_plans.emplace(key, h);
   out = h;
   return JPM_OK;

@@ -121,6 +121,9 @@ __device__ __forceinline__ float green(float kk) { return kk == 0.0f ? 0.0f : -(
 __global__ void __launch_bounds__(256)
 force_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid hg, Tabs t, float r2,
              float scale, int deconv) {
+  // blockIdx.y: spectrum of a batch of equal shapes (in stride H, out 3 H)
+  dk += size_t(blockIdx.y) * hg.H;
+  out += 3 * size_t(blockIdx.y) * hg.H;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
     uint32_t i, j, k;
     hg.coords(e, i, j, k);
@@ -210,7 +213,16 @@ __device__ __forceinline__ float interp(float x, const float *__restrict__ xp, c
 
 __global__ void __launch_bounds__(256)
 linear_field_kernel(float2 *__restrict__ fk, HalfGrid hg, Tabs t, float3 box, float3 nm, float mcells, float vol,
-                    const float *__restrict__ ktab, const float *__restrict__ pktab, int ntab, float scale) {
+                    const float *__restrict__ ktab, const float *__restrict__ pktab, int ntab, float scale,
+                    const float *__restrict__ boxes) {
+  // blockIdx.y: spectrum of a batch of equal shapes (stride H) with its own
+  // box (boxes[3 y..3 y+2], when given) and P(k) table (pktab + y ntab)
+  fk += size_t(blockIdx.y) * hg.H;
+  pktab += size_t(blockIdx.y) * ntab;
+  if (boxes) {
+    box = make_float3(boxes[3 * blockIdx.y], boxes[3 * blockIdx.y + 1], boxes[3 * blockIdx.y + 2]);
+    vol = float(double(box.x) * double(box.y) * double(box.z));
+  }
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
     uint32_t i, j, k;
     hg.coords(e, i, j, k);
@@ -358,6 +370,9 @@ lpt2_source_kernel(const float *__restrict__ phi, uint32_t m, float *__restrict_
 __global__ void __launch_bounds__(256)
 force_adjoint_kernel(const float2 *__restrict__ in3, float2 *__restrict__ out, HalfGrid hg, Tabs t, float r2,
                      float scale, int deconv) {
+  // blockIdx.y: batch of equal shapes (in stride 3 H, out H)
+  in3 += 3 * size_t(blockIdx.y) * hg.H;
+  out += size_t(blockIdx.y) * hg.H;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
     uint32_t i, j, k;
     hg.coords(e, i, j, k);
@@ -542,8 +557,64 @@ int kspace_linear_field(hipStream_t s, int nx, int ny, int nz, void *fk, float L
   {
     JPM_TIMED(s, "linear_field_kernel");
     hipLaunchKernelGGL(linear_field_kernel, dim3(grid_for(hg.H, 256, 256 * 128)), dim3(256), 0, s,
-                       static_cast<float2 *>(fk), hg, t, box, nm, mcells, vol, ktab, pktab, ntab, scale);
+                       static_cast<float2 *>(fk), hg, t, box, nm, mcells, vol, ktab, pktab, ntab, scale,
+                       static_cast<const float *>(nullptr));
   }
+  JPM_LAUNCHED("linear_field_kernel");
+  return JPM_OK;
+}
+
+// Batched forms over `batch` spectra of one shape (stride H; the force
+// passes' three-component spectra stride 3 H).
+int kspace_force_batch(hipStream_t s, int nx, int ny, int nz, int batch, const void *dk, void *out3, float scale,
+                       uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(dk && out3 && batch >= 1 && batch <= 65535, "bad batched force pass arguments");
+  Tabs t;
+  rc = get_tabs(s, nx, ny, nz, flags, t);
+  if (rc) return rc;
+  HalfGrid hg = make_half(nx, ny, nz);
+  JPM_TIMED(s, "force_kernel");
+  hipLaunchKernelGGL(force_kernel, dim3(grid_for(hg.H, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
+                     static_cast<const float2 *>(dk), static_cast<float2 *>(out3), hg, t, 0.0f, scale,
+                     int((flags & JPM_K_DECONV) != 0));
+  JPM_LAUNCHED("force_kernel");
+  return JPM_OK;
+}
+
+int kspace_force_adjoint_batch(hipStream_t s, int nx, int ny, int nz, int batch, const void *in3, void *out,
+                               float scale, uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(in3 && out && batch >= 1 && batch <= 65535, "bad batched force adjoint arguments");
+  Tabs t;
+  rc = get_tabs(s, nx, ny, nz, flags, t);
+  if (rc) return rc;
+  HalfGrid hg = make_half(nx, ny, nz);
+  JPM_TIMED(s, "force_adjoint_kernel");
+  hipLaunchKernelGGL(force_adjoint_kernel, dim3(grid_for(hg.H, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
+                     static_cast<const float2 *>(in3), static_cast<float2 *>(out), hg, t, 0.0f, scale,
+                     int((flags & JPM_K_DECONV) != 0));
+  JPM_LAUNCHED("force_adjoint_kernel");
+  return JPM_OK;
+}
+
+int kspace_linear_field_batch(hipStream_t s, int nx, int ny, int nz, int batch, void *fk, const float *boxes,
+                              const float *ktab, const float *pktabs, int ntab, float scale) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(fk && boxes && ktab && pktabs && batch >= 1 && batch <= 65535, "bad batched linear field arguments");
+  JPM_REQUIRE(ntab >= 2, "power-spectrum table needs at least 2 entries (got %d)", ntab);
+  Tabs t;
+  rc = get_tabs(s, nx, ny, nz, 0, t);
+  if (rc) return rc;
+  HalfGrid hg = make_half(nx, ny, nz);
+  const float3 nm = make_float3(float(nx), float(ny), float(nz));
+  JPM_TIMED(s, "linear_field_kernel");
+  hipLaunchKernelGGL(linear_field_kernel, dim3(grid_for(hg.H, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
+                     static_cast<float2 *>(fk), hg, t, make_float3(1.0f, 1.0f, 1.0f), nm,
+                     float(cells(nx, ny, nz)), 1.0f, ktab, pktabs, ntab, scale, boxes);
   JPM_LAUNCHED("linear_field_kernel");
   return JPM_OK;
 }

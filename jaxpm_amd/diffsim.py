@@ -209,6 +209,61 @@ class LptNBody:
         return _autograd.CicPaint.apply(zeros, xS, None, 1.0, not self.absolute, self.deterministic)
 
 
+class _BatchLinearField(torch.autograd.Function):
+    """Concatenated noise -> concatenated Gaussian fields (symmetric map)."""
+
+    @staticmethod
+    def forward(ctx, noise, st, boxes, ktab, pktabs):
+        ctx.args = (st, boxes, ktab, pktabs)
+        return st.linear_field(noise.contiguous(), boxes, ktab, pktabs)
+
+    @staticmethod
+    def backward(ctx, g):
+        st, boxes, ktab, pktabs = ctx.args
+        return st.linear_field(g.contiguous(), boxes, ktab, pktabs), None, None, None, None
+
+
+class _BatchLatticeForces(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, field, st):
+        ctx.st = st
+        return st.lattice_forces(field.contiguous())
+
+    @staticmethod
+    def backward(ctx, g):
+        return ctx.st.lattice_forces_vjp(g.contiguous()), None
+
+
+class _BatchScale(torch.autograd.Function):
+    """y_b = c[b] x_b with a float32 device array c (jpm_batch_axpy_f32)."""
+
+    @staticmethod
+    def forward(ctx, x, st, c):
+        ctx.st, ctx.c = st, c
+        y = torch.zeros_like(x)
+        st.axpy(c, x.contiguous(), y)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        y = torch.zeros_like(g)
+        ctx.st.axpy(ctx.c, g.contiguous(), y)
+        return y, None, None
+
+
+class _BatchPaint(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pos, st):
+        ctx.st = st
+        ctx.save_for_backward(pos)
+        return st.paint(pos.contiguous())
+
+    @staticmethod
+    def backward(ctx, g):
+        (pos,) = ctx.saved_tensors
+        return ctx.st.paint_vjp(pos, g.contiguous()), None
+
+
 class BatchLptNBody:
     """The forward model of :class:`LptNBody` for a batch of simulations with
     their own meshes, boxes and cosmologies, the N-body stage stepped by the
@@ -236,6 +291,43 @@ class BatchLptNBody:
 
     def parameter_matrix(self):
         return torch.stack([s.parameter_vector() for s in self.specs])
+
+    def _batched_ic(self, dev):
+        """Device tables of the batched first-order initial conditions at the
+        base cosmologies: boxes, P(k) tables and the two LPT factors, plus the
+        concatenated particle lattices."""
+        if getattr(self, "_ic", None) is None:
+            q = [s.q(s.parameter_vector().numpy()) for s in self.specs]
+            boxes = torch.tensor([s.box for s in self.specs], dtype=torch.float32, device=dev)
+            ktab = torch.as_tensor(self.specs[0].ktab, dtype=torch.float32, device=dev)
+            pk = torch.as_tensor(np.stack([qb[s.slices["pk"]] for qb, s in zip(q, self.specs)]).astype(np.float32),
+                                 device=dev)
+            lc = np.stack([qb[s.slices["lpt"]] for qb, s in zip(q, self.specs)])
+            d1 = torch.as_tensor(lc[:, 0].astype(np.float32), device=dev)
+            c1 = torch.as_tensor(lc[:, 1].astype(np.float32), device=dev)
+            lat = torch.cat([_lattice(s.shape, dev).reshape(-1, 3) for s in self.specs])
+            kdk = torch.as_tensor(np.stack([qb[s.slices["kdk"]] for qb, s in zip(q, self.specs)]))
+            self._ic = (boxes, ktab, pk, d1, c1, lat, kdk)
+        return self._ic
+
+    def forward_fixed_cosmology(self, noise):
+        """The model at the base cosmologies, every stage batched: concatenated
+        noise [M] -> concatenated final densities [M].  Differentiable in the
+        noise; first-order LPT with absolute painting, one particle per cell
+        (what the 48-simulation benchmark runs)."""
+        if self.specs[0].order != 1 or not self.absolute:
+            raise ValueError("the fully batched model covers 1LPT with absolute painting")
+        if any(s.n != int(np.prod(s.shape)) or np.any(s.ktab != self.specs[0].ktab) for s in self.specs):
+            raise ValueError("the fully batched model needs one particle per cell and one k table")
+        st = self.stepper
+        boxes, ktab, pk, d1, c1, lat, kdk = self._batched_ic(noise.device)
+        ic = _BatchLinearField.apply(noise, st, boxes, ktab, pk)
+        psi = _BatchLatticeForces.apply(ic, st)
+        dx = _BatchScale.apply(psi, st, d1)
+        p = _BatchScale.apply(dx, st, c1)
+        x0 = lat + dx
+        xS, _ = integrate.BatchNBody.apply(x0, p, kdk, self.engine, self.checkpoint_every)
+        return _BatchPaint.apply(xS, st)
 
     def __call__(self, noises, params):
         if len(noises) != self.B:

@@ -410,6 +410,45 @@ class BatchStepper:
         call("jpm_batch_dot_f32", stream(), self.handle, ptr(x), ptr(y), ptr(out), ptr(self.dot_ws),
              self.dot_ws.numel())
 
+    def _lpt_ws(self):
+        if getattr(self, "lpt_ws", None) is None:
+            self.lpt_ws = torch.empty(size("jpm_batch_lpt_workspace_size", self.handle), dtype=torch.uint8,
+                                      device=self.dev)
+        return self.lpt_ws
+
+    def linear_field(self, noise, boxes, ktab, pktabs):
+        """Every simulation's Gaussian field from its noise (concatenated
+        meshes [M]); boxes [B, 3], ktab [ntab], pktabs [B, ntab] on the device."""
+        ws = self._lpt_ws()
+        out = torch.empty_like(noise)
+        call("jpm_batch_linear_field_f32", stream(), self.handle, ptr(noise), ptr(out), ptr(boxes), ptr(ktab),
+             ptr(pktabs), ktab.numel(), ptr(ws), ws.numel())
+        return out
+
+    def lattice_forces(self, field):
+        """[N, 3] first-order LPT forces at each simulation's lattice."""
+        ws = self._lpt_ws()
+        psi = torch.empty((self.N, 3), dtype=torch.float32, device=self.dev)
+        call("jpm_batch_lattice_forces_f32", stream(), self.handle, ptr(field), ptr(psi), ptr(ws), ws.numel())
+        return psi
+
+    def lattice_forces_vjp(self, gpsi):
+        ws = self._lpt_ws()
+        g = torch.empty(self.M, dtype=torch.float32, device=self.dev)
+        call("jpm_batch_lattice_forces_vjp_f32", stream(), self.handle, ptr(gpsi), ptr(g), ptr(ws), ws.numel())
+        return g
+
+    def paint(self, pos):
+        """Concatenated CIC densities [M] of every simulation's particles."""
+        mesh = torch.empty(self.M, dtype=torch.float32, device=self.dev)
+        call("jpm_batch_paint_f32", stream(), self.handle, ptr(pos), ptr(mesh), ptr(self.ws), self.ws.numel())
+        return mesh
+
+    def paint_vjp(self, pos, gmesh):
+        gpos = torch.empty((self.N, 3), dtype=torch.float32, device=self.dev)
+        call("jpm_batch_paint_vjp_f32", stream(), self.handle, ptr(pos), ptr(gmesh), ptr(gpos))
+        return gpos
+
 
 class BatchSchedule:
     """Per-simulation leapfrog coefficients as the float32 device arrays the
