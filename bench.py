@@ -462,7 +462,8 @@ def main():
                    "mesh": n, "particles": N, "checkpoint_every": args.checkpoint,
                    "sort_tuning": {"chunk": 2 ** (9 + tuning[0]) if tuning[0] else 8192,
                                    "slice": 2 ** (11 + tuning[1]) if tuning[1] else 16384,
-                                   "vjp_density_window": "caches" if len(tuning) > 2 and tuning[2] else "lds"}},
+                                   "vjp_density_window": "caches" if len(tuning) > 2 and tuning[2] else "lds",
+                                   "xcd_placement": "slabs" if len(tuning) > 3 and tuning[3] else "round-robin"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": PMC_TRAFFIC_BYTES.get((dom, n)),

@@ -51,7 +51,7 @@ extern "C" {
 
 /* Pipeline flags (jpm_pm_forces_f32, jpm_force_*, jpm_batch_create): paint
  * flags in bits 0..7, k-space flags shifted left by 8, and the tile sort's
- * tuning in bits 16..24 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
+ * tuning in bits 16..25 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
  * accumulation/gather chunk (c = 1..6, default 8192), JPM_TUNE_SLICE(s):
  * 2^(11+s) particles per counting/scatter workgroup (s = 1..7, default
  * 16384).  The choice changes speed, never results beyond float rounding;
@@ -60,6 +60,8 @@ extern "C" {
 #define JPM_TUNE_SLICE(s) ((uint32_t)(s) << 20)
 #define JPM_TUNE_RHO_GLOBAL (1u << 24)  /* force-VJP gather: density adjoint
                                            read through the caches, not LDS */
+#define JPM_TUNE_XCD_SLABS (1u << 25)   /* tile workgroups placed so each XCD
+                                           walks one contiguous run of tiles */
 
 /* k-space flags */
 #define JPM_K_GRAD_EXACT 1u  /* gradient_kernel order=0 (i k, len//2 zeroed)

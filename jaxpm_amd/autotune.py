@@ -1,9 +1,10 @@
 """Autotuner for the tile sort and the gathers: picks the chunk width
 (particles per accumulation / gather workgroup), the slice width (particles
-per counting / scatter workgroup) and where the force-VJP gather reads the
-density adjoint (an LDS window or the caches) by timing one force evaluation
-with kick and one force VJP on the caller's own particle state, for every
-candidate.
+per counting / scatter workgroup), where the force-VJP gather reads the
+density adjoint (an LDS window or the caches) and how tile workgroups are
+placed on the XCDs (round-robin or one contiguous run per XCD) by timing one
+force evaluation with kick and one force VJP on the caller's own particle
+state, for every candidate.
 
 Results are cached per (device, mesh, particle count, mode) in memory and in
 ``$XDG_CACHE_HOME/jaxpm_amd/sort_tuning.json`` (``~/.cache`` by default), so
@@ -20,6 +21,7 @@ from . import _lib, integrate
 CHUNK_CODES = (2, 3, 4, 5)      # 2048, 4096, 8192, 16384 particles
 SLICE_CODES = (2, 3, 4, 5)      # 8192, 16384, 32768, 65536 particles
 RHO_MODES = (0, 1)              # density adjoint in LDS / through the caches
+SLAB_MODES = (0, 1)             # tiles dealt round-robin to the XCDs / in runs
 
 _memo = {}
 
@@ -69,14 +71,14 @@ def _key(kind, shapes, counts, relative):
 
 
 def _candidates():
-    return [(c, s, rg) for c in CHUNK_CODES for s in SLICE_CODES for rg in RHO_MODES]
+    return [(c, s, rg, xs) for c in CHUNK_CODES for s in SLICE_CODES for rg in RHO_MODES for xs in SLAB_MODES]
 
 
 def _cached(key, use_cache):
     if not use_cache:
         return None
     hit = _memo.get(key) or _load().get(key)
-    if hit and len(hit.get("best", ())) == 3:
+    if hit and len(hit.get("best", ())) == 4:
         _memo[key] = hit
         return tuple(hit["best"])
     return None
@@ -90,7 +92,7 @@ def _finish(key, timings):
 
 
 def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
-    """Best (chunk_code, slice_code, rho_global) for a single simulation at
+    """Best (chunk_code, slice_code, rho_global, xcd_slabs) for a single simulation at
     positions ``pos`` ([n, 3] device tensor); returns (codes, {codes: ms})."""
     shape = tuple(int(s) for s in mesh_shape)
     n = pos.reshape(-1, 3).shape[0]
@@ -104,7 +106,7 @@ def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
     for tune in _candidates():
         st = integrate.Stepper(shape, n, relative=relative, sort_tuning=tune)
         if not st.sorted:
-            return (0, 0, 0), {}
+            return (0, 0, 0, 0), {}
         field = st.new_field(keep_sort=True)
         v = torch.zeros_like(x)
         g = torch.zeros_like(x)
@@ -119,7 +121,7 @@ def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
 
 
 def tune_batch(mesh_shapes, counts, pos, relative=False, reps=3, use_cache=True):
-    """Best (chunk_code, slice_code, rho_global) for a batch (BatchStepper
+    """Best (chunk_code, slice_code, rho_global, xcd_slabs) for a batch (BatchStepper
     layout)."""
     shapes = [tuple(int(s) for s in sh) for sh in mesh_shapes]
     counts = [int(c) for c in counts]

@@ -45,6 +45,7 @@ int batch_create(int B, const int *dims, const int64_t *counts, uint32_t flags, 
   bt->chunk = tuned_chunk(flags);
   bt->slice = tuned_slice(flags);
   bt->rho_lds = tuned_rho_lds(flags);
+  bt->slabs = tuned_slabs(flags);
   uint64_t K = 0, N = 0, M = 0, H = 0, G = 0;
   uint32_t maxK = 0;
   for (int b = 0; b < B; ++b) {

@@ -335,10 +335,14 @@ __global__ void __launch_bounds__(256)
 bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
                  const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, G g, uint32_t M,
                  const float *__restrict__ wmax_dev, float wmax_host, uint32_t n, float *__restrict__ partial,
-                 float *__restrict__ overflow) {
+                 float *__restrict__ overflow, int accumulate, float *__restrict__ mesh, uint32_t per, bool slabs) {
   __shared__ unsigned long long win[kWinCells];
-  const uint32_t c = blockIdx.x;
-  const int ch = int(blockIdx.y);
+  // (chunk, channel) of this workgroup: the nc channels of one chunk go to
+  // one XCD back to back (workgroups b, b + 8, ...), so their reads of the
+  // chunk's particles and weights share that XCD's L2
+  const uint32_t xcd = blockIdx.x & 7u, run = blockIdx.x >> 3;
+  const uint32_t c = slabs ? xcd * per + run / uint32_t(nc) : (run / uint32_t(nc)) * 8u + xcd;
+  const int ch = int(run % uint32_t(nc));
   if (c >= *nchunks_total) return;
   const ChunkDesc d = desc[c];
   const SimRef &sim = g.tile_sim(d.tile);
@@ -347,11 +351,24 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
   int itx, ity, itz;
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+  // Software pipeline over this thread's particles q, q + B, ...: particle
+  // q deposits while the weight of q + B (whose record arrived an iteration
+  // ago) and the record of q + 2B load.
+  const uint32_t cnt = d.count, B = blockDim.x;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 rec1 = threadIdx.x < cnt ? spos[d.start + threadIdx.x] : zero4;
+  float4 rec2 = threadIdx.x + B < cnt ? spos[d.start + threadIdx.x + B] : zero4;
+  float nw = wscalar;
+  if (weight && threadIdx.x < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
   for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) win[i] = 0ull;
   const double scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
   __syncthreads();
-  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
-    const float4 rec = spos[d.start + q];
+  for (uint32_t q = threadIdx.x; q < cnt; q += B) {
+    const float4 rec = rec1;
+    const float wp = nw;
+    rec1 = rec2;
+    if (weight && q + B < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
+    if (q + 2 * B < cnt) rec2 = spos[d.start + q + 2 * B];
     const float x = rec.x, y = rec.y, z = rec.z;
     const uint32_t pidx = __float_as_uint(rec.w);
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
@@ -375,7 +392,6 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
       ly[0] = ay.l0; ly[1] = ay.l1; wy[0] = ay.w0; wy[1] = ay.w1;
       lz[0] = az.l0; lz[1] = az.l1; wz[0] = az.w0; wz[1] = az.w1;
     }
-    const float wp = weight ? weight[size_t(pidx) * nc + ch] : wscalar;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
@@ -398,7 +414,52 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
   // particle lands in which chunk depends on the scatter's atomic order, so
   // only exact sums across its chunks are reproducible bit for bit.
   const size_t slot = (size_t(c) * nc + ch) * kWinCells;
-  if (d.pad == 0) {
+  if (!REL && d.pad == 0) {
+    // Absolute positions never leave the window, so the cells of a
+    // single-chunk tile hold their own deposits complete: they go straight
+    // to the mesh, and only the +1 halo (the low faces of the upper
+    // neighbours) is kept for the reduce.
+    const double inv = 1.0 / scale;
+    const int Lx = tile_len(itx, t.tx, t.nx), Ly = tile_len(ity, t.ty, t.ny), Lz = tile_len(itz, t.tz, t.nz);
+    float *out = partial + slot * 2;
+    float *dst = mesh + size_t(ch) * M + cell0;
+    if (Lx == kTile && Ly == kTile && Lz == kTile) {
+      // interior: each 16 lanes write one 64-byte row of the mesh
+      for (int e = threadIdx.x; e < kTile * kTile * kTile; e += blockDim.x) {
+        const int lx = e >> 8, ly = (e >> 4) & 15, lz = e & 15;
+        const float v = float(double((long long)win[(lx * kWin + ly) * kWin + lz]) * inv);
+        float *cellp = dst + (uint32_t(ox + lx) * uint32_t(t.ny) + uint32_t(oy + ly)) * uint32_t(t.nz) + uint32_t(oz + lz);
+        *cellp = accumulate ? *cellp + v : v;
+      }
+      // halo: the lx = 16 plane, then ly = 16 (lx < 16), then lz = 16 (lx, ly < 16)
+      constexpr int kH0 = kWin * kWin, kH1 = kH0 + kTile * kWin, kH = kH1 + kTile * kTile;
+      for (int e = threadIdx.x; e < kH; e += blockDim.x) {
+        int lx, ly, lz;
+        if (e < kH0) {
+          lx = kTile; ly = e / kWin; lz = e % kWin;
+        } else if (e < kH1) {
+          const int f = e - kH0;
+          lx = f / kWin; ly = kTile; lz = f % kWin;
+        } else {
+          const int f = e - kH1;
+          lx = f >> 4; ly = f & 15; lz = kTile;
+        }
+        const int i = (lx * kWin + ly) * kWin + lz;
+        out[i] = float(double((long long)win[i]) * inv);
+      }
+    } else {
+      for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) {
+        const int lz = i % kWin, lxy = i / kWin, ly = lxy % kWin, lx = lxy / kWin;
+        const float v = float(double((long long)win[i]) * inv);
+        if (lx < Lx && ly < Ly && lz < Lz) {
+          float *cellp = dst + (uint32_t(ox + lx) * uint32_t(t.ny) + uint32_t(oy + ly)) * uint32_t(t.nz) + uint32_t(oz + lz);
+          *cellp = accumulate ? *cellp + v : v;
+        } else {
+          out[i] = v;
+        }
+      }
+    }
+  } else if (d.pad == 0) {
     const double inv = 1.0 / scale;
     float *out = partial + slot * 2;
     for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) out[i] = float(double((long long)win[i]) * inv);
@@ -415,19 +476,23 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
 // faces, the +1 halo of the lower neighbour(s): along an axis where the local
 // coordinate is 0, the neighbour at -1 (periodic) contributes at its own
 // local index = its tile length.  The (up to 8) sources and each tile's
-// chunks are summed in a fixed order.
-template <class G>
+// chunks are summed in a fixed order.  With DIRECT, a single-chunk tile's
+// own deposits are already in the mesh (the accumulation wrote them), and
+// only its low-face cells take the neighbours' halos, in the same order.
+template <class G, bool DIRECT>
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
                   const uint32_t *__restrict__ nchunk, G g, uint32_t M, int nc, const float *__restrict__ overflow,
                   int has_overflow, const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n,
-                  int accumulate, float *__restrict__ mesh) {
+                  int accumulate, float *__restrict__ mesh, uint32_t K, uint32_t per, bool slabs) {
+  const uint32_t gtile = slab_item(blockIdx.x, per, slabs);
+  if (gtile >= K) return;
   const unsigned long long *ipart = reinterpret_cast<const unsigned long long *>(partial);
   const double inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
-  const SimRef &sim = g.tile_sim(blockIdx.x);
+  const SimRef &sim = g.tile_sim(gtile);
   const TileGrid &t = sim.t;
   const uint32_t tile0 = sim.tile0, cell0 = sim.cell0;
-  const uint32_t tile = blockIdx.x - tile0;
+  const uint32_t tile = gtile - tile0;
   const int ch = int(blockIdx.y);
   const uint32_t txy = t.div_tz.div(tile);
   const int itz = int(tile - txy * uint32_t(t.tz));
@@ -437,35 +502,78 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
   // lower neighbours (periodic) and their lengths
   const int px = itx ? itx - 1 : t.tx - 1, py = ity ? ity - 1 : t.ty - 1, pz = itz ? itz - 1 : t.tz - 1;
   const int PLx = tile_len(px, t.tx, t.nx), PLy = tile_len(py, t.ty, t.ny), PLz = tile_len(pz, t.tz, t.nz);
-  const int cells_in_tile = Lx * Ly * Lz;
+  // the (up to 8) source tiles, q = sx | sy << 1 | sz << 2 (s = 1: the
+  // lower neighbour along that axis): first chunk and chunk count
+  __shared__ uint32_t src_off[8], src_cnt[8];
+  if (threadIdx.x < 8) {
+    const int q = int(threadIdx.x);
+    const int jx = (q & 1) ? px : itx, jy = (q & 2) ? py : ity, jz = (q & 4) ? pz : itz;
+    const uint32_t src = tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
+    src_off[q] = chunk_off[src];
+    src_cnt[q] = nchunk[src];
+  }
+  __syncthreads();
+  const bool faces_only = DIRECT && src_cnt[0] == 1;
+  const bool full = Lx == kTile && Ly == kTile && Lz == kTile;
+  // faces: the a = 0 plane, then b = 0 with a > 0, then c = 0 with a, b > 0
+  const int nf0 = Ly * Lz, nf1 = (Lx - 1) * Lz, nf2 = (Lx - 1) * (Ly - 1);
+  const int cells_in_tile = faces_only ? nf0 + nf1 + nf2 : Lx * Ly * Lz;
   for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
-    const int c = e % Lz;
-    const int ab = e / Lz;
-    const int b = ab % Ly;
-    const int a = ab / Ly;
-    float acc = 0.0f;
+    int a, b, c;
+    if (full) {
+      // the same enumeration with the divisors known at compile time
+      if (!faces_only) {
+        a = e >> 8; b = (e >> 4) & 15; c = e & 15;
+      } else if (e < 256) {
+        a = 0; b = e >> 4; c = e & 15;
+      } else if (e < 496) {
+        const int f = e - 256;
+        a = 1 + (f >> 4); b = 0; c = f & 15;
+      } else {
+        const int f = e - 496;
+        a = 1 + f / 15; b = 1 + f % 15; c = 0;
+      }
+    } else if (!faces_only) {
+      c = e % Lz;
+      const int ab = e / Lz;
+      b = ab % Ly;
+      a = ab / Ly;
+    } else if (e < nf0) {
+      a = 0; b = e / Lz; c = e % Lz;
+    } else if (e < nf0 + nf1) {
+      const int f = e - nf0;
+      a = 1 + f / Lz; b = 0; c = f % Lz;
+    } else {
+      const int f = e - nf0 - nf1;
+      a = 1 + f / (Ly - 1); b = 1 + f % (Ly - 1); c = 0;
+    }
+    const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
+                                      uint32_t(t.nz) + uint32_t(itz * kTile + c);
+    float *dst = mesh + size_t(ch) * M + cell;
+    float acc = faces_only ? *dst : 0.0f;
     const int nxs = a == 0 ? 2 : 1, nys = b == 0 ? 2 : 1, nzs = c == 0 ? 2 : 1;
     for (int sx = 0; sx < nxs; ++sx)
       for (int sy = 0; sy < nys; ++sy)
         for (int sz = 0; sz < nzs; ++sz) {
-          const int jx = sx ? px : itx, jy = sy ? py : ity, jz = sz ? pz : itz;
+          if (faces_only && !(sx | sy | sz)) continue;  // own deposits: already in acc
+          const int q = sx | (sy << 1) | (sz << 2);
           const int la = sx ? PLx : a, lb = sy ? PLy : b, lc = sz ? PLz : c;
-          const uint32_t src = tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
           const int li = (la * kWin + lb) * kWin + lc;
-          const uint32_t c0 = chunk_off[src], cn = nchunk[src];
+          const uint32_t c0 = src_off[q], cn = src_cnt[q];
           if (cn == 1) {
             acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + li];
           } else if (cn > 1) {
-            unsigned long long s = 0ull;
-            for (uint32_t q = 0; q < cn; ++q) s += ipart[(size_t(c0 + q) * nc + ch) * kWinCells + li];
-            acc += float(double((long long)s) * inv);
+            unsigned long long sum = 0ull;
+            for (uint32_t k = 0; k < cn; ++k) sum += ipart[(size_t(c0 + k) * nc + ch) * kWinCells + li];
+            acc += float(double((long long)sum) * inv);
           }
         }
+    if (faces_only) {
+      *dst = acc;
+      continue;
+    }
     float v = acc;
-    const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
-                                      uint32_t(t.nz) + uint32_t(itz * kTile + c);
     if (has_overflow) v += overflow[size_t(ch) * M + cell];
-    float *dst = mesh + size_t(ch) * M + cell;
     *dst = accumulate ? *dst + v : v;
   }
 }
@@ -490,6 +598,7 @@ static int set_lds(K kernel, size_t bytes) {
 // per-simulation tile count (the LDS histogram size).
 struct SortDims {
   uint32_t K, n, G, slice, chunk, maxchunks, maxK, B;
+  bool slabs;
 };
 
 static SortDims one_dims(const TileGrid &t, int64_t n, uint32_t flags) {
@@ -503,6 +612,7 @@ static SortDims one_dims(const TileGrid &t, int64_t n, uint32_t flags) {
   d.maxchunks = uint32_t((nn + d.chunk - 1) / d.chunk + t.K);
   d.maxK = t.K;
   d.B = 1;
+  d.slabs = tuned_slabs(flags);
   return d;
 }
 
@@ -572,6 +682,7 @@ static BinsT<G> view_of(const G &g, const SortDims &d, uint32_t M, bool rel, con
   b.maxchunks = d.maxchunks;
   b.chunk = d.chunk;
   b.rel = rel;
+  b.slabs = d.slabs;
   b.spos = reinterpret_cast<const float4 *>(base + w.spos);
   b.desc = reinterpret_cast<const ChunkDesc *>(base + w.desc);
   b.nchunk = reinterpret_cast<const uint32_t *>(base + w.nchunk);
@@ -704,15 +815,22 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
   {
     JPM_TIMED(s, nc == 1 ? "bin_accum_kernel[nc=1]" : "bin_accum_kernel[nc=3]");
     auto kern = b.rel ? bin_accum_kernel<true, G> : bin_accum_kernel<false, G>;
-    hipLaunchKernelGGL(kern, dim3(b.maxchunks, nc), dim3(256), 0, s, b.spos, weight, nc, wscalar, b.desc,
-                       b.total_chunks, b.g, b.M, wmax_dev, wmax_host, b.n, partial, overflow);
+    uint32_t per;
+    (void)slab_grid(b.maxchunks, b.slabs, per);
+    const unsigned grid = 8u * per * unsigned(nc);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, b.spos, weight, nc, wscalar, b.desc,
+                       b.total_chunks, b.g, b.M, wmax_dev, wmax_host, b.n, partial, overflow, int(accumulate), mesh,
+                       per, b.slabs);
   }
   JPM_LAUNCHED("bin_accum_kernel");
   {
     JPM_TIMED(s, nc == 1 ? "bin_reduce_kernel[nc=1]" : "bin_reduce_kernel[nc=3]");
-    auto kern = bin_reduce_kernel<G>;
-    hipLaunchKernelGGL(kern, dim3(b.K, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
-                       int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh);
+    // absolute positions: single-chunk tiles wrote their own cells already
+    auto kern = b.rel ? bin_reduce_kernel<G, false> : bin_reduce_kernel<G, true>;
+    uint32_t per;
+    const unsigned grid = slab_grid(b.K, b.slabs, per);
+    hipLaunchKernelGGL(kern, dim3(grid, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
+                       int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh, b.K, per, b.slabs);
   }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;
@@ -757,6 +875,7 @@ static SortDims batch_dims(const SimBatch &bt) {
   d.maxchunks = uint32_t((uint64_t(bt.N) + bt.chunk - 1) / bt.chunk + bt.K + uint32_t(bt.B));
   d.maxK = bt.maxK;
   d.B = uint32_t(bt.B);
+  d.slabs = bt.slabs;
   return d;
 }
 

@@ -130,8 +130,8 @@ bool binned_paint_supported(int, int, int);
 // Paint used inside the pipelines: the binned LDS paint unless JPM_ATOMIC or
 // the mesh has more tiles than the binned paint's LDS histogram holds.
 // Pipeline flags: paint flags in bits 0..7, k-space flags in bits 8..15,
-// sort and gather tuning in bits 16..24 (which travel with the paint flags).
-static uint32_t paint_flags(uint32_t flags) { return flags & 0x01ff00ffu; }
+// sort and gather tuning in bits 16..25 (which travel with the paint flags).
+static uint32_t paint_flags(uint32_t flags) { return flags & 0x03ff00ffu; }
 static uint32_t kspace_flags(uint32_t flags) { return (flags >> 8) & 0xffu; }
 
 static bool use_atomic(int nx, int ny, int nz, uint32_t pflags) {

@@ -74,16 +74,34 @@ __device__ __forceinline__ int wrapn(int c, int n) {
   return r < 0 ? r + n : r;
 }
 
+// c mod n for c in [-n, 2n): one compare-and-add instead of a division.
+__device__ __forceinline__ int wrap_near(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
+// Every window coordinate o - halo + a (o < n, halo <= 2, a < 21) lies in
+// [-n, 2n) once each mesh edge is at least 21 cells.
+__device__ __forceinline__ bool near_wrap_ok(const TileGrid &t) { return t.nx >= kPw && t.ny >= kPw && t.nz >= kPw; }
+
 // Load phi over [o - halo, o - halo + edge)^3 (periodic) into win.
 __device__ __forceinline__ void load_window(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
                                             int halo, int edge, float *win) {
   const int cells = edge * edge * edge;
+  const bool near = near_wrap_ok(t);
+#pragma unroll 4
   for (int e = threadIdx.x; e < cells; e += blockDim.x) {
     const int c = e % edge;
     const int ab = e / edge;
     const int b = ab % edge;
     const int a = ab / edge;
-    const int gx = wrapn(ox - halo + a, t.nx), gy = wrapn(oy - halo + b, t.ny), gz = wrapn(oz - halo + c, t.nz);
+    int gx, gy, gz;
+    if (near) {
+      gx = wrap_near(ox - halo + a, t.nx);
+      gy = wrap_near(oy - halo + b, t.ny);
+      gz = wrap_near(oz - halo + c, t.nz);
+    } else {
+      gx = wrapn(ox - halo + a, t.nx);
+      gy = wrapn(oy - halo + b, t.ny);
+      gz = wrapn(oz - halo + c, t.nz);
+    }
     win[e] = phi[(uint32_t(gx) * uint32_t(t.ny) + uint32_t(gy)) * uint32_t(t.nz) + uint32_t(gz)];
   }
 }
@@ -169,9 +187,9 @@ __device__ __forceinline__ void stencil_global(const float *__restrict__ phi, co
 template <bool REL, int MODE, class G>
 __global__ void __launch_bounds__(256)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
-                float *__restrict__ vel) {
+                float *__restrict__ vel, uint32_t per) {
   __shared__ float win[kPwCells];
-  const uint32_t c = blockIdx.x;
+  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
   const SimRef &sim = bins.g.tile_sim(d.tile);
@@ -240,10 +258,10 @@ template <bool REL, class G, bool RLDS>
 __global__ void __launch_bounds__(256)
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
                 const float *__restrict__ cot, float scale_s, const float *__restrict__ scale_dev,
-                float *__restrict__ gpos) {
+                float *__restrict__ gpos, uint32_t per) {
   __shared__ float win[kPwCells];
   __shared__ float rwin[RLDS ? kWinCells : 1];
-  const uint32_t c = blockIdx.x;
+  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
   const SimRef &sim = bins.g.tile_sim(d.tile);
@@ -255,15 +273,38 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   int itx, ity, itz;
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+  // Software pipeline over this thread's particles q, q + B, q + 2B, ...:
+  // particle q computes while the cotangent and gpos of q + B (whose record
+  // arrived an iteration ago) and the record of q + 2B load, so no load waits
+  // on another within an iteration.
+  const uint32_t cnt = d.count, B = blockDim.x;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 rec1 = threadIdx.x < cnt ? bins.spos[d.start + threadIdx.x] : zero4;
+  float4 rec2 = threadIdx.x + B < cnt ? bins.spos[d.start + threadIdx.x + B] : zero4;
+  float nu[3] = {0.0f, 0.0f, 0.0f}, ng[3] = {0.0f, 0.0f, 0.0f};
+  if (threadIdx.x < cnt) {
+    const size_t i3 = 3 * size_t(__float_as_uint(rec1.w));
+    nu[0] = cot[i3]; nu[1] = cot[i3 + 1]; nu[2] = cot[i3 + 2];
+    ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
+  }
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   if (RLDS) load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
+  const bool near = near_wrap_ok(t);
   __syncthreads();
-  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
-    const float4 rec = bins.spos[d.start + q];
+  for (uint32_t q = threadIdx.x; q < cnt; q += B) {
+    const float4 rec = rec1;
     const uint32_t idx = __float_as_uint(rec.w);
+    const float u0 = nu[0], u1 = nu[1], u2 = nu[2];
+    const float g0 = ng[0], g1 = ng[1], g2 = ng[2];
+    rec1 = rec2;
+    if (q + B < cnt) {
+      const size_t i3 = 3 * size_t(__float_as_uint(rec1.w));
+      nu[0] = cot[i3]; nu[1] = cot[i3 + 1]; nu[2] = cot[i3 + 2];
+      ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
+    }
+    if (q + 2 * B < cnt) rec2 = bins.spos[d.start + q + 2 * B];
     CornerSet cs;
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
-    const float u0 = cot[3 * size_t(idx)], u1 = cot[3 * size_t(idx) + 1], u2 = cot[3 * size_t(idx) + 2];
     float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
     if (!REL || corners_in_window(cs)) {
       float F8[3][8];
@@ -275,6 +316,10 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
         float rv;
         if (RLDS) {
           rv = rwin[((la0 + bx) * kWin + lb0 + by) * kWin + lc0 + bz];
+        } else if (near) {
+          const int x = wrap_near(ox + la0 + bx, t.nx), y = wrap_near(oy + lb0 + by, t.ny),
+                    z = wrap_near(oz + lc0 + bz, t.nz);
+          rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
         } else {
           const int x = wrapn(ox + la0 + bx, t.nx), y = wrapn(oy + lb0 + by, t.ny), z = wrapn(oz + lc0 + bz, t.nz);
           rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
@@ -323,9 +368,9 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
       }
     }
     float *g = gpos + 3 * size_t(idx);
-    g[0] += scale * gd[0] + gr[0];
-    g[1] += scale * gd[1] + gr[1];
-    g[2] += scale * gd[2] + gr[2];
+    g[0] = g0 + (scale * gd[0] + gr[0]);
+    g[1] = g1 + (scale * gd[1] + gr[1]);
+    g[2] = g2 + (scale * gd[2] + gr[2]);
   }
 }
 
@@ -333,18 +378,19 @@ template <class G>
 static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float gamma, const float *gamma_dev,
                        float *vel, int write_forces) {
   JPM_REQUIRE(phi && vel, "null pointer argument");
-  const dim3 grid(b.maxchunks), block(256);
+  uint32_t per;
+  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
   JPM_TIMED(s, "bin_kick_kernel");
   if (b.rel) {
     if (write_forces)
-      hipLaunchKernelGGL((bin_kick_kernel<true, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+      hipLaunchKernelGGL((bin_kick_kernel<true, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
     else
-      hipLaunchKernelGGL((bin_kick_kernel<true, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+      hipLaunchKernelGGL((bin_kick_kernel<true, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
   } else {
     if (write_forces)
-      hipLaunchKernelGGL((bin_kick_kernel<false, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+      hipLaunchKernelGGL((bin_kick_kernel<false, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
     else
-      hipLaunchKernelGGL((bin_kick_kernel<false, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel);
+      hipLaunchKernelGGL((bin_kick_kernel<false, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
   }
   JPM_LAUNCHED("bin_kick_kernel");
   return JPM_OK;
@@ -354,11 +400,12 @@ template <class G>
 static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const float *rho_bar, const float *cot,
                        float scale, const float *scale_dev, float *gpos, bool rho_lds) {
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
-  const dim3 grid(b.maxchunks), block(256);
+  uint32_t per;
+  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
   JPM_TIMED(s, "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
-  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos);
+  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos, per);
   JPM_LAUNCHED("bin_grad_kernel");
   return JPM_OK;
 }

@@ -26,6 +26,23 @@ inline uint32_t tuned_chunk(uint32_t flags) {
 // instead of an LDS window.
 constexpr uint32_t kTuneRhoGlobal = 1u << 24;
 inline bool tuned_rho_lds(uint32_t flags) { return !(flags & kTuneRhoGlobal); }
+// Bit 25: XCD slabs.  The dispatcher deals workgroup b to XCD b mod 8; with
+// slabs, the chunk (or tile) of workgroup b is (b mod 8) * per + b / 8, so
+// each XCD walks one contiguous run of tiles and neighbouring tiles, whose
+// windows overlap, share that XCD's L2.
+constexpr uint32_t kTuneXcdSlabs = 1u << 25;
+inline bool tuned_slabs(uint32_t flags) { return (flags & kTuneXcdSlabs) != 0; }
+
+// Workgroups for `items` work items and the per-XCD run length.
+inline unsigned slab_grid(uint32_t items, bool slabs, uint32_t &per) {
+  per = (items + 7u) / 8u;
+  if (per == 0) per = 1;
+  return slabs ? 8u * per : (items ? items : 1u);
+}
+
+__device__ __forceinline__ uint32_t slab_item(uint32_t b, uint32_t per, bool slabs) {
+  return slabs ? (b & 7u) * per + (b >> 3) : b;
+}
 inline uint32_t tuned_slice(uint32_t flags) {
   const uint32_t v = (flags >> 20) & 0xfu;
   return (v >= 1 && v <= 7) ? (1u << (11 + v)) : kSlice;
@@ -129,7 +146,7 @@ template <class G>
 struct BinsT {
   G g;
   uint32_t K, n, M, maxchunks, chunk;
-  bool rel;
+  bool rel, slabs;
   const float4 *spos;
   const ChunkDesc *desc;
   const uint32_t *nchunk, *choff, *total_chunks;
@@ -149,7 +166,7 @@ struct SimBatch {
   int B = 0;
   bool rel = false;
   uint32_t K = 0, N = 0, M = 0, H = 0, G = 0, maxK = 0, chunk = kChunk, slice = kSlice;
-  bool rho_lds = true;
+  bool rho_lds = true, slabs = false;
   std::vector<SimRef> host;
   std::vector<SimClass> classes;
   SimRef *dev = nullptr;

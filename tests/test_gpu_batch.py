@@ -168,3 +168,28 @@ def test_fully_batched_model_matches_per_simulation(gpu):
         assert rel_err(rho[off:off + m], r1.detach().cpu().numpy().reshape(-1)) < 1e-5, b
         assert rel_err(g[off:off + m], n1.grad.cpu().numpy().reshape(-1)) < 1e-4, b
         off += m
+
+
+def test_batch_tuning_changes_speed_not_results(gpu):
+    """Chunk, slice, density-window and XCD-placement choices give the same
+    batched kick and force VJP up to float rounding."""
+    from jaxpm_amd import integrate
+    xs, vs = states(SHAPES, False, 56)
+    r = np.random.default_rng(57)
+    n = [x.shape[0] for x in xs]
+    x = torch.as_tensor(np.concatenate(xs), device="cuda")
+    u = torch.as_tensor(r.normal(size=x.shape).astype(np.float32), device="cuda")
+    out = {}
+    for tune in [(0, 0), (1, 2, 0, 1), (2, 1, 1, 1), (0, 0, 1, 0)]:
+        st = integrate.BatchStepper(SHAPES, n, sort_tuning=tune)
+        f = st.new_field(keep_sort=True)
+        gam = torch.full((st.B,), 0.7, dtype=torch.float32, device="cuda")
+        one = torch.ones(st.B, dtype=torch.float32, device="cuda")
+        v = torch.zeros_like(x)
+        st.field_kick(x, v, gam, f)
+        g = torch.zeros_like(x)
+        st.vjp_field(f, x, u, one, one, g)
+        out[tune] = (v.cpu().numpy(), g.cpu().numpy())
+    for tune, (v, g) in out.items():
+        assert rel_err(v, out[(0, 0)][0]) < 1e-6, tune
+        assert rel_err(g, out[(0, 0)][1]) < 1e-6, tune

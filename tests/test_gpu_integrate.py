@@ -165,7 +165,7 @@ def test_sort_tuning_changes_speed_not_results(gpu, tmp_path, monkeypatch):
     xt = torch.as_tensor(pos, device="cuda")
     u = torch.as_tensor(r.normal(size=(n, 3)).astype(np.float32), device="cuda")
     out = {}
-    for tune in [(0, 0), (1, 1), (6, 7), (2, 5, 1), (0, 0, 1)]:
+    for tune in [(0, 0), (1, 1), (6, 7), (2, 5, 1), (0, 0, 1), (3, 2, 0, 1), (1, 1, 1, 1)]:
         print(f"tuning {tune}", flush=True)
         st = integrate.Stepper(shape, n, sort_tuning=tune)
         f = st.new_field(keep_sort=True)
