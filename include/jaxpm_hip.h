@@ -170,9 +170,15 @@ int jpm_fft_c2r_f32(void *stream, int nx, int ny, int nz, int batch,
 /* Complex-to-complex transform of `batch` [nx][ny][nz] complex64 arrays,
  * direction -1 forward / +1 inverse, unnormalised; in == out allowed.
  * The c2c jaxdecomp.pfft3d / pifft3d of distributed.py:37-42 for complex
- * input. */
+ * input.  Computed from one batched R2C of the real and imaginary parts
+ * (Z = X + iY, the inverse as conj(FFT(conj Z))) in a workspace of
+ * jpm_fft_c2c_workspace_size bytes: no complex-to-complex hipFFT plan is
+ * ever created, because creating one changed the results of R2C plans of
+ * other shapes created after it in the same process. */
+size_t jpm_fft_c2c_workspace_size(int nx, int ny, int nz, int batch);
 int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch,
-                    const void *in, void *out, int direction);
+                    const void *in, void *out, int direction, void *ws,
+                    size_t ws_bytes);
 
 /* ------------------------------------------------------------ k-space */
 

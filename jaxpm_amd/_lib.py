@@ -56,7 +56,8 @@ SIGNATURES = {
     "jpm_cic_grad_pos_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _vp, _f, _vp, _u32, _i]),
     "jpm_fft_r2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "jpm_fft_c2r_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
-    "jpm_fft_c2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i]),
+    "jpm_fft_c2c_workspace_size": (_sz, [_i, _i, _i, _i]),
+    "jpm_fft_c2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i, _vp, _sz]),
     "jpm_kspace_force_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
     "jpm_kspace_force_adjoint_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _f, _f, _u32]),
     "jpm_kspace_compensate_f32": (_i, [_vp, _i, _i, _i, _vp, _f]),

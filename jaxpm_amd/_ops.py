@@ -170,7 +170,8 @@ def c2c(xk, inverse=False):
     nx, ny, nz = _dims(xk.shape[-3:])
     batch = xk.numel() // (nx * ny * nz)
     out = torch.empty_like(xk)
-    call("jpm_fft_c2c_f32", stream(), nx, ny, nz, batch, ptr(xk), ptr(out), 1 if inverse else -1)
+    ws = torch.empty(size("jpm_fft_c2c_workspace_size", nx, ny, nz, batch), dtype=torch.uint8, device=xk.device)
+    call("jpm_fft_c2c_f32", stream(), nx, ny, nz, batch, ptr(xk), ptr(out), 1 if inverse else -1, ptr(ws), ws.numel())
     return out
 
 

@@ -51,7 +51,8 @@ int cic_read(hipStream_t, int, int, int, const float *, int, int64_t, const floa
              uint32_t);
 int fft_r2c(hipStream_t, int, int, int, int, const float *, void *);
 int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
-int fft_c2c(hipStream_t, int, int, int, int, const void *, void *, int);
+int fft_c2c(hipStream_t, int, int, int, int, const void *, void *, int, void *, size_t);
+size_t fft_c2c_workspace_size(int, int, int, int);
 void free_plans();
 void free_tabs();
 int kspace_force(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
@@ -518,8 +519,14 @@ int jpm_fft_c2r_f32(void *stream, int nx, int ny, int nz, int batch, void *in, f
   JPM_GUARD(fft_c2r(as_stream(stream), nx, ny, nz, batch, in, out));
 }
 
-int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch, const void *in, void *out, int direction) {
-  JPM_GUARD(fft_c2c(as_stream(stream), nx, ny, nz, batch, in, out, direction));
+size_t jpm_fft_c2c_workspace_size(int nx, int ny, int nz, int batch) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || batch < 1) return 0;
+  return fft_c2c_workspace_size(nx, ny, nz, batch);
+}
+
+int jpm_fft_c2c_f32(void *stream, int nx, int ny, int nz, int batch, const void *in, void *out, int direction,
+                    void *ws, size_t ws_bytes) {
+  JPM_GUARD(fft_c2c(as_stream(stream), nx, ny, nz, batch, in, out, direction, ws, ws_bytes));
 }
 
 int jpm_kspace_force_f32(void *stream, int nx, int ny, int nz, const void *delta_k, void *out3, float r_split,

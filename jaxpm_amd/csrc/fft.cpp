@@ -1,4 +1,4 @@
-// hipFFT plan cache: one R2C / C2R / C2C plan per (shape, type, batch),
+// hipFFT plan cache: one R2C / C2R plan per (shape, type, batch),
 // created on first use and kept until jpm_shutdown.  Plans are bound to the
 // caller's stream under the cache mutex right before execution, so two
 // threads never race on one plan's stream.
@@ -122,28 +122,37 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
   return JPM_OK;
 }
 
-int fft_c2c(hipStream_t s, int nx, int ny, int nz, int batch, const void *in, void *out, int direction) {
+int c2c_split(hipStream_t s, int64_t m, int batch, const void *z, float *xy, float sign);
+int c2c_combine(hipStream_t s, int nx, int ny, int nz, int batch, const void *xyh, void *out, float sign);
+
+// Workspace of fft_c2c: the real and imaginary parts of every array as real
+// fields [2 batch][M], then their half spectra [2 batch][H].
+size_t fft_c2c_workspace_size(int nx, int ny, int nz, int batch) {
+  const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
+  return align_up(2 * size_t(batch) * M * 4) + 2 * size_t(batch) * H * 8;
+}
+
+// Z = FFT(x + iy) = X + iY with X, Y the R2C spectra of the real fields x, y
+// (extended to the full grid by X(-k) = conj X(k)); the inverse transform is
+// conj(FFT(conj Z)).  Only R2C plans are used.
+int fft_c2c(hipStream_t s, int nx, int ny, int nz, int batch, const void *in, void *out, int direction, void *ws,
+            size_t ws_bytes) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
-  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
+  JPM_REQUIRE(batch >= 1 && batch <= 32767, "batch %d out of range", batch);
   JPM_REQUIRE(direction == -1 || direction == 1, "direction must be -1 (forward) or +1 (inverse)");
   JPM_REQUIRE(in && out, "null FFT buffer");
-  const size_t M = size_t(cells(nx, ny, nz));
-  const int sub = sub_batch(nx, ny, nz, batch);
-  std::lock_guard<std::mutex> lk(g_mu);
-  for (int b0 = 0; b0 < batch; b0 += sub) {
-    const int nb = batch - b0 < sub ? batch - b0 : sub;
-    hipfftHandle h;
-    rc = plan_locked(nx, ny, nz, HIPFFT_C2C, nb, h);
-    if (rc) return rc;
-    hipfftResult r = hipfftSetStream(h, s);
-    if (r == HIPFFT_SUCCESS)
-      r = hipfftExecC2C(h, static_cast<hipfftComplex *>(const_cast<void *>(in)) + size_t(b0) * M,
-                        static_cast<hipfftComplex *>(out) + size_t(b0) * M,
-                        direction == -1 ? HIPFFT_FORWARD : HIPFFT_BACKWARD);
-    if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2C: %s", fft_err(r));
-  }
-  return JPM_OK;
+  const size_t need = fft_c2c_workspace_size(nx, ny, nz, batch);
+  JPM_REQUIRE(ws && ws_bytes >= need, "c2c FFT needs %zu workspace bytes, got %zu", need, ws_bytes);
+  const int64_t M = cells(nx, ny, nz);
+  float *xy = static_cast<float *>(ws);
+  void *xyh = static_cast<char *>(ws) + align_up(2 * size_t(batch) * size_t(M) * 4);
+  const float sign = direction == -1 ? 1.0f : -1.0f;
+  rc = c2c_split(s, M, batch, in, xy, sign);
+  if (rc) return rc;
+  rc = fft_r2c(s, nx, ny, nz, 2 * batch, xy, xyh);
+  if (rc) return rc;
+  return c2c_combine(s, nx, ny, nz, batch, xyh, out, sign);
 }
 
 void free_plans() {

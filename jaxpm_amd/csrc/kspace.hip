@@ -732,6 +732,71 @@ int lpt2_source_vjp(hipStream_t s, int64_t m, const float *phi6, const float *gb
   return JPM_OK;
 }
 
+// Complex arrays [batch][M] -> real fields [2 batch][M]: x = Re z, y =
+// sign Im z (sign -1 conjugates, for the inverse transform).
+__global__ void __launch_bounds__(256)
+c2c_split_kernel(const float2 *__restrict__ z, float *__restrict__ xy, uint32_t M, float sign) {
+  const float2 *zb = z + size_t(blockIdx.y) * M;
+  float *x = xy + 2 * size_t(blockIdx.y) * M;
+  float *y = x + M;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
+    const float2 v = zb[e];
+    x[e] = v.x;
+    y[e] = sign * v.y;
+  }
+}
+
+// Half spectra X, Y of the pair (x, y) of array b -> full spectrum of
+// x + iy: Z(k) = X(k) + i Y(k), with X(k) = conj X(-k) above nz/2; the
+// imaginary part times sign (-1: the conjugate, for the inverse).
+__global__ void __launch_bounds__(256)
+c2c_combine_kernel(const float2 *__restrict__ xyh, float2 *__restrict__ out, uint32_t M, uint32_t H, uint32_t nx,
+                   uint32_t ny, uint32_t nz, FastDiv div_nz, FastDiv div_ny, float sign) {
+  const float2 *X = xyh + 2 * size_t(blockIdx.y) * H;
+  const float2 *Y = X + H;
+  float2 *o = out + size_t(blockIdx.y) * M;
+  const uint32_t nzh = nz / 2 + 1;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
+    const uint32_t r = div_nz.div(e);
+    const uint32_t k = e - r * nz;
+    const uint32_t i = div_ny.div(r);
+    const uint32_t j = r - i * ny;
+    float2 a, b;
+    if (k < nzh) {
+      const size_t h = (size_t(i) * ny + j) * nzh + k;
+      a = X[h];
+      b = Y[h];
+    } else {
+      const uint32_t mi = i ? nx - i : 0, mj = j ? ny - j : 0;
+      const size_t h = (size_t(mi) * ny + mj) * nzh + (nz - k);
+      a = make_float2(X[h].x, -X[h].y);
+      b = make_float2(Y[h].x, -Y[h].y);
+    }
+    o[e] = make_float2(a.x - b.y, sign * (a.y + b.x));
+  }
+}
+
+int c2c_split(hipStream_t s, int64_t m, int batch, const void *z, float *xy, float sign) {
+  JPM_REQUIRE(m > 0 && m < (int64_t(1) << 32) && batch >= 1 && batch <= 65535, "c2c split of %d x %lld", batch,
+              (long long)m);
+  hipLaunchKernelGGL(c2c_split_kernel, dim3(grid_for(m, 256, 256 * 64 / batch + 1), batch), dim3(256), 0, s,
+                     static_cast<const float2 *>(z), xy, uint32_t(m), sign);
+  JPM_LAUNCHED("c2c_split_kernel");
+  return JPM_OK;
+}
+
+int c2c_combine(hipStream_t s, int nx, int ny, int nz, int batch, const void *xyh, void *out, float sign) {
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(m < (int64_t(1) << 32) && batch >= 1 && batch <= 65535, "c2c combine of %d x %lld", batch,
+              (long long)m);
+  hipLaunchKernelGGL(c2c_combine_kernel, dim3(grid_for(m, 256, 256 * 64 / batch + 1), batch), dim3(256), 0, s,
+                     static_cast<const float2 *>(xyh), static_cast<float2 *>(out), uint32_t(m),
+                     uint32_t(half_cells(nx, ny, nz)), uint32_t(nx), uint32_t(ny), uint32_t(nz),
+                     FastDiv(uint32_t(nz)), FastDiv(uint32_t(ny)), sign);
+  JPM_LAUNCHED("c2c_combine_kernel");
+  return JPM_OK;
+}
+
 int spectrum_full_to_half(hipStream_t s, int nx, int ny, int nz, const void *full, void *half) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
