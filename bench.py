@@ -35,6 +35,14 @@ HBM_PEAK_GBS = 8000.0
 PMC_TRAFFIC_BYTES = {("bin_kick_kernel", 256): 1153.3e6, ("bin_grad_kernel", 256): 1710.6e6}
 
 
+def fixed_tuning(args):
+    """The sort tuning codes given on the command line, (0, 0) with
+    --no-tune, or None: run the autotuner."""
+    if args.tuning:
+        return tuple(int(v) for v in args.tuning.split(","))
+    return (0, 0) if args.no_tune else None
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +57,8 @@ def parse():
                     help="pm: the default forward+adjoint step benchmark; diffsim: the differentiable "
                          "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8)")
     ap.add_argument("--no-tune", action="store_true", help="skip the sort autotuner (default widths)")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="fixed sort tuning 'chunk,slice,rho_global,xcd_slabs' codes instead of the autotuner")
     ap.add_argument("--plain-steps", type=int, default=5,
                     help="leapfrog steps of the plain stored-trajectory autodiff comparison (0: skip)")
     return ap.parse_args()
@@ -162,8 +172,8 @@ def batch48(args):
     nr = np.random.default_rng(7)
     noises = [torch.as_tensor(nr.standard_normal(sh, dtype=np.float32), device=dev) for sh in shapes]
     tnoise = [torch.as_tensor(nr.standard_normal(sh, dtype=np.float32), device=dev) for sh in shapes]
-    tuning = (0, 0)
-    if not args.no_tune:
+    tuning = fixed_tuning(args)
+    if tuning is None:
         from jaxpm_amd import autotune
         lat = torch.cat([torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
                                                      indexing="ij"), -1).reshape(-1, 3) for n in sizes])
@@ -198,6 +208,7 @@ def batch48(args):
 
     reps = max(1, min(args.steps, 5))
     t_batch, g_batch = timed(batched, reps)
+    print(f"batch48: batched forward+adjoint {t_batch:.3f} s per pass", file=sys.stderr, flush=True)
     out = {"metric": "forward+adjoint simulations/s, 48 simulations (16 each at 64^3, 96^3, 128^3), 1LPT + 20 "
                      "leapfrog steps", "value": round(len(sizes) / t_batch, 3), "unit": "sims/s", "n_gpus": 1,
            "steps": reps, "warmup": 1, "ms_per_step": round(1000 * t_batch, 2), "higher_is_better": True,
@@ -222,10 +233,12 @@ def batch48(args):
                 return grads
 
             t_one, g_one = timed(one_by_one, 1)
+            print(f"batch48: one by one ({method}) {t_one:.3f} s", file=sys.stderr, flush=True)
             rel = max(float((a - b).abs().max() / b.abs().max()) for a, b in zip(g_batch, g_one))
+            rms = max(float((a - b).square().mean().sqrt() / b.square().mean().sqrt()) for a, b in zip(g_batch, g_one))
             out[f"one_by_one_{method}"] = {"sims_per_s": round(len(sizes) / t_one, 3),
                                            "speedup_of_batched": round(t_one / t_batch, 3),
-                                           "grad_max_rel_diff": rel}
+                                           "grad_max_rel_diff": rel, "grad_rms_rel_diff": rms}
             del singles
             torch.cuda.empty_cache()
     out["peak_mem_gb"] = round(torch.cuda.max_memory_allocated() / 1e9, 2)
@@ -270,6 +283,7 @@ def large(args):
     run()                      # plans, tables
     torch.cuda.synchronize()
     t_first = time.perf_counter() - t0
+    print(f"large: first forward+adjoint run (with plan set-up) {t_first:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     loss, g = run()
@@ -326,8 +340,8 @@ def main():
                                     lattice.reshape(-1, 3) + 0.5 * torch.randn((N, 3), generator=gen, device=dev))
     sigma = 0.1
 
-    tuning = (0, 0)
-    if not args.no_tune:
+    tuning = fixed_tuning(args)
+    if tuning is None:
         from jaxpm_amd import autotune
         tuning, _ = autotune.tune_stepper(shape, x0)
     stepper = integrate.Stepper(shape, N, sort_tuning=tuning)
@@ -366,7 +380,10 @@ def main():
     win = 4 * M * (17 / 16) ** 3   # tile windows (16^3 tiles + 1-cell CIC halo)
     algo = {
         "bin_kick_kernel": 36 * N + 4 * M,             # pos, vel r+w, potential
+        "bin_kick_kernel[drift]": 48 * N + 4 * M,      # ... and the drifted positions
+        "bin_kick_kernel[forces]": 24 * N + 4 * M,     # pos, forces, potential
         "bin_grad_kernel": 48 * N + 8 * M,             # pos, cot, gpos r+w, potential, rho_bar
+        "bin_grad_kernel[drift]": 60 * N + 8 * M,      # ... and the drift adjoint's cot write
         "bin_count_kernel": 12 * N,
         "bin_scatter_kernel": 12 * N + 16 * N,
         "bin_accum_kernel[nc=1]": 16 * N + win,

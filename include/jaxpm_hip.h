@@ -390,6 +390,28 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz,
                           size_t ws_bytes, const void *sorted,
                           size_t sorted_bytes);
 
+/* The leapfrog's kick fused with the drift that follows it (ode.py:142-146:
+ * a kick, then the next step's drift): jpm_force_kick_f32 with
+ * write_forces = 0, then pos_out[p] = pos[p] + beta * vel[p] with the new
+ * velocity, also written to pos_keep when it is not NULL.  pos_out may be
+ * pos.  Same workspace and keep_sort as jpm_force_kick_f32. */
+int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz,
+                             const float *pos, int64_t n, float *vel,
+                             float gamma, float beta, float *pos_out,
+                             float *pos_keep, float *phi, float r_split,
+                             uint32_t flags, void *ws, size_t ws_bytes,
+                             void *keep_sort, size_t keep_bytes);
+
+/* The adjoint of that fused step, reverse order: jpm_force_vjp_pot_f32 into
+ * gpos, then the drift's adjoint cot[p] += beta * gpos[p] (cot updated in
+ * place).  Same workspace as jpm_force_vjp_pot_f32. */
+int jpm_force_vjp_pot_drift_f32(void *stream, int nx, int ny, int nz,
+                                const float *phi, const float *pos, int64_t n,
+                                float *cot, float scale, float beta,
+                                float *gpos, float r_split, uint32_t flags,
+                                void *ws, size_t ws_bytes, const void *sorted,
+                                size_t sorted_bytes);
+
 /* Workspace bytes for jpm_dot_f32. */
 size_t jpm_dot_workspace_size(void);
 
@@ -456,6 +478,24 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
                             float *gpos, float r_split, uint32_t kflags,
                             void *ws, size_t ws_bytes, const void *sorted,
                             size_t sorted_bytes);
+
+/* The fused kick + drift and its adjoint (jpm_force_kick_drift_f32,
+ * jpm_force_vjp_pot_drift_f32) for every simulation, beta a device array
+ * [nsims]. */
+int jpm_batch_force_kick_drift_f32(void *stream, const void *batch,
+                                   const float *pos, float *vel,
+                                   const float *gamma, const float *beta,
+                                   float *pos_out, float *pos_keep, float *phi,
+                                   float r_split, uint32_t kflags, void *ws,
+                                   size_t ws_bytes, void *keep_sort,
+                                   size_t keep_bytes);
+int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch,
+                                  const float *phi, const float *pos,
+                                  float *cot, const float *scale,
+                                  const float *pot_scale, const float *beta,
+                                  float *gpos, float r_split, uint32_t kflags,
+                                  void *ws, size_t ws_bytes,
+                                  const void *sorted, size_t sorted_bytes);
 
 /* y_b += a[b] x_b over each simulation's 3 n_b floats (the drift of
  * ode.py:140 with per-simulation coefficients). */

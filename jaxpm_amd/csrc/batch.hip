@@ -21,14 +21,15 @@ int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
 int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *, float, float, const float *,
                            uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
-int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int);
+int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
+               float *);
 int kspace_force_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_force_adjoint_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_linear_field_batch(hipStream_t, int, int, int, int, void *, const float *, const float *, const float *, int,
                               float);
 int grad_pos_batch(hipStream_t, const SimRef *, int, bool, const float *, const float *, float *);
 int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *,
-               bool);
+               bool, const float *, float *);
 
 // ------------------------------------------------------------ description
 
@@ -157,10 +158,13 @@ static int check_kflags(uint32_t kflags) {
 }
 
 // For every simulation b: phi_b = potential of the CIC density of its
-// particles; vel_b += gamma[b] F_b(pos_b) (or = F_b with write_forces).
+// particles; vel_b += gamma[b] F_b(pos_b) (or = F_b with write_forces).  With
+// xout, the drift follows: xout_b = pos_b + beta[b] vel_b (also into xkeep
+// when given; xout may be pos).
 int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float *vel, const float *gamma, float *phi,
                      float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes, void *keep,
-                     size_t keep_bytes) {
+                     size_t keep_bytes, const float *beta, float *xout, float *xkeep) {
+  JPM_REQUIRE(!xout || (beta && !write_forces), "the fused drift needs beta and a kick");
   if (int rc = check_kflags(kflags)) return rc;
   JPM_REQUIRE(pos && vel && phi && gamma, "null pointer argument");
   const BatchWs w = batch_ws(bt, 1);
@@ -187,14 +191,15 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
       return rc;
     if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, phi + c.cell0))) return rc;
   }
-  return batch_kick(s, b, phi, gamma, vel, write_forces);
+  return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
 }
 
 // For every simulation b: gpos_b += scale[b] J_b^T cot_b, with pot_scale[b] =
-// scale[b] / M_b applied inside the potential.
+// scale[b] / M_b applied inside the potential.  With beta, the adjoint of
+// the drift that preceded the kick follows: cot_b += beta[b] gpos_b.
 int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const float *pos, const float *cot,
                     const float *scale, const float *pot_scale, float *gpos, float r_split, uint32_t kflags, void *ws,
-                    size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+                    size_t ws_bytes, const void *sorted, size_t sorted_bytes, const float *beta, float *cot_out) {
   if (int rc = check_kflags(kflags)) return rc;
   JPM_REQUIRE(phi && pos && cot && scale && pot_scale && gpos, "null pointer argument");
   const BatchWs w = batch_ws(bt, 3);
@@ -223,7 +228,7 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
       return rc;
     if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, rho + c.cell0))) return rc;
   }
-  return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds);
+  return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds, beta, cot_out);
 }
 
 // ---------------------------------------- initial conditions and final paint

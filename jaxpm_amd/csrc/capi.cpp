@@ -80,9 +80,10 @@ int batch_destroy(SimBatch *);
 size_t batch_kick_ws(const SimBatch &);
 size_t batch_vjp_ws(const SimBatch &);
 int batch_force_kick(hipStream_t, const SimBatch &, const float *, float *, const float *, float *, float, uint32_t,
-                     int, void *, size_t, void *, size_t);
+                     int, void *, size_t, void *, size_t, const float *, float *, float *);
 int batch_force_vjp(hipStream_t, const SimBatch &, const float *, const float *, const float *, const float *,
-                    const float *, float *, float, uint32_t, void *, size_t, const void *, size_t);
+                    const float *, float *, float, uint32_t, void *, size_t, const void *, size_t, const float *,
+                    float *);
 int batch_axpy(hipStream_t, const SimBatch &, const float *, const float *, float *);
 size_t batch_dot_ws(const SimBatch &);
 int batch_dot(hipStream_t, const SimBatch &, const float *, const float *, double *, void *, size_t);
@@ -158,8 +159,9 @@ int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, floa
 
 int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t);
 size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
-int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int);
-int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool);
+int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *);
+int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool, float,
+              float *);
 
 // ------------------------------------------------------ sorted pipelines
 //
@@ -195,9 +197,11 @@ static bool sorted_supported(int nx, int ny, int nz, uint32_t flags) {
 
 static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
                       float *phi, float r_split, uint32_t flags, int write_forces, void *ws, size_t ws_bytes,
-                      void *keep_sort, size_t keep_bytes) {
+                      void *keep_sort, size_t keep_bytes, float beta = 0.0f, float *xout = nullptr,
+                      float *xkeep = nullptr) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
+  JPM_REQUIRE(!xout || !write_forces, "the fused drift follows a kick, not a force write");
   const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
@@ -224,12 +228,13 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   if (rc) return rc;
   rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, phi);
   if (rc) return rc;
-  return bins_kick(s, b, phi, gamma, vel, write_forces);
+  return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
 }
 
 static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
                          const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
-                         size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+                         size_t ws_bytes, const void *sorted, size_t sorted_bytes, float beta = 0.0f,
+                         float *cot_out = nullptr) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
@@ -263,7 +268,7 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   if (rc) return rc;
   rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, rho);
   if (rc) return rc;
-  return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags));
+  return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags), beta, cot_out);
 }
 
 // Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
@@ -633,7 +638,16 @@ int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos, 
                              void *keep_sort, size_t keep_bytes) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, write_forces,
-                             ws, ws_bytes, keep_sort, keep_bytes));
+                             ws, ws_bytes, keep_sort, keep_bytes, nullptr, nullptr, nullptr));
+}
+
+int jpm_batch_force_kick_drift_f32(void *stream, const void *batch, const float *pos, float *vel, const float *gamma,
+                                   const float *beta, float *pos_out, float *pos_keep, float *phi, float r_split,
+                                   uint32_t kflags, void *ws, size_t ws_bytes, void *keep_sort, size_t keep_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  if (!pos_out || !beta) return fail(JPM_E_INVALID, "pos_out and beta are required");
+  JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, 0, ws,
+                             ws_bytes, keep_sort, keep_bytes, beta, pos_out, pos_keep));
 }
 
 int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi, const float *pos, const float *cot,
@@ -641,7 +655,17 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi, c
                             void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_force_vjp(as_stream(stream), *as_batch(batch), phi, pos, cot, scale, pot_scale, gpos, r_split,
-                            kflags, ws, ws_bytes, sorted, sorted_bytes));
+                            kflags, ws, ws_bytes, sorted, sorted_bytes, nullptr, nullptr));
+}
+
+int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch, const float *phi, const float *pos, float *cot,
+                                  const float *scale, const float *pot_scale, const float *beta, float *gpos,
+                                  float r_split, uint32_t kflags, void *ws, size_t ws_bytes, const void *sorted,
+                                  size_t sorted_bytes) {
+  if (!batch) return fail(JPM_E_INVALID, "null batch");
+  if (!beta || !cot) return fail(JPM_E_INVALID, "beta and cot are required");
+  JPM_GUARD(batch_force_vjp(as_stream(stream), *as_batch(batch), phi, pos, cot, scale, pot_scale, gpos, r_split,
+                            kflags, ws, ws_bytes, sorted, sorted_bytes, beta, cot));
 }
 
 int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a, const float *x, float *y) {
@@ -763,6 +787,22 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz, const float *phi
                           size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
   JPM_GUARD(force_vjp_pot(as_stream(stream), nx, ny, nz, phi, pos, n, cot, scale, gpos, r_split, flags, ws,
                           ws_bytes, sorted, sorted_bytes));
+}
+
+int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, float *vel,
+                             float gamma, float beta, float *pos_out, float *pos_keep, float *phi, float r_split,
+                             uint32_t flags, void *ws, size_t ws_bytes, void *keep_sort, size_t keep_bytes) {
+  if (!pos_out) return fail(JPM_E_INVALID, "pos_out is null");
+  JPM_GUARD(force_kick(as_stream(stream), nx, ny, nz, pos, n, vel, gamma, phi, r_split, flags, 0, ws, ws_bytes,
+                       keep_sort, keep_bytes, beta, pos_out, pos_keep));
+}
+
+int jpm_force_vjp_pot_drift_f32(void *stream, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
+                                float *cot, float scale, float beta, float *gpos, float r_split, uint32_t flags,
+                                void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+  if (!cot) return fail(JPM_E_INVALID, "cot is null");
+  JPM_GUARD(force_vjp_pot(as_stream(stream), nx, ny, nz, phi, pos, n, cot, scale, gpos, r_split, flags, ws,
+                          ws_bytes, sorted, sorted_bytes, beta, cot));
 }
 
 size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {

@@ -182,12 +182,23 @@ __device__ __forceinline__ void stencil_global(const float *__restrict__ phi, co
   F[2] = -(kC8 * (at(x, y, z + 1) - at(x, y, z - 1)) - kC1 * (at(x, y, z + 2) - at(x, y, z - 2)));
 }
 
-// MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F.  gamma is
-// per simulation (gamma_dev[sim]) when gamma_dev is given.
+// The leapfrog drift that may follow a kick, fused into it: x = x + beta v
+// with the new velocity (beta per simulation, beta_dev[sim], when given),
+// written to xout and, when given, to xkeep.
+struct DriftArgs {
+  float beta;
+  const float *beta_dev;
+  float *xout, *xkeep;
+};
+
+// MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F;  MODE 2:
+// MODE 0 followed by the drift x[idx] = x + beta vel[idx], x being the
+// sorted copy of the positions the forces were computed at.  gamma is per
+// simulation (gamma_dev[sim]) when gamma_dev is given.
 template <bool REL, int MODE, class G>
 __global__ void __launch_bounds__(256)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
-                float *__restrict__ vel, uint32_t per) {
+                float *__restrict__ vel, uint32_t per, DriftArgs dr) {
   __shared__ float win[kPwCells];
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
@@ -197,6 +208,7 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
   const uint32_t part0 = sim.part0;
   const float *phi = phi_all + sim.cell0;
   const float gamma = gamma_dev ? gamma_dev[sim.idx] : gamma_s;
+  const float beta = MODE == 2 ? (dr.beta_dev ? dr.beta_dev[sim.idx] : dr.beta) : 0.0f;
   int itx, ity, itz;
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
@@ -241,6 +253,22 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
       v[0] += gamma * s[0];
       v[1] += gamma * s[1];
       v[2] += gamma * s[2];
+    } else if (MODE == 2) {
+      const float v0 = v[0] + gamma * s[0], v1 = v[1] + gamma * s[1], v2 = v[2] + gamma * s[2];
+      v[0] = v0;
+      v[1] = v1;
+      v[2] = v2;
+      const float x0 = rec.x + beta * v0, x1 = rec.y + beta * v1, x2 = rec.z + beta * v2;
+      float *xo = dr.xout + 3 * size_t(idx);
+      xo[0] = x0;
+      xo[1] = x1;
+      xo[2] = x2;
+      if (dr.xkeep) {
+        float *xk = dr.xkeep + 3 * size_t(idx);
+        xk[0] = x0;
+        xk[1] = x1;
+        xk[2] = x2;
+      }
     } else {
       v[0] = gamma * s[0];
       v[1] = gamma * s[1];
@@ -254,11 +282,15 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
 // RLDS: the density-adjoint window in LDS (true) or read from the mesh
 // through the caches (false: 37 KiB of LDS per workgroup instead of 57 KiB,
 // twice the resident workgroups).
+// With cot_out (the same array as cot, so neither is restrict), the adjoint
+// of the drift that preceded the kick follows: cot[idx] += beta gpos[idx]
+// with the new gpos (beta per simulation, beta_dev[sim], when given).
 template <bool REL, class G, bool RLDS>
 __global__ void __launch_bounds__(256)
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
-                const float *__restrict__ cot, float scale_s, const float *__restrict__ scale_dev,
-                float *__restrict__ gpos, uint32_t per) {
+                const float *cot, float scale_s, const float *__restrict__ scale_dev,
+                float *__restrict__ gpos, uint32_t per, float beta_s, const float *__restrict__ beta_dev,
+                float *cot_out) {
   __shared__ float win[kPwCells];
   __shared__ float rwin[RLDS ? kWinCells : 1];
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
@@ -270,6 +302,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   const float *phi = phi_all + sim.cell0;
   const float *rho_bar = rho_all + sim.cell0;
   const float scale = scale_dev ? scale_dev[sim.idx] : scale_s;
+  const float beta = beta_dev ? beta_dev[sim.idx] : beta_s;
   int itx, ity, itz;
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
@@ -368,65 +401,74 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
       }
     }
     float *g = gpos + 3 * size_t(idx);
-    g[0] = g0 + (scale * gd[0] + gr[0]);
-    g[1] = g1 + (scale * gd[1] + gr[1]);
-    g[2] = g2 + (scale * gd[2] + gr[2]);
+    const float n0 = g0 + (scale * gd[0] + gr[0]), n1 = g1 + (scale * gd[1] + gr[1]),
+                n2 = g2 + (scale * gd[2] + gr[2]);
+    g[0] = n0;
+    g[1] = n1;
+    g[2] = n2;
+    if (cot_out) {
+      float *co = cot_out + 3 * size_t(idx);
+      co[0] = u0 + beta * n0;
+      co[1] = u1 + beta * n1;
+      co[2] = u2 + beta * n2;
+    }
   }
 }
 
 template <class G>
 static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float gamma, const float *gamma_dev,
-                       float *vel, int write_forces) {
+                       float *vel, int mode, DriftArgs dr) {
   JPM_REQUIRE(phi && vel, "null pointer argument");
+  JPM_REQUIRE(mode != 2 || dr.xout, "the fused drift needs its position output");
   uint32_t per;
   const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
-  JPM_TIMED(s, "bin_kick_kernel");
-  if (b.rel) {
-    if (write_forces)
-      hipLaunchKernelGGL((bin_kick_kernel<true, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
-    else
-      hipLaunchKernelGGL((bin_kick_kernel<true, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
-  } else {
-    if (write_forces)
-      hipLaunchKernelGGL((bin_kick_kernel<false, 1, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
-    else
-      hipLaunchKernelGGL((bin_kick_kernel<false, 0, G>), grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per);
-  }
+  JPM_TIMED(s, mode == 2 ? "bin_kick_kernel[drift]" : mode == 1 ? "bin_kick_kernel[forces]" : "bin_kick_kernel");
+  auto kern = b.rel ? (mode == 1 ? bin_kick_kernel<true, 1, G> : mode == 2 ? bin_kick_kernel<true, 2, G>
+                                                                          : bin_kick_kernel<true, 0, G>)
+                    : (mode == 1 ? bin_kick_kernel<false, 1, G> : mode == 2 ? bin_kick_kernel<false, 2, G>
+                                                                           : bin_kick_kernel<false, 0, G>);
+  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, gamma, gamma_dev, vel, per, dr);
   JPM_LAUNCHED("bin_kick_kernel");
   return JPM_OK;
 }
 
 template <class G>
 static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const float *rho_bar, const float *cot,
-                       float scale, const float *scale_dev, float *gpos, bool rho_lds) {
+                       float scale, const float *scale_dev, float *gpos, bool rho_lds, float beta,
+                       const float *beta_dev, float *cot_out) {
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
+  JPM_REQUIRE(!cot_out || cot_out == cot, "the fused drift adjoint updates the cotangent in place");
   uint32_t per;
   const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
-  JPM_TIMED(s, "bin_grad_kernel");
+  JPM_TIMED(s, cot_out ? "bin_grad_kernel[drift]" : "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
-  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos, per);
+  hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos, per, beta, beta_dev,
+                     cot_out);
   JPM_LAUNCHED("bin_grad_kernel");
   return JPM_OK;
 }
 
 int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float *gamma_dev, float *vel,
-               int write_forces) {
-  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, write_forces);
+               int write_forces, const float *beta_dev, float *xout, float *xkeep) {
+  const int mode = write_forces ? 1 : (xout ? 2 : 0);
+  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, mode, DriftArgs{0.0f, beta_dev, xout, xkeep});
 }
 
 int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
-               const float *scale_dev, float *gpos, bool rho_lds) {
-  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos, rho_lds);
+               const float *scale_dev, float *gpos, bool rho_lds, const float *beta_dev, float *cot_out) {
+  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos, rho_lds, 0.0f, beta_dev, cot_out);
 }
 
-int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces) {
-  return kick_sorted(s, b, phi, gamma, nullptr, vel, write_forces);
+int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces, float beta,
+              float *xout, float *xkeep) {
+  const int mode = write_forces ? 1 : (xout ? 2 : 0);
+  return kick_sorted(s, b, phi, gamma, nullptr, vel, mode, DriftArgs{beta, nullptr, xout, xkeep});
 }
 
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
-              float *gpos, bool rho_lds) {
-  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds);
+              float *gpos, bool rho_lds, float beta, float *cot_out) {
+  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds, beta, nullptr, cot_out);
 }
 
 }  // namespace jpm

@@ -119,6 +119,35 @@ class Stepper:
             out.zero_()
             self.kick(buf, x, out, 1.0)
 
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None):
+        """v += gamma F(x), then the drift xout = x + beta v (also into
+        ``xkeep`` when given); ``xout`` may be ``x``.  One launch sequence on
+        the sorted pipeline (the drift rides on the kick's gather)."""
+        buf, srt = field
+        if self.sorted:
+            self._timed("field_kick", "jpm_force_kick_drift_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
+                        ptr(v), float(gamma), float(beta), ptr(xout), ptr(xkeep), ptr(buf), self.r_split,
+                        self.flags, ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+        else:
+            self.field_kick(x, v, gamma, field)
+            if xout is not x:
+                xout.copy_(x)
+            self.axpy(beta, v, xout)
+            if xkeep is not None:
+                xkeep.copy_(xout)
+
+    def vjp_field_drift(self, field, x, cot, scale, beta, gpos):
+        """gpos += scale J(x)^T cot, then the adjoint of the drift before the
+        kick, cot += beta gpos."""
+        buf, srt = field
+        if self.sorted:
+            self._timed("vjp", "jpm_force_vjp_pot_drift_f32", stream(), self.nx, self.ny, self.nz, ptr(buf), ptr(x),
+                        self.n, ptr(cot), float(scale), float(beta), ptr(gpos), self.r_split, self.flags,
+                        ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+        else:
+            self.vjp_field(field, x, cot, scale, gpos)
+            self.axpy(beta, gpos, cot)
+
     def vjp_field(self, field, x, cot, scale, gpos):
         """gpos += scale J(x)^T cot given the evaluation's ``field``."""
         buf, srt = field
@@ -199,6 +228,29 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
         stepper.field_kick(x, v, gamma[i], phi)
 
 
+def _run_fused(stepper, x, v, beta, gamma, start, stop, phi, keep=None, drift_out=True):
+    """Kicks [start, stop) of the fused form, in place: on entry x is y_start
+    (the positions kick ``start`` evaluates its force at, i.e. after drift
+    ``start``) and v the velocity before that kick.  Each kick carries the
+    next drift, y_{i+1} = y_i + beta_{i+1} v, except after the segment's
+    last kick when ``drift_out`` is false or no drift follows (the final
+    step).  With ``keep`` each step's y_i and field go to its buffers (the
+    drift writes y_{i+1} into the next buffer directly)."""
+    S = len(beta)
+    for i in range(start, stop):
+        field = phi
+        if keep is not None:
+            kp = keep[i - start]
+            if i == start:
+                kp[0].copy_(x)
+            field = kp[1]
+        if i + 1 < S and (i + 1 < stop or drift_out):
+            xkeep = keep[i + 1 - start][0] if keep is not None and i + 1 < stop else None
+            stepper.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x, xkeep)
+        else:
+            stepper.field_kick(x, v, gamma[i], field)
+
+
 def _keep_buffers(st, like, count, forces):
     """Per-step adjoint data of a segment: positions, the force evaluation's
     field (potential + kept tile sort) and, for coefficient cotangents, the
@@ -237,14 +289,32 @@ class NBody(torch.autograd.Function):
         v = v0.detach().clone().contiguous()
         phi0 = st.new_field(keep_sort=grad)
         f0 = None
-        if want_c:
-            f0 = torch.empty_like(x)
-            st.forces(x, f0, phi0)
-            st.axpy(k0, f0, v)
-        else:
-            st.field_kick(x, v, k0, phi0)
         k = S if checkpoint_every is None else max(1, int(checkpoint_every))
         last = ((S - 1) // k) * k
+        ctx.stepper, ctx.sched, ctx.k = st, (k0, beta, gamma), k
+        ctx.c_meta = (coeffs.dtype, coeffs.device)
+        ctx.fused = not want_c
+        if ctx.fused:
+            # every kick carries the drift that follows it; checkpoints hold
+            # (y, v) just before a kick
+            x0c = x.clone() if grad else None
+            st.field_kick_drift(x, v, k0, beta[0], phi0, x)
+            ckpts = {0: (x.clone(), v.clone())}
+            keep = _keep_buffers(st, x, min(k, S), False) if grad else None
+            phi = st.new_field()
+            for s0 in range(0, S, k):
+                s1 = min(S, s0 + k)
+                _run_fused(st, x, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None)
+                if s1 < S:
+                    ckpts[s1] = (x.clone(), v.clone())
+            ctx.ckpts = ckpts
+            ctx.kept = (keep, phi0, x0c, last)
+            return x, v
+        # with coefficient cotangents: separate forces and axpy, keeping the
+        # forces and drift velocities the cotangents are dot products with
+        f0 = torch.empty_like(x)
+        st.forces(x, f0, phi0)
+        st.axpy(k0, f0, v)
         ckpts = {0: (x.clone(), v.clone())}   # x0 and v after the first half kick
         keep = _keep_buffers(st, x, min(k, S), want_c) if grad else None
         phi = st.new_field()
@@ -255,13 +325,14 @@ class NBody(torch.autograd.Function):
                 _run(st, x, v, k0, beta, gamma, s0, min(S, s0 + k), phi)
             if s0 + k < S:
                 ckpts[s0 + k] = (x.clone(), v.clone())
-        ctx.stepper, ctx.sched, ctx.k, ctx.ckpts = st, (k0, beta, gamma), k, ckpts
+        ctx.ckpts = ckpts
         ctx.kept = (keep, phi0, f0, last)
-        ctx.c_meta = (coeffs.dtype, coeffs.device)
         return x, v
 
     @staticmethod
     def backward(ctx, gx, gv):
+        if ctx.fused:
+            return NBody._backward_fused(ctx, gx, gv)
         st = ctx.stepper
         k0, beta, gamma = ctx.sched
         S = len(beta)
@@ -293,6 +364,32 @@ class NBody(torch.autograd.Function):
         ctx.ckpts = ctx.kept = None
         gc = dots.to(device=ctx.c_meta[1], dtype=ctx.c_meta[0]) if want_c else None
         return lx, lv, gc, None, None
+
+    @staticmethod
+    def _backward_fused(ctx, gx, gv):
+        """The adjoint of the fused forward: per kick i, in reverse, the force
+        VJP at y_i followed by the adjoint of drift i (lv += beta_i lx), one
+        launch sequence; then the first half kick's VJP at x0."""
+        st = ctx.stepper
+        k0, beta, gamma = ctx.sched
+        S = len(beta)
+        keep, phi0, x0, last = ctx.kept
+        like = ctx.ckpts[0][0]
+        lx = (gx if gx is not None else torch.zeros_like(like)).detach().clone().contiguous()
+        lv = (gv if gv is not None else torch.zeros_like(like)).detach().clone().contiguous()
+        phi = st.new_field()
+        for s0 in reversed(range(0, S, ctx.k)):
+            s1 = min(S, s0 + ctx.k)
+            if s0 != last:
+                yc, vc = ctx.ckpts[s0]
+                y, v = yc.clone(), vc.clone()
+                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, drift_out=False)
+            for i in reversed(range(s0, s1)):
+                kp = keep[i - s0]
+                st.vjp_field_drift(kp[1], kp[0], lv, gamma[i], beta[i], lx)
+        st.vjp_field(phi0, x0, lv, k0, lx)
+        ctx.ckpts = ctx.kept = None
+        return lx, lv, None, None, None
 
 
 def leapfrog_autodiff(x0, v0, coeffs, mesh_shape, paint_absolute_pos=True):
@@ -395,6 +492,21 @@ class BatchStepper:
     def forces(self, x, out, field, ones):
         """out = F(x) (``ones`` a float32 device array of 1.0, [B])."""
         self._kick(x, out, ones, field, True)
+
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None):
+        """v_b += gamma[b] F_b(x_b), then xout_b = x_b + beta[b] v_b (also
+        into ``xkeep``); gamma, beta float32 device arrays [B]."""
+        buf, srt = field
+        call("jpm_batch_force_kick_drift_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(beta),
+             ptr(xout), ptr(xkeep), ptr(buf), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(), ptr(srt),
+             0 if srt is None else srt.numel())
+
+    def vjp_field_drift(self, field, x, cot, scale, pot_scale, beta, gpos):
+        """vjp_field, then the drift adjoint cot_b += beta[b] gpos_b."""
+        buf, srt = field
+        call("jpm_batch_force_vjp_drift_f32", stream(), self.handle, ptr(buf), ptr(x), ptr(cot), ptr(scale),
+             ptr(pot_scale), ptr(beta), ptr(gpos), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(),
+             ptr(srt), 0 if srt is None else srt.numel())
 
     def vjp_field(self, field, x, cot, scale, pot_scale, gpos):
         buf, srt = field
@@ -538,6 +650,20 @@ class BatchLeapfrog:
         x, v = self.x, self.v
         x.copy_(self.x0)
         v.copy_(self.v0)
+        if not self.want_c:
+            # every kick carries the drift that follows it (see _run_fused);
+            # checkpoints hold (y, v) just before a kick
+            st.field_kick_drift(x, v, sc.k0, sc.beta[0], self.phi0, x)
+            for s0 in range(0, self.S, self.k):
+                s1 = min(self.S, s0 + self.k)
+                if s0 == 0:
+                    self.ckpts[0][0].copy_(x)
+                    self.ckpts[0][1].copy_(v)
+                _run_fused(st, x, v, sc.beta, sc.gamma, s0, s1, self.phi, self.keep if s0 == self.last else None)
+                if s1 < self.S:
+                    self.ckpts[s1][0].copy_(x)
+                    self.ckpts[s1][1].copy_(v)
+            return
         # the first half kick and the last segment keep their adjoint data
         if self.want_c:
             st.forces(x, self.f0, self.phi0, sc.ones)
@@ -561,6 +687,19 @@ class BatchLeapfrog:
         lx, lv, dots = self.lx, self.lv, self.dots
         lx.copy_(self.gx)
         lv.copy_(self.gv)
+        if not self.want_c:
+            for s0 in reversed(range(0, S, self.k)):
+                s1 = min(S, s0 + self.k)
+                if s0 != self.last:
+                    self.xs.copy_(self.ckpts[s0][0])
+                    self.vs.copy_(self.ckpts[s0][1])
+                    _run_fused(st, self.xs, self.vs, sc.beta, sc.gamma, s0, s1, self.phi, self.keep,
+                               drift_out=False)
+                for i in reversed(range(s0, s1)):
+                    kp = self.keep[i - s0]
+                    st.vjp_field_drift(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], sc.beta[i], lx)
+            st.vjp_field(self.phi0, self.x0, lv, sc.k0, sc.k0_pot, lx)
+            return
         if self.want_c:
             dots.zero_()
         for s0 in reversed(range(0, S, self.k)):

@@ -3,9 +3,12 @@
     python -m jaxpm_amd.traffic_probe [mesh]
 
 Runs, on a 256^3 (default) workload: a plain device copy of a known size
-(the calibration launch), then the tile-sorted force evaluations with kick
-(`jpm_force_kick_f32`) of a 10-step leapfrog from 2LPT initial conditions to
-the clustered state at a=1, and 5 force VJPs (`jpm_force_vjp_pot_f32`) there.
+(the calibration launch), then the tile-sorted force evaluations of a 10-step
+leapfrog from 2LPT initial conditions to the clustered state at a=1, each
+kick carrying the next drift (`jpm_force_kick_drift_f32`, the last one a
+plain kick), and 5 force VJPs with the drift adjoint
+(`jpm_force_vjp_pot_drift_f32`) there: the launches of the leapfrog's
+forward and adjoint sweeps.
 Meant to be run under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`;
 the per-dispatch counters of a kernel divided by those of the copy give its
 HBM traffic in units of the copy's known byte count.
@@ -42,15 +45,18 @@ def main():
         steps = np.linspace(0.1, 1.0, 11)
         k0, beta, gamma = integrate.kdk_schedule(cosmo, steps)
         field = st.new_field(keep_sort=True)
-        st.field_kick(x, v, k0, field)
-        for i in range(len(beta)):
-            st.axpy(beta[i], v, x)
-            st.field_kick(x, v, gamma[i], field)
+        S = len(beta)
+        st.field_kick_drift(x, v, k0, beta[0], field, x)
+        for i in range(S):
+            if i + 1 < S:
+                st.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x)
+            else:
+                st.field_kick(x, v, gamma[i], field)
         # force VJPs at the final state, reusing its kept tile sort
         cot = torch.randn((N, 3), generator=gen, device=dev)
         gpos = torch.zeros_like(x)
         for _ in range(5):
-            st.vjp_field(field, x, cot, float(gamma[-1]), gpos)
+            st.vjp_field_drift(field, x, cot, float(gamma[-1]), float(beta[-1]), gpos)
     torch.cuda.synchronize()
     print(f"traffic probe done: copy of {4 * N} bytes, {len(beta) + 1} force evaluations and 5 force VJPs at "
           f"{n}^3", flush=True)

@@ -193,3 +193,29 @@ def test_batch_tuning_changes_speed_not_results(gpu):
     for tune, (v, g) in out.items():
         assert rel_err(v, out[(0, 0)][0]) < 1e-6, tune
         assert rel_err(g, out[(0, 0)][1]) < 1e-6, tune
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_fused_kick_drift_matches_separate_drift(gpu, graph):
+    """The kick-drift fused sweeps (no coefficient cotangents) against the
+    sweeps with separate drifts (coefficient cotangents on): the same float
+    operations, so states and adjoints agree bit for bit."""
+    from jaxpm_amd import integrate
+    xs, vs = states(SHAPES, False, 58)
+    r = np.random.default_rng(59)
+    S = 5
+    c = np.concatenate([r.uniform(0.02, 0.06, (len(SHAPES), 1)), r.uniform(0.1, 0.3, (len(SHAPES), S)),
+                        r.uniform(0.03, 0.08, (len(SHAPES), S))], 1)
+    st = integrate.BatchStepper(SHAPES, [x.shape[0] for x in xs])
+    x0 = torch.as_tensor(np.concatenate(xs), device="cuda")
+    v0 = torch.as_tensor(np.concatenate(vs), device="cuda")
+    gx = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    gv = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    out = []
+    for want_c in (False, True):
+        lf = integrate.BatchLeapfrog(st, c, 2, coeff_grads=want_c, graph=graph)
+        x, v = [t.clone() for t in lf.forward(x0, v0)]
+        lx, lv = [t.clone() for t in lf.backward(gx, gv)[:2]]
+        out.append((x, v, lx, lv))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
