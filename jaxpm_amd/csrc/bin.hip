@@ -331,7 +331,7 @@ bin_scatter_global_kernel(const float *__restrict__ pos, uint32_t slice, G g, ui
 // ----------------------------------------------------------------- accum
 
 template <bool REL, class G>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kWindowThreads)
 bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
                  const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, G g, uint32_t M,
                  const float *__restrict__ wmax_dev, float wmax_host, uint32_t n, float *__restrict__ partial,
@@ -356,8 +356,8 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
   // ago) and the record of q + 2B load.
   const uint32_t cnt = d.count, B = blockDim.x;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float4 rec1 = threadIdx.x < cnt ? spos[d.start + threadIdx.x] : zero4;
-  float4 rec2 = threadIdx.x + B < cnt ? spos[d.start + threadIdx.x + B] : zero4;
+  float4 rec1 = threadIdx.x < cnt ? load_global(spos + (d.start + threadIdx.x)) : zero4;
+  float4 rec2 = threadIdx.x + B < cnt ? load_global(spos + (d.start + threadIdx.x + B)) : zero4;
   float nw = wscalar;
   if (weight && threadIdx.x < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
   for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) win[i] = 0ull;
@@ -368,7 +368,7 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
     const float wp = nw;
     rec1 = rec2;
     if (weight && q + B < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
-    if (q + 2 * B < cnt) rec2 = spos[d.start + q + 2 * B];
+    if (q + 2 * B < cnt) rec2 = load_global(spos + (d.start + q + 2 * B));
     const float x = rec.x, y = rec.y, z = rec.z;
     const uint32_t pidx = __float_as_uint(rec.w);
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
@@ -818,7 +818,7 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     uint32_t per;
     (void)slab_grid(b.maxchunks, b.slabs, per);
     const unsigned grid = 8u * per * unsigned(nc);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, b.spos, weight, nc, wscalar, b.desc,
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kWindowThreads), 0, s, b.spos, weight, nc, wscalar, b.desc,
                        b.total_chunks, b.g, b.M, wmax_dev, wmax_host, b.n, partial, overflow, int(accumulate), mesh,
                        per, b.slabs);
   }

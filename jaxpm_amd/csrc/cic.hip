@@ -188,8 +188,18 @@ __global__ void __launch_bounds__(256)
 abs_max_kernel(const float *__restrict__ w, uint32_t n, float *__restrict__ out) {
   __shared__ float sh[4];
   float m = 0.0f;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x)
-    m = fmaxf(m, fabsf(w[p]));
+  // 16-byte loads over the aligned body, scalar loads for the rest
+  const uint32_t head = uint32_t((16u - (reinterpret_cast<uintptr_t>(w) & 15u)) & 15u) / 4u;
+  const uint32_t h = head < n ? head : n;
+  const uint32_t nv = (n - h) / 4u;
+  const float4 *w4 = reinterpret_cast<const float4 *>(w + h);
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  for (uint32_t p = tid; p < nv; p += stride) {
+    const float4 q = w4[p];
+    m = fmaxf(m, fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fmaxf(fabsf(q.z), fabsf(q.w))));
+  }
+  for (uint32_t p = tid; p < h; p += stride) m = fmaxf(m, fabsf(w[p]));
+  for (uint32_t p = h + 4u * nv + tid; p < n; p += stride) m = fmaxf(m, fabsf(w[p]));
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -401,7 +411,7 @@ int abs_max(hipStream_t s, const float *w, uint32_t n, float *out) {
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, out, 1u);
   {
     JPM_TIMED(s, "abs_max_kernel");
-    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, w, n, out);
+    hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for((int64_t(n) + 3) / 4, 256, 2048)), dim3(256), 0, s, w, n, out);
   }
   JPM_LAUNCHED("abs_max_kernel");
   return JPM_OK;

@@ -196,7 +196,7 @@ struct DriftArgs {
 // sorted copy of the positions the forces were computed at.  gamma is per
 // simulation (gamma_dev[sim]) when gamma_dev is given.
 template <bool REL, int MODE, class G>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kWindowThreads)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
                 float *__restrict__ vel, uint32_t per, DriftArgs dr) {
   __shared__ float win[kPwCells];
@@ -212,10 +212,30 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
   int itx, ity, itz;
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+  // Software pipeline over this thread's particles q, q + B, q + 2B, ...:
+  // an iteration computes particle q (record and velocity already loaded),
+  // loads the velocity of q + B (whose record arrived an iteration ago) and
+  // the record of q + 2B, so no load waits on another within an iteration.
+  const uint32_t cnt = d.count, B = blockDim.x;
+  const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float4 rec1 = threadIdx.x < cnt ? load_global(bins.spos + (d.start + threadIdx.x)) : zero4;
+  float4 rec2 = threadIdx.x + B < cnt ? load_global(bins.spos + (d.start + threadIdx.x + B)) : zero4;
+  float nv0 = 0.0f, nv1 = 0.0f, nv2 = 0.0f;
+  if (MODE != 1 && threadIdx.x < cnt) {
+    const float *v = vel + 3 * size_t(__float_as_uint(rec1.w));
+    nv0 = v[0]; nv1 = v[1]; nv2 = v[2];
+  }
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   __syncthreads();
-  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
-    const float4 rec = bins.spos[d.start + q];
+  for (uint32_t q = threadIdx.x; q < cnt; q += B) {
+    const float4 rec = rec1;
+    const float pv0 = nv0, pv1 = nv1, pv2 = nv2;
+    rec1 = rec2;
+    if (MODE != 1 && q + B < cnt) {
+      const float *v = vel + 3 * size_t(__float_as_uint(rec1.w));
+      nv0 = v[0]; nv1 = v[1]; nv2 = v[2];
+    }
+    if (q + 2 * B < cnt) rec2 = load_global(bins.spos + (d.start + q + 2 * B));
     CornerSet cs;
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     float s[3] = {0.0f, 0.0f, 0.0f};
@@ -250,11 +270,11 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
     const uint32_t idx = __float_as_uint(rec.w);
     float *v = vel + 3 * size_t(idx);
     if (MODE == 0) {
-      v[0] += gamma * s[0];
-      v[1] += gamma * s[1];
-      v[2] += gamma * s[2];
+      v[0] = pv0 + gamma * s[0];
+      v[1] = pv1 + gamma * s[1];
+      v[2] = pv2 + gamma * s[2];
     } else if (MODE == 2) {
-      const float v0 = v[0] + gamma * s[0], v1 = v[1] + gamma * s[1], v2 = v[2] + gamma * s[2];
+      const float v0 = pv0 + gamma * s[0], v1 = pv1 + gamma * s[1], v2 = pv2 + gamma * s[2];
       v[0] = v0;
       v[1] = v1;
       v[2] = v2;
@@ -312,8 +332,8 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   // on another within an iteration.
   const uint32_t cnt = d.count, B = blockDim.x;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  float4 rec1 = threadIdx.x < cnt ? bins.spos[d.start + threadIdx.x] : zero4;
-  float4 rec2 = threadIdx.x + B < cnt ? bins.spos[d.start + threadIdx.x + B] : zero4;
+  float4 rec1 = threadIdx.x < cnt ? load_global(bins.spos + (d.start + threadIdx.x)) : zero4;
+  float4 rec2 = threadIdx.x + B < cnt ? load_global(bins.spos + (d.start + threadIdx.x + B)) : zero4;
   float nu[3] = {0.0f, 0.0f, 0.0f}, ng[3] = {0.0f, 0.0f, 0.0f};
   if (threadIdx.x < cnt) {
     const size_t i3 = 3 * size_t(__float_as_uint(rec1.w));
@@ -335,7 +355,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
       nu[0] = cot[i3]; nu[1] = cot[i3 + 1]; nu[2] = cot[i3 + 2];
       ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
     }
-    if (q + 2 * B < cnt) rec2 = bins.spos[d.start + q + 2 * B];
+    if (q + 2 * B < cnt) rec2 = load_global(bins.spos + (d.start + q + 2 * B));
     CornerSet cs;
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
@@ -421,7 +441,7 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
   JPM_REQUIRE(phi && vel, "null pointer argument");
   JPM_REQUIRE(mode != 2 || dr.xout, "the fused drift needs its position output");
   uint32_t per;
-  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
+  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, mode == 2 ? "bin_kick_kernel[drift]" : mode == 1 ? "bin_kick_kernel[forces]" : "bin_kick_kernel");
   auto kern = b.rel ? (mode == 1 ? bin_kick_kernel<true, 1, G> : mode == 2 ? bin_kick_kernel<true, 2, G>
                                                                           : bin_kick_kernel<true, 0, G>)

@@ -14,6 +14,9 @@ constexpr int kWinCells = kWin * kWin * kWin;   // 4913
 constexpr int kChunk = 8192;                    // default max particles per chunk
 constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
 constexpr uint32_t kSlice = 16384;              // default particles per count / scatter workgroup
+// Accumulation / force-gather workgroup size: their LDS windows (38-39 KiB)
+// allow 4 workgroups per CU, so 512 threads keep 32 waves resident.
+constexpr int kWindowThreads = 512;
 
 // Sort tuning carried in bits 16..24 of the pipeline flags (0 = default):
 // chunk = 2^(9 + c) particles for c = bits 16..19 in 1..6 (1024..32768),
@@ -46,6 +49,15 @@ __device__ __forceinline__ uint32_t slab_item(uint32_t b, uint32_t per, bool sla
 inline uint32_t tuned_slice(uint32_t flags) {
   const uint32_t v = (flags >> 20) & 0xfu;
   return (v >= 1 && v <= 7) ? (1u << (11 + v)) : kSlice;
+}
+
+// A load through the global address space.  For some pointers held in
+// structs the compiler otherwise emits a flat load, which also counts
+// against the LDS wait counter: every LDS wait after it would wait for the
+// memory load too, and a load issued early to hide its latency would not.
+__device__ __forceinline__ float4 load_global(const float4 *p) {
+  const __attribute__((address_space(1))) float *q = (const __attribute__((address_space(1))) float *)(p);
+  return make_float4(q[0], q[1], q[2], q[3]);
 }
 
 struct TileGrid {

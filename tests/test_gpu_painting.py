@@ -188,6 +188,12 @@ def test_binned_paint_absolute(gpu, shape, deterministic):
     m = dev(np.zeros(shape))
     _ops.paint(m, dev(pos), weight=dev(w), deterministic=deterministic, zero=True)
     close(host(m), op.cic_paint(np.zeros(shape), pos, w))
+    # weights not 16-byte aligned (the weight bound's vector loads start
+    # after a scalar head)
+    wpad = dev(np.concatenate([[0.0], w]).astype(np.float32))
+    m = dev(np.zeros(shape))
+    _ops.paint(m, dev(pos), weight=wpad[1:], deterministic=deterministic, zero=True)
+    close(host(m), op.cic_paint(np.zeros(shape), pos, w))
 
 
 @pytest.mark.parametrize("shape", BINNED_SHAPES)
