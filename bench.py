@@ -28,11 +28,14 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from hardware counters (FETCH_SIZE and WRITE_SIZE
-# passes of `python -m jaxpm_amd.traffic_probe 256`, each scaled by the same
-# counter's reading for a plain copy of a known 64 MiB: x1.999 for reads,
-# x1.000 for writes), by kernel and mesh side.  The commands and the counter
-# values are in profiles/sorted_gathers_256_traffic.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel", 256): 1153.3e6, ("bin_grad_kernel", 256): 1710.6e6}
+# passes of `python -m jaxpm_amd.traffic_probe 256 <tuning>`, each scaled by
+# the same counter's reading for a plain copy of a known 64 MiB: x1.999 for
+# reads, x1.000 for writes), by kernel, mesh side and where the force-VJP
+# gather reads the density adjoint ("lds" / "caches"; None: either).  The
+# commands and the counter values are in profiles/sorted_gathers_256_traffic.md.
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256, None): 1403.9e6,
+                     ("bin_grad_kernel[drift]", 256, "caches"): 2364.4e6,
+                     ("bin_grad_kernel[drift]", 256, "lds"): 1943.6e6}
 
 
 def fixed_tuning(args):
@@ -461,6 +464,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(n, n, x0.cpu().numpy(), v0.cpu().numpy())
 
+    rho_mode = "caches" if len(tuning) > 2 and tuning[2] else "lds"
     out = {
         "metric": "fwd+adjoint PM steps/s at 256^3 mesh; CIC paint/read GB/s as % HBM peak",
         "value": round(steps_per_s, 3),
@@ -483,7 +487,7 @@ def main():
                                    "xcd_placement": "slabs" if len(tuning) > 3 and tuning[3] else "round-robin"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC_BYTES.get((dom, n)),
+                     "traffic": PMC_TRAFFIC_BYTES.get((dom, n, rho_mode), PMC_TRAFFIC_BYTES.get((dom, n, None))),
                      "algorithmic_bytes": int(algo[dom]), "ms_per_launch": round(t_dom, 4)},
         "kernels": kinfo,
         "plain_autodiff": plain,

@@ -1,6 +1,6 @@
 """Launch a fixed set of kernels for hardware-counter passes.
 
-    python -m jaxpm_amd.traffic_probe [mesh]
+    python -m jaxpm_amd.traffic_probe [mesh] [chunk,slice,rho_global,xcd_slabs]
 
 Runs, on a 256^3 (default) workload: a plain device copy of a known size
 (the calibration launch), then the tile-sorted force evaluations of a 10-step
@@ -23,6 +23,7 @@ def main():
     from jaxpm_amd import _lib, integrate, pm
     from jaxpm_amd import cosmology as pc
     n = int(sys.argv[1]) if len(sys.argv) > 1 else 256
+    tuning = tuple(int(c) for c in sys.argv[2].split(",")) if len(sys.argv) > 2 else (0, 0)
     dev = _lib.device()
     shape = (n, n, n)
     N = n ** 3
@@ -41,7 +42,7 @@ def main():
         dx, p, _ = pm.lpt(cosmo, ic, lattice, a=0.1, order=2)
         x = (lattice + dx).reshape(-1, 3).contiguous()
         v = p.reshape(-1, 3).contiguous()
-        st = integrate.Stepper(shape, N)
+        st = integrate.Stepper(shape, N, sort_tuning=tuning)
         steps = np.linspace(0.1, 1.0, 11)
         k0, beta, gamma = integrate.kdk_schedule(cosmo, steps)
         field = st.new_field(keep_sort=True)
@@ -59,7 +60,7 @@ def main():
             st.vjp_field_drift(field, x, cot, float(gamma[-1]), float(beta[-1]), gpos)
     torch.cuda.synchronize()
     print(f"traffic probe done: copy of {4 * N} bytes, {len(beta) + 1} force evaluations and 5 force VJPs at "
-          f"{n}^3", flush=True)
+          f"{n}^3, sort tuning {tuning}", flush=True)
 
 
 if __name__ == "__main__":
