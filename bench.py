@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--checkpoint", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48", "large"], default="pm",
+    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48", "large", "kernels"], default="pm",
                     help="pm: the default forward+adjoint step benchmark; diffsim: the differentiable "
                          "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8)")
     ap.add_argument("--no-tune", action="store_true", help="skip the sort autotuner (default widths)")
@@ -248,6 +248,81 @@ def batch48(args):
     print(json.dumps(out))
 
 
+def kernels(args):
+    """Kernel microbenchmark, forward only: cic_paint, cic_read and one
+    pm_forces on a 256^3 mesh with 256^3 particles, in two position states --
+    uniform random in the box, and clustered (2LPT at a=0.1 from seed-2 noise,
+    box 1024 Mpc/h, then 20 leapfrog steps to a=1).  GB/s on algorithmic
+    bytes: paint 12N + 4M (positions in, mesh out), read 12N + 4M + 4N.
+    Checks that need no oracle (parity with the oracle is the test suite's):
+    the paint conserves mass, the binned paint equals the atomic one, and
+    pm_forces of the tile-sorted pipeline equals the unfused one (atomic
+    paint, three force meshes, separate read)."""
+    import torch
+
+    from jaxpm_amd import _lib, _ops, integrate, painting, pm
+    from jaxpm_amd import cosmology as pc
+    dev = _lib.device()
+    n = args.mesh
+    shape = (n, n, n)
+    N = M = n ** 3
+    gen = torch.Generator(device=dev).manual_seed(2)
+    uniform = torch.rand((N, 3), generator=gen, device=dev) * n
+    cosmo = pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    k, pk = pc.pk_table(cosmo)
+    with torch.no_grad():
+        ic = pm.linear_field(shape, (args.box,) * 3, pm.TabulatedPk(k, pk), torch.randn(shape, generator=gen,
+                                                                                          device=dev))
+        lattice = torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
+                                             indexing="ij"), -1)
+        dx, p, _ = pm.lpt(cosmo, ic, lattice, a=0.1, order=2)
+        x0 = (lattice + dx).reshape(-1, 3).contiguous()
+        v0 = p.reshape(-1, 3).contiguous()
+        st = integrate.Stepper(shape, N)
+        sched = integrate.schedule_tensor(integrate.kdk_schedule(cosmo, np.linspace(0.1, 1.0, 21)))
+        clustered, _ = integrate.NBody.apply(x0, v0, sched, st, 5)
+    del ic, lattice, dx, p, x0, v0, st
+    reps = args.kernel_reps
+    paint_bytes, read_bytes = 12 * N + 4 * M, 12 * N + 4 * M + 4 * N
+    out_states = {}
+    for name, pos in (("uniform", uniform), ("clustered", clustered)):
+        mesh = torch.zeros(shape, device=dev)
+        t_paint = kernel_time_ms(lambda: painting.cic_paint(mesh, pos), reps)
+        t_atomic = kernel_time_ms(lambda: _ops.paint(torch.zeros(shape, device=dev), pos, method="atomic",
+                                                     zero=True), reps)
+        rho = painting.cic_paint(mesh, pos)
+        rho_atomic = _ops.paint(torch.zeros(shape, device=dev), pos, method="atomic", zero=True)
+        t_read = kernel_time_ms(lambda: painting.cic_read(rho, pos), reps)
+        t_forces = kernel_time_ms(lambda: pm.pm_forces(pos, mesh_shape=shape), reps)
+        f_fused = pm.pm_forces(pos, mesh_shape=shape)
+        f_plain = _ops.pm_forces_fused(pos, shape, atomic=True)
+        torch.cuda.synchronize()
+        out_states[name] = {
+            "cic_paint": {"ms": round(t_paint, 4), "GB_s": round(paint_bytes / (t_paint * 1e-3) / 1e9, 1),
+                          "frac_hbm": round(paint_bytes / (t_paint * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "what": "tile sort + LDS windows + face reduce, deterministic"},
+            "cic_paint_atomic": {"ms": round(t_atomic, 4),
+                                 "GB_s": round(paint_bytes / (t_atomic * 1e-3) / 1e9, 1),
+                                 "frac_hbm": round(paint_bytes / (t_atomic * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "what": "one global float atomic per corner"},
+            "cic_read": {"ms": round(t_read, 4), "GB_s": round(read_bytes / (t_read * 1e-3) / 1e9, 1),
+                         "frac_hbm": round(read_bytes / (t_read * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "pm_forces": {"ms": round(t_forces, 4)},
+            "checks": {"mass_rel_err": abs(float(rho.double().sum()) - N) / N,
+                       "binned_vs_atomic_max_rel": float((rho - rho_atomic).abs().max() / rho.abs().max()),
+                       "pm_forces_sorted_vs_unfused_max_rel": float((f_fused - f_plain).abs().max() /
+                                                                    f_plain.abs().max())}}
+    c = out_states["clustered"]
+    out = {"metric": "CIC paint / read GB/s (as % HBM peak) and pm_forces ms at 256^3, uniform and clustered",
+           "value": c["cic_paint"]["GB_s"], "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+           "ms_per_step": c["cic_paint"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"cic_paint / cic_read / pm_forces, {n}^3 particles on {n}^3 mesh, uniform "
+                                  "random and 2LPT + 20-step clustered positions", "mesh": n, "particles": N},
+           "states": out_states}
+    print(json.dumps(out))
+
+
 def large(args):
     """Large single-GPU run: 512^3 particles (a lattice of spacing 2 cells)
     on a 1024^3 mesh, box 1000 Mpc/h, 2LPT at a=0.1 and 30 leapfrog steps to
@@ -318,6 +393,8 @@ def main():
         return batch48(args)
     if args.workload == "large":
         return large(args)
+    if args.workload == "kernels":
+        return kernels(args)
     import torch
 
     from jaxpm_amd import _lib, _ops, integrate, painting, pm

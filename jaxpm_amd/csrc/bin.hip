@@ -330,6 +330,16 @@ bin_scatter_global_kernel(const float *__restrict__ pos, uint32_t slice, G g, ui
 
 // ----------------------------------------------------------------- accum
 
+// The +1 halo of a full (16^3) single-chunk tile in absolute mode is stored
+// compactly: the lx = 16 plane (17 x 17, index ly * 17 + lz), then the ly = 16
+// cells with lx < 16 (16 x 17), then the lz = 16 cells with lx, ly < 16
+// (16 x 16) -- 817 floats.
+constexpr int kH0 = kWin * kWin, kH1 = kH0 + kTile * kWin, kH = kH1 + kTile * kTile;
+
+__device__ __forceinline__ int halo_index(int lx, int ly, int lz) {
+  return lx == kTile ? ly * kWin + lz : (ly == kTile ? kH0 + lx * kWin + lz : kH1 + lx * kTile + ly);
+}
+
 template <bool REL, class G>
 __global__ void __launch_bounds__(kWindowThreads)
 bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
@@ -431,8 +441,9 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
         float *cellp = dst + (uint32_t(ox + lx) * uint32_t(t.ny) + uint32_t(oy + ly)) * uint32_t(t.nz) + uint32_t(oz + lz);
         *cellp = accumulate ? *cellp + v : v;
       }
-      // halo: the lx = 16 plane, then ly = 16 (lx < 16), then lz = 16 (lx, ly < 16)
-      constexpr int kH0 = kWin * kWin, kH1 = kH0 + kTile * kWin, kH = kH1 + kTile * kTile;
+      // halo, compact (halo_index): the lx = 16 plane, then ly = 16 (lx <
+      // 16), then lz = 16 (lx, ly < 16), so both this write and the reduce's
+      // reads of each face run along consecutive floats
       for (int e = threadIdx.x; e < kH; e += blockDim.x) {
         int lx, ly, lz;
         if (e < kH0) {
@@ -444,8 +455,7 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
           const int f = e - kH1;
           lx = f >> 4; ly = f & 15; lz = kTile;
         }
-        const int i = (lx * kWin + ly) * kWin + lz;
-        out[i] = float(double((long long)win[i]) * inv);
+        out[e] = float(double((long long)win[(lx * kWin + ly) * kWin + lz]) * inv);
       }
     } else {
       for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) {
@@ -561,7 +571,10 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
           const int li = (la * kWin + lb) * kWin + lc;
           const uint32_t c0 = src_off[q], cn = src_cnt[q];
           if (cn == 1) {
-            acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + li];
+            // a full single-chunk source (absolute mode) holds its halo compactly
+            const bool compact = DIRECT && (sx ? PLx : Lx) == kTile && (sy ? PLy : Ly) == kTile &&
+                                 (sz ? PLz : Lz) == kTile;
+            acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + (compact ? halo_index(la, lb, lc) : li)];
           } else if (cn > 1) {
             unsigned long long sum = 0ull;
             for (uint32_t k = 0; k < cn; ++k) sum += ipart[(size_t(c0 + k) * nc + ch) * kWinCells + li];
