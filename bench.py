@@ -293,6 +293,8 @@ def kernels(args):
         rho = painting.cic_paint(mesh, pos)
         rho_atomic = _ops.paint(torch.zeros(shape, device=dev), pos, method="atomic", zero=True)
         t_read = kernel_time_ms(lambda: painting.cic_read(rho, pos), reps)
+        t_read_sorted = kernel_time_ms(lambda: painting.cic_read(rho, pos, method="sorted"), reps)
+        same_read = bool(torch.equal(painting.cic_read(rho, pos), painting.cic_read(rho, pos, method="sorted")))
         t_forces = kernel_time_ms(lambda: pm.pm_forces(pos, mesh_shape=shape), reps)
         f_fused = pm.pm_forces(pos, mesh_shape=shape)
         f_plain = _ops.pm_forces_fused(pos, shape, atomic=True)
@@ -306,9 +308,15 @@ def kernels(args):
                                  "frac_hbm": round(paint_bytes / (t_atomic * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                  "what": "one global float atomic per corner"},
             "cic_read": {"ms": round(t_read, 4), "GB_s": round(read_bytes / (t_read * 1e-3) / 1e9, 1),
-                         "frac_hbm": round(read_bytes / (t_read * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                         "frac_hbm": round(read_bytes / (t_read * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "what": "direct 8-corner gather"},
+            "cic_read_sorted": {"ms": round(t_read_sorted, 4),
+                                "GB_s": round(read_bytes / (t_read_sorted * 1e-3) / 1e9, 1),
+                                "frac_hbm": round(read_bytes / (t_read_sorted * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "what": "tile sort + LDS windows (method='sorted')"},
             "pm_forces": {"ms": round(t_forces, 4)},
             "checks": {"mass_rel_err": abs(float(rho.double().sum()) - N) / N,
+                       "sorted_read_bitwise_equal": same_read,
                        "binned_vs_atomic_max_rel": float((rho - rho_atomic).abs().max() / rho.abs().max()),
                        "pm_forces_sorted_vs_unfused_max_rel": float((f_fused - f_plain).abs().max() /
                                                                     f_plain.abs().max())}}

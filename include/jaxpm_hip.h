@@ -350,6 +350,17 @@ int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3,
                       float scale, float *gpos, float r_split, uint32_t flags,
                       void *ws, size_t ws_bytes);
 
+/* cic_read (painting.py:78-106) through a tile sort of the positions, for
+ * particles in no spatial order: absolute positions, 1 to 3 meshes
+ * (comp_stride floats apart), out[p * ncomp + c], bit for bit the values of
+ * jpm_cic_read_f32.  Workspace of jpm_cic_read_sorted_workspace_size bytes
+ * (0: the mesh is too large for the tile sort). */
+size_t jpm_cic_read_sorted_workspace_size(int nx, int ny, int nz, int64_t n);
+int jpm_cic_read_sorted_f32(void *stream, int nx, int ny, int nz,
+                            const float *mesh, int ncomp, int64_t comp_stride,
+                            const float *pos, int64_t n, float *out,
+                            uint32_t flags, void *ws, size_t ws_bytes);
+
 /* Workspace bytes for jpm_force_kick_f32 (0 when three meshes exceed 2^32
  * cells or flags ask for the exact gradient / atomic paint, which the sorted
  * pipeline does not cover). */

@@ -41,10 +41,10 @@ class CicRead(torch.autograd.Function):
     218-236).  meshes: (nc, nx, ny, nz)."""
 
     @staticmethod
-    def forward(ctx, meshes, pos, relative, deterministic):
+    def forward(ctx, meshes, pos, relative, deterministic, method="direct"):
         ctx.save_for_backward(meshes, pos)
         ctx.relative, ctx.deterministic = relative, deterministic
-        return _ops.read(meshes, pos, relative=relative)
+        return _ops.read(meshes, pos, relative=relative, method=method)
 
     @staticmethod
     def backward(ctx, g):
@@ -56,7 +56,7 @@ class CicRead(torch.autograd.Function):
                                   deterministic=ctx.deterministic)
         if ctx.needs_input_grad[1]:
             gp = _ops.grad_pos(meshes, pos, cot=g, relative=ctx.relative).reshape(pos.shape)
-        return gm, gp, None, None
+        return gm, gp, None, None, None
 
 
 class ForceField(torch.autograd.Function):

@@ -51,6 +51,8 @@ SIGNATURES = {
     "jpm_cic_paint_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_cic_paint_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _vp, _u32, _vp, _sz]),
     "jpm_cic_read_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _u32]),
+    "jpm_cic_read_sorted_workspace_size": (_sz, [_i, _i, _i, _i64]),
+    "jpm_cic_read_sorted_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _u32, _vp, _sz]),
     "jpm_cic_paint_multi_workspace_size": (_sz, [_i, _i, _i, _i, _u32]),
     "jpm_cic_paint_multi_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _i, _vp, _u32, _vp, _sz]),
     "jpm_cic_grad_pos_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _vp, _f, _vp, _u32, _i]),

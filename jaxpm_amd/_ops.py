@@ -102,8 +102,12 @@ def paint_multi(shape, pos, weights, relative=False, deterministic=False, method
     return out
 
 
-def read(meshes, pos, relative=False):
-    """meshes (nc,nx,ny,nz) or (nx,ny,nz) -> values (n,nc) or (n,)."""
+def read(meshes, pos, relative=False, method="direct"):
+    """meshes (nc,nx,ny,nz) or (nx,ny,nz) -> values (n,nc) or (n,).
+
+    method "direct" (8 corner loads per particle, the default) or "sorted"
+    (a tile sort first, then LDS windows: faster for particles in no spatial
+    order; absolute positions, up to 3 meshes; the same values)."""
     meshes = _f32(meshes, "mesh")
     single = meshes.dim() == 3
     m4 = meshes.unsqueeze(0) if single else meshes
@@ -112,6 +116,18 @@ def read(meshes, pos, relative=False):
     pos = _f32(pos, "positions").reshape(-1, 3)
     n = pos.shape[0]
     out = torch.empty((n, nc), dtype=torch.float32, device=pos.device)
+    if method == "sorted":
+        if relative or nc > 3:
+            raise ValueError("the sorted read takes absolute positions and at most 3 meshes")
+        wsb = size("jpm_cic_read_sorted_workspace_size", nx, ny, nz, n)
+        if wsb == 0 and n > 0:
+            raise ValueError(f"mesh {(nx, ny, nz)} is too large for the tile-sorted read")
+        ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=pos.device)
+        call("jpm_cic_read_sorted_f32", stream(), nx, ny, nz, ptr(m4), nc, nx * ny * nz, ptr(pos), n, ptr(out), 0,
+             ptr(ws), wsb)
+        return out.reshape(n) if single else out
+    if method != "direct":
+        raise ValueError(f"unknown read method {method!r}")
     call("jpm_cic_read_f32", stream(), nx, ny, nz, ptr(m4), nc, nx * ny * nz, ptr(pos), n, ptr(out),
          _lib.RELATIVE if relative else 0)
     return out.reshape(n) if single else out

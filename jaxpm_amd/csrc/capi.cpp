@@ -162,6 +162,7 @@ size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
 int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *);
 int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool, float,
               float *);
+int bins_read(hipStream_t, const Bins &, const float *, int, uint32_t, float *);
 
 // ------------------------------------------------------ sorted pipelines
 //
@@ -229,6 +230,30 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, phi);
   if (rc) return rc;
   return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
+}
+
+// cic_read of up to 3 meshes through a tile sort of the positions (absolute
+// mode): for particles in no spatial order, whose direct 8-corner gathers
+// would each touch separate cache lines.
+static int cic_read_sorted(hipStream_t s, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+                           const float *pos, int64_t n, float *out, uint32_t flags, void *ws, size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(!(flags & JPM_RELATIVE), "the tile-sorted read takes absolute positions");
+  JPM_REQUIRE(binned_paint_supported(nx, ny, nz), "mesh too large for the tile sort");
+  JPM_REQUIRE(ncomp >= 1 && ncomp <= 3, "ncomp=%d must be 1..3", ncomp);
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  const int64_t m = cells(nx, ny, nz);
+  JPM_REQUIRE(ncomp == 1 || (comp_stride >= m && int64_t(ncomp - 1) * comp_stride + m <= (int64_t(1) << 32)),
+              "bad component stride %lld", (long long)comp_stride);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(mesh && pos && out, "null pointer argument");
+  const size_t need = bin_sort_workspace_size(nx, ny, nz, n, 0);
+  JPM_REQUIRE(ws && ws_bytes >= need, "tile-sorted read needs %zu workspace bytes, got %zu", need, ws_bytes);
+  Bins b;
+  rc = bin_sort(s, nx, ny, nz, pos, n, 0, ws, ws_bytes, b);
+  if (rc) return rc;
+  return bins_read(s, b, mesh, ncomp, uint32_t(ncomp == 1 ? m : comp_stride), out);
 }
 
 static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
@@ -803,6 +828,17 @@ int jpm_force_vjp_pot_drift_f32(void *stream, int nx, int ny, int nz, const floa
   if (!cot) return fail(JPM_E_INVALID, "cot is null");
   JPM_GUARD(force_vjp_pot(as_stream(stream), nx, ny, nz, phi, pos, n, cot, scale, gpos, r_split, flags, ws,
                           ws_bytes, sorted, sorted_bytes, beta, cot));
+}
+
+size_t jpm_cic_read_sorted_workspace_size(int nx, int ny, int nz, int64_t n) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !binned_paint_supported(nx, ny, nz)) return 0;
+  return bin_sort_workspace_size(nx, ny, nz, n, 0);
+}
+
+int jpm_cic_read_sorted_f32(void *stream, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
+                            const float *pos, int64_t n, float *out, uint32_t flags, void *ws, size_t ws_bytes) {
+  JPM_GUARD(cic_read_sorted(as_stream(stream), nx, ny, nz, mesh, ncomp, comp_stride, pos, n, out, flags, ws,
+                            ws_bytes));
 }
 
 size_t jpm_cic_paint_binned_workspace_size(int nx, int ny, int nz, int64_t n, int nc, uint32_t flags) {

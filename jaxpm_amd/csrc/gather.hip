@@ -435,6 +435,52 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   }
 }
 
+// cic_read (painting.py:78-106) of nc meshes over the tile-sorted particles:
+// each workgroup loads the tile's 17^3 window of every mesh into LDS and
+// sums the 8 corners of its particles in painting.py's corner order
+// (000,100,010,001,110,101,011,111), the order and float operations of the
+// direct read, so both give the same bits.  Absolute positions only.
+template <int NC, class G>
+__global__ void __launch_bounds__(kWindowThreads)
+bin_read_kernel(BinsT<G> bins, const float *__restrict__ mesh_all, uint32_t cstride, float *__restrict__ out,
+                uint32_t per) {
+  __shared__ float win[NC * kWinCells];
+  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
+  if (c >= *bins.total_chunks) return;
+  const ChunkDesc d = bins.desc[c];
+  const SimRef &sim = bins.g.tile_sim(d.tile);
+  const TileGrid &t = sim.t;
+  const float *mesh = mesh_all + sim.cell0;
+  int itx, ity, itz;
+  tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
+  const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
+#pragma unroll
+  for (int ch = 0; ch < NC; ++ch) load_window(mesh + size_t(ch) * cstride, t, ox, oy, oz, 0, kWin, win + ch * kWinCells);
+  __syncthreads();
+  const int xs[8] = {0, 1, 0, 0, 1, 1, 0, 1};
+  const int ys[8] = {0, 0, 1, 0, 1, 0, 1, 1};
+  const int zs[8] = {0, 0, 0, 1, 0, 1, 1, 1};
+  for (uint32_t q = threadIdx.x; q < d.count; q += blockDim.x) {
+    const float4 rec = load_global(bins.spos + (d.start + q));
+    const LAxis ax = laxis_abs(rec.x, ox, t.nx), ay = laxis_abs(rec.y, oy, t.ny), az = laxis_abs(rec.z, oz, t.nz);
+    int li[8];
+    float w[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      li[o] = ((xs[o] ? ax.l1 : ax.l0) * kWin + (ys[o] ? ay.l1 : ay.l0)) * kWin + (zs[o] ? az.l1 : az.l0);
+      w[o] = (xs[o] ? ax.w1 : ax.w0) * (ys[o] ? ay.w1 : ay.w0) * (zs[o] ? az.w1 : az.w0);
+    }
+    const uint32_t idx = __float_as_uint(rec.w);
+#pragma unroll
+    for (int ch = 0; ch < NC; ++ch) {
+      float sum = 0.0f;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) sum += win[ch * kWinCells + li[o]] * w[o];
+      out[size_t(idx) * NC + ch] = sum;
+    }
+  }
+}
+
 template <class G>
 static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float gamma, const float *gamma_dev,
                        float *vel, int mode, DriftArgs dr) {
@@ -489,6 +535,19 @@ int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
               float *gpos, bool rho_lds, float beta, float *cot_out) {
   return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds, beta, nullptr, cot_out);
+}
+
+int bins_read(hipStream_t s, const Bins &b, const float *mesh, int nc, uint32_t cstride, float *out) {
+  JPM_REQUIRE(mesh && out, "null pointer argument");
+  JPM_REQUIRE(!b.rel, "the tile-sorted read takes absolute positions");
+  JPM_REQUIRE(nc >= 1 && nc <= 3, "the tile-sorted read takes 1 to 3 meshes (got %d)", nc);
+  uint32_t per;
+  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  JPM_TIMED(s, "bin_read_kernel");
+  auto kern = nc == 1 ? bin_read_kernel<1, OneSim> : nc == 2 ? bin_read_kernel<2, OneSim> : bin_read_kernel<3, OneSim>;
+  hipLaunchKernelGGL(kern, grid, block, 0, s, b, mesh, cstride, out, per);
+  JPM_LAUNCHED("bin_read_kernel");
+  return JPM_OK;
 }
 
 }  // namespace jpm
