@@ -483,6 +483,8 @@ def main():
         "axpy_kernel": 36 * N,
         "hipfft_r2c": 4 * M + 8 * H,
         "hipfft_c2r": 8 * H + 4 * M,
+        "fft_r2c": 4 * M + 8 * H,
+        "fft_c2r": 8 * H + 4 * M,
         "abs_max_kernel": 12 * N,
     }
     kinfo = {}
@@ -495,7 +497,7 @@ def main():
             entry.update({"GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)})
         kinfo[name] = entry
     # dominant hand-written kernel by total time in the timed region
-    dom = max((k for k in per_kernel if k in algo and not k.startswith("hipfft")),
+    dom = max((k for k in per_kernel if k in algo and not k.startswith(("hipfft", "fft_"))),
               key=lambda k: per_kernel[k][1])
     t_dom = per_kernel[dom][1] / per_kernel[dom][0]
     achieved = algo[dom] / (t_dom * 1e-3) / 1e9

@@ -4,6 +4,7 @@
 // threads never race on one plan's stream.
 #include <hipfft/hipfft.h>
 
+#include <cstdint>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -77,11 +78,25 @@ static int sub_batch(int nx, int ny, int nz, int batch) {
   return int(sub < batch ? sub : batch);
 }
 
+bool native_fft_ok(int nx, int ny, int nz);
+int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out);
+int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out);
+
+// Power-of-two meshes use the library's own transforms (fftk.hip); other
+// shapes, or real buffers not 8-byte aligned, go to hipFFT.
+static bool use_native(int nx, int ny, int nz, const void *real) {
+  return native_fft_ok(nx, ny, nz) && (reinterpret_cast<uintptr_t>(real) & 7u) == 0;
+}
+
 int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
+  if (use_native(nx, ny, nz, in)) {
+    JPM_TIMED(s, "fft_r2c");
+    return native_r2c(s, nx, ny, nz, batch, in, out);
+  }
   const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
   const int sub = sub_batch(nx, ny, nz, batch);
   std::lock_guard<std::mutex> lk(g_mu);
@@ -105,6 +120,10 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
   if (rc) return rc;
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
+  if (use_native(nx, ny, nz, out)) {
+    JPM_TIMED(s, "fft_c2r");
+    return native_c2r(s, nx, ny, nz, batch, in, out);
+  }
   const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
   const int sub = sub_batch(nx, ny, nz, batch);
   std::lock_guard<std::mutex> lk(g_mu);
@@ -155,10 +174,15 @@ int fft_c2c(hipStream_t s, int nx, int ny, int nz, int batch, const void *in, vo
   return c2c_combine(s, nx, ny, nz, batch, xyh, out, sign);
 }
 
+void free_twiddles();
+
 void free_plans() {
-  std::lock_guard<std::mutex> lk(g_mu);
-  for (auto &kv : g_plans) hipfftDestroy(kv.second);
-  g_plans.clear();
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &kv : g_plans) hipfftDestroy(kv.second);
+    g_plans.clear();
+  }
+  free_twiddles();
 }
 
 }  // namespace jpm

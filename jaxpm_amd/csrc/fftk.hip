@@ -1,0 +1,432 @@
+// Native 3-D real-to-complex / complex-to-real FFTs for power-of-two meshes,
+// the transforms of jaxdecomp.pfft3d / pifft3d (distributed.py:37-42) with
+// hipFFT's conventions: unnormalised, e^{-i} forward, half spectrum
+// [nx][ny][nz/2+1].
+//
+// Each pass reads and writes the data once:
+//   R2C  z: every row of nz reals as nz/2 complex points (even, odd
+//           samples), an nz/2-point FFT, then the split into the nz/2 + 1
+//           half-spectrum values;  y, x: in-place column FFTs over kz blocks.
+//   C2R  x, y: in-place inverse column FFTs;  z: the merge of each
+//           half-spectrum row into nz/2 complex points, an inverse FFT, the
+//           real row (imaginary parts of kz = 0 and nz/2 dropped, as irfft).
+// A workgroup stages up to 4096 complex points (KB columns of length L) in
+// LDS and runs a Stockham FFT of radix-16 stages (then one radix-8, 4 or 2
+// stage) with twiddles from a table computed in double precision.
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace jpm {
+
+constexpr int kFftThreads = 512;
+constexpr int kFftPoints = 4096;   // complex points per workgroup
+constexpr int kFftMaxL = 1024;
+
+// exp(-2 pi i e / L) for e in [0, L), L = 2 .. 2048 (powers of two), each
+// table at offset L - 2 of one device buffer.
+static std::mutex g_tw_mu;
+static float2 *g_tw = nullptr;
+
+static int twiddles(const float2 **out) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  if (!g_tw) {
+    std::vector<float2> h;
+    for (int L = 2; L <= 2 * kFftMaxL; L *= 2)
+      for (int e = 0; e < L; ++e) {
+        const double a = -2.0 * M_PI * double(e) / double(L);
+        h.push_back(make_float2(float(std::cos(a)), float(std::sin(a))));
+      }
+    float *buf = nullptr;
+    JPM_HIP(hipMalloc(&buf, h.size() * sizeof(float2)));
+    const hipError_t e = hipMemcpy(buf, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(buf);
+      return fail(JPM_E_HIP, "uploading FFT twiddles failed: %s", hipGetErrorString(e));
+    }
+    g_tw = reinterpret_cast<float2 *>(buf);
+  }
+  *out = g_tw;
+  return JPM_OK;
+}
+
+void free_twiddles() {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  if (g_tw) (void)hipFree(g_tw);
+  g_tw = nullptr;
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// 4-point DFT in place, e^{-i} (forward) or e^{+i} (INV).
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+  const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), d = csub(a1, a3);
+  const float2 t3 = INV ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);   // d * (+-i)
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// a * W_16^e (e^{-2 pi i e / 16}, conjugated for INV) for the constant
+// exponents of the radix-8 and radix-16 butterflies.
+template <bool INV, int E>
+__device__ __forceinline__ float2 w16(float2 a) {
+  constexpr float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+  constexpr int e = E & 15;
+  float cr, ci;   // W = cr - i ci (forward)
+  if constexpr (e == 0) { return a; }
+  else if constexpr (e == 4) { return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x); }
+  else if constexpr (e == 8) { return make_float2(-a.x, -a.y); }
+  else if constexpr (e == 12) { return INV ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x); }
+  else {
+    constexpr float tc[16] = {1.0f, c1, h, s1, 0.0f, -s1, -h, -c1, -1.0f, -c1, -h, -s1, 0.0f, s1, h, c1};
+    constexpr float ts[16] = {0.0f, s1, h, c1, 1.0f, c1, h, s1, 0.0f, -s1, -h, -c1, -1.0f, -c1, -h, -s1};
+    cr = tc[e];
+    ci = INV ? -ts[e] : ts[e];
+    return make_float2(a.x * cr + a.y * ci, a.y * cr - a.x * ci);
+  }
+}
+
+// R-point DFT of a[0..R) in place, natural order, R = 2, 4, 8 or 16:
+// a length-R transform as (R/4 or 2) x 4 with the W_R twiddles between.
+template <int R, bool INV>
+__device__ __forceinline__ void dftR(float2 *a) {
+  if constexpr (R == 2) {
+    const float2 b0 = cadd(a[0], a[1]), b1 = csub(a[0], a[1]);
+    a[0] = b0;
+    a[1] = b1;
+  } else if constexpr (R == 4) {
+    dft4<INV>(a[0], a[1], a[2], a[3]);
+  } else if constexpr (R == 8) {
+    // m = 2 m1 + m2 (m1 < 4, m2 < 2); q = q1 + 4 q2
+    dft4<INV>(a[0], a[2], a[4], a[6]);
+    dft4<INV>(a[1], a[3], a[5], a[7]);
+    a[3] = w16<INV, 2>(a[3]);   // W_8^{m2 q1} = W_16^{2 m2 q1}
+    a[5] = w16<INV, 4>(a[5]);
+    a[7] = w16<INV, 6>(a[7]);
+    float2 b[8];
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1) {
+      b[q1] = cadd(a[2 * q1], a[2 * q1 + 1]);
+      b[q1 + 4] = csub(a[2 * q1], a[2 * q1 + 1]);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = b[q];
+  } else {
+    // m = 4 m1 + m2; q = q1 + 4 q2
+#pragma unroll
+    for (int m2 = 0; m2 < 4; ++m2) dft4<INV>(a[m2], a[4 + m2], a[8 + m2], a[12 + m2]);
+    // y[m2][q1] = a[4 q1 + m2] now; twiddle by W_16^{m2 q1}
+    a[5] = w16<INV, 1>(a[5]);
+    a[9] = w16<INV, 2>(a[9]);
+    a[13] = w16<INV, 3>(a[13]);
+    a[6] = w16<INV, 2>(a[6]);
+    a[10] = w16<INV, 4>(a[10]);
+    a[14] = w16<INV, 6>(a[14]);
+    a[7] = w16<INV, 3>(a[7]);
+    a[11] = w16<INV, 6>(a[11]);
+    a[15] = w16<INV, 9>(a[15]);
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1) dft4<INV>(a[4 * q1], a[4 * q1 + 1], a[4 * q1 + 2], a[4 * q1 + 3]);
+    // X[q1 + 4 q2] = a[4 q1 + q2]: transpose the 4 x 4 block
+    float2 b[16];
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1)
+#pragma unroll
+      for (int q2 = 0; q2 < 4; ++q2) b[q1 + 4 * q2] = a[4 * q1 + q2];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a[q] = b[q];
+  }
+}
+
+// Largest radix of the next stage: 16 while at least 16 points remain, then
+// what is left (8, 4 or 2).
+constexpr int stage_radix(int rem) { return rem >= 16 ? 16 : rem; }
+
+// FFTs of KB columns of length L held in LDS as buf[l * PITCH + c]
+// (c < KB), in place; L, KB and PITCH are compile-time, so every index is
+// shifts and masks and the stage loop unrolls.  Stockham stage with span Ns
+// and radix R: butterfly j of a column reads elements j + r L/R, twiddles
+// them by W_L^{r k L/(Ns R)} (k = j mod Ns), and writes element
+// (j / Ns) Ns R + k + r Ns.  Every thread keeps its butterflies in registers
+// between the read and the write half of a stage.  L * KB <= 4096, so a
+// thread holds at most 16 points.  tw is the length-L table in LDS.
+template <int L, int KB, int PITCH, bool INV, int NS>
+__device__ __forceinline__ void fft_stage(float2 *buf, const float2 *tw) {
+  constexpr int R = stage_radix(L / NS);
+  constexpr int nb = L / R, step = L / (NS * R), total = nb * KB;
+  constexpr int per = (total + kFftThreads - 1) / kFftThreads;
+  float2 x[per][R];
+  int dst[per], cc[per];
+#pragma unroll
+  for (int it = 0; it < per; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads == 0 || f < total) {
+      const int c = f & (KB - 1), j = f / KB, k = j & (NS - 1);
+      float2 a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = buf[(j + r * nb) * PITCH + c];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[(r * k * step) & (L - 1)];
+          if (INV) w = conjf2(w);
+          a[r] = cmul(a[r], w);
+        }
+      }
+      dftR<R, INV>(a);
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[it][r] = a[r];
+      dst[it] = (j / NS) * NS * R + k;
+      cc[it] = c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < per; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads == 0 || f < total) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[(dst[it] + r * NS) * PITCH + cc[it]] = x[it][r];
+    }
+  }
+  __syncthreads();
+  constexpr int next = NS * R;
+  if constexpr (next < L) fft_stage<L, KB, PITCH, INV, next>(buf, tw);
+}
+
+template <int L, int KB, int PITCH, bool INV>
+__device__ __forceinline__ void lds_fft(float2 *buf, const float2 *tw) {
+  if constexpr (L > 1) fft_stage<L, KB, PITCH, INV, 1>(buf, tw);
+}
+
+// The length-L twiddle table into LDS; the barrier after the data load
+// orders it before the first stage that reads it.
+template <int L>
+__device__ __forceinline__ void load_twiddles(float2 *tws, const float2 *__restrict__ tw_all) {
+  for (int e = threadIdx.x; e < L; e += kFftThreads) tws[e] = tw_all[(L - 2) + e];
+}
+
+template <int L>
+struct FftCols {
+  static constexpr int KB = (kFftPoints / L) > 64 ? 64 : (kFftPoints / L);
+};
+
+// Column FFTs along one axis of a half spectrum, in place: column c of
+// workgroup (blockIdx.x, outer = blockIdx.y, batch item blockIdx.z) is kz =
+// blockIdx.x * KB + c, element l at item + outer * ostride + l * lstride + kz.
+template <int L, bool INV>
+__global__ void __launch_bounds__(kFftThreads)
+fft_cols_kernel(float2 *__restrict__ data, int nzh, size_t lstride, size_t ostride, size_t bstride,
+                const float2 *__restrict__ tw_all, uint32_t nkzb, uint32_t nouter, uint32_t nblk, uint32_t per) {
+  constexpr int KB = FftCols<L>::KB, total = L * KB;
+  __shared__ float2 buf[total];
+  __shared__ float2 tw[L];
+  // workgroup b -> block (b mod 8) per + b / 8: the dispatcher deals
+  // workgroups to the 8 XCDs in turn, so each XCD walks one contiguous run of
+  // blocks and the kz blocks of a row, which share cache lines at their
+  // edges, run on one XCD
+  const uint32_t item = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (item >= nblk) return;
+  const uint32_t kzb = item % nkzb, rest = item / nkzb, outer = rest % nouter, bi = rest / nouter;
+  load_twiddles<L>(tw, tw_all);
+  float2 *base = data + size_t(bi) * bstride + size_t(outer) * ostride;
+  const int kz0 = int(kzb) * KB;
+  constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int c = f & (KB - 1), l = f / KB;
+    buf[f] = kz0 + c < nzh ? base[size_t(l) * lstride + kz0 + c] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  lds_fft<L, KB, KB, INV>(buf, tw);
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int c = f & (KB - 1), l = f / KB;
+    if (kz0 + c < nzh) base[size_t(l) * lstride + kz0 + c] = buf[f];
+  }
+}
+
+// z pass of the R2C: rows r = blockIdx.x * KB + c (< nrows) of item
+// blockIdx.y, each nz = 2M reals read as M complex points.
+template <int M>
+__global__ void __launch_bounds__(kFftThreads)
+fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int nrows, size_t in_bstride,
+                    size_t out_bstride, const float2 *__restrict__ tw_all) {
+  constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB;
+  __shared__ float2 buf[M * PITCH];
+  __shared__ float2 twM[M];
+  load_twiddles<M>(twM, tw_all);
+  const float2 *twN = tw_all + (2 * M - 2);
+  const int r0 = blockIdx.x * KB;
+  const float2 *src = in + size_t(blockIdx.y) * in_bstride;
+  float2 *dst = out + size_t(blockIdx.y) * out_bstride;
+  // rows are contiguous: lanes along l
+  constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int l = f & (M - 1), c = f / M;
+    buf[l * PITCH + c] = r0 + c < nrows ? src[size_t(r0 + c) * M + l] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  lds_fft<M, KB, PITCH, false>(buf, twM);
+  // X[k] = E[k] + W_{2M}^k O[k], E = (Z[k] + conj Z[M-k]) / 2,
+  // O = (Z[k] - conj Z[M-k]) / 2i, k = 0 .. M
+  constexpr int nzh = M + 1, tot2 = nzh * KB;
+  for (int f = threadIdx.x; f < tot2; f += kFftThreads) {
+    const int c = f / nzh, k = f - c * nzh;
+    if (r0 + c >= nrows) continue;
+    const float2 z = buf[(k & (M - 1)) * PITCH + c];
+    const float2 zm = buf[((M - k) & (M - 1)) * PITCH + c];
+    const float2 E = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+    const float2 O = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+    dst[size_t(r0 + c) * nzh + k] = cadd(E, cmul(twN[k], O));
+  }
+}
+
+// z pass of the C2R: half-spectrum rows (M + 1 values) -> rows of 2M reals.
+template <int M>
+__global__ void __launch_bounds__(kFftThreads)
+fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int nrows, size_t in_bstride,
+                    size_t out_bstride, const float2 *__restrict__ tw_all) {
+  constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB, nzh = M + 1;
+  __shared__ float2 buf[M * PITCH];
+  __shared__ float2 twM[M];
+  load_twiddles<M>(twM, tw_all);
+  const float2 *twN = tw_all + (2 * M - 2);
+  const int r0 = blockIdx.x * KB;
+  const float2 *src = in + size_t(blockIdx.y) * in_bstride;
+  float2 *dst = out + size_t(blockIdx.y) * out_bstride;
+  // Z'[k] = E' + i O', E' = X[k] + conj X[M-k], O' = (X[k] - conj X[M-k]) W_{2M}^{-k};
+  // X[0] and X[M] enter with their real parts only
+  constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int k = f & (M - 1), c = f / M;
+    float2 z = make_float2(0.0f, 0.0f);
+    if (r0 + c < nrows) {
+      const float2 *row = src + size_t(r0 + c) * nzh;
+      float2 a = row[k], b = row[M - k];
+      if (k == 0) {
+        a.y = 0.0f;
+        b.y = 0.0f;
+      }
+      const float2 bc = conjf2(b);
+      const float2 E = cadd(a, bc);
+      const float2 O = cmul(csub(a, bc), conjf2(twN[k]));
+      z = make_float2(E.x - O.y, E.y + O.x);
+    }
+    buf[k * PITCH + c] = z;
+  }
+  __syncthreads();
+  lds_fft<M, KB, PITCH, true>(buf, twM);
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int l = f & (M - 1), c = f / M;
+    if (r0 + c < nrows) dst[size_t(r0 + c) * M + l] = buf[l * PITCH + c];
+  }
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// The native transforms cover power-of-two meshes with 2 <= nx, ny <= 1024
+// and 4 <= nz <= 2048.
+bool native_fft_ok(int nx, int ny, int nz) {
+  return pow2(nx) && pow2(ny) && pow2(nz) && nx >= 2 && ny >= 2 && nz >= 4 && nx <= kFftMaxL && ny <= kFftMaxL &&
+         nz <= 2 * kFftMaxL;
+}
+
+// Launch helpers over the compile-time lengths 2 .. 1024.
+#define JPM_FFT_LENGTHS(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+
+static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int outer, size_t lstride,
+                     size_t ostride, size_t bstride, int batch, const float2 *tw) {
+  switch (L) {
+#define JPM_COLS(N)                                                                                    \
+  case N: {                                                                                            \
+    const uint32_t nkzb = (nzh + FftCols<N>::KB - 1) / FftCols<N>::KB;                                \
+    const uint32_t nblk = nkzb * uint32_t(outer) * uint32_t(batch), per = (nblk + 7) / 8;              \
+    auto kern = inv ? fft_cols_kernel<N, true> : fft_cols_kernel<N, false>;                             \
+    hipLaunchKernelGGL(kern, dim3(8 * per), dim3(kFftThreads), 0, s, data, nzh, lstride, ostride,      \
+                       bstride, tw, nkzb, uint32_t(outer), nblk, per);                                 \
+    break;                                                                                             \
+  }
+    JPM_FFT_LENGTHS(JPM_COLS)
+#undef JPM_COLS
+    default:
+      return fail(JPM_E_INVALID, "native FFT length %d", L);
+  }
+  JPM_LAUNCHED("fft_cols_kernel");
+  return JPM_OK;
+}
+
+int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
+  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
+  const float2 *tw;
+  int rc = twiddles(&tw);
+  if (rc) return rc;
+  const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
+  const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
+  float2 *o = static_cast<float2 *>(out);
+  const float2 *i2 = reinterpret_cast<const float2 *>(in);
+  switch (M) {
+#define JPM_R2C(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL(fft_r2c_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
+                       dim3(kFftThreads), 0, s, i2, o, nrows, Mc, H, tw);                                     \
+    break;
+    JPM_FFT_LENGTHS(JPM_R2C)
+#undef JPM_R2C
+    default:
+      return fail(JPM_E_INVALID, "native FFT length %d", nz);
+  }
+  JPM_LAUNCHED("fft_r2c_rows_kernel");
+  if ((rc = cols_pass(s, false, o, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
+  return cols_pass(s, false, o, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw);
+}
+
+int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out) {
+  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
+  const float2 *tw;
+  int rc = twiddles(&tw);
+  if (rc) return rc;
+  const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
+  const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
+  float2 *h = static_cast<float2 *>(in);
+  if ((rc = cols_pass(s, true, h, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw))) return rc;
+  if ((rc = cols_pass(s, true, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
+  float2 *o2 = reinterpret_cast<float2 *>(out);
+  switch (M) {
+#define JPM_C2R(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL(fft_c2r_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
+                       dim3(kFftThreads), 0, s, h, o2, nrows, H, Mc, tw);                                     \
+    break;
+    JPM_FFT_LENGTHS(JPM_C2R)
+#undef JPM_C2R
+    default:
+      return fail(JPM_E_INVALID, "native FFT length %d", nz);
+  }
+  JPM_LAUNCHED("fft_c2r_rows_kernel");
+  return JPM_OK;
+}
+
+}  // namespace jpm
