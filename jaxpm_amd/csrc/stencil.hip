@@ -63,7 +63,8 @@ fd4_gradient_kernel(const float *__restrict__ phi, Mesh3 g, float scale, float *
 // Adjoint of fd4_gradient_kernel: out = scale * sum_d (D_d g_d), since
 // (-D_d)^T = D_d.
 __global__ void __launch_bounds__(256)
-fd4_divergence_kernel(const float *__restrict__ g3, size_t chan_stride, Mesh3 g, float scale, float *__restrict__ out) {
+fd4_divergence_kernel(const float *__restrict__ g3, size_t chan_stride, Mesh3 g, float scale, float *__restrict__ out,
+                      uint32_t per) {
   // blockIdx.y: mesh of a batch of equal shapes (stride M); the three input
   // channels lie chan_stride floats apart
   const uint32_t M = g.nx * g.ny * g.nz;
@@ -71,7 +72,13 @@ fd4_divergence_kernel(const float *__restrict__ g3, size_t chan_stride, Mesh3 g,
   out += size_t(blockIdx.y) * M;
   const float c8 = scale * (8.0f / 12.0f), c1 = scale * (1.0f / 12.0f);
   const float *gx = g3, *gy = g3 + chan_stride, *gz = g3 + 2 * chan_stride;
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < M; e += gridDim.x * blockDim.x) {
+  // workgroup b takes the 256 cells of block (b mod 8) per + b / 8: the
+  // dispatcher deals workgroups to the 8 XCDs in turn, so each XCD sweeps one
+  // contiguous slab of x planes and the reads at x +- 1, 2 hit its L2
+  const uint32_t blk = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  {
+    const uint32_t e = blk * blockDim.x + threadIdx.x;
+    if (e >= M) return;
     const uint32_t r = g.div_nz.div(e);
     const uint32_t k = e - r * g.nz;
     const uint32_t i = g.div_ny.div(r);
@@ -124,8 +131,9 @@ int fd4_divergence_batch(hipStream_t s, int nx, int ny, int nz, int batch, const
   JPM_REQUIRE(3 * m <= (int64_t(1) << 32), "mesh too large");
   {
     JPM_TIMED(s, "fd4_divergence_kernel");
-    hipLaunchKernelGGL(fd4_divergence_kernel, dim3(grid_for(m, 256, 256 * 128 / batch + 1), batch), dim3(256), 0, s,
-                       g3, chan_stride, make_mesh3(nx, ny, nz), scale, out);
+    const uint32_t blocks = uint32_t((m + 255) / 256), per = (blocks + 7) / 8;
+    hipLaunchKernelGGL(fd4_divergence_kernel, dim3(8 * per, batch), dim3(256), 0, s, g3, chan_stride,
+                       make_mesh3(nx, ny, nz), scale, out, per);
   }
   JPM_LAUNCHED("fd4_divergence_kernel");
   return JPM_OK;
