@@ -161,6 +161,69 @@ __device__ __forceinline__ void stencil8_lds(const float *win, int a, int b, int
   }
 }
 
+// u . (-D phi) at the 8 CIC corners, f[o] = (u0 F[0][o] + u1 F[1][o]) + u2
+// F[2][o] with F as stencil8_lds computes it: the same float operations in
+// the same order, accumulated axis by axis so that only 8 sums stay live.
+__device__ __forceinline__ void stencil8_dot_lds(const float *win, int a, int b, int c, float u0, float u1, float u2,
+                                                 float f[8]) {
+  const int sx = kPw * kPw, sy = kPw;
+  const int e0 = (a * kPw + b) * kPw + c;
+  float C[8];
+#pragma unroll
+  for (int o = 0; o < 8; ++o) C[o] = win[e0 + (o & 1) * sx + ((o >> 1) & 1) * sy + ((o >> 2) & 1)];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = q & 1, k = q >> 1;
+    const int ex = e0 + j * sy + k;
+    const float m2 = win[ex - 2 * sx], m1 = win[ex - sx], p2 = win[ex + 2 * sx], p3 = win[ex + 3 * sx];
+    const float c0 = C[(j << 1) | (k << 2)], c1 = C[1 | (j << 1) | (k << 2)];
+    f[(j << 1) | (k << 2)] = u0 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    f[1 | (j << 1) | (k << 2)] = u0 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = q & 1, k = q >> 1;
+    const int ey = e0 + i * sx + k;
+    const float m2 = win[ey - 2 * sy], m1 = win[ey - sy], p2 = win[ey + 2 * sy], p3 = win[ey + 3 * sy];
+    const float c0 = C[i | (k << 2)], c1 = C[i | 2 | (k << 2)];
+    f[i | (k << 2)] = f[i | (k << 2)] + u1 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    f[i | 2 | (k << 2)] = f[i | 2 | (k << 2)] + u1 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = q & 1, j = q >> 1;
+    const int ez = e0 + i * sx + j * sy;
+    const float m2 = win[ez - 2], m1 = win[ez - 1], p2 = win[ez + 2], p3 = win[ez + 3];
+    const float c0 = C[i | (j << 1)], c1 = C[i | (j << 1) | 4];
+    f[i | (j << 1)] = f[i | (j << 1)] + u2 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
+    f[i | (j << 1) | 4] = f[i | (j << 1) | 4] + u2 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  }
+}
+
+// The 8 corner values of a mesh around an absolute position (cells floor(x)
+// and floor(x) + 1 per axis, periodic), corner o = bx | by << 1 | bz << 2,
+// read from global memory.  A floor cell already inside the mesh skips the
+// periodic wrap's division.
+__device__ __forceinline__ void corners_global_abs(const float *__restrict__ mesh, const TileGrid &t, float4 r,
+                                                   float out[8]) {
+  const float p[3] = {r.x, r.y, r.z};
+  const int n[3] = {t.nx, t.ny, t.nz};
+  uint32_t lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int fi = int(floorf(p[a]));
+    const int c = uint32_t(fi) < uint32_t(n[a]) ? fi : wrapi(fi, n[a]);
+    lo[a] = uint32_t(c);
+    hi[a] = c + 1 == n[a] ? 0u : uint32_t(c + 1);
+  }
+  const uint32_t ny = uint32_t(t.ny), nz = uint32_t(t.nz);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const uint32_t x = (q & 1) ? hi[0] : lo[0], y = (q & 2) ? hi[1] : lo[1], z = (q & 4) ? hi[2] : lo[2];
+    out[q] = mesh[(x * ny + y) * nz + z];
+  }
+}
+
 // The particle's 8 corners form one 2^3 block inside the window (always for
 // absolute positions; relative ones can wrap or round outside).
 __device__ __forceinline__ bool corners_in_window(const CornerSet &cs) {
@@ -329,16 +392,21 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   // Software pipeline over this thread's particles q, q + B, q + 2B, ...:
   // particle q computes while the cotangent and gpos of q + B (whose record
   // arrived an iteration ago) and the record of q + 2B load, so no load waits
-  // on another within an iteration.
+  // on another within an iteration.  With absolute positions and the density
+  // adjoint read through the caches (PF), the 8 density-adjoint corners of
+  // q + B load in the same iteration, a particle ahead as well.
+  constexpr bool PF = !REL && !RLDS;
   const uint32_t cnt = d.count, B = blockDim.x;
   const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float4 rec1 = threadIdx.x < cnt ? load_global(bins.spos + (d.start + threadIdx.x)) : zero4;
   float4 rec2 = threadIdx.x + B < cnt ? load_global(bins.spos + (d.start + threadIdx.x + B)) : zero4;
   float nu[3] = {0.0f, 0.0f, 0.0f}, ng[3] = {0.0f, 0.0f, 0.0f};
+  float nr[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (threadIdx.x < cnt) {
     const size_t i3 = 3 * size_t(__float_as_uint(rec1.w));
     nu[0] = cot[i3]; nu[1] = cot[i3 + 1]; nu[2] = cot[i3 + 2];
     ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
+    if (PF) corners_global_abs(rho_bar, t, rec1, nr);
   }
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   if (RLDS) load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
@@ -349,25 +417,31 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     const uint32_t idx = __float_as_uint(rec.w);
     const float u0 = nu[0], u1 = nu[1], u2 = nu[2];
     const float g0 = ng[0], g1 = ng[1], g2 = ng[2];
+    float r8[8];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) r8[o] = nr[o];
     rec1 = rec2;
     if (q + B < cnt) {
       const size_t i3 = 3 * size_t(__float_as_uint(rec1.w));
       nu[0] = cot[i3]; nu[1] = cot[i3 + 1]; nu[2] = cot[i3 + 2];
       ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
+      if (PF) corners_global_abs(rho_bar, t, rec1, nr);
     }
     if (q + 2 * B < cnt) rec2 = load_global(bins.spos + (d.start + q + 2 * B));
     CornerSet cs;
     corner_set<REL>(rec, t, part0, ox, oy, oz, cs);
     float gd[3] = {0.0f, 0.0f, 0.0f}, gr[3] = {0.0f, 0.0f, 0.0f};
     if (!REL || corners_in_window(cs)) {
-      float F8[3][8];
+      float f8[8];
       const int la0 = cs.l[0][0], lb0 = cs.l[1][0], lc0 = cs.l[2][0];
-      stencil8_lds(win, la0 + kHalo, lb0 + kHalo, lc0 + kHalo, F8);
+      stencil8_dot_lds(win, la0 + kHalo, lb0 + kHalo, lc0 + kHalo, u0, u1, u2, f8);
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
         float rv;
-        if (RLDS) {
+        if (PF) {
+          rv = r8[o];
+        } else if (RLDS) {
           rv = rwin[((la0 + bx) * kWin + lb0 + by) * kWin + lc0 + bz];
         } else if (near) {
           const int x = wrap_near(ox + la0 + bx, t.nx), y = wrap_near(oy + lb0 + by, t.ny),
@@ -377,7 +451,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
           const int x = wrapn(ox + la0 + bx, t.nx), y = wrapn(oy + lb0 + by, t.ny), z = wrapn(oz + lc0 + bz, t.nz);
           rv = rho_bar[(uint32_t(x) * uint32_t(t.ny) + uint32_t(y)) * uint32_t(t.nz) + uint32_t(z)];
         }
-        const float f = u0 * F8[0][o] + u1 * F8[1][o] + u2 * F8[2][o];
+        const float f = f8[o];
         const float dwx = cs.d[0][bx] * cs.w[1][by] * cs.w[2][bz];
         const float dwy = cs.w[0][bx] * cs.d[1][by] * cs.w[2][bz];
         const float dwz = cs.w[0][bx] * cs.w[1][by] * cs.d[2][bz];
