@@ -436,6 +436,28 @@ int jpm_dot_f32(void *stream, int64_t n, const float *x, const float *y,
  * of the leapfrog built on make_ode_fn (ode.py:139-140) and its adjoint. */
 int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y);
 
+/* Workspace bytes of jpm_tile_order_u32. */
+size_t jpm_tile_order_workspace_size(int nx, int ny, int nz, int64_t n);
+
+/* order[k] = index of the particle at slot k of the tile sort of absolute
+ * positions pos[n][3]: particles grouped by 16^3-cell tile.  No counterpart
+ * in the reference: the leapfrog driver (integrate.NBody, standing in for
+ * the odeint/diffrax drive of make_ode_fn, ode.py:122-147, and its jax.grad)
+ * runs its whole time loop on particles in this order, so that each tile's
+ * particles are adjacent in memory, and returns results in the caller's
+ * order.  The order within a tile may vary from call to call. */
+int jpm_tile_order_u32(void *stream, int nx, int ny, int nz, const float *pos,
+                       int64_t n, uint32_t *order, void *ws, size_t ws_bytes);
+
+/* Rows of 3 floats reordered, out of place: out[k] = in[order[k]]
+ * (scatter = 0) or out[order[k]] = in[k] (scatter = 1, the inverse), for
+ * in0 -> out0 and, when in1 is not NULL, in1 -> out1 in the same launch.
+ * Rows whose order entry is >= n are skipped.  No counterpart in the
+ * reference (see jpm_tile_order_u32). */
+int jpm_permute_rows_f32(void *stream, int64_t n, const uint32_t *order,
+                         const float *in0, float *out0, const float *in1,
+                         float *out1, int scatter);
+
 /* vel[n][3] += gamma * read(phi3, pos): the kick of ode.py:142-143 fused
  * into the 3-component CIC read. */
 int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3,
@@ -535,6 +557,12 @@ int jpm_batch_lattice_forces_f32(void *stream, const void *batch, const float *f
 int jpm_batch_lattice_forces_vjp_f32(void *stream, const void *batch,
                                      const float *gpsi, float *gfield, void *ws,
                                      size_t ws_bytes);
+
+/* jpm_tile_order_u32 over every simulation of an absolute-position batch
+ * (workspace of jpm_batch_sort_size bytes); each simulation's slots stay
+ * within its own particle range. */
+int jpm_batch_tile_order_u32(void *stream, const void *batch, const float *pos,
+                             uint32_t *order, void *ws, size_t ws_bytes);
 
 /* cic_paint of every simulation's particles into zeroed meshes [M]
  * (workspace of jpm_batch_force_kick_workspace_size), and the position

@@ -91,6 +91,7 @@ SIGNATURES = {
     "jpm_batch_force_vjp_drift_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp,
                                            _sz]),
     "jpm_batch_axpy_f32": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "jpm_batch_tile_order_u32": (_i, [_vp, _vp, _vp, _vp, _vp, _sz]),
     "jpm_batch_dot_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz]),
     "jpm_batch_lpt_workspace_size": (_sz, [_vp]),
     "jpm_batch_linear_field_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _sz]),
@@ -115,6 +116,9 @@ SIGNATURES = {
     "jpm_force_vjp_pot_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _f, _vp, _f, _u32, _vp, _sz, _vp,
                                          _sz]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
+    "jpm_tile_order_workspace_size": (_sz, [_i, _i, _i, _i64]),
+    "jpm_tile_order_u32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _sz]),
+    "jpm_permute_rows_f32": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _i]),
     "jpm_kick_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _f, _vp, _u32]),
     "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_cic_paint_binned_workspace_size": (_sz, [_i, _i, _i, _i64, _i, _u32]),

@@ -409,6 +409,41 @@ def dot_into(out, i, x, y):
          ws.numel())
 
 
+def tile_order(pos, mesh_shape, ws=None):
+    """int32 tensor (uint32 bits) order[k] = index of the particle at slot k
+    of the tile sort of absolute positions ``pos`` (jpm_tile_order_u32).
+    ``ws``: an optional uint8 workspace of at least
+    jpm_tile_order_workspace_size bytes."""
+    pos = _f32(pos, "positions")
+    nx, ny, nz = _dims(mesh_shape)
+    n = pos.numel() // 3
+    need = size("jpm_tile_order_workspace_size", nx, ny, nz, n)
+    if need == 0:
+        raise ValueError(f"mesh {tuple(mesh_shape)} too large for the tile sort")
+    if ws is None or ws.numel() < need:
+        ws = torch.empty(need, dtype=torch.uint8, device=pos.device)
+    order = torch.empty(n, dtype=torch.int32, device=pos.device)
+    call("jpm_tile_order_u32", stream(), nx, ny, nz, ptr(pos), n, ptr(order), ptr(ws), ws.numel())
+    return order
+
+
+def permute_rows(order, *arrays, scatter=False):
+    """New (n, 3) float32 tensors, one per array: out[k] = x[order[k]], or
+    with ``scatter`` out[order[k]] = x[k] (the inverse); one or two arrays
+    per launch (jpm_permute_rows_f32)."""
+    if order.dtype != torch.int32 or not order.is_contiguous():
+        raise ValueError("order must be a contiguous int32 tensor")
+    xs = [_f32(x, "rows") for x in arrays]
+    if any(3 * order.numel() != x.numel() for x in xs):
+        raise ValueError("order must have one entry per row")
+    outs = [torch.empty_like(x) for x in xs]
+    for i in range(0, len(xs), 2):
+        a, b = xs[i:i + 2], outs[i:i + 2]
+        call("jpm_permute_rows_f32", stream(), order.numel(), ptr(order), ptr(a[0]), ptr(b[0]),
+             ptr(a[1]) if len(a) > 1 else None, ptr(b[1]) if len(b) > 1 else None, 1 if scatter else 0)
+    return outs[0] if len(outs) == 1 else tuple(outs)
+
+
 def axpy(a, x, y):
     """y += a x in place (jpm_axpy_f32); float32 tensors of one size."""
     x = _f32(x, "x")

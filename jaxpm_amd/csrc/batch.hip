@@ -353,6 +353,21 @@ int batch_lattice_forces_vjp(hipStream_t s, const SimBatch &bt, const float *gps
   return JPM_OK;
 }
 
+// The tile order of an absolute-position batch: each simulation's slots stay
+// within its own particle range, since its tiles follow the tiles of the
+// simulations before it.
+int batch_tile_order(hipStream_t s, const SimBatch &bt, const float *pos, uint32_t *order, void *ws,
+                     size_t ws_bytes) {
+  JPM_REQUIRE(!bt.rel, "the tile order takes absolute positions (relative ones are tied to their lattice index)");
+  JPM_REQUIRE(pos && order, "null pointer argument");
+  const size_t need = batch_sort_workspace_size(bt);
+  JPM_REQUIRE(ws && ws_bytes >= need, "tile order needs %zu workspace bytes, got %zu", need, ws_bytes);
+  BatchBins b;
+  int rc = batch_sort(s, bt, pos, ws, ws_bytes, b);
+  if (rc) return rc;
+  return bins_order(s, b, order);
+}
+
 int batch_paint_mesh(hipStream_t s, const SimBatch &bt, const float *pos, float *mesh, void *ws, size_t ws_bytes) {
   JPM_REQUIRE(pos && mesh, "null pointer argument");
   const BatchWs w = batch_ws(bt, 1);

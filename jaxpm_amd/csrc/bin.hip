@@ -796,6 +796,25 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
   return sort_into(s, g, one_dims(t, n, flags), uint32_t(M), rel, pos, ws, ws_bytes, out);
 }
 
+// order[k] = index of the particle the tile sort put at slot k: the index
+// bits of the sorted records.
+__global__ void __launch_bounds__(256)
+record_order_kernel(const float4 *__restrict__ spos, uint32_t n, uint32_t *__restrict__ order) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) order[k] = __float_as_uint(spos[k].w);
+}
+
+template <class G>
+int bins_order(hipStream_t s, const BinsT<G> &b, uint32_t *order) {
+  JPM_REQUIRE(order, "null pointer argument");
+  JPM_TIMED(s, "record_order_kernel");
+  hipLaunchKernelGGL(record_order_kernel, dim3((b.n + 255) / 256), dim3(256), 0, s, b.spos, b.n, order);
+  JPM_LAUNCHED("record_order_kernel");
+  return JPM_OK;
+}
+template int bins_order<OneSim>(hipStream_t, const Bins &, uint32_t *);
+template int bins_order<SimTable>(hipStream_t, const BatchBins &, uint32_t *);
+
 // nc meshes (stride b.M) = [mesh +] window sums of the sorted particles,
 // channel c with weight[p*nc + c] (or wscalar when weight is NULL).
 template <class G>

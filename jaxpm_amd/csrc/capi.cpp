@@ -93,12 +93,14 @@ int batch_linear_field(hipStream_t, const SimBatch &, const float *, float *, co
 int batch_lattice_forces(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
 int batch_lattice_forces_vjp(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
 int batch_paint_mesh(hipStream_t, const SimBatch &, const float *, float *, void *, size_t);
+int batch_tile_order(hipStream_t, const SimBatch &, const float *, uint32_t *, void *, size_t);
 int batch_paint_vjp(hipStream_t, const SimBatch &, const float *, const float *, float *);
 int dot(hipStream_t, int64_t, const float *, const float *, double *, void *, size_t);
 int power_spectrum(hipStream_t, int, int, int, const void *, const void *, double, double, double, const double *,
                    int, const int *, int, double, double, double, double *, void *, size_t);
 
 int axpy(hipStream_t, int64_t, float, const float *, float *);
+int permute_rows(hipStream_t, int64_t, const uint32_t *, const float *, float *, const float *, float *, int);
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
 int fd4_gradient(hipStream_t, int, int, int, const float *, float, float *);
 int fd4_divergence(hipStream_t, int, int, int, const float *, float, float *);
@@ -254,6 +256,22 @@ static int cic_read_sorted(hipStream_t s, int nx, int ny, int nz, const float *m
   rc = bin_sort(s, nx, ny, nz, pos, n, 0, ws, ws_bytes, b);
   if (rc) return rc;
   return bins_read(s, b, mesh, ncomp, uint32_t(ncomp == 1 ? m : comp_stride), out);
+}
+
+static int tile_order(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t *order, void *ws,
+                      size_t ws_bytes) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(binned_paint_supported(nx, ny, nz), "mesh too large for the tile sort");
+  JPM_REQUIRE(n >= 0 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(pos && order, "null pointer argument");
+  const size_t need = bin_sort_workspace_size(nx, ny, nz, n, 0);
+  JPM_REQUIRE(ws && ws_bytes >= need, "tile order needs %zu workspace bytes, got %zu", need, ws_bytes);
+  Bins b;
+  rc = bin_sort(s, nx, ny, nz, pos, n, 0, ws, ws_bytes, b);
+  if (rc) return rc;
+  return bins_order(s, b, order);
 }
 
 static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
@@ -719,6 +737,12 @@ int jpm_batch_lattice_forces_vjp_f32(void *stream, const void *batch, const floa
   JPM_GUARD(batch_lattice_forces_vjp(as_stream(stream), *as_batch(batch), gpsi, gfield, ws, ws_bytes));
 }
 
+int jpm_batch_tile_order_u32(void *stream, const void *batch, const float *pos, uint32_t *order, void *ws,
+                             size_t ws_bytes) {
+  JPM_REQUIRE(batch, "null batch handle");
+  JPM_GUARD(batch_tile_order(as_stream(stream), *as_batch(batch), pos, order, ws, ws_bytes));
+}
+
 int jpm_batch_paint_f32(void *stream, const void *batch, const float *pos, float *mesh, void *ws, size_t ws_bytes) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_paint_mesh(as_stream(stream), *as_batch(batch), pos, mesh, ws, ws_bytes));
@@ -773,6 +797,21 @@ int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3, c
 
 int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y) {
   JPM_GUARD(axpy(as_stream(stream), n, a, x, y));
+}
+
+size_t jpm_tile_order_workspace_size(int nx, int ny, int nz, int64_t n) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !binned_paint_supported(nx, ny, nz)) return 0;
+  return bin_sort_workspace_size(nx, ny, nz, n, 0);
+}
+
+int jpm_tile_order_u32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t *order, void *ws,
+                       size_t ws_bytes) {
+  JPM_GUARD(tile_order(as_stream(stream), nx, ny, nz, pos, n, order, ws, ws_bytes));
+}
+
+int jpm_permute_rows_f32(void *stream, int64_t n, const uint32_t *order, const float *in0, float *out0,
+                         const float *in1, float *out1, int scatter) {
+  JPM_GUARD(permute_rows(as_stream(stream), n, order, in0, out0, in1, out1, scatter));
 }
 
 int jpm_kick_f32(void *stream, int nx, int ny, int nz, const float *phi3, const float *pos, int64_t n, float gamma,

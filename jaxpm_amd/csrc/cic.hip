@@ -688,6 +688,45 @@ kick_kernel(const float *__restrict__ phi, uint32_t cstride, const float *__rest
   }
 }
 
+// Rows of 3 floats in another order, one or two arrays per launch:
+// out[k] = in[order[k]] (gather) or out[order[k]] = in[k] (scatter, the
+// inverse).  Entries of order >= n are skipped, so a malformed order cannot
+// address outside the arrays.
+template <bool SCATTER>
+__global__ void __launch_bounds__(256)
+permute_rows_kernel(const uint32_t *__restrict__ order, const float *__restrict__ in0, float *__restrict__ out0,
+                    const float *__restrict__ in1, float *__restrict__ out1, uint32_t n) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t row = order[k];
+  if (row >= n) return;
+  const size_t src = 3 * size_t(SCATTER ? k : row), dst = 3 * size_t(SCATTER ? row : k);
+  const float a0 = in0[src], a1 = in0[src + 1], a2 = in0[src + 2];
+  if (in1) {
+    const float b0 = in1[src], b1 = in1[src + 1], b2 = in1[src + 2];
+    out1[dst] = b0;
+    out1[dst + 1] = b1;
+    out1[dst + 2] = b2;
+  }
+  out0[dst] = a0;
+  out0[dst + 1] = a1;
+  out0[dst + 2] = a2;
+}
+
+int permute_rows(hipStream_t s, int64_t n, const uint32_t *order, const float *in0, float *out0, const float *in1,
+                 float *out1, int scatter) {
+  JPM_REQUIRE(n >= 0 && n <= int64_t(0xffffffffu) - 255, "row count %lld out of range", (long long)n);
+  if (n == 0) return JPM_OK;
+  JPM_REQUIRE(order && in0 && out0, "null pointer argument");
+  JPM_REQUIRE(!in1 == !out1, "the second array needs both its input and its output");
+  JPM_REQUIRE(in0 != out0 && (!in1 || in1 != out1), "the permutation is out of place");
+  JPM_TIMED(s, "permute_rows_kernel");
+  hipLaunchKernelGGL(scatter ? permute_rows_kernel<true> : permute_rows_kernel<false>,
+                     dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, order, in0, out0, in1, out1, uint32_t(n));
+  JPM_LAUNCHED("permute_rows_kernel");
+  return JPM_OK;
+}
+
 int axpy(hipStream_t s, int64_t n, float a, const float *x, float *y) {
   JPM_REQUIRE(n >= 0 && n <= (int64_t(1) << 32) - 1, "length %lld out of range", (long long)n);
   if (n == 0) return JPM_OK;

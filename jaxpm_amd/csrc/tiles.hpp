@@ -196,6 +196,8 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
              size_t ws_bytes, Bins &out);
 size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags);
 Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws);
+template <class G>
+int bins_order(hipStream_t s, const BinsT<G> &b, uint32_t *order);
 size_t batch_sort_workspace_size(const SimBatch &bt);
 size_t batch_paint_workspace_size(const SimBatch &bt, int nc);
 int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out);

@@ -49,7 +49,7 @@ class Stepper:
     """Device buffers and C-ABI calls of one mesh / particle configuration."""
 
     def __init__(self, mesh_shape, n, relative=False, deterministic=False, r_split=0.0, kflags=0, atomic=False,
-                 sort_tuning=None):
+                 sort_tuning=None, reorder=True):
         self.shape = tuple(int(s) for s in mesh_shape)
         self.nx, self.ny, self.nz = self.shape
         self.n = int(n)
@@ -80,6 +80,20 @@ class Stepper:
             wb = max(size("jpm_force_field_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags),
                      size("jpm_force_vjp_workspace_size", self.nx, self.ny, self.nz, self.n, self.flags))
         self.ws = torch.empty(wb, dtype=torch.uint8, device=self.dev)
+        # NBody runs its time loop on the particles in the tile order of the
+        # initial positions (absolute positions on the sorted pipeline): each
+        # tile's particles then sit together in memory, so the gathers' reads
+        # and writes by particle index fill whole cache lines.  Results are
+        # the same bits either way (the paint sums in fixed point, the
+        # gathers are per particle).
+        self.reorder = bool(reorder) and self.sorted and not relative
+
+    def tile_order(self, x):
+        """order[k] = index of the particle at slot k of the tile sort of x."""
+        return _ops.tile_order(x, self.shape, self.ws)
+
+    def permute(self, order, *arrays, scatter=False):
+        return _ops.permute_rows(order, *arrays, scatter=scatter)
 
     def new_phi(self):
         return torch.empty((3, *self.shape), dtype=torch.float32, device=self.dev)
@@ -285,8 +299,14 @@ class NBody(torch.autograd.Function):
         S = len(beta)
         grad = any(ctx.needs_input_grad[:3])
         want_c = ctx.needs_input_grad[2]
-        x = x0.detach().clone().contiguous()
-        v = v0.detach().clone().contiguous()
+        # the fused sweeps run on the particles in tile order (Stepper.reorder)
+        order = st.tile_order(x0.detach()) if st.reorder and not want_c else None
+        if order is not None:
+            x, v = st.permute(order, x0.detach(), v0.detach())
+        else:
+            x = x0.detach().clone().contiguous()
+            v = v0.detach().clone().contiguous()
+        ctx.order = order
         phi0 = st.new_field(keep_sort=grad)
         f0 = None
         k = S if checkpoint_every is None else max(1, int(checkpoint_every))
@@ -309,6 +329,8 @@ class NBody(torch.autograd.Function):
                     ckpts[s1] = (x.clone(), v.clone())
             ctx.ckpts = ckpts
             ctx.kept = (keep, phi0, x0c, last)
+            if order is not None:
+                return st.permute(order, x, v, scatter=True)
             return x, v
         # with coefficient cotangents: separate forces and axpy, keeping the
         # forces and drift velocities the cotangents are dot products with
@@ -375,8 +397,13 @@ class NBody(torch.autograd.Function):
         S = len(beta)
         keep, phi0, x0, last = ctx.kept
         like = ctx.ckpts[0][0]
-        lx = (gx if gx is not None else torch.zeros_like(like)).detach().clone().contiguous()
-        lv = (gv if gv is not None else torch.zeros_like(like)).detach().clone().contiguous()
+        order = ctx.order
+        gx = (gx if gx is not None else torch.zeros_like(like)).detach()
+        gv = (gv if gv is not None else torch.zeros_like(like)).detach()
+        if order is not None:
+            lx, lv = st.permute(order, gx, gv)
+        else:
+            lx, lv = gx.clone().contiguous(), gv.clone().contiguous()
         phi = st.new_field()
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
@@ -388,7 +415,9 @@ class NBody(torch.autograd.Function):
                 kp = keep[i - s0]
                 st.vjp_field_drift(kp[1], kp[0], lv, gamma[i], beta[i], lx)
         st.vjp_field(phi0, x0, lv, k0, lx)
-        ctx.ckpts = ctx.kept = None
+        ctx.ckpts = ctx.kept = ctx.order = None
+        if order is not None:
+            return (*st.permute(order, lx, lv, scatter=True), None, None, None)
         return lx, lv, None, None, None
 
 

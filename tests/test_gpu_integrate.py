@@ -181,3 +181,59 @@ def test_sort_tuning_changes_speed_not_results(gpu, tmp_path, monkeypatch):
     assert best in timings and len(timings) == len(autotune._candidates())
     again, t2 = autotune.tune_stepper(shape, xt)
     assert again == best and t2 == {}
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (17, 33, 20)])
+def test_tile_order_and_permutation(gpu, shape):
+    """jpm_tile_order_u32 is a permutation that groups the particles by tile
+    (tiles in increasing key order, keys from floor cells computed on the
+    host), and
+    jpm_permute_rows_f32's gather and scatter are inverse to each other."""
+    from jaxpm_amd import _ops
+    x0, _ = _state(shape, 40)
+    x0[::7] += np.float32(shape[0])          # some positions a period away
+    x0[::11] -= np.float32(shape[1])
+    xt = torch.as_tensor(x0, device="cuda")
+    order = _ops.tile_order(xt, shape).cpu().numpy().view(np.uint32)
+    n = x0.shape[0]
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    cells = np.floor(x0.astype(np.float64)).astype(np.int64) % np.array(shape)
+    t = [(s + 15) // 16 for s in shape]
+    key = ((cells[:, 0] // 16) * t[1] + cells[:, 1] // 16) * t[2] + cells[:, 2] // 16
+    assert np.all(np.diff(key[order]) >= 0)
+    ot = torch.as_tensor(order.view(np.int32), device="cuda")
+    g = _ops.permute_rows(ot, xt)
+    assert np.array_equal(g.cpu().numpy(), x0[order])
+    assert torch.equal(_ops.permute_rows(ot, g, scatter=True), xt)
+    y = -2.0 * xt
+    g2, h2 = _ops.permute_rows(ot, xt, y)
+    assert torch.equal(g2, g)
+    assert np.array_equal(h2.cpu().numpy(), y.cpu().numpy()[order])
+    a, b = _ops.permute_rows(ot, g2, h2, scatter=True)
+    assert torch.equal(a, xt) and torch.equal(b, y)
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (17, 33, 20)])
+def test_tile_ordered_sweeps_give_the_same_bits(gpu, shape):
+    """NBody on particles in tile order (the default for absolute positions)
+    against the caller's order: the same final state and gradients, bit for
+    bit, since the paint sums in fixed point and the gathers are per
+    particle."""
+    from jaxpm_amd import cosmology as pc
+    from jaxpm_amd import integrate
+    x0, v0 = _state(shape, 41)
+    r = np.random.default_rng(42)
+    gx = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    gv = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    sched = integrate.schedule_tensor(integrate.kdk_schedule(pc.Cosmology(), np.linspace(0.1, 1.0, 6)))
+    out = []
+    for reorder in (True, False):
+        st = integrate.Stepper(shape, x0.shape[0], reorder=reorder)
+        assert st.reorder == reorder
+        xt = torch.as_tensor(x0, device="cuda").requires_grad_(True)
+        vt = torch.as_tensor(v0, device="cuda").requires_grad_(True)
+        x, v = integrate.NBody.apply(xt, vt, sched, st, 2)
+        ((x * gx).sum() + (v * gv).sum()).backward()
+        out.append([t.detach().cpu().numpy() for t in (x, v, xt.grad, vt.grad)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
