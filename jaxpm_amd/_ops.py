@@ -427,16 +427,23 @@ def tile_order(pos, mesh_shape, ws=None):
     return order
 
 
-def permute_rows(order, *arrays, scatter=False):
-    """New (n, 3) float32 tensors, one per array: out[k] = x[order[k]], or
-    with ``scatter`` out[order[k]] = x[k] (the inverse); one or two arrays
-    per launch (jpm_permute_rows_f32)."""
+def permute_rows(order, *arrays, scatter=False, out=None):
+    """(n, 3) float32 tensors, one per array: out[k] = x[order[k]], or with
+    ``scatter`` out[order[k]] = x[k] (the inverse); one or two arrays per
+    launch (jpm_permute_rows_f32).  ``out``: the output tensors (new ones
+    when None)."""
     if order.dtype != torch.int32 or not order.is_contiguous():
         raise ValueError("order must be a contiguous int32 tensor")
     xs = [_f32(x, "rows") for x in arrays]
     if any(3 * order.numel() != x.numel() for x in xs):
         raise ValueError("order must have one entry per row")
-    outs = [torch.empty_like(x) for x in xs]
+    if out is None:
+        outs = [torch.empty_like(x) for x in xs]
+    else:
+        outs = list(out)
+        if len(outs) != len(xs) or any(o.dtype != torch.float32 or not o.is_contiguous() or o.numel() != x.numel()
+                                       for o, x in zip(outs, xs)):
+            raise ValueError("one contiguous float32 output of the input's size per array")
     for i in range(0, len(xs), 2):
         a, b = xs[i:i + 2], outs[i:i + 2]
         call("jpm_permute_rows_f32", stream(), order.numel(), ptr(order), ptr(a[0]), ptr(b[0]),

@@ -466,7 +466,7 @@ class BatchStepper:
     must be contiguous; arrays are the concatenation of the simulations in
     order: particles [N, 3], meshes [M]."""
 
-    def __init__(self, mesh_shapes, counts, relative=False, r_split=0.0, kflags=0, sort_tuning=None):
+    def __init__(self, mesh_shapes, counts, relative=False, r_split=0.0, kflags=0, sort_tuning=None, reorder=True):
         import ctypes
         self.shapes = [tuple(int(s) for s in sh) for sh in mesh_shapes]
         self.counts = [int(c) for c in counts]
@@ -496,6 +496,17 @@ class BatchStepper:
         self.ws = torch.empty(wb, dtype=torch.uint8, device=self.dev)
         self.dot_ws = torch.empty(size("jpm_batch_dot_workspace_size", h), dtype=torch.uint8, device=self.dev)
         self.trace = None
+        # BatchLeapfrog runs its sweeps on the particles in tile order, as
+        # NBody does (Stepper.reorder); absolute positions only
+        self.reorder = bool(reorder) and not self.relative
+
+    def tile_order(self, x, order):
+        """order[k] = index of the particle at slot k of the batch's tile
+        sort of x; each simulation's slots stay within its own range."""
+        ws = self.ws if self.ws.numel() >= self.sort_bytes else \
+            torch.empty(self.sort_bytes, dtype=torch.uint8, device=self.dev)
+        call("jpm_batch_tile_order_u32", stream(), self.handle, ptr(x), ptr(order), ptr(ws), ws.numel())
+        return order
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -670,6 +681,11 @@ class BatchLeapfrog:
         self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c)
         self.gx, self.gv, self.lx, self.lv, self.xs, self.vs, self.f0 = e(), e(), e(), e(), e(), e(), e()
         self.dots = torch.zeros((1 + 2 * S, st.B), dtype=torch.float64, device=st.dev)
+        # the fused sweeps (no coefficient cotangents) run on the particles in
+        # tile order; forward / backward permute in and out around them
+        self.order = torch.empty(st.N, dtype=torch.int32, device=st.dev) if st.reorder and not self.want_c else None
+        if self.order is not None:
+            self.xo, self.vo, self.lxo, self.lvo = e(), e(), e(), e()
         self.graphs = None
         if graph:
             self._capture()
@@ -770,25 +786,39 @@ class BatchLeapfrog:
             _lib.timing(was)
 
     def forward(self, x0, v0):
-        """Run the forward sweep from (x0, v0); returns the static (x, v)
+        """Run the forward sweep from (x0, v0); returns static (x, v)
         buffers (valid until the next call)."""
-        self.x0.copy_(x0)
-        self.v0.copy_(v0)
+        order = self.order
+        if order is not None:
+            # the order of this call's initial positions, outside the graph
+            self.st.tile_order(x0, order)
+            _ops.permute_rows(order, x0, v0, out=(self.x0, self.v0))
+        else:
+            self.x0.copy_(x0)
+            self.v0.copy_(v0)
         if self.graphs:
             self.graphs[0].replay()
         else:
             self._fwd()
+        if order is not None:
+            return _ops.permute_rows(order, self.x, self.v, scatter=True, out=(self.xo, self.vo))
         return self.x, self.v
 
     def backward(self, gx, gv):
         """The adjoint of the last forward for output cotangents (gx, gv);
-        returns the static (lx, lv, dots) buffers."""
-        self.gx.copy_(gx)
-        self.gv.copy_(gv)
+        returns static (lx, lv, dots) buffers."""
+        order = self.order
+        if order is not None:
+            _ops.permute_rows(order, gx, gv, out=(self.gx, self.gv))
+        else:
+            self.gx.copy_(gx)
+            self.gv.copy_(gv)
         if self.graphs:
             self.graphs[1].replay()
         else:
             self._bwd()
+        if order is not None:
+            return (*_ops.permute_rows(order, self.lx, self.lv, scatter=True, out=(self.lxo, self.lvo)), self.dots)
         return self.lx, self.lv, self.dots
 
 

@@ -214,8 +214,29 @@ def test_fused_kick_drift_matches_separate_drift(gpu, graph):
     out = []
     for want_c in (False, True):
         lf = integrate.BatchLeapfrog(st, c, 2, coeff_grads=want_c, graph=graph)
+        # the fused sweeps run on the particles in tile order, the others in
+        # the caller's order: the same bits either way
+        assert (lf.order is not None) == (not want_c)
         x, v = [t.clone() for t in lf.forward(x0, v0)]
         lx, lv = [t.clone() for t in lf.backward(gx, gv)[:2]]
         out.append((x, v, lx, lv))
     for a, b in zip(*out):
         assert torch.equal(a, b)
+
+
+def test_batch_tile_order(gpu):
+    """jpm_batch_tile_order_u32: a permutation that keeps each simulation's
+    slots within its own particle range and groups them by tile."""
+    from jaxpm_amd import integrate
+    xs, _ = states(SHAPES, False, 60)
+    st = integrate.BatchStepper(SHAPES, [x.shape[0] for x in xs])
+    x = torch.as_tensor(np.concatenate(xs), device="cuda")
+    order = st.tile_order(x, torch.empty(st.N, dtype=torch.int32, device="cuda")).cpu().numpy()
+    for b, sh in enumerate(SHAPES):
+        p0, n = int(st.part0[b]), st.counts[b]
+        o = order[p0:p0 + n]
+        assert np.array_equal(np.sort(o), np.arange(p0, p0 + n))
+        cells = np.floor(xs[b].astype(np.float64)).astype(np.int64) % np.array(sh)
+        t = [(s + 15) // 16 for s in sh]
+        key = ((cells[:, 0] // 16) * t[1] + cells[:, 1] // 16) * t[2] + cells[:, 2] // 16
+        assert np.all(np.diff(key[o - p0]) >= 0)
