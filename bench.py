@@ -485,7 +485,10 @@ def main():
         "hipfft_c2r": 8 * H + 4 * M,
         "fft_r2c": 4 * M + 8 * H,
         "fft_c2r": 8 * H + 4 * M,
+        "fft_potential": 4 * M + 4 * M,                # the mesh in, the potential out
         "abs_max_kernel": 12 * N,
+        "permute_rows_kernel": 2 * 24 * N,             # two (N, 3) arrays, read and written
+        "record_order_kernel": 16 * N + 4 * N,
     }
     kinfo = {}
     for name, (cnt, tot) in per_kernel.items():

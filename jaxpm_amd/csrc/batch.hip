@@ -20,6 +20,8 @@ int fft_r2c(hipStream_t, int, int, int, int, const float *, void *);
 int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
 int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *, float, float, const float *,
                            uint32_t);
+int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
+                    uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
                float *);
@@ -185,11 +187,9 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
   if (rc) return rc;
   for (const SimClass &c : bt.classes) {
     const float inv_m = float(1.0 / double(cells(c.nx, c.ny, c.nz)));
-    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, mesh + c.cell0, spec + c.half0))) return rc;
-    if ((rc = kspace_potential_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, spec2 + c.half0, r_split, inv_m,
-                                     nullptr, kflags)))
+    if ((rc = potential_solve(s, c.nx, c.ny, c.nz, c.count, mesh + c.cell0, spec + c.half0, spec2 + c.half0,
+                              phi + c.cell0, r_split, inv_m, nullptr, kflags)))
       return rc;
-    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, phi + c.cell0))) return rc;
   }
   return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
 }
@@ -222,11 +222,9 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
   for (const SimClass &c : bt.classes) {
     if ((rc = fd4_divergence_batch(s, c.nx, c.ny, c.nz, c.count, g3 + c.cell0, size_t(bt.M), 1.0f, rho + c.cell0)))
       return rc;
-    if ((rc = fft_r2c(s, c.nx, c.ny, c.nz, c.count, rho + c.cell0, spec + c.half0))) return rc;
-    if ((rc = kspace_potential_batch(s, c.nx, c.ny, c.nz, c.count, spec + c.half0, spec2 + c.half0, r_split, 0.0f,
-                                     pot_scale + c.first, kflags)))
+    if ((rc = potential_solve(s, c.nx, c.ny, c.nz, c.count, rho + c.cell0, spec + c.half0, spec2 + c.half0,
+                              rho + c.cell0, r_split, 0.0f, pot_scale + c.first, kflags)))
       return rc;
-    if ((rc = fft_c2r(s, c.nx, c.ny, c.nz, c.count, spec2 + c.half0, rho + c.cell0))) return rc;
   }
   return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds, beta, cot_out);
 }

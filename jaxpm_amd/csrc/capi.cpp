@@ -102,6 +102,8 @@ int power_spectrum(hipStream_t, int, int, int, const void *, const void *, doubl
 int axpy(hipStream_t, int64_t, float, const float *, float *);
 int permute_rows(hipStream_t, int64_t, const uint32_t *, const float *, float *, const float *, float *, int);
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
+int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
+                    uint32_t);
 int fd4_gradient(hipStream_t, int, int, int, const float *, float, float *);
 int fd4_divergence(hipStream_t, int, int, int, const float *, float, float *);
 
@@ -225,11 +227,8 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   float *mesh = reinterpret_cast<float *>(base + w.mesh);
   rc = bins_paint(s, b, nullptr, 1.0f, 1, mesh, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint);
   if (rc) return rc;
-  rc = fft_r2c(s, nx, ny, nz, 1, mesh, base + w.spec);
-  if (rc) return rc;
-  rc = kspace_potential(s, nx, ny, nz, base + w.spec, base + w.spec2, r_split, float(1.0 / double(m)), kflags);
-  if (rc) return rc;
-  rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, phi);
+  rc = potential_solve(s, nx, ny, nz, 1, mesh, base + w.spec, base + w.spec2, phi, r_split, float(1.0 / double(m)),
+                       nullptr, kflags);
   if (rc) return rc;
   return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
 }
@@ -304,12 +303,8 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   if (rc) return rc;
   rc = fd4_divergence(s, nx, ny, nz, g3, 1.0f, rho);
   if (rc) return rc;
-  rc = fft_r2c(s, nx, ny, nz, 1, rho, base + w.spec);
-  if (rc) return rc;
-  rc = kspace_potential(s, nx, ny, nz, base + w.spec, base + w.spec2, r_split, float(double(scale) / double(m)),
-                        kflags);
-  if (rc) return rc;
-  rc = fft_c2r(s, nx, ny, nz, 1, base + w.spec2, rho);
+  rc = potential_solve(s, nx, ny, nz, 1, rho, base + w.spec, base + w.spec2, rho, r_split,
+                       float(double(scale) / double(m)), nullptr, kflags);
   if (rc) return rc;
   return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags), beta, cot_out);
 }

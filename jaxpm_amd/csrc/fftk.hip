@@ -16,7 +16,7 @@
 #include <mutex>
 #include <vector>
 
-#include "common.hpp"
+#include "kspace.hpp"
 
 namespace jpm {
 
@@ -259,6 +259,60 @@ fft_cols_kernel(float2 *__restrict__ data, int nzh, size_t lstride, size_t ostri
   }
 }
 
+// The x pass of a potential solve, in place: the forward column FFT, the
+// potential multiplier at each mode, the inverse column FFT.  Columns as in
+// fft_cols_kernel with the x axis as the column axis (element l = kx) and
+// outer = ky; the same float operations as the forward x pass,
+// potential_kernel and the inverse x pass run one after another.
+template <int L>
+__global__ void __launch_bounds__(kFftThreads)
+fft_cols_potential_kernel(float2 *__restrict__ data, int nzh, size_t lstride, size_t ostride, size_t bstride,
+                          const float2 *__restrict__ tw_all, uint32_t nkzb, uint32_t nouter, uint32_t nblk,
+                          uint32_t per, Tabs t, float r2, float scale_s, const float *__restrict__ scale_dev,
+                          int deconv) {
+  constexpr int KB = FftCols<L>::KB, total = L * KB;
+  __shared__ float2 buf[total];
+  __shared__ float2 tw[L];
+  const uint32_t item = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+  if (item >= nblk) return;
+  const uint32_t kzb = item % nkzb, rest = item / nkzb, outer = rest % nouter, bi = rest / nouter;
+  load_twiddles<L>(tw, tw_all);
+  float2 *base = data + size_t(bi) * bstride + size_t(outer) * ostride;
+  const int kz0 = int(kzb) * KB;
+  constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int c = f & (KB - 1), l = f / KB;
+    buf[f] = kz0 + c < nzh ? base[size_t(l) * lstride + kz0 + c] : make_float2(0.0f, 0.0f);
+  }
+  __syncthreads();
+  lds_fft<L, KB, KB, false>(buf, tw);
+  const float scale = scale_dev ? scale_dev[bi] : scale_s;
+  const uint32_t jy = t.oy + outer;
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int c = f & (KB - 1), l = f / KB;
+    if (kz0 + c < nzh) {
+      const float fac = potential_factor(t, uint32_t(l), jy, t.oz + uint32_t(kz0 + c), r2, deconv, scale);
+      const float2 v = buf[f];
+      buf[f] = make_float2(v.x * fac, v.y * fac);
+    }
+  }
+  __syncthreads();
+  lds_fft<L, KB, KB, true>(buf, tw);
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads != 0 && f >= total) break;
+    const int c = f & (KB - 1), l = f / KB;
+    if (kz0 + c < nzh) base[size_t(l) * lstride + kz0 + c] = buf[f];
+  }
+}
+
 // z pass of the R2C: rows r = blockIdx.x * KB + c (< nrows) of item
 // blockIdx.y, each nz = 2M reals read as M complex points.
 template <int M>
@@ -378,20 +432,16 @@ static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int 
   return JPM_OK;
 }
 
-int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
-  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
-  const float2 *tw;
-  int rc = twiddles(&tw);
-  if (rc) return rc;
+static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, float2 *out,
+                    const float2 *tw) {
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
-  float2 *o = static_cast<float2 *>(out);
   const float2 *i2 = reinterpret_cast<const float2 *>(in);
   switch (M) {
 #define JPM_R2C(N)                                                                                            \
   case N:                                                                                                     \
     hipLaunchKernelGGL(fft_r2c_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
-                       dim3(kFftThreads), 0, s, i2, o, nrows, Mc, H, tw);                                     \
+                       dim3(kFftThreads), 0, s, i2, out, nrows, Mc, H, tw);                                   \
     break;
     JPM_FFT_LENGTHS(JPM_R2C)
 #undef JPM_R2C
@@ -399,6 +449,37 @@ int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in
       return fail(JPM_E_INVALID, "native FFT length %d", nz);
   }
   JPM_LAUNCHED("fft_r2c_rows_kernel");
+  return JPM_OK;
+}
+
+static int rows_c2r(hipStream_t s, int nx, int ny, int nz, int batch, float2 *in, float *out, const float2 *tw) {
+  const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
+  const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
+  float2 *o2 = reinterpret_cast<float2 *>(out);
+  switch (M) {
+#define JPM_C2R(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL(fft_c2r_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
+                       dim3(kFftThreads), 0, s, in, o2, nrows, H, Mc, tw);                                    \
+    break;
+    JPM_FFT_LENGTHS(JPM_C2R)
+#undef JPM_C2R
+    default:
+      return fail(JPM_E_INVALID, "native FFT length %d", nz);
+  }
+  JPM_LAUNCHED("fft_c2r_rows_kernel");
+  return JPM_OK;
+}
+
+int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out) {
+  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
+  const float2 *tw;
+  int rc = twiddles(&tw);
+  if (rc) return rc;
+  const int nzh = nz / 2 + 1;
+  const size_t H = size_t(nx) * ny * nzh;
+  float2 *o = static_cast<float2 *>(out);
+  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, o, tw))) return rc;
   if ((rc = cols_pass(s, false, o, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
   return cols_pass(s, false, o, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw);
 }
@@ -408,25 +489,47 @@ int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float
   const float2 *tw;
   int rc = twiddles(&tw);
   if (rc) return rc;
-  const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
-  const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
+  const int nzh = nz / 2 + 1;
+  const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(in);
   if ((rc = cols_pass(s, true, h, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw))) return rc;
   if ((rc = cols_pass(s, true, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
-  float2 *o2 = reinterpret_cast<float2 *>(out);
-  switch (M) {
-#define JPM_C2R(N)                                                                                            \
-  case N:                                                                                                     \
-    hipLaunchKernelGGL(fft_c2r_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
-                       dim3(kFftThreads), 0, s, h, o2, nrows, H, Mc, tw);                                     \
-    break;
-    JPM_FFT_LENGTHS(JPM_C2R)
-#undef JPM_C2R
-    default:
-      return fail(JPM_E_INVALID, "native FFT length %d", nz);
+  return rows_c2r(s, nx, ny, nz, batch, h, out, tw);
+}
+
+// out = C2R(scale G . R2C(in)) for a batch of meshes, the spectrum in spec
+// (H complex per mesh): R2C rows and y columns, the x columns with the
+// multiplier between their forward and inverse transforms, inverse y
+// columns, C2R rows.  in and out may be the same array.
+int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
+                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv) {
+  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
+  const float2 *tw;
+  int rc = twiddles(&tw);
+  if (rc) return rc;
+  const int nzh = nz / 2 + 1;
+  const size_t H = size_t(nx) * ny * nzh;
+  float2 *h = static_cast<float2 *>(spec);
+  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw))) return rc;
+  if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
+  switch (nx) {
+#define JPM_POT(N)                                                                                        \
+  case N: {                                                                                               \
+    const uint32_t nkzb = (nzh + FftCols<N>::KB - 1) / FftCols<N>::KB;                                   \
+    const uint32_t nblk = nkzb * uint32_t(ny) * uint32_t(batch), per = (nblk + 7) / 8;                    \
+    hipLaunchKernelGGL(fft_cols_potential_kernel<N>, dim3(8 * per), dim3(kFftThreads), 0, s, h, nzh,      \
+                       size_t(ny) * nzh, size_t(nzh), H, tw, nkzb, uint32_t(ny), nblk, per, t, r2, scale, \
+                       scale_dev, deconv);                                                                \
+    break;                                                                                                \
   }
-  JPM_LAUNCHED("fft_c2r_rows_kernel");
-  return JPM_OK;
+    JPM_FFT_LENGTHS(JPM_POT)
+#undef JPM_POT
+    default:
+      return fail(JPM_E_INVALID, "native FFT length %d", nx);
+  }
+  JPM_LAUNCHED("fft_cols_potential_kernel");
+  if ((rc = cols_pass(s, true, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
+  return rows_c2r(s, nx, ny, nz, batch, h, out, tw);
 }
 
 }  // namespace jpm

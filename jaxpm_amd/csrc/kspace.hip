@@ -8,25 +8,14 @@
 #include <tuple>
 #include <vector>
 
-#include "common.hpp"
+#include "kspace.hpp"
 
 namespace jpm {
-
-// Per-axis tables, each of length nx + ny + nzh (x entries, then y, then z):
-//   k    : wave number 2*pi*fftfreq(n) in radians per cell (fftk)
-//   k2   : k^2, or (k sinc(k/2pi))^2 with JPM_K_LAPL_FD (invlaplace_kernel)
-//   k2e  : k^2 (longrange_kernel always uses the plain k^2)
-//   g    : gradient_kernel coefficient a, the kernel being i*a
-//   s    : sinc(k/2pi) (cic_compensation = (sx sy sz)^-2)
-struct Tabs {
-  const float *k, *k2, *k2e, *g, *s;
-  uint32_t oy, oz;  // offsets of the y and z entries
-};
 
 static std::mutex g_tab_mu;
 static std::map<std::tuple<int, int, int, uint32_t>, float *> g_tabs;
 
-static int get_tabs(hipStream_t st, int nx, int ny, int nz, uint32_t flags, Tabs &t) {
+int get_tabs(hipStream_t st, int nx, int ny, int nz, uint32_t flags, Tabs &t) {
   const int nzh = nz / 2 + 1;
   const size_t len = size_t(nx) + ny + nzh;
   const uint32_t key_flags = flags & (JPM_K_GRAD_EXACT | JPM_K_LAPL_FD);
@@ -115,9 +104,6 @@ static HalfGrid make_half(int nx, int ny, int nz) {
   return h;
 }
 
-// invlaplace_kernel (kernels.py:90-92): -1/kk, 0 at kk == 0.
-__device__ __forceinline__ float green(float kk) { return kk == 0.0f ? 0.0f : -(1.0f / kk); }
-
 __global__ void __launch_bounds__(256)
 force_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGrid hg, Tabs t, float r2,
              float scale, int deconv) {
@@ -162,18 +148,7 @@ potential_kernel(const float2 *__restrict__ dk, float2 *__restrict__ out, HalfGr
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < hg.H; e += gridDim.x * blockDim.x) {
     uint32_t i, j, k;
     hg.coords(e, i, j, k);
-    const uint32_t jy = t.oy + j, kz = t.oz + k;
-    float kk = (t.k2[i] + t.k2[jy]) + t.k2[kz];
-    float f = green(kk);
-    if (r2 != 0.0f) {
-      float kke = (t.k2e[i] + t.k2e[jy]) + t.k2e[kz];
-      f = f * expf(-kke * r2);
-    }
-    if (deconv) {
-      float p = (t.s[i] * t.s[jy]) * t.s[kz];
-      f = f / (p * p);
-    }
-    f = f * scale;
+    const float f = potential_factor(t, i, t.oy + j, t.oz + k, r2, deconv, scale);
     const float2 v = dk[e];
     out[e] = make_float2(v.x * f, v.y * f);
   }
@@ -518,6 +493,37 @@ int kspace_potential_batch(hipStream_t s, int nx, int ny, int nz, int batch, con
   }
   JPM_LAUNCHED("potential_kernel");
   return JPM_OK;
+}
+
+int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *out);
+int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out);
+bool native_fft_ok(int nx, int ny, int nz);
+int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
+                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv);
+
+// out = C2R(potential(R2C(in))) for a batch of equal meshes (in and out may
+// be the same array): the forward transform, potential_kernel's multiplier
+// and the inverse transform.  spec and spec2 hold H complex values per mesh;
+// power-of-two meshes apply the multiplier inside the FFT's x pass and use
+// spec only, other shapes run hipFFT, potential_kernel into spec2, hipFFT.
+int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, void *spec2,
+                    float *out, float r_split, float scale, const float *scale_dev, uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(in && spec && spec2 && out, "null pointer argument");
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d meshes", batch);
+  const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 7u) == 0;
+  if (native_fft_ok(nx, ny, nz) && aligned) {
+    Tabs t;
+    rc = get_tabs(s, nx, ny, nz, flags, t);
+    if (rc) return rc;
+    JPM_TIMED(s, "fft_potential");
+    return native_potential(s, nx, ny, nz, batch, in, spec, out, t, r_split * r_split, scale, scale_dev,
+                            int((flags & JPM_K_DECONV) != 0));
+  }
+  if ((rc = fft_r2c(s, nx, ny, nz, batch, in, spec))) return rc;
+  if ((rc = kspace_potential_batch(s, nx, ny, nz, batch, spec, spec2, r_split, scale, scale_dev, flags))) return rc;
+  return fft_c2r(s, nx, ny, nz, batch, spec2, out);
 }
 
 int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void *out, float r_split, float scale,
