@@ -4,9 +4,10 @@
 
 Runs, on a 256^3 (default) workload: a plain device copy of a known size
 (the calibration launch), then the tile-sorted force evaluations of a 10-step
-leapfrog from 2LPT initial conditions to the clustered state at a=1, each
-kick carrying the next drift (`jpm_force_kick_drift_f32`, the last one a
-plain kick), and 5 force VJPs with the drift adjoint
+leapfrog from 2LPT initial conditions to the clustered state at a=1, on
+the particles in the tile order of the initial positions as NBody runs
+them, each kick carrying the next drift (`jpm_force_kick_drift_f32`, the
+last one a plain kick), and 5 force VJPs with the drift adjoint
 (`jpm_force_vjp_pot_drift_f32`) there: the launches of the leapfrog's
 forward and adjoint sweeps.
 Meant to be run under `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE`;
@@ -43,6 +44,8 @@ def main():
         x = (lattice + dx).reshape(-1, 3).contiguous()
         v = p.reshape(-1, 3).contiguous()
         st = integrate.Stepper(shape, N, sort_tuning=tuning)
+        if st.reorder:
+            x, v = st.permute(st.tile_order(x), x, v)
         steps = np.linspace(0.1, 1.0, 11)
         k0, beta, gamma = integrate.kdk_schedule(cosmo, steps)
         field = st.new_field(keep_sort=True)
