@@ -249,8 +249,9 @@ def batch48(args):
 
 
 def kernels(args):
-    """Kernel microbenchmark, forward only: cic_paint, cic_read and one
-    pm_forces on a 256^3 mesh with 256^3 particles, in two position states --
+    """Kernel microbenchmark, forward only: cic_paint, cic_read, one
+    pm_forces and one lens-plane density (1024^2, with and without smoothing)
+    on a 256^3 mesh with 256^3 particles, in two position states --
     uniform random in the box, and clustered (2LPT at a=0.1 from seed-2 noise,
     box 1024 Mpc/h, then 20 leapfrog steps to a=1).  GB/s on algorithmic
     bytes: paint 12N + 4M (positions in, mesh out), read 12N + 4M + 4N.
@@ -260,7 +261,7 @@ def kernels(args):
     paint, three force meshes, separate read)."""
     import torch
 
-    from jaxpm_amd import _lib, _ops, integrate, painting, pm
+    from jaxpm_amd import _lib, _ops, integrate, lensing, painting, pm
     from jaxpm_amd import cosmology as pc
     dev = _lib.device()
     n = args.mesh
@@ -284,6 +285,7 @@ def kernels(args):
     del ic, lattice, dx, p, x0, v0, st
     reps = args.kernel_reps
     paint_bytes, read_bytes = 12 * N + 4 * M, 12 * N + 4 * M + 4 * N
+    plane_bytes = 12 * N + 4 * 1024 ** 2           # positions in, plane out
     out_states = {}
     for name, pos in (("uniform", uniform), ("clustered", clustered)):
         mesh = torch.zeros(shape, device=dev)
@@ -298,6 +300,14 @@ def kernels(args):
         t_forces = kernel_time_ms(lambda: pm.pm_forces(pos, mesh_shape=shape), reps)
         f_fused = pm.pm_forces(pos, mesh_shape=shape)
         f_plain = _ops.pm_forces_fused(pos, shape, atomic=True)
+        # lens plane of the central quarter of the box at 1024^2 (lensing.py:11-44)
+        R, center, width = 1024, n / 2, n / 4
+        t_plane = kernel_time_ms(lambda: lensing.density_plane(pos, shape, center, width, R), reps)
+        t_plane_s = kernel_time_ms(lambda: lensing.density_plane(pos, shape, center, width, R, smoothing_sigma=1.0),
+                                   reps)
+        plane = lensing.density_plane(pos, shape, center, width, R)
+        in_slab = int(((pos[:, 2] > center - width / 2) & (pos[:, 2] <= center + width / 2)).sum())
+        plane_mass = float(plane.double().sum()) * (n / R) ** 2 * width
         torch.cuda.synchronize()
         out_states[name] = {
             "cic_paint": {"ms": round(t_paint, 4), "GB_s": round(paint_bytes / (t_paint * 1e-3) / 1e9, 1),
@@ -315,7 +325,14 @@ def kernels(args):
                                 "frac_hbm": round(read_bytes / (t_read_sorted * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                                 "what": "tile sort + LDS windows (method='sorted')"},
             "pm_forces": {"ms": round(t_forces, 4)},
+            "density_plane": {"ms": round(t_plane, 4), "GB_s": round(plane_bytes / (t_plane * 1e-3) / 1e9, 1),
+                              "frac_hbm": round(plane_bytes / (t_plane * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "what": f"{R}^2 plane of the slab {center - width / 2:g} < z <= "
+                                      f"{center + width / 2:g}: wrap, select, 2-D CIC paint, normalise"},
+            "density_plane_smoothed": {"ms": round(t_plane_s, 4), "what": "the same plus gaussian_smoothing, "
+                                       "sigma 1 pixel (2-D R2C, filter, C2R)"},
             "checks": {"mass_rel_err": abs(float(rho.double().sum()) - N) / N,
+                       "density_plane_mass_rel_err": abs(plane_mass - in_slab) / max(in_slab, 1),
                        "sorted_read_bitwise_equal": same_read,
                        "binned_vs_atomic_max_rel": float((rho - rho_atomic).abs().max() / rho.abs().max()),
                        "pm_forces_sorted_vs_unfused_max_rel": float((f_fused - f_plain).abs().max() /

@@ -436,6 +436,57 @@ int jpm_dot_f32(void *stream, int64_t n, const float *x, const float *y,
  * of the leapfrog built on make_ode_fn (ode.py:139-140) and its adjoint. */
 int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y);
 
+/* ------------------------------------------------------- lens planes */
+
+/* mesh[nx][ny] += the weighted 2-D CIC deposits of n particles whose (x, y)
+ * are pos[p * stride], pos[p * stride + 1] (stride >= 2, so the first two
+ * coordinates of [n][3] positions can be painted in place): cic_paint_2d
+ * (painting.py:131-158), corner order (0,0), (1,0), (0,1), (1,1), weights
+ * (1 - |x - cx|)(1 - |y - cy|) times weight[p] when weight is not NULL,
+ * cells int32(c) mod n.  Float atomics: the summation order varies, as
+ * lax.scatter_add's does on a GPU. */
+int jpm_cic_paint_2d_f32(void *stream, int nx, int ny, const float *pos,
+                         int64_t n, int stride, const float *weight,
+                         float *mesh);
+
+/* Adjoint of jpm_cic_paint_2d_f32 for a mesh cotangent g[nx][ny] (what
+ * jax.grad forms through painting.py:131-158): gpos (2 floats per particle
+ * at stride gstride) = weight[p] sum_o g[idx_o] dW_o/d(x, y) and gweight[p] =
+ * sum_o g[idx_o] W_o.  Either output may be NULL. */
+int jpm_cic_paint_2d_vjp_f32(void *stream, int nx, int ny, const float *pos,
+                             int64_t n, int stride, const float *weight,
+                             const float *g, float *gpos, int gstride,
+                             float *gweight);
+
+/* density_plane without smoothing (lensing.py:11-40) into plane[res][res]:
+ * (x, y) = jnp.mod(pos[:, :2], box_nx) / box_nx * res, weight 1 for
+ * center - width/2 < z <= center + width/2 (the bounds rounded to float32),
+ * 0 otherwise (those particles are skipped: their deposits are zeros), the
+ * 2-D CIC paint into zeros, then division by (box_nx / res)(box_ny / res)
+ * width. */
+int jpm_density_plane_f32(void *stream, const float *pos, int64_t n,
+                          int box_nx, int box_ny, double center, double width,
+                          int res, float *plane);
+
+/* Adjoint of jpm_density_plane_f32 for a plane cotangent g[res][res], which
+ * is overwritten (divided by the normalisation): gpos[n][3] = the position
+ * cotangent (z and the particles outside the slab get 0). */
+int jpm_density_plane_vjp_f32(void *stream, const float *pos, int64_t n,
+                              int box_nx, int box_ny, double center,
+                              double width, int res, float *g, float *gpos);
+
+/* Workspace bytes of jpm_gaussian_smoothing_f32 (one half spectrum). */
+size_t jpm_gaussian_smoothing_workspace_size(int n0, int n1);
+
+/* out[n0][n1] = Re ifft2(fft2(im) filter) with the filter of
+ * utils.gaussian_smoothing (utils.py:208-222): exp(-|k|^2 / (2 s^2)),
+ * |k| the norm of the fftfreq wave numbers, s = 1 / (2 pi sigma).  The
+ * filter is real and even, so the operation is its own adjoint.  im and out
+ * may be the same array. */
+int jpm_gaussian_smoothing_f32(void *stream, int n0, int n1, const float *im,
+                               float *out, double sigma, void *ws,
+                               size_t ws_bytes);
+
 /* Workspace bytes of jpm_tile_order_u32. */
 size_t jpm_tile_order_workspace_size(int nx, int ny, int nz, int64_t n);
 

@@ -192,3 +192,51 @@ class Compensate(torch.autograd.Function):
     def backward(ctx, g):
         fk = _ops.kspace_compensate(_ops.r2c(g.contiguous()), ctx.shape, 1.0 / g.numel())
         return _ops.c2r(fk, ctx.shape, clobber=True)
+
+
+class CicPaint2d(torch.autograd.Function):
+    """out = mesh + paint2d(pos, w) (painting.py:131-158)."""
+
+    @staticmethod
+    def forward(ctx, mesh, pos2, weight):
+        out = _ops.paint_2d(mesh.clone(), pos2, weight)
+        ctx.save_for_backward(pos2, weight)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        pos2, weight = ctx.saved_tensors
+        g = g.contiguous()
+        gpos, gw = _ops.paint_2d_vjp(g, pos2, weight, want_pos=ctx.needs_input_grad[1],
+                                     want_weight=ctx.needs_input_grad[2] and weight is not None)
+        return (g if ctx.needs_input_grad[0] else None), gpos, gw
+
+
+class DensityPlane(torch.autograd.Function):
+    """The unsmoothed density plane of lensing.py:11-40."""
+
+    @staticmethod
+    def forward(ctx, pos, box_shape, center, width, res):
+        ctx.save_for_backward(pos)
+        ctx.args = (box_shape, center, width, res)
+        return _ops.density_plane(pos, box_shape, center, width, res)
+
+    @staticmethod
+    def backward(ctx, g):
+        (pos,) = ctx.saved_tensors
+        gpos = _ops.density_plane_vjp(g.contiguous(), pos, *ctx.args) if ctx.needs_input_grad[0] else None
+        return gpos, None, None, None, None
+
+
+class GaussianSmoothing(torch.autograd.Function):
+    """utils.py:208-222; the filter is real and even, so the backward is the
+    same filter applied to the cotangent."""
+
+    @staticmethod
+    def forward(ctx, im, sigma):
+        ctx.sigma = sigma
+        return _ops.gaussian_smoothing(im, sigma)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _ops.gaussian_smoothing(g.contiguous(), ctx.sigma), None

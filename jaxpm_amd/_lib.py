@@ -116,6 +116,12 @@ SIGNATURES = {
     "jpm_force_vjp_pot_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _f, _vp, _f, _u32, _vp, _sz, _vp,
                                          _sz]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
+    "jpm_cic_paint_2d_f32": (_i, [_vp, _i, _i, _vp, _i64, _i, _vp, _vp]),
+    "jpm_cic_paint_2d_vjp_f32": (_i, [_vp, _i, _i, _vp, _i64, _i, _vp, _vp, _vp, _i, _vp]),
+    "jpm_density_plane_f32": (_i, [_vp, _vp, _i64, _i, _i, _d, _d, _i, _vp]),
+    "jpm_density_plane_vjp_f32": (_i, [_vp, _vp, _i64, _i, _i, _d, _d, _i, _vp, _vp]),
+    "jpm_gaussian_smoothing_workspace_size": (_sz, [_i, _i]),
+    "jpm_gaussian_smoothing_f32": (_i, [_vp, _i, _i, _vp, _vp, _d, _vp, _sz]),
     "jpm_tile_order_workspace_size": (_sz, [_i, _i, _i, _i64]),
     "jpm_tile_order_u32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _sz]),
     "jpm_permute_rows_f32": (_i, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _i]),

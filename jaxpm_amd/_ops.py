@@ -409,6 +409,68 @@ def dot_into(out, i, x, y):
          ws.numel())
 
 
+def paint_2d(mesh, pos2, weight=None):
+    """mesh[nx][ny] += the 2-D CIC deposits of pos2 (n, 2), times weight (n)
+    when given, in place (jpm_cic_paint_2d_f32)."""
+    if mesh.dim() != 2 or mesh.dtype != torch.float32 or not mesh.is_contiguous():
+        raise ValueError("the plane must be a contiguous 2-D float32 tensor")
+    pos2 = _f32(pos2, "positions")
+    n = pos2.numel() // 2
+    w = None if weight is None else _f32(weight, "weight")
+    if w is not None and w.numel() != n:
+        raise ValueError("Weight shape must match particle shape")
+    call("jpm_cic_paint_2d_f32", stream(), mesh.shape[0], mesh.shape[1], ptr(pos2), n, 2, ptr(w), ptr(mesh))
+    return mesh
+
+
+def paint_2d_vjp(g, pos2, weight=None, want_pos=True, want_weight=False):
+    """(gpos (n, 2) or None, gweight (n) or None) of the 2-D paint for the
+    plane cotangent g (jpm_cic_paint_2d_vjp_f32)."""
+    g = _f32(g, "cotangent")
+    pos2 = _f32(pos2, "positions")
+    n = pos2.numel() // 2
+    w = None if weight is None else _f32(weight, "weight")
+    gpos = torch.empty((n, 2), dtype=torch.float32, device=g.device) if want_pos else None
+    gw = torch.empty(n, dtype=torch.float32, device=g.device) if want_weight else None
+    if want_pos or want_weight:
+        call("jpm_cic_paint_2d_vjp_f32", stream(), g.shape[0], g.shape[1], ptr(pos2), n, 2, ptr(w), ptr(g),
+             ptr(gpos), 2, ptr(gw))
+    return gpos, gw
+
+
+def density_plane(pos, box_shape, center, width, res):
+    """A new res x res plane (jpm_density_plane_f32) of positions (n, 3)."""
+    pos = _f32(pos, "positions")
+    plane = torch.empty((res, res), dtype=torch.float32, device=pos.device)
+    call("jpm_density_plane_f32", stream(), ptr(pos), pos.numel() // 3, int(box_shape[0]), int(box_shape[1]),
+         float(center), float(width), int(res), ptr(plane))
+    return plane
+
+
+def density_plane_vjp(g, pos, box_shape, center, width, res):
+    """The position cotangent (n, 3) of density_plane for the plane
+    cotangent g (jpm_density_plane_vjp_f32; g is copied first)."""
+    pos = _f32(pos, "positions")
+    g = _f32(g, "cotangent").clone()
+    gpos = torch.empty_like(pos)
+    call("jpm_density_plane_vjp_f32", stream(), ptr(pos), pos.numel() // 3, int(box_shape[0]), int(box_shape[1]),
+         float(center), float(width), int(res), ptr(g), ptr(gpos))
+    return gpos
+
+
+def gaussian_smoothing(im, sigma):
+    """A new image: the Gaussian filter of utils.gaussian_smoothing applied
+    to the 2-D float32 image im (jpm_gaussian_smoothing_f32)."""
+    im = _f32(im, "image")
+    if im.dim() != 2:
+        raise ValueError("gaussian_smoothing takes a 2-D image")
+    n0, n1 = im.shape
+    ws = torch.empty(size("jpm_gaussian_smoothing_workspace_size", n0, n1), dtype=torch.uint8, device=im.device)
+    out = torch.empty_like(im)
+    call("jpm_gaussian_smoothing_f32", stream(), n0, n1, ptr(im), ptr(out), float(sigma), ptr(ws), ws.numel())
+    return out
+
+
 def tile_order(pos, mesh_shape, ws=None):
     """int32 tensor (uint32 bits) order[k] = index of the particle at slot k
     of the tile sort of absolute positions ``pos`` (jpm_tile_order_u32).

@@ -101,6 +101,13 @@ int power_spectrum(hipStream_t, int, int, int, const void *, const void *, doubl
 
 int axpy(hipStream_t, int64_t, float, const float *, float *);
 int permute_rows(hipStream_t, int64_t, const uint32_t *, const float *, float *, const float *, float *, int);
+int paint_2d(hipStream_t, int, int, const float *, int64_t, int, const float *, float *);
+int paint_2d_vjp(hipStream_t, int, int, const float *, int64_t, int, const float *, const float *, float *, int,
+                 float *);
+int density_plane(hipStream_t, const float *, int64_t, int, int, double, double, int, float *);
+int density_plane_vjp(hipStream_t, const float *, int64_t, int, int, double, double, int, float *, float *);
+size_t gaussian_smoothing_ws(int, int);
+int gaussian_smoothing(hipStream_t, int, int, const float *, float *, double, void *, size_t);
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
 int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
                     uint32_t);
@@ -792,6 +799,33 @@ int jpm_force_vjp_f32(void *stream, int nx, int ny, int nz, const float *phi3, c
 
 int jpm_axpy_f32(void *stream, int64_t n, float a, const float *x, float *y) {
   JPM_GUARD(axpy(as_stream(stream), n, a, x, y));
+}
+
+int jpm_cic_paint_2d_f32(void *stream, int nx, int ny, const float *pos, int64_t n, int stride,
+                         const float *weight, float *mesh) {
+  JPM_GUARD(paint_2d(as_stream(stream), nx, ny, pos, n, stride, weight, mesh));
+}
+
+int jpm_cic_paint_2d_vjp_f32(void *stream, int nx, int ny, const float *pos, int64_t n, int stride,
+                             const float *weight, const float *g, float *gpos, int gstride, float *gweight) {
+  JPM_GUARD(paint_2d_vjp(as_stream(stream), nx, ny, pos, n, stride, weight, g, gpos, gstride, gweight));
+}
+
+int jpm_density_plane_f32(void *stream, const float *pos, int64_t n, int box_nx, int box_ny, double center,
+                          double width, int res, float *plane) {
+  JPM_GUARD(density_plane(as_stream(stream), pos, n, box_nx, box_ny, center, width, res, plane));
+}
+
+int jpm_density_plane_vjp_f32(void *stream, const float *pos, int64_t n, int box_nx, int box_ny, double center,
+                              double width, int res, float *g, float *gpos) {
+  JPM_GUARD(density_plane_vjp(as_stream(stream), pos, n, box_nx, box_ny, center, width, res, g, gpos));
+}
+
+size_t jpm_gaussian_smoothing_workspace_size(int n0, int n1) { return gaussian_smoothing_ws(n0, n1); }
+
+int jpm_gaussian_smoothing_f32(void *stream, int n0, int n1, const float *im, float *out, double sigma, void *ws,
+                               size_t ws_bytes) {
+  JPM_GUARD(gaussian_smoothing(as_stream(stream), n0, n1, im, out, sigma, ws, ws_bytes));
 }
 
 size_t jpm_tile_order_workspace_size(int nx, int ny, int nz, int64_t n) {

@@ -401,10 +401,10 @@ fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
 
 static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// The native transforms cover power-of-two meshes with 2 <= nx, ny <= 1024
-// and 4 <= nz <= 2048.
+// The native transforms cover power-of-two meshes with 1 <= nx <= 1024, 2 <=
+// ny <= 1024 and 4 <= nz <= 2048 (nx = 1: 2-D images, no x pass).
 bool native_fft_ok(int nx, int ny, int nz) {
-  return pow2(nx) && pow2(ny) && pow2(nz) && nx >= 2 && ny >= 2 && nz >= 4 && nx <= kFftMaxL && ny <= kFftMaxL &&
+  return pow2(nx) && pow2(ny) && pow2(nz) && ny >= 2 && nz >= 4 && nx <= kFftMaxL && ny <= kFftMaxL &&
          nz <= 2 * kFftMaxL;
 }
 
@@ -481,6 +481,7 @@ int native_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in
   float2 *o = static_cast<float2 *>(out);
   if ((rc = rows_r2c(s, nx, ny, nz, batch, in, o, tw))) return rc;
   if ((rc = cols_pass(s, false, o, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
+  if (nx == 1) return JPM_OK;
   return cols_pass(s, false, o, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw);
 }
 
@@ -492,7 +493,7 @@ int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float
   const int nzh = nz / 2 + 1;
   const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(in);
-  if ((rc = cols_pass(s, true, h, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw))) return rc;
+  if (nx > 1 && (rc = cols_pass(s, true, h, nx, nzh, ny, size_t(ny) * nzh, size_t(nzh), H, batch, tw))) return rc;
   if ((rc = cols_pass(s, true, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
   return rows_c2r(s, nx, ny, nz, batch, h, out, tw);
 }
@@ -503,7 +504,7 @@ int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float
 // columns, C2R rows.  in and out may be the same array.
 int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
                      const Tabs &t, float r2, float scale, const float *scale_dev, int deconv) {
-  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && batch >= 1 && batch <= 65535, "native FFT shape");
+  JPM_REQUIRE(native_fft_ok(nx, ny, nz) && nx >= 2 && batch >= 1 && batch <= 65535, "native FFT shape");
   const float2 *tw;
   int rc = twiddles(&tw);
   if (rc) return rc;

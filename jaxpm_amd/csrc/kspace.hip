@@ -513,7 +513,7 @@ int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const floa
   JPM_REQUIRE(in && spec && spec2 && out, "null pointer argument");
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d meshes", batch);
   const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 7u) == 0;
-  if (native_fft_ok(nx, ny, nz) && aligned) {
+  if (native_fft_ok(nx, ny, nz) && nx >= 2 && aligned) {
     Tabs t;
     rc = get_tabs(s, nx, ny, nz, flags, t);
     if (rc) return rc;

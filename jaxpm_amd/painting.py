@@ -100,6 +100,25 @@ def cic_read_dx(grid_mesh, disp, halo_size=0, sharding=None):
     return vals.reshape(shape)
 
 
+def cic_paint_2d(mesh, positions, weight):
+    """Weighted 2-D CIC paint (painting.py:131-158): ``mesh`` [nx, ny] plus
+    the deposits of ``positions`` (..., 2), each times ``weight`` (one value
+    per particle, or None)."""
+    m = to_device(mesh)
+    if m.dim() != 2:
+        raise ValueError(f"mesh must be 2-D, got shape {tuple(m.shape)}")
+    pos = to_device(positions)
+    if pos.shape[-1] != 2:
+        raise ValueError(f"positions must end in a dimension of 2, got {tuple(pos.shape)}")
+    pos = pos.reshape(-1, 2)
+    w = None
+    if weight is not None:
+        w = to_device(weight).reshape(-1)
+        if w.numel() != pos.shape[0]:
+            raise ValueError("Weight shape must match particle shape")
+    return _autograd.CicPaint2d.apply(m, pos, w)
+
+
 def compensate_cic(field):
     """painting.py:263-275: divide by the CIC window, Re IFFT[FFT(field)
     prod_d sinc(k_d/2pi)^-2]."""

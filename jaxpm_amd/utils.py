@@ -107,3 +107,17 @@ def pktranscoh(mesh0, mesh1, box_shape, kedges=None):
     ks, pk0 = power_spectrum(mesh0, box_shape=box_shape, kedges=kedges)
     ks, pk1 = power_spectrum(mesh1, box_shape=box_shape, kedges=kedges)
     return ks, pk0, pk1, (pk1 / pk0) ** .5, pk01 / (pk0 * pk1) ** .5
+
+
+def gaussian_smoothing(im, sigma):
+    """utils.py:208-222: the image filtered by exp(-|k|^2 / (2 s^2)), s =
+    1 / (2 pi sigma), |k| the norm of the fftfreq wave numbers, through the
+    library's 2-D transforms.  The reference builds its filter with
+    ``meshgrid``'s 'xy' order, which broadcasts against the image only when
+    it is square; non-square images raise ValueError here."""
+    from . import _autograd
+    from ._convert import to_device
+    t = to_device(im)
+    if t.dim() != 2 or t.shape[0] != t.shape[1]:
+        raise ValueError(f"gaussian_smoothing takes a square 2-D image, got shape {tuple(t.shape)}")
+    return _autograd.GaussianSmoothing.apply(t, float(sigma))

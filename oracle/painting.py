@@ -130,6 +130,31 @@ def cic_read_dx(grid_mesh, disp, dtype=np.float64):
 
 
 
+# Corner order of the 2-D painter, painting.py:141.
+CORNERS_2D = np.array([[0, 0], [1, 0], [0, 1], [1, 1]])
+
+
+def cic_paint_2d(mesh, positions, weight, dtype=np.float64):
+    """Weighted 2-D CIC paint, painting.py:131-158: corners floor + offset,
+    kernel (1 - |x - cx|)(1 - |y - cy|) times ``weight`` (when not None),
+    int32 cast then ``mod mesh.shape``, scatter_add into a copy of ``mesh``."""
+    m = np.asarray(mesh)
+    pos = np.asarray(positions, dtype).reshape(-1, 2)
+    fl = np.floor(pos)
+    shp = np.array(m.shape, np.int64)
+    out = m.astype(np.float64).reshape(-1).copy()
+    wt = None if weight is None else np.asarray(weight, dtype).reshape(-1)
+    for o in CORNERS_2D:
+        c = fl + o.astype(dtype)
+        k = dtype(1) - np.abs(pos - c)
+        k = k[:, 0] * k[:, 1]
+        if wt is not None:
+            k = k * wt
+        ci = np.mod(c.astype(np.int32).astype(np.int64), shp)
+        out += np.bincount(ci[:, 0] * shp[1] + ci[:, 1], weights=k.astype(np.float64), minlength=out.size)
+    return out.reshape(m.shape).astype(dtype)
+
+
 def compensate_cic(field):
     """painting.py:263-275: ifft(fft(field) * cic_compensation(kvec)).real."""
     from .kernels import cic_compensation, fftk

@@ -106,3 +106,18 @@ def coherence(mesh0, mesh1, box_shape, kedges=None):
     ks, pk0 = power_spectrum(mesh0, box_shape=box_shape, kedges=kedges)
     ks, pk1 = power_spectrum(mesh1, box_shape=box_shape, kedges=kedges)
     return ks, pk01 / (pk0 * pk1) ** .5
+
+
+def gaussian_smoothing(im, sigma):
+    """utils.py:208-222: ifft2(fft2(im) * filter).real, filter =
+    norm.pdf(|k|, 0, 1 / (2 pi sigma)) / its value at k = 0, |k| the norm of
+    the fftfreq wave numbers (meshgrid in 'xy' order, utils.py:214-216,
+    which broadcasts against the image only for square images)."""
+    im = np.asarray(im, np.float64)
+    if im.ndim != 2 or im.shape[0] != im.shape[1]:
+        raise ValueError("gaussian_smoothing takes a square 2-D image")
+    kx, ky = np.meshgrid(np.fft.fftfreq(im.shape[0]), np.fft.fftfreq(im.shape[1]))
+    k = np.sqrt(kx ** 2 + ky ** 2)
+    s = 1.0 / (2.0 * np.pi * sigma)
+    filt = np.exp(-0.5 * (k / s) ** 2)
+    return np.fft.ifft2(np.fft.fft2(im) * filt).real
