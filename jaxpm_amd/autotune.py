@@ -1,10 +1,9 @@
 """Autotuner for the tile sort and the gathers: picks the chunk width
 (particles per accumulation / gather workgroup), the slice width (particles
-per counting / scatter workgroup), where the force-VJP gather reads the
-density adjoint (an LDS window or the caches) and how tile workgroups are
-placed on the XCDs (round-robin or one contiguous run per XCD) by timing one
+per counting / scatter workgroup) and how tile workgroups are placed on the
+XCDs (round-robin or one contiguous run per XCD) by timing one
 force evaluation with kick and one force VJP on the caller's own particle
-state, for every candidate.
+state (in the tile order the leapfrog runs it in), for every candidate.
 
 Results are cached per (device, mesh, particle count, mode) in memory and in
 ``$XDG_CACHE_HOME/jaxpm_amd/sort_tuning.json`` (``~/.cache`` by default), so
@@ -16,11 +15,17 @@ import os
 
 import torch
 
-from . import _lib, integrate
+from . import _lib, _ops, integrate
 
 CHUNK_CODES = (2, 3, 4, 5)      # 2048, 4096, 8192, 16384 particles
 SLICE_CODES = (2, 3, 4, 5)      # 8192, 16384, 32768, 65536 particles
-RHO_MODES = (0, 1)              # density adjoint in LDS / through the caches
+# Where the force-VJP gather reads the density adjoint: only the LDS window
+# (0) is searched.  With 512-thread workgroups both forms keep 4 waves per
+# SIMD resident and the window moves a fifth fewer bytes; the cache form (1)
+# wins on nearly uniform states (by ~4% at 256^3 2LPT initial conditions)
+# and loses on clustered ones (by ~13% at a=1), where a leapfrog spends most
+# of its steps, and the search only sees the initial state.
+RHO_MODES = (0,)
 SLAB_MODES = (0, 1)             # tiles dealt round-robin to the XCDs / in runs
 
 _memo = {}
@@ -101,12 +106,17 @@ def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
     if hit:
         return hit, {}
     x = pos.reshape(-1, 3).contiguous()
+    probe = integrate.Stepper(shape, n, relative=relative)
+    if not probe.sorted:
+        return (0, 0, 0, 0), {}
+    if probe.reorder:
+        # the particle order NBody's sweeps run in
+        x = probe.permute(probe.tile_order(x), x)
+    del probe
     cot = torch.randn_like(x)
     timings = {}
     for tune in _candidates():
         st = integrate.Stepper(shape, n, relative=relative, sort_tuning=tune)
-        if not st.sorted:
-            return (0, 0, 0, 0), {}
         field = st.new_field(keep_sort=True)
         v = torch.zeros_like(x)
         g = torch.zeros_like(x)
@@ -130,6 +140,12 @@ def tune_batch(mesh_shapes, counts, pos, relative=False, reps=3, use_cache=True)
     if hit:
         return hit, {}
     x = pos.reshape(-1, 3).contiguous()
+    probe = integrate.BatchStepper(shapes, counts, relative=relative)
+    if probe.reorder:
+        # the particle order BatchLeapfrog's sweeps run in
+        order = probe.tile_order(x, torch.empty(probe.N, dtype=torch.int32, device=x.device))
+        x = _ops.permute_rows(order, x)
+    del probe
     cot = torch.randn_like(x)
     timings = {}
     for tune in _candidates():

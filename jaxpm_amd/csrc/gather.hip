@@ -363,13 +363,15 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
 // gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx, scale
 // per simulation (scale_dev[sim]) when scale_dev is given.
 // RLDS: the density-adjoint window in LDS (true) or read from the mesh
-// through the caches (false: 37 KiB of LDS per workgroup instead of 57 KiB,
-// twice the resident workgroups).
+// through the caches (false: 37 KiB of LDS per workgroup instead of 57 KiB).
+// With 512-thread workgroups both forms keep 4 waves per SIMD resident (two
+// workgroups per CU; the cache form is held there by its 84 VGPRs), so the
+// LDS window, which moves a fifth fewer bytes, is usually the faster.
 // With cot_out (the same array as cot, so neither is restrict), the adjoint
 // of the drift that preceded the kick follows: cot[idx] += beta gpos[idx]
 // with the new gpos (beta per simulation, beta_dev[sim], when given).
 template <bool REL, class G, bool RLDS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kWindowThreads)
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
                 const float *cot, float scale_s, const float *__restrict__ scale_dev,
                 float *__restrict__ gpos, uint32_t per, float beta_s, const float *__restrict__ beta_dev,
@@ -579,7 +581,7 @@ static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
   JPM_REQUIRE(!cot_out || cot_out == cot, "the fused drift adjoint updates the cotangent in place");
   uint32_t per;
-  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(256);
+  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, cot_out ? "bin_grad_kernel[drift]" : "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
