@@ -246,6 +246,63 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
   }
 }
 
+// The whole tile table of a sort of at most kTableMaxK tiles in one
+// single-workgroup launch: tile starts and scatter cursors (exclusive scan of
+// the counts), chunks per tile and their offsets (exclusive scan), the chunk
+// descriptors, and the totals [particles, chunks].  Thread t owns the
+// consecutive tiles [t per, (t + 1) per); both scans run over the threads'
+// sums in LDS.  The same values as count -> scans -> tile_chunks ->
+// chunk_desc, which larger tile grids use.
+constexpr int kTableThreads = 1024;
+constexpr uint32_t kTableMaxK = 32768;
+
+__global__ void __launch_bounds__(kTableThreads)
+tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk, uint32_t *__restrict__ tstart,
+                  uint32_t *__restrict__ cursor, uint32_t *__restrict__ nchunk, uint32_t *__restrict__ choff,
+                  uint32_t *__restrict__ totals, ChunkDesc *__restrict__ desc) {
+  __shared__ uint32_t sp[kTableThreads], sc[kTableThreads];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (K + kTableThreads - 1) / kTableThreads;
+  const uint32_t k0 = min(K, t * per), k1 = min(K, k0 + per);
+  uint32_t np = 0, nc = 0;
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint32_t c = count[k];
+    np += c;
+    nc += (c + chunk - 1) / chunk;
+  }
+  sp[t] = np;
+  sc[t] = nc;
+  __syncthreads();
+  for (uint32_t off = 1; off < kTableThreads; off <<= 1) {
+    const uint32_t x = t >= off ? sp[t - off] : 0u, y = t >= off ? sc[t - off] : 0u;
+    __syncthreads();
+    sp[t] += x;
+    sc[t] += y;
+    __syncthreads();
+  }
+  uint32_t pb = sp[t] - np, cb = sc[t] - nc;
+  if (t == kTableThreads - 1) {
+    totals[0] = sp[t];
+    totals[1] = sc[t];
+  }
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint32_t cnt = count[k], ncnk = (cnt + chunk - 1) / chunk;
+    tstart[k] = pb;
+    cursor[k] = pb;
+    nchunk[k] = ncnk;
+    choff[k] = cb;
+    for (uint32_t o = 0; o < cnt; o += chunk, ++cb) {
+      ChunkDesc d;
+      d.tile = k;
+      d.start = pb + o;
+      d.count = min(chunk, cnt - o);
+      d.pad = cnt > chunk ? 1u : 0u;
+      desc[cb] = d;
+    }
+    pb += cnt;
+  }
+}
+
 // -------------------------------------------------------------- scatter
 
 template <bool REL, class G>
@@ -748,18 +805,24 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
     JPM_LAUNCHED("bin_count_global_kernel");
   }
   // 2. tile starts, scatter cursors, chunk table
-  JPM_HIP(hipMemcpyAsync(tstart, count, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
-  rc = exclusive_scan(s, tstart, d.K, sums, totals);
-  if (rc) return rc;
-  JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
-  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K, d.chunk, nchunk);
-  JPM_LAUNCHED("tile_chunks_kernel");
-  JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
-  rc = exclusive_scan(s, choff, d.K, sums, totals + 1);
-  if (rc) return rc;
-  hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, tstart, count, choff, d.K,
-                     d.chunk, reinterpret_cast<ChunkDesc *>(base + w.desc));
-  JPM_LAUNCHED("chunk_desc_kernel");
+  if (d.K <= kTableMaxK) {
+    hipLaunchKernelGGL(tile_table_kernel, dim3(1), dim3(kTableThreads), 0, s, count, d.K, d.chunk, tstart, cursor,
+                       nchunk, choff, totals, reinterpret_cast<ChunkDesc *>(base + w.desc));
+    JPM_LAUNCHED("tile_table_kernel");
+  } else {
+    JPM_HIP(hipMemcpyAsync(tstart, count, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+    rc = exclusive_scan(s, tstart, d.K, sums, totals);
+    if (rc) return rc;
+    JPM_HIP(hipMemcpyAsync(cursor, tstart, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K, d.chunk, nchunk);
+    JPM_LAUNCHED("tile_chunks_kernel");
+    JPM_HIP(hipMemcpyAsync(choff, nchunk, size_t(d.K) * 4, hipMemcpyDeviceToDevice, s));
+    rc = exclusive_scan(s, choff, d.K, sums, totals + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, tstart, count, choff, d.K,
+                       d.chunk, reinterpret_cast<ChunkDesc *>(base + w.desc));
+    JPM_LAUNCHED("chunk_desc_kernel");
+  }
   // 3. scatter particles into tile order
   if (lds) {
     auto kern = rel ? bin_scatter_kernel<true, G> : bin_scatter_kernel<false, G>;

@@ -408,8 +408,8 @@ class NBody(torch.autograd.Function):
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
             if s0 != last:
-                yc, vc = ctx.ckpts[s0]
-                y, v = yc.clone(), vc.clone()
+                # each checkpoint is restored once: recompute in place
+                y, v = ctx.ckpts.pop(s0)
                 _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, drift_out=False)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
