@@ -415,13 +415,20 @@ int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz,
 
 /* The adjoint of that fused step, reverse order: jpm_force_vjp_pot_f32 into
  * gpos, then the drift's adjoint cot[p] += beta * gpos[p] (cot updated in
- * place).  Same workspace as jpm_force_vjp_pot_f32. */
+ * place).  Same workspace as jpm_force_vjp_pot_f32.
+ * cot_max_in: NULL, or a device float holding max_p,d |cot[p][d]| of the
+ * cotangent given (the bound of the cotangent paint's fixed-point sums),
+ * which skips that reduction; cot_max_out: NULL, or a device float that
+ * receives max |cot| of the updated cotangent.  A leapfrog adjoint passes
+ * the same float as both from its second step on (the value is exact, so
+ * the results are the same bits as without it). */
 int jpm_force_vjp_pot_drift_f32(void *stream, int nx, int ny, int nz,
                                 const float *phi, const float *pos, int64_t n,
                                 float *cot, float scale, float beta,
                                 float *gpos, float r_split, uint32_t flags,
                                 void *ws, size_t ws_bytes, const void *sorted,
-                                size_t sorted_bytes);
+                                size_t sorted_bytes, const float *cot_max_in,
+                                float *cot_max_out);
 
 /* Workspace bytes for jpm_dot_f32. */
 size_t jpm_dot_workspace_size(void);
@@ -565,7 +572,7 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
 
 /* The fused kick + drift and its adjoint (jpm_force_kick_drift_f32,
  * jpm_force_vjp_pot_drift_f32) for every simulation, beta a device array
- * [nsims]. */
+ * [nsims]; cot_max_in / cot_max_out as there, over the whole batch. */
 int jpm_batch_force_kick_drift_f32(void *stream, const void *batch,
                                    const float *pos, float *vel,
                                    const float *gamma, const float *beta,
@@ -579,7 +586,8 @@ int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch,
                                   const float *pot_scale, const float *beta,
                                   float *gpos, float r_split, uint32_t kflags,
                                   void *ws, size_t ws_bytes,
-                                  const void *sorted, size_t sorted_bytes);
+                                  const void *sorted, size_t sorted_bytes,
+                                  const float *cot_max_in, float *cot_max_out);
 
 /* y_b += a[b] x_b over each simulation's 3 n_b floats (the drift of
  * ode.py:140 with per-simulation coefficients). */

@@ -89,7 +89,7 @@ SIGNATURES = {
     "jpm_batch_force_kick_drift_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp,
                                             _sz]),
     "jpm_batch_force_vjp_drift_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp,
-                                           _sz]),
+                                           _sz, _vp, _vp]),
     "jpm_batch_axpy_f32": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "jpm_batch_tile_order_u32": (_i, [_vp, _vp, _vp, _vp, _vp, _sz]),
     "jpm_batch_dot_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz]),
@@ -114,7 +114,7 @@ SIGNATURES = {
     "jpm_force_kick_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _f, _vp, _vp, _vp, _f, _u32, _vp, _sz,
                                       _vp, _sz]),
     "jpm_force_vjp_pot_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _f, _vp, _f, _u32, _vp, _sz, _vp,
-                                         _sz]),
+                                         _sz, _vp, _vp]),
     "jpm_axpy_f32": (_i, [_vp, _i64, _f, _vp, _vp]),
     "jpm_cic_paint_2d_f32": (_i, [_vp, _i, _i, _vp, _i64, _i, _vp, _vp]),
     "jpm_cic_paint_2d_vjp_f32": (_i, [_vp, _i, _i, _vp, _i64, _i, _vp, _vp, _vp, _i, _vp]),

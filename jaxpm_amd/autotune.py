@@ -89,8 +89,20 @@ def _cached(key, use_cache):
     return None
 
 
+# Candidates within this factor of the fastest count as tied; ties go to the
+# largest chunk width.  The search times the initial state, where tiles hold
+# about their mean count and chunk widths barely matter; as the particles
+# cluster, heavy tiles split into more chunks of a narrow width, each
+# reloading its windows (4096 against 16384 particles: 234 against 270
+# batch48 simulations/s), while timing noise alone can pick the narrow one.
+TIE = 1.03
+
+
 def _finish(key, timings):
-    best = min(timings, key=timings.get)
+    fastest = min(timings.values())
+    tied = [t for t, ms in timings.items() if ms <= TIE * fastest]
+    widest = max(t[0] for t in tied)
+    best = min((t for t in tied if t[0] == widest), key=timings.get)
     _memo[key] = {"best": list(best), "ms": timings[best]}
     _store(key, _memo[key])
     return best, timings

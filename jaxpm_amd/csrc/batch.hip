@@ -31,7 +31,7 @@ int kspace_linear_field_batch(hipStream_t, int, int, int, int, void *, const flo
                               float);
 int grad_pos_batch(hipStream_t, const SimRef *, int, bool, const float *, const float *, float *);
 int batch_grad(hipStream_t, const BatchBins &, const float *, const float *, const float *, const float *, float *,
-               bool, const float *, float *);
+               bool, const float *, float *, float *);
 
 // ------------------------------------------------------------ description
 
@@ -199,7 +199,8 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
 // the drift that preceded the kick follows: cot_b += beta[b] gpos_b.
 int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const float *pos, const float *cot,
                     const float *scale, const float *pot_scale, float *gpos, float r_split, uint32_t kflags, void *ws,
-                    size_t ws_bytes, const void *sorted, size_t sorted_bytes, const float *beta, float *cot_out) {
+                    size_t ws_bytes, const void *sorted, size_t sorted_bytes, const float *beta, float *cot_out,
+                    const float *cot_max_in, float *cot_max_out) {
   if (int rc = check_kflags(kflags)) return rc;
   JPM_REQUIRE(phi && pos && cot && scale && pot_scale && gpos, "null pointer argument");
   const BatchWs w = batch_ws(bt, 3);
@@ -217,7 +218,7 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
   float *g3 = reinterpret_cast<float *>(base + w.g3);
   float *rho = reinterpret_cast<float *>(base + w.mesh);
   float2 *spec = reinterpret_cast<float2 *>(base + w.spec), *spec2 = reinterpret_cast<float2 *>(base + w.spec2);
-  rc = batch_paint(s, bt, b, cot, 3, g3, base + w.paint, w.mesh - w.paint);
+  rc = batch_paint(s, bt, b, cot, 3, g3, base + w.paint, w.mesh - w.paint, cot_max_in);
   if (rc) return rc;
   for (const SimClass &c : bt.classes) {
     if ((rc = fd4_divergence_batch(s, c.nx, c.ny, c.nz, c.count, g3 + c.cell0, size_t(bt.M), 1.0f, rho + c.cell0)))
@@ -226,7 +227,7 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
                               rho + c.cell0, r_split, 0.0f, pot_scale + c.first, kflags)))
       return rc;
   }
-  return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds, beta, cot_out);
+  return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds, beta, cot_out, cot_max_out);
 }
 
 // ---------------------------------------- initial conditions and final paint

@@ -880,9 +880,11 @@ template int bins_order<SimTable>(hipStream_t, const BatchBins &, uint32_t *);
 
 // nc meshes (stride b.M) = [mesh +] window sums of the sorted particles,
 // channel c with weight[p*nc + c] (or wscalar when weight is NULL).
+// wmax_in: NULL, or a device float holding max |weight| already (the
+// fixed-point scale's bound), which skips that pass.
 template <class G>
 static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, float wscalar, int nc, float *mesh,
-                        uint32_t flags, void *ws, size_t ws_bytes) {
+                        uint32_t flags, void *ws, size_t ws_bytes, const float *wmax_in = nullptr) {
   SortDims d{};
   d.maxchunks = b.maxchunks;
   JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
@@ -900,12 +902,12 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     if (rc) return rc;
   }
   const float wmax_host = fabsf(wscalar);
-  if (weight) {
+  if (weight && !wmax_in) {
     rc = abs_max(s, weight, uint32_t(int64_t(b.n) * nc), reinterpret_cast<float *>(base + owmax));
     if (rc) return rc;
   }
   auto *partial = reinterpret_cast<float *>(base + opart);
-  const float *wmax_dev = reinterpret_cast<const float *>(base + owmax);
+  const float *wmax_dev = wmax_in ? wmax_in : reinterpret_cast<const float *>(base + owmax);
   float *overflow = reinterpret_cast<float *>(base + oover);
   {
     JPM_TIMED(s, nc == 1 ? "bin_accum_kernel[nc=1]" : "bin_accum_kernel[nc=3]");
@@ -932,8 +934,8 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
 }
 
 int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
-               void *ws, size_t ws_bytes) {
-  return paint_sorted(s, b, weight, wscalar, nc, mesh, flags, ws, ws_bytes);
+               void *ws, size_t ws_bytes, const float *wmax_in) {
+  return paint_sorted(s, b, weight, wscalar, nc, mesh, flags, ws, ws_bytes, wmax_in);
 }
 
 int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,
@@ -955,7 +957,8 @@ int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_
   Bins b;
   rc = bin_sort(s, nx, ny, nz, pos, n, flags, ws, sort_bytes, b);
   if (rc) return rc;
-  return bins_paint(s, b, weight, wscalar, nc, mesh, flags, static_cast<char *>(ws) + sort_bytes, ws_bytes - sort_bytes);
+  return bins_paint(s, b, weight, wscalar, nc, mesh, flags, static_cast<char *>(ws) + sort_bytes, ws_bytes - sort_bytes,
+                    nullptr);
 }
 
 // ------------------------------------------------------------- batches
@@ -992,8 +995,8 @@ BatchBins batch_view(const SimBatch &bt, const void *ws) {
 }
 
 int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
-                void *ws, size_t ws_bytes) {
-  return paint_sorted(s, b, weight, 1.0f, nc, mesh, JPM_ZERO_MESH, ws, ws_bytes);
+                void *ws, size_t ws_bytes, const float *wmax_in) {
+  return paint_sorted(s, b, weight, 1.0f, nc, mesh, JPM_ZERO_MESH, ws, ws_bytes, wmax_in);
 }
 
 }  // namespace jpm

@@ -83,7 +83,7 @@ int batch_force_kick(hipStream_t, const SimBatch &, const float *, float *, cons
                      int, void *, size_t, void *, size_t, const float *, float *, float *);
 int batch_force_vjp(hipStream_t, const SimBatch &, const float *, const float *, const float *, const float *,
                     const float *, float *, float, uint32_t, void *, size_t, const void *, size_t, const float *,
-                    float *);
+                    float *, const float *, float *);
 int batch_axpy(hipStream_t, const SimBatch &, const float *, const float *, float *);
 size_t batch_dot_ws(const SimBatch &);
 int batch_dot(hipStream_t, const SimBatch &, const float *, const float *, double *, void *, size_t);
@@ -168,11 +168,12 @@ static int pipeline_paint(hipStream_t s, int nx, int ny, int nz, const float *po
 }
 int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, float, float *, uint32_t);
 
-int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t);
+int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t,
+               const float *wmax_in = nullptr);
 size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
 int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *);
 int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool, float,
-              float *);
+              float *, float *);
 int bins_read(hipStream_t, const Bins &, const float *, int, uint32_t, float *);
 
 // ------------------------------------------------------ sorted pipelines
@@ -283,7 +284,7 @@ static int tile_order(hipStream_t s, int nx, int ny, int nz, const float *pos, i
 static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
                          const float *cot, float scale, float *gpos, float r_split, uint32_t flags, void *ws,
                          size_t ws_bytes, const void *sorted, size_t sorted_bytes, float beta = 0.0f,
-                         float *cot_out = nullptr) {
+                         float *cot_out = nullptr, const float *cot_max_in = nullptr, float *cot_max_out = nullptr) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
@@ -306,14 +307,14 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   }
   float *g3 = reinterpret_cast<float *>(base + w.g3);
   float *rho = reinterpret_cast<float *>(base + w.mesh);
-  rc = bins_paint(s, b, cot, 1.0f, 3, g3, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint);
+  rc = bins_paint(s, b, cot, 1.0f, 3, g3, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint, cot_max_in);
   if (rc) return rc;
   rc = fd4_divergence(s, nx, ny, nz, g3, 1.0f, rho);
   if (rc) return rc;
   rc = potential_solve(s, nx, ny, nz, 1, rho, base + w.spec, base + w.spec2, rho, r_split,
                        float(double(scale) / double(m)), nullptr, kflags);
   if (rc) return rc;
-  return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags), beta, cot_out);
+  return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags), beta, cot_out, cot_max_out);
 }
 
 // Workspace of force_field: [mesh][half spectrum][3 half spectra][paint scratch].
@@ -700,17 +701,17 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi, c
                             void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_force_vjp(as_stream(stream), *as_batch(batch), phi, pos, cot, scale, pot_scale, gpos, r_split,
-                            kflags, ws, ws_bytes, sorted, sorted_bytes, nullptr, nullptr));
+                            kflags, ws, ws_bytes, sorted, sorted_bytes, nullptr, nullptr, nullptr, nullptr));
 }
 
 int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch, const float *phi, const float *pos, float *cot,
                                   const float *scale, const float *pot_scale, const float *beta, float *gpos,
                                   float r_split, uint32_t kflags, void *ws, size_t ws_bytes, const void *sorted,
-                                  size_t sorted_bytes) {
+                                  size_t sorted_bytes, const float *cot_max_in, float *cot_max_out) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   if (!beta || !cot) return fail(JPM_E_INVALID, "beta and cot are required");
   JPM_GUARD(batch_force_vjp(as_stream(stream), *as_batch(batch), phi, pos, cot, scale, pot_scale, gpos, r_split,
-                            kflags, ws, ws_bytes, sorted, sorted_bytes, beta, cot));
+                            kflags, ws, ws_bytes, sorted, sorted_bytes, beta, cot, cot_max_in, cot_max_out));
 }
 
 int jpm_batch_axpy_f32(void *stream, const void *batch, const float *a, const float *x, float *y) {
@@ -892,10 +893,11 @@ int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz, const float *
 
 int jpm_force_vjp_pot_drift_f32(void *stream, int nx, int ny, int nz, const float *phi, const float *pos, int64_t n,
                                 float *cot, float scale, float beta, float *gpos, float r_split, uint32_t flags,
-                                void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes) {
+                                void *ws, size_t ws_bytes, const void *sorted, size_t sorted_bytes,
+                                const float *cot_max_in, float *cot_max_out) {
   if (!cot) return fail(JPM_E_INVALID, "cot is null");
   JPM_GUARD(force_vjp_pot(as_stream(stream), nx, ny, nz, phi, pos, n, cot, scale, gpos, r_split, flags, ws,
-                          ws_bytes, sorted, sorted_bytes, beta, cot));
+                          ws_bytes, sorted, sorted_bytes, beta, cot, cot_max_in, cot_max_out));
 }
 
 size_t jpm_cic_read_sorted_workspace_size(int nx, int ny, int nz, int64_t n) {

@@ -375,9 +375,10 @@ __global__ void __launch_bounds__(kWindowThreads)
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
                 const float *cot, float scale_s, const float *__restrict__ scale_dev,
                 float *__restrict__ gpos, uint32_t per, float beta_s, const float *__restrict__ beta_dev,
-                float *cot_out) {
+                float *cot_out, float *__restrict__ cot_max) {
   __shared__ float win[kPwCells];
   __shared__ float rwin[RLDS ? kWinCells : 1];
+  __shared__ float wmax[kWindowThreads / 64];
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
@@ -413,6 +414,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
   if (RLDS) load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
   const bool near = near_wrap_ok(t);
+  float cmax = 0.0f;
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
@@ -504,9 +506,23 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     g[2] = n2;
     if (cot_out) {
       float *co = cot_out + 3 * size_t(idx);
-      co[0] = u0 + beta * n0;
-      co[1] = u1 + beta * n1;
-      co[2] = u2 + beta * n2;
+      const float c0 = u0 + beta * n0, c1 = u1 + beta * n1, c2 = u2 + beta * n2;
+      co[0] = c0;
+      co[1] = c1;
+      co[2] = c2;
+      cmax = fmaxf(cmax, fmaxf(fabsf(c0), fmaxf(fabsf(c1), fabsf(c2))));
+    }
+  }
+  if (cot_max) {
+    // max |updated cotangent| for the next force VJP's fixed-point bound: one
+    // atomic per workgroup; values are >= 0, so the unsigned bit pattern
+    // orders like the float
+    for (int off = 32; off > 0; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = cmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) cmax = fmaxf(cmax, wmax[w]);
+      atomicMax(reinterpret_cast<unsigned int *>(cot_max), __float_as_uint(cmax));
     }
   }
 }
@@ -574,19 +590,26 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
   return JPM_OK;
 }
 
+int zero_f32(hipStream_t s, float *p, int64_t m);
+
 template <class G>
 static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const float *rho_bar, const float *cot,
                        float scale, const float *scale_dev, float *gpos, bool rho_lds, float beta,
-                       const float *beta_dev, float *cot_out) {
+                       const float *beta_dev, float *cot_out, float *cot_max) {
   JPM_REQUIRE(phi && rho_bar && cot && gpos, "null pointer argument");
   JPM_REQUIRE(!cot_out || cot_out == cot, "the fused drift adjoint updates the cotangent in place");
+  JPM_REQUIRE(!cot_max || cot_out, "the cotangent maximum is of the updated cotangent");
+  if (cot_max) {
+    int rc = zero_f32(s, cot_max, 1);
+    if (rc) return rc;
+  }
   uint32_t per;
   const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, cot_out ? "bin_grad_kernel[drift]" : "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
   hipLaunchKernelGGL(kern, grid, block, 0, s, b, phi, rho_bar, cot, scale, scale_dev, gpos, per, beta, beta_dev,
-                     cot_out);
+                     cot_out, cot_max);
   JPM_LAUNCHED("bin_grad_kernel");
   return JPM_OK;
 }
@@ -598,8 +621,9 @@ int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float 
 }
 
 int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
-               const float *scale_dev, float *gpos, bool rho_lds, const float *beta_dev, float *cot_out) {
-  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos, rho_lds, 0.0f, beta_dev, cot_out);
+               const float *scale_dev, float *gpos, bool rho_lds, const float *beta_dev, float *cot_out,
+               float *cot_max) {
+  return grad_sorted(s, b, phi, rho_bar, cot, 0.0f, scale_dev, gpos, rho_lds, 0.0f, beta_dev, cot_out, cot_max);
 }
 
 int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces, float beta,
@@ -609,8 +633,8 @@ int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float
 }
 
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,
-              float *gpos, bool rho_lds, float beta, float *cot_out) {
-  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds, beta, nullptr, cot_out);
+              float *gpos, bool rho_lds, float beta, float *cot_out, float *cot_max) {
+  return grad_sorted(s, b, phi, rho_bar, cot, scale, nullptr, gpos, rho_lds, beta, nullptr, cot_out, cot_max);
 }
 
 int bins_read(hipStream_t s, const Bins &b, const float *mesh, int nc, uint32_t cstride, float *out) {

@@ -203,7 +203,7 @@ size_t batch_paint_workspace_size(const SimBatch &bt, int nc);
 int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out);
 BatchBins batch_view(const SimBatch &bt, const void *ws);
 int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
-                void *ws, size_t ws_bytes);
+                void *ws, size_t ws_bytes, const float *wmax_in = nullptr);
 
 // One corner axis of a particle relative to its tile origin: local window
 // index (or -1 when the corner is dropped / outside the window) and weight.

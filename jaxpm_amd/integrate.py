@@ -87,6 +87,8 @@ class Stepper:
         # the same bits either way (the paint sums in fixed point, the
         # gathers are per particle).
         self.reorder = bool(reorder) and self.sorted and not relative
+        # max |cotangent| kept from one force VJP to the next (see vjp_field_drift)
+        self.cot_max = torch.zeros(1, dtype=torch.float32, device=self.dev)
 
     def tile_order(self, x):
         """order[k] = index of the particle at slot k of the tile sort of x."""
@@ -150,14 +152,18 @@ class Stepper:
             if xkeep is not None:
                 xkeep.copy_(xout)
 
-    def vjp_field_drift(self, field, x, cot, scale, beta, gpos):
+    def vjp_field_drift(self, field, x, cot, scale, beta, gpos, chain=False):
         """gpos += scale J(x)^T cot, then the adjoint of the drift before the
-        kick, cot += beta gpos."""
+        kick, cot += beta gpos.  The call leaves max |cot| of the updated
+        cotangent in ``self.cot_max``; ``chain`` says the cotangent is the one
+        the previous vjp_field_drift updated (unchanged since), whose kept
+        maximum then bounds the cotangent paint without another pass."""
         buf, srt = field
         if self.sorted:
             self._timed("vjp", "jpm_force_vjp_pot_drift_f32", stream(), self.nx, self.ny, self.nz, ptr(buf), ptr(x),
                         self.n, ptr(cot), float(scale), float(beta), ptr(gpos), self.r_split, self.flags,
-                        ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+                        ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel(),
+                        ptr(self.cot_max) if chain else None, ptr(self.cot_max))
         else:
             self.vjp_field(field, x, cot, scale, gpos)
             self.axpy(beta, gpos, cot)
@@ -413,7 +419,7 @@ class NBody(torch.autograd.Function):
                 _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, drift_out=False)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
-                st.vjp_field_drift(kp[1], kp[0], lv, gamma[i], beta[i], lx)
+                st.vjp_field_drift(kp[1], kp[0], lv, gamma[i], beta[i], lx, chain=i < S - 1)
         st.vjp_field(phi0, x0, lv, k0, lx)
         ctx.ckpts = ctx.kept = ctx.order = None
         if order is not None:
@@ -499,6 +505,7 @@ class BatchStepper:
         # BatchLeapfrog runs its sweeps on the particles in tile order, as
         # NBody does (Stepper.reorder); absolute positions only
         self.reorder = bool(reorder) and not self.relative
+        self.cot_max = torch.zeros(1, dtype=torch.float32, device=self.dev)
 
     def tile_order(self, x, order):
         """order[k] = index of the particle at slot k of the batch's tile
@@ -541,12 +548,13 @@ class BatchStepper:
              ptr(xout), ptr(xkeep), ptr(buf), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(), ptr(srt),
              0 if srt is None else srt.numel())
 
-    def vjp_field_drift(self, field, x, cot, scale, pot_scale, beta, gpos):
-        """vjp_field, then the drift adjoint cot_b += beta[b] gpos_b."""
+    def vjp_field_drift(self, field, x, cot, scale, pot_scale, beta, gpos, chain=False):
+        """vjp_field, then the drift adjoint cot_b += beta[b] gpos_b; ``chain``
+        and ``self.cot_max`` as in Stepper.vjp_field_drift."""
         buf, srt = field
         call("jpm_batch_force_vjp_drift_f32", stream(), self.handle, ptr(buf), ptr(x), ptr(cot), ptr(scale),
              ptr(pot_scale), ptr(beta), ptr(gpos), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(),
-             ptr(srt), 0 if srt is None else srt.numel())
+             ptr(srt), 0 if srt is None else srt.numel(), ptr(self.cot_max) if chain else None, ptr(self.cot_max))
 
     def vjp_field(self, field, x, cot, scale, pot_scale, gpos):
         buf, srt = field
@@ -742,7 +750,8 @@ class BatchLeapfrog:
                                drift_out=False)
                 for i in reversed(range(s0, s1)):
                     kp = self.keep[i - s0]
-                    st.vjp_field_drift(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], sc.beta[i], lx)
+                    st.vjp_field_drift(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], sc.beta[i], lx,
+                                       chain=i < S - 1)
             st.vjp_field(self.phi0, self.x0, lv, sc.k0, sc.k0_pot, lx)
             return
         if self.want_c:

@@ -38,3 +38,13 @@ def test_cosmology_quantities_are_smooth():
     # P(k) is quadratic in sigma8 (linear_matter_power normalisation)
     c2 = base.replace(sigma8=1.6)
     np.testing.assert_allclose(pc.linear_matter_power(c2, k), 4 * pc.linear_matter_power(base, k), rtol=1e-12)
+
+
+def test_autotune_ties_go_to_the_widest_chunk(tmp_path, monkeypatch):
+    """Candidates within 3% of the fastest are tied; the widest chunk wins,
+    then the fastest among those; a clear winner keeps its place."""
+    monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
+    from jaxpm_amd import autotune
+    t = {(3, 2, 0, 0): 1.00, (5, 2, 0, 0): 1.02, (5, 3, 0, 1): 1.01, (4, 2, 0, 0): 0.99, (2, 2, 0, 0): 1.2}
+    assert autotune._finish("tie", t)[0] == (5, 3, 0, 1)
+    assert autotune._finish("clear", {(3, 2, 0, 0): 1.0, (5, 2, 0, 0): 1.2})[0] == (3, 2, 0, 0)
