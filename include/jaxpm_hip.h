@@ -405,13 +405,22 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz,
  * a kick, then the next step's drift): jpm_force_kick_f32 with
  * write_forces = 0, then pos_out[p] = pos[p] + beta * vel[p] with the new
  * velocity, also written to pos_keep when it is not NULL.  pos_out may be
- * pos.  Same workspace and keep_sort as jpm_force_kick_f32. */
+ * pos.  Same workspace and keep_sort as jpm_force_kick_f32.
+ * Absolute positions: counts_out, when not NULL, receives the tile counts
+ * of pos_out (jpm_tile_count(nx, ny, nz) uint32), and a later call whose
+ * pos is that pos_out, unchanged, may pass them as counts_in, which skips
+ * the counting pass of its tile sort (NULL: count).  The two arrays must
+ * differ. */
 int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz,
                              const float *pos, int64_t n, float *vel,
                              float gamma, float beta, float *pos_out,
                              float *pos_keep, float *phi, float r_split,
                              uint32_t flags, void *ws, size_t ws_bytes,
-                             void *keep_sort, size_t keep_bytes);
+                             void *keep_sort, size_t keep_bytes,
+                             const uint32_t *counts_in, uint32_t *counts_out);
+
+/* Tiles (16^3 cells) of a mesh: the length of the tile-count arrays. */
+size_t jpm_tile_count(int nx, int ny, int nz);
 
 /* The adjoint of that fused step, reverse order: jpm_force_vjp_pot_f32 into
  * gpos, then the drift's adjoint cot[p] += beta * gpos[p] (cot updated in
@@ -542,6 +551,9 @@ int jpm_batch_create(int nsims, const int *dims, const int64_t *counts,
                      uint32_t flags, void **batch);
 int jpm_batch_destroy(void *batch);
 
+/* Tiles of all simulations of the batch (the tile-count arrays' length). */
+size_t jpm_batch_tile_count(const void *batch);
+
 /* Bytes of a kept tile sort / of the force_kick, force_vjp and dot
  * workspaces of this batch. */
 size_t jpm_batch_sort_size(const void *batch);
@@ -572,14 +584,17 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
 
 /* The fused kick + drift and its adjoint (jpm_force_kick_drift_f32,
  * jpm_force_vjp_pot_drift_f32) for every simulation, beta a device array
- * [nsims]; cot_max_in / cot_max_out as there, over the whole batch. */
+ * [nsims]; counts_in / counts_out (jpm_batch_tile_count(batch) uint32) and
+ * cot_max_in / cot_max_out as there, over the whole batch. */
 int jpm_batch_force_kick_drift_f32(void *stream, const void *batch,
                                    const float *pos, float *vel,
                                    const float *gamma, const float *beta,
                                    float *pos_out, float *pos_keep, float *phi,
                                    float r_split, uint32_t kflags, void *ws,
                                    size_t ws_bytes, void *keep_sort,
-                                   size_t keep_bytes);
+                                   size_t keep_bytes,
+                                   const uint32_t *counts_in,
+                                   uint32_t *counts_out);
 int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch,
                                   const float *phi, const float *pos,
                                   float *cot, const float *scale,

@@ -24,7 +24,7 @@ int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void
                     uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
-               float *);
+               float *, uint32_t *);
 int kspace_force_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_force_adjoint_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_linear_field_batch(hipStream_t, int, int, int, int, void *, const float *, const float *, const float *, int,
@@ -165,8 +165,10 @@ static int check_kflags(uint32_t kflags) {
 // when given; xout may be pos).
 int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float *vel, const float *gamma, float *phi,
                      float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes, void *keep,
-                     size_t keep_bytes, const float *beta, float *xout, float *xkeep) {
+                     size_t keep_bytes, const float *beta, float *xout, float *xkeep, const uint32_t *counts_in,
+                     uint32_t *counts_out) {
   JPM_REQUIRE(!xout || (beta && !write_forces), "the fused drift needs beta and a kick");
+  JPM_REQUIRE(!counts_out || (xout && !bt.rel), "tile counts of drifted absolute positions only");
   if (int rc = check_kflags(kflags)) return rc;
   JPM_REQUIRE(pos && vel && phi && gamma, "null pointer argument");
   const BatchWs w = batch_ws(bt, 1);
@@ -176,9 +178,9 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
   int rc;
   if (keep) {
     JPM_REQUIRE(keep_bytes >= batch_sort_workspace_size(bt), "keep_sort buffer too small (%zu bytes)", keep_bytes);
-    rc = batch_sort(s, bt, pos, keep, keep_bytes, b);
+    rc = batch_sort(s, bt, pos, keep, keep_bytes, b, counts_in);
   } else {
-    rc = batch_sort(s, bt, pos, base + w.sort, w.paint - w.sort, b);
+    rc = batch_sort(s, bt, pos, base + w.sort, w.paint - w.sort, b, counts_in);
   }
   if (rc) return rc;
   float *mesh = reinterpret_cast<float *>(base + w.mesh);
@@ -191,7 +193,7 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
                               phi + c.cell0, r_split, inv_m, nullptr, kflags)))
       return rc;
   }
-  return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep);
+  return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out);
 }
 
 // For every simulation b: gpos_b += scale[b] J_b^T cot_b, with pot_scale[b] =

@@ -769,10 +769,12 @@ Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws
 }
 
 // Sort the particles of every simulation into its tiles (workspace of
-// sort_ws(d).total bytes).
+// sort_ws(d).total bytes).  counts_in: NULL, or the tile counts of these
+// positions already (d.K entries, e.g. from the kick that drifted them),
+// which skips the counting pass.
 template <class G>
 static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, bool rel, const float *pos, void *ws,
-                     size_t ws_bytes, BinsT<G> &out) {
+                     size_t ws_bytes, BinsT<G> &out, const uint32_t *counts_in = nullptr) {
   const SortWs w = sort_ws(d);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "tile sort needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   JPM_REQUIRE(pos != nullptr, "positions are null");
@@ -789,20 +791,24 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
   int rc;
 
   // 1. tile counts
-  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K);
-  JPM_LAUNCHED("zero_u32_kernel");
-  if (lds) {
-    auto kern = rel ? bin_count_kernel<true, G> : bin_count_kernel<false, G>;
-    rc = set_lds(kern, lds_hist);
-    if (rc) return rc;
-    JPM_TIMED(s, "bin_count_kernel");
-    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, count);
-    JPM_LAUNCHED("bin_count_kernel");
+  if (counts_in) {
+    count = const_cast<uint32_t *>(counts_in);   // only read below
   } else {
-    auto kern = rel ? bin_count_global_kernel<true, G> : bin_count_global_kernel<false, G>;
-    JPM_TIMED(s, "bin_count_kernel");
-    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), 0, s, pos, d.slice, g, count);
-    JPM_LAUNCHED("bin_count_global_kernel");
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, count, d.K);
+    JPM_LAUNCHED("zero_u32_kernel");
+    if (lds) {
+      auto kern = rel ? bin_count_kernel<true, G> : bin_count_kernel<false, G>;
+      rc = set_lds(kern, lds_hist);
+      if (rc) return rc;
+      JPM_TIMED(s, "bin_count_kernel");
+      hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), lds_hist, s, pos, d.slice, g, count);
+      JPM_LAUNCHED("bin_count_kernel");
+    } else {
+      auto kern = rel ? bin_count_global_kernel<true, G> : bin_count_global_kernel<false, G>;
+      JPM_TIMED(s, "bin_count_kernel");
+      hipLaunchKernelGGL(kern, dim3(d.G), dim3(kBinThreads), 0, s, pos, d.slice, g, count);
+      JPM_LAUNCHED("bin_count_global_kernel");
+    }
   }
   // 2. tile starts, scatter cursors, chunk table
   if (d.K <= kTableMaxK) {
@@ -845,7 +851,7 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
 
 // Sort n particles into tiles (workspace `ws` of bin_sort_workspace_size).
 int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
-             size_t ws_bytes, Bins &out) {
+             size_t ws_bytes, Bins &out, const uint32_t *counts_in) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(n >= 1 && n <= kMaxCells, "particle count %lld out of range", (long long)n);
@@ -856,7 +862,7 @@ int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n,
   JPM_REQUIRE(!rel || n == M, "relative mode needs one particle per lattice node (n=%lld, mesh %lld)",
               (long long)n, (long long)M);
   OneSim g{one_sim(nx, ny, nz, n)};
-  return sort_into(s, g, one_dims(t, n, flags), uint32_t(M), rel, pos, ws, ws_bytes, out);
+  return sort_into(s, g, one_dims(t, n, flags), uint32_t(M), rel, pos, ws, ws_bytes, out, counts_in);
 }
 
 // order[k] = index of the particle the tile sort put at slot k: the index
@@ -984,9 +990,16 @@ size_t batch_paint_workspace_size(const SimBatch &bt, int nc) {
   return accum_ws(batch_dims(bt), bt.M, nc, bt.rel, &a, &b, &c);
 }
 
-int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out) {
+int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out,
+               const uint32_t *counts_in) {
   SimTable g{bt.dev, bt.B};
-  return sort_into(s, g, batch_dims(bt), bt.M, bt.rel, pos, ws, ws_bytes, out);
+  return sort_into(s, g, batch_dims(bt), bt.M, bt.rel, pos, ws, ws_bytes, out, counts_in);
+}
+
+int zero_u32(hipStream_t s, uint32_t *p, uint32_t n) {
+  hipLaunchKernelGGL(zero_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+  JPM_LAUNCHED("zero_u32_kernel");
+  return JPM_OK;
 }
 
 BatchBins batch_view(const SimBatch &bt, const void *ws) {

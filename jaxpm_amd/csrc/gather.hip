@@ -247,12 +247,37 @@ __device__ __forceinline__ void stencil_global(const float *__restrict__ phi, co
 
 // The leapfrog drift that may follow a kick, fused into it: x = x + beta v
 // with the new velocity (beta per simulation, beta_dev[sim], when given),
-// written to xout and, when given, to xkeep.
+// written to xout and, when given, to xkeep.  next_count: NULL, or the tile
+// counts (zeroed) that the drifted positions add to, for the next sort of
+// them (absolute positions).
 struct DriftArgs {
   float beta;
   const float *beta_dev;
   float *xout, *xkeep;
+  uint32_t *next_count;
 };
+
+// Tile of the floor cell of an absolute position, per axis, and its offset
+// from tile (itx, ity, itz) folded periodically into {-1, 0, 1} when the two
+// tiles are neighbours.  Returns whether all three offsets are.
+__device__ __forceinline__ bool tile_offset(float x, float y, float z, const TileGrid &t, int itx, int ity, int itz,
+                                            int &jx, int &jy, int &jz, int &d) {
+  jx = floor_cell_abs(x, t.nx) / kTile;
+  jy = floor_cell_abs(y, t.ny) / kTile;
+  jz = floor_cell_abs(z, t.nz) / kTile;
+  const int j[3] = {jx, jy, jz}, i[3] = {itx, ity, itz}, T[3] = {t.tx, t.ty, t.tz};
+  bool near = true;
+  d = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int o = j[a] - i[a];
+    if (o > 1) o -= T[a];
+    if (o < -1) o += T[a];
+    near = near && o >= -1 && o <= 1;
+    d = d * 3 + (o + 1);
+  }
+  return near;
+}
 
 // MODE 0: vel[idx] += gamma * F;  MODE 1: out[idx] = gamma * F;  MODE 2:
 // MODE 0 followed by the drift x[idx] = x + beta vel[idx], x being the
@@ -263,8 +288,11 @@ __global__ void __launch_bounds__(kWindowThreads)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
                 float *__restrict__ vel, uint32_t per, DriftArgs dr) {
   __shared__ float win[kPwCells];
+  __shared__ uint32_t ncount[27];   // drifted particles per neighbour tile
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
+  const bool count_next = MODE == 2 && !REL && dr.next_count;
+  if (count_next && threadIdx.x < 27) ncount[threadIdx.x] = 0u;
   const ChunkDesc d = bins.desc[c];
   const SimRef &sim = bins.g.tile_sim(d.tile);
   const TileGrid &t = sim.t;
@@ -352,10 +380,31 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
         xk[1] = x1;
         xk[2] = x2;
       }
+      if (count_next) {
+        int jx, jy, jz, d;
+        if (tile_offset(x0, x1, x2, t, itx, ity, itz, jx, jy, jz, d))
+          atomicAdd(&ncount[d], 1u);
+        else
+          atomicAdd(&dr.next_count[sim.tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) +
+                                   uint32_t(jz)], 1u);
+      }
     } else {
       v[0] = gamma * s[0];
       v[1] = gamma * s[1];
       v[2] = gamma * s[2];
+    }
+  }
+  if (count_next) {
+    __syncthreads();
+    if (threadIdx.x < 27 && ncount[threadIdx.x]) {
+      const int o = int(threadIdx.x);
+      int jx = itx + o / 9 - 1, jy = ity + (o / 3) % 3 - 1, jz = itz + o % 3 - 1;
+      jx = jx < 0 ? jx + t.tx : (jx >= t.tx ? jx - t.tx : jx);
+      jy = jy < 0 ? jy + t.ty : (jy >= t.ty ? jy - t.ty : jy);
+      jz = jz < 0 ? jz + t.tz : (jz >= t.tz ? jz - t.tz : jz);
+      atomicAdd(&dr.next_count[sim.tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) +
+                               uint32_t(jz)],
+                ncount[threadIdx.x]);
     }
   }
 }
@@ -578,6 +627,11 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
                        float *vel, int mode, DriftArgs dr) {
   JPM_REQUIRE(phi && vel, "null pointer argument");
   JPM_REQUIRE(mode != 2 || dr.xout, "the fused drift needs its position output");
+  JPM_REQUIRE(!dr.next_count || (mode == 2 && !b.rel), "tile counts of drifted absolute positions only");
+  if (dr.next_count) {
+    int rc = zero_u32(s, dr.next_count, b.K);
+    if (rc) return rc;
+  }
   uint32_t per;
   const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, mode == 2 ? "bin_kick_kernel[drift]" : mode == 1 ? "bin_kick_kernel[forces]" : "bin_kick_kernel");
@@ -615,9 +669,9 @@ static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const
 }
 
 int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float *gamma_dev, float *vel,
-               int write_forces, const float *beta_dev, float *xout, float *xkeep) {
+               int write_forces, const float *beta_dev, float *xout, float *xkeep, uint32_t *next_count) {
   const int mode = write_forces ? 1 : (xout ? 2 : 0);
-  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, mode, DriftArgs{0.0f, beta_dev, xout, xkeep});
+  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, mode, DriftArgs{0.0f, beta_dev, xout, xkeep, next_count});
 }
 
 int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
@@ -627,9 +681,9 @@ int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float 
 }
 
 int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces, float beta,
-              float *xout, float *xkeep) {
+              float *xout, float *xkeep, uint32_t *next_count) {
   const int mode = write_forces ? 1 : (xout ? 2 : 0);
-  return kick_sorted(s, b, phi, gamma, nullptr, vel, mode, DriftArgs{beta, nullptr, xout, xkeep});
+  return kick_sorted(s, b, phi, gamma, nullptr, vel, mode, DriftArgs{beta, nullptr, xout, xkeep, next_count});
 }
 
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,

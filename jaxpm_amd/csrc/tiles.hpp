@@ -193,14 +193,16 @@ inline SimRef one_sim(int nx, int ny, int nz, int64_t n) {
 }
 
 int bin_sort(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, uint32_t flags, void *ws,
-             size_t ws_bytes, Bins &out);
+             size_t ws_bytes, Bins &out, const uint32_t *counts_in = nullptr);
 size_t bin_sort_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t flags);
 Bins bins_view(int nx, int ny, int nz, int64_t n, uint32_t flags, const void *ws);
 template <class G>
 int bins_order(hipStream_t s, const BinsT<G> &b, uint32_t *order);
 size_t batch_sort_workspace_size(const SimBatch &bt);
 size_t batch_paint_workspace_size(const SimBatch &bt, int nc);
-int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out);
+int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out,
+               const uint32_t *counts_in = nullptr);
+int zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
 BatchBins batch_view(const SimBatch &bt, const void *ws);
 int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
                 void *ws, size_t ws_bytes, const float *wmax_in = nullptr);

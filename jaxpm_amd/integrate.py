@@ -89,6 +89,21 @@ class Stepper:
         self.reorder = bool(reorder) and self.sorted and not relative
         # max |cotangent| kept from one force VJP to the next (see vjp_field_drift)
         self.cot_max = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        # tile counts of the last drifted positions and of the ones before
+        # (see field_kick_drift), absolute positions on the sorted pipeline
+        self._tile_counts = None
+        if self.sorted and not relative:
+            k = size("jpm_tile_count", self.nx, self.ny, self.nz)
+            self._tile_counts = [torch.empty(k, dtype=torch.int32, device=self.dev) for _ in range(2)]
+
+    def _counts(self, chain):
+        """(counts_in, counts_out) of the next drift and flips the pair."""
+        tc = self._tile_counts
+        if tc is None:
+            return None, None
+        cin = tc[0] if chain else None
+        tc.reverse()
+        return cin, tc[0]
 
     def tile_order(self, x):
         """order[k] = index of the particle at slot k of the tile sort of x."""
@@ -135,15 +150,20 @@ class Stepper:
             out.zero_()
             self.kick(buf, x, out, 1.0)
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
         """v += gamma F(x), then the drift xout = x + beta v (also into
         ``xkeep`` when given); ``xout`` may be ``x``.  One launch sequence on
-        the sorted pipeline (the drift rides on the kick's gather)."""
+        the sorted pipeline (the drift rides on the kick's gather).  With
+        absolute positions the kick also counts xout's particles per tile for
+        the next sort; ``chain`` says x is the previous field_kick_drift's
+        xout, unchanged since, whose counts then replace the counting pass."""
         buf, srt = field
         if self.sorted:
+            cin, cout = self._counts(chain)
             self._timed("field_kick", "jpm_force_kick_drift_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
                         ptr(v), float(gamma), float(beta), ptr(xout), ptr(xkeep), ptr(buf), self.r_split,
-                        self.flags, ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+                        self.flags, ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel(),
+                        ptr(cin), ptr(cout))
         else:
             self.field_kick(x, v, gamma, field)
             if xout is not x:
@@ -248,14 +268,16 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
         stepper.field_kick(x, v, gamma[i], phi)
 
 
-def _run_fused(stepper, x, v, beta, gamma, start, stop, phi, keep=None, drift_out=True):
+def _run_fused(stepper, x, v, beta, gamma, start, stop, phi, keep=None, drift_out=True, chain=False):
     """Kicks [start, stop) of the fused form, in place: on entry x is y_start
     (the positions kick ``start`` evaluates its force at, i.e. after drift
     ``start``) and v the velocity before that kick.  Each kick carries the
     next drift, y_{i+1} = y_i + beta_{i+1} v, except after the segment's
     last kick when ``drift_out`` is false or no drift follows (the final
     step).  With ``keep`` each step's y_i and field go to its buffers (the
-    drift writes y_{i+1} into the next buffer directly)."""
+    drift writes y_{i+1} into the next buffer directly).  ``chain``: x is the
+    output of the stepper's previous drift (its tile counts are kept); every
+    later kick of the segment follows a drift of its own."""
     S = len(beta)
     for i in range(start, stop):
         field = phi
@@ -266,7 +288,7 @@ def _run_fused(stepper, x, v, beta, gamma, start, stop, phi, keep=None, drift_ou
             field = kp[1]
         if i + 1 < S and (i + 1 < stop or drift_out):
             xkeep = keep[i + 1 - start][0] if keep is not None and i + 1 < stop else None
-            stepper.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x, xkeep)
+            stepper.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x, xkeep, chain=chain or i > start)
         else:
             stepper.field_kick(x, v, gamma[i], field)
 
@@ -330,7 +352,7 @@ class NBody(torch.autograd.Function):
             phi = st.new_field()
             for s0 in range(0, S, k):
                 s1 = min(S, s0 + k)
-                _run_fused(st, x, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None)
+                _run_fused(st, x, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None, chain=True)
                 if s1 < S:
                     ckpts[s1] = (x.clone(), v.clone())
             ctx.ckpts = ckpts
@@ -506,6 +528,10 @@ class BatchStepper:
         # NBody does (Stepper.reorder); absolute positions only
         self.reorder = bool(reorder) and not self.relative
         self.cot_max = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        self._tile_counts = None
+        if not self.relative:
+            k = size("jpm_batch_tile_count", h)
+            self._tile_counts = [torch.empty(k, dtype=torch.int32, device=self.dev) for _ in range(2)]
 
     def tile_order(self, x, order):
         """order[k] = index of the particle at slot k of the batch's tile
@@ -540,13 +566,15 @@ class BatchStepper:
         """out = F(x) (``ones`` a float32 device array of 1.0, [B])."""
         self._kick(x, out, ones, field, True)
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
         """v_b += gamma[b] F_b(x_b), then xout_b = x_b + beta[b] v_b (also
-        into ``xkeep``); gamma, beta float32 device arrays [B]."""
+        into ``xkeep``); gamma, beta float32 device arrays [B]; ``chain`` as
+        in Stepper.field_kick_drift."""
         buf, srt = field
+        cin, cout = Stepper._counts(self, chain)
         call("jpm_batch_force_kick_drift_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(beta),
              ptr(xout), ptr(xkeep), ptr(buf), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(), ptr(srt),
-             0 if srt is None else srt.numel())
+             0 if srt is None else srt.numel(), ptr(cin), ptr(cout))
 
     def vjp_field_drift(self, field, x, cot, scale, pot_scale, beta, gpos, chain=False):
         """vjp_field, then the drift adjoint cot_b += beta[b] gpos_b; ``chain``
@@ -712,7 +740,8 @@ class BatchLeapfrog:
                 if s0 == 0:
                     self.ckpts[0][0].copy_(x)
                     self.ckpts[0][1].copy_(v)
-                _run_fused(st, x, v, sc.beta, sc.gamma, s0, s1, self.phi, self.keep if s0 == self.last else None)
+                _run_fused(st, x, v, sc.beta, sc.gamma, s0, s1, self.phi, self.keep if s0 == self.last else None,
+                           chain=True)
                 if s1 < self.S:
                     self.ckpts[s1][0].copy_(x)
                     self.ckpts[s1][1].copy_(v)

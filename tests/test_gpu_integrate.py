@@ -216,9 +216,10 @@ def test_tile_order_and_permutation(gpu, shape):
 @pytest.mark.parametrize("shape", [(32, 32, 32), (17, 33, 20)])
 def test_tile_ordered_sweeps_give_the_same_bits(gpu, shape):
     """NBody on particles in tile order (the default for absolute positions)
-    against the caller's order: the same final state and gradients, bit for
-    bit, since the paint sums in fixed point and the gathers are per
-    particle."""
+    against the caller's order, and with the sorts' tile counts taken from
+    the kicks that drifted the positions against counted afresh: the same
+    final state and gradients, bit for bit, since the paint sums in fixed
+    point, the gathers are per particle and the counts are the same."""
     from jaxpm_amd import cosmology as pc
     from jaxpm_amd import integrate
     x0, v0 = _state(shape, 41)
@@ -227,13 +228,18 @@ def test_tile_ordered_sweeps_give_the_same_bits(gpu, shape):
     gv = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
     sched = integrate.schedule_tensor(integrate.kdk_schedule(pc.Cosmology(), np.linspace(0.1, 1.0, 6)))
     out = []
-    for reorder in (True, False):
+    # tile order with the kicks' tile counts chained into the next sorts,
+    # tile order with every sort counting, the caller's order
+    for reorder, chained in ((True, True), (True, False), (False, True)):
         st = integrate.Stepper(shape, x0.shape[0], reorder=reorder)
         assert st.reorder == reorder
+        if not chained:
+            st._tile_counts = None
         xt = torch.as_tensor(x0, device="cuda").requires_grad_(True)
         vt = torch.as_tensor(v0, device="cuda").requires_grad_(True)
         x, v = integrate.NBody.apply(xt, vt, sched, st, 2)
         ((x * gx).sum() + (v * gv).sum()).backward()
         out.append([t.detach().cpu().numpy() for t in (x, v, xt.grad, vt.grad)])
-    for a, b in zip(*out):
-        assert np.array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
