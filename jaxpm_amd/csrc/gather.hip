@@ -262,9 +262,9 @@ struct DriftArgs {
 // tiles are neighbours.  Returns whether all three offsets are.
 __device__ __forceinline__ bool tile_offset(float x, float y, float z, const TileGrid &t, int itx, int ity, int itz,
                                             int &jx, int &jy, int &jz, int &d) {
-  jx = floor_cell_abs(x, t.nx) / kTile;
-  jy = floor_cell_abs(y, t.ny) / kTile;
-  jz = floor_cell_abs(z, t.nz) / kTile;
+  jx = int(uint32_t(floor_cell_abs(x, t.nx)) / uint32_t(kTile));
+  jy = int(uint32_t(floor_cell_abs(y, t.ny)) / uint32_t(kTile));
+  jz = int(uint32_t(floor_cell_abs(z, t.nz)) / uint32_t(kTile));
   const int j[3] = {jx, jy, jz}, i[3] = {itx, ity, itz}, T[3] = {t.tx, t.ty, t.tz};
   bool near = true;
   d = 0;
@@ -381,12 +381,21 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
         xk[2] = x2;
       }
       if (count_next) {
+        // most particles stay in their tile: those add up across the
+        // wavefront and one lane adds them, the others add one each
         int jx, jy, jz, d;
-        if (tile_offset(x0, x1, x2, t, itx, ity, itz, jx, jy, jz, d))
-          atomicAdd(&ncount[d], 1u);
-        else
-          atomicAdd(&dr.next_count[sim.tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) +
-                                   uint32_t(jz)], 1u);
+        const bool nb = tile_offset(x0, x1, x2, t, itx, ity, itz, jx, jy, jz, d);
+        const bool own = nb && d == 13;
+        const unsigned long long act = __ballot(1), mown = __ballot(own);
+        if (int(threadIdx.x & 63) == __ffsll((long long)act) - 1 && mown)
+          atomicAdd(&ncount[13], uint32_t(__popcll(mown)));
+        if (!own) {
+          if (nb)
+            atomicAdd(&ncount[d], 1u);
+          else
+            atomicAdd(&dr.next_count[sim.tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) +
+                                     uint32_t(jz)], 1u);
+        }
       }
     } else {
       v[0] = gamma * s[0];

@@ -84,6 +84,8 @@ inline TileGrid make_tiles(int nx, int ny, int nz) {
 }
 
 __device__ __forceinline__ int wrapi(int c, int n) {
+  // cells already inside [0, n) (nearly all of them) skip the division
+  if (uint32_t(c) < uint32_t(n)) return c;
   int r = c % n;
   return r < 0 ? r + n : r;
 }
