@@ -20,6 +20,7 @@ struct Axis {
 };
 
 __device__ __forceinline__ int wrap_index(int c, int n) {
+  if (uint32_t(c) < uint32_t(n)) return c;   // inside the mesh: no division
   int r = c % n;
   return r < 0 ? r + n : r;
 }

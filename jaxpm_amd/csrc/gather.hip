@@ -70,6 +70,7 @@ __device__ __forceinline__ void corner_set(float4 rec, const TileGrid &t, uint32
 }
 
 __device__ __forceinline__ int wrapn(int c, int n) {
+  if (uint32_t(c) < uint32_t(n)) return c;
   int r = c % n;
   return r < 0 ? r + n : r;
 }
