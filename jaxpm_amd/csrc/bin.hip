@@ -34,9 +34,8 @@ constexpr int kScanBlock = 256 * kScanItems;
 // Floor cell of particle p (global index; `local` its index within its
 // simulation, which relative mode decodes into the lattice node).
 template <bool REL>
-__device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, uint32_t local,
-                                            const TileGrid &t, int &cx, int &cy, int &cz) {
-  const float x = pos[3 * size_t(p)], y = pos[3 * size_t(p) + 1], z = pos[3 * size_t(p) + 2];
+__device__ __forceinline__ void floor_cells_xyz(float x, float y, float z, uint32_t local, const TileGrid &t, int &cx,
+                                                int &cy, int &cz) {
   if (REL) {
     uint32_t r = t.div_nz.div(local);
     uint32_t k = local - r * uint32_t(t.nz);
@@ -50,6 +49,12 @@ __device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint3
     cy = floor_cell_abs(y, t.ny);
     cz = floor_cell_abs(z, t.nz);
   }
+}
+
+template <bool REL>
+__device__ __forceinline__ void floor_cells(const float *__restrict__ pos, uint32_t p, uint32_t local,
+                                            const TileGrid &t, int &cx, int &cy, int &cz) {
+  floor_cells_xyz<REL>(pos[3 * size_t(p)], pos[3 * size_t(p) + 1], pos[3 * size_t(p) + 2], local, t, cx, cy, cz);
 }
 
 __device__ __forceinline__ uint32_t tile_key(int cx, int cy, int cz, const TileGrid &t) {
@@ -347,6 +352,74 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t 
     const int my_head = below ? 63 - __clzll((long long)below) : 0;
     base = __shfl(base, my_head);
     if (active) spos[base + uint32_t(lane - my_head)] = make_float4(x, y, z, __uint_as_float(p));
+  }
+}
+
+// bin_scatter_kernel with the slice held in registers: 1024 threads, PER
+// particles each (slice = 1024 PER), loaded and keyed once for both the
+// histogram and the scatter, so the positions are read once.
+constexpr int kScatterRegThreads = 1024;
+
+template <bool REL, class G, int PER>
+__global__ void __launch_bounds__(kScatterRegThreads)
+bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ cursor,
+                       float4 *__restrict__ spos) {
+  extern __shared__ uint32_t cur[];
+  const SimRef &s = g.block_sim(blockIdx.x);
+  const uint32_t K = s.t.K, tile0 = s.tile0;
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x) cur[k] = 0;
+  uint32_t p0, p1;
+  block_range(s, blockIdx.x, slice, p0, p1);
+  const int lane = threadIdx.x & 63;
+  float x[PER], y[PER], z[PER];
+  uint32_t key[PER];
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    key[it] = 0xffffffffu;
+    x[it] = y[it] = z[it] = 0.0f;
+    if (p < p1) {
+      x[it] = pos[3 * size_t(p)];
+      y[it] = pos[3 * size_t(p) + 1];
+      z[it] = pos[3 * size_t(p) + 2];
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    if (p < p1) {
+      int cx, cy, cz;
+      floor_cells_xyz<REL>(x[it], y[it], z[it], p - s.part0, s.t, cx, cy, cz);
+      key[it] = tile_key(cx, cy, cz, s.t);
+    }
+  }
+  __syncthreads();
+  // LDS histogram of the slice, one atomic per run of equal keys in a wavefront
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const bool active = p < p1;
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key[it], active);
+    if (active && ((heads >> lane) & 1ull)) atomicAdd(&cur[key[it]], run_length(heads, act));
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < K; k += blockDim.x)
+    if (cur[k]) cur[k] = atomicAdd(&cursor[tile0 + k], cur[k]);
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < PER; ++it) {
+    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const bool active = p < p1;
+    const unsigned long long act = __ballot(active);
+    const unsigned long long heads = run_heads(key[it], active);
+    const bool head = active && ((heads >> lane) & 1ull);
+    uint32_t base = 0;
+    if (head) base = atomicAdd(&cur[key[it]], run_length(heads, act));
+    const unsigned long long below = heads & ((2ull << lane) - 1ull);
+    const int my_head = below ? 63 - __clzll((long long)below) : 0;
+    base = __shfl(base, my_head);
+    if (active) spos[base + uint32_t(lane - my_head)] = make_float4(x[it], y[it], z[it], __uint_as_float(p));
   }
 }
 
@@ -830,7 +903,17 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
     JPM_LAUNCHED("chunk_desc_kernel");
   }
   // 3. scatter particles into tile order
-  if (lds) {
+  if (lds && (d.slice == 8u * kScatterRegThreads || d.slice == 16u * kScatterRegThreads)) {
+    const bool p8 = d.slice == 8u * kScatterRegThreads;
+    auto kern = rel ? (p8 ? bin_scatter_reg_kernel<true, G, 8> : bin_scatter_reg_kernel<true, G, 16>)
+                    : (p8 ? bin_scatter_reg_kernel<false, G, 8> : bin_scatter_reg_kernel<false, G, 16>);
+    rc = set_lds(kern, lds_hist);
+    if (rc) return rc;
+    JPM_TIMED(s, "bin_scatter_kernel");
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kScatterRegThreads), lds_hist, s, pos, d.slice, g, cursor,
+                       reinterpret_cast<float4 *>(base + w.spos));
+    JPM_LAUNCHED("bin_scatter_reg_kernel");
+  } else if (lds) {
     auto kern = rel ? bin_scatter_kernel<true, G> : bin_scatter_kernel<false, G>;
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
