@@ -1013,10 +1013,14 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     JPM_TIMED(s, nc == 1 ? "bin_reduce_kernel[nc=1]" : "bin_reduce_kernel[nc=3]");
     // absolute positions: single-chunk tiles wrote their own cells already
     auto kern = b.rel ? bin_reduce_kernel<G, false> : bin_reduce_kernel<G, true>;
+    // always in XCD slabs: tiles consecutive in z then run on one XCD, and
+    // their low-z faces, single floats a tile apart along each mesh row,
+    // share that XCD's L2 lines (0.14 -> 0.10 ms for 3 channels at 256^3)
     uint32_t per;
-    const unsigned grid = slab_grid(b.K, b.slabs, per);
+    const unsigned grid = slab_grid(b.K, true, per);
     hipLaunchKernelGGL(kern, dim3(grid, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
-                       int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh, b.K, per, b.slabs);
+                       int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh, b.K, per,
+                       true);
   }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;
