@@ -33,9 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # reads, x1.000 for writes), by kernel, mesh side and where the force-VJP
 # gather reads the density adjoint ("lds" / "caches"; None: either).  The
 # commands and the counter values are in profiles/sorted_gathers_256_traffic.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256, None): 1272.8e6,
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256, None): 1267.8e6,
                      ("bin_grad_kernel[drift]", 256, "caches"): 2121.3e6,
-                     ("bin_grad_kernel[drift]", 256, "lds"): 1878.7e6}
+                     ("bin_grad_kernel[drift]", 256, "lds"): 1879.4e6}
 
 
 def fixed_tuning(args):

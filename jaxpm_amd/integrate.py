@@ -268,38 +268,44 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
         stepper.field_kick(x, v, gamma[i], phi)
 
 
-def _run_fused(stepper, x, v, beta, gamma, start, stop, phi, keep=None, drift_out=True, chain=False):
-    """Kicks [start, stop) of the fused form, in place: on entry x is y_start
+def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=None, out=None, chain=False):
+    """Kicks [start, stop) of the fused form.  On entry ``y`` holds y_start
     (the positions kick ``start`` evaluates its force at, i.e. after drift
-    ``start``) and v the velocity before that kick.  Each kick carries the
-    next drift, y_{i+1} = y_i + beta_{i+1} v, except after the segment's
-    last kick when ``drift_out`` is false or no drift follows (the final
-    step).  With ``keep`` each step's y_i and field go to its buffers (the
-    drift writes y_{i+1} into the next buffer directly).  ``chain``: x is the
-    output of the stepper's previous drift (its tile counts are kept); every
-    later kick of the segment follows a drift of its own."""
+    ``start``) and is only read; v is the velocity before that kick and is
+    kicked in place.  Each kick carries the next drift,
+    y_{i+1} = y_i + beta_{i+1} v, written straight into the buffer y_{i+1}
+    lives in: with ``keep``, keep[i + 1 - start][0] (each step's positions
+    and field stay there for the adjoint), otherwise ``work`` (in place
+    there after the first drift).  The segment's last kick drifts into
+    ``out`` when given and a drift follows (none after the final step), and
+    drifts nowhere otherwise.  So checkpoints and kept positions are the
+    buffers the drifts write, never copies.  Returns the buffer of the
+    positions the segment's last kick evaluated.  ``chain``: y is the output
+    of the stepper's previous drift (its tile counts are kept); every later
+    kick of the segment follows a drift of its own."""
     S = len(beta)
+    cur = y
     for i in range(start, stop):
-        field = phi
-        if keep is not None:
-            kp = keep[i - start]
-            if i == start:
-                kp[0].copy_(x)
-            field = kp[1]
-        if i + 1 < S and (i + 1 < stop or drift_out):
-            xkeep = keep[i + 1 - start][0] if keep is not None and i + 1 < stop else None
-            stepper.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x, xkeep, chain=chain or i > start)
+        field = keep[i - start][1] if keep is not None else phi
+        if i + 1 < S and (i + 1 < stop or out is not None):
+            nxt = out if i + 1 == stop else (keep[i + 1 - start][0] if keep is not None else work)
+            stepper.field_kick_drift(cur, v, gamma[i], beta[i + 1], field, nxt, chain=chain or i > start)
+            if i + 1 < stop:
+                cur = nxt
         else:
-            stepper.field_kick(x, v, gamma[i], field)
+            stepper.field_kick(cur, v, gamma[i], field)
+    return cur
 
 
-def _keep_buffers(st, like, count, forces):
+def _keep_buffers(st, like, count, forces, first_pos=True):
     """Per-step adjoint data of a segment: positions, the force evaluation's
     field (potential + kept tile sort) and, for coefficient cotangents, the
-    drift velocity and the forces."""
+    drift velocity and the forces.  Without ``first_pos`` the first step's
+    positions get no buffer (the fused sweeps read them from the segment's
+    checkpoint)."""
     out = []
-    for _ in range(count):
-        b = [torch.empty_like(like), st.new_field(keep_sort=True)]
+    for j in range(count):
+        b = [torch.empty_like(like) if first_pos or j > 0 else None, st.new_field(keep_sort=True)]
         if forces:
             b += [torch.empty_like(like), torch.empty_like(like)]
         out.append(b)
@@ -343,23 +349,31 @@ class NBody(torch.autograd.Function):
         ctx.c_meta = (coeffs.dtype, coeffs.device)
         ctx.fused = not want_c
         if ctx.fused:
-            # every kick carries the drift that follows it; checkpoints hold
-            # (y, v) just before a kick
-            x0c = x.clone() if grad else None
-            st.field_kick_drift(x, v, k0, beta[0], phi0, x)
-            ckpts = {0: (x.clone(), v.clone())}
-            keep = _keep_buffers(st, x, min(k, S), False) if grad else None
+            # every kick carries the drift that follows it, written into the
+            # buffer of the positions it makes; checkpoints hold (y, v) just
+            # before a kick, y being the buffer the drift wrote (no copy)
+            y = torch.empty_like(x) if grad else x
+            st.field_kick_drift(x, v, k0, beta[0], phi0, y)
+            x0c = x if grad else None
+            keep = _keep_buffers(st, x, min(k, S), False, first_pos=False) if grad else None
+            work = torch.empty_like(x) if grad else x
             phi = st.new_field()
+            ckpts = {}
             for s0 in range(0, S, k):
                 s1 = min(S, s0 + k)
-                _run_fused(st, x, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None, chain=True)
-                if s1 < S:
-                    ckpts[s1] = (x.clone(), v.clone())
+                if grad:
+                    # the kept last segment is never recomputed: no velocity
+                    ckpts[s0] = (y, v.clone() if s0 != last else None)
+                out = (torch.empty_like(x) if grad else work) if s1 < S else None
+                yl = _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None,
+                                work=work, out=out, chain=True)
+                y = out
             ctx.ckpts = ckpts
             ctx.kept = (keep, phi0, x0c, last)
             if order is not None:
-                return st.permute(order, x, v, scatter=True)
-            return x, v
+                return st.permute(order, yl, v, scatter=True)
+            # kept positions are overwritten by the adjoint's recompute
+            return (yl.clone() if grad else yl), v
         # with coefficient cotangents: separate forces and axpy, keeping the
         # forces and drift velocities the cotangents are dot products with
         f0 = torch.empty_like(x)
@@ -435,13 +449,13 @@ class NBody(torch.autograd.Function):
         phi = st.new_field()
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
+            y, v = ctx.ckpts.pop(s0)
             if s0 != last:
                 # each checkpoint is restored once: recompute in place
-                y, v = ctx.ckpts.pop(s0)
-                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, drift_out=False)
+                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
-                st.vjp_field_drift(kp[1], kp[0], lv, gamma[i], beta[i], lx, chain=i < S - 1)
+                st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, gamma[i], beta[i], lx, chain=i < S - 1)
         st.vjp_field(phi0, x0, lv, k0, lx)
         ctx.ckpts = ctx.kept = ctx.order = None
         if order is not None:
@@ -709,13 +723,20 @@ class BatchLeapfrog:
         self.k = S if checkpoint_every is None else max(1, int(checkpoint_every))
         self.want_c = bool(coeff_grads)
         e = lambda: torch.empty((st.N, 3), dtype=torch.float32, device=st.dev)  # noqa: E731
-        self.x0, self.v0, self.x, self.v = e(), e(), e(), e()
-        self.ckpts = {s0: (e(), e()) for s0 in range(0, S, self.k)}
+        self.x0, self.x, self.v = e(), e(), e()
         self.last = ((S - 1) // self.k) * self.k
+        # the fused sweeps keep no velocity for the kept last segment and
+        # read each segment's first positions from its checkpoint
+        fused = not self.want_c
+        self.ckpts = {s0: (e(), e() if not (fused and s0 == self.last) else None) for s0 in range(0, S, self.k)}
         self.phi = st.new_field()
         self.phi0 = st.new_field(keep_sort=True)
-        self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c)
-        self.gx, self.gv, self.lx, self.lv, self.xs, self.vs, self.f0 = e(), e(), e(), e(), e(), e(), e()
+        self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c, first_pos=not fused)
+        self.lx, self.lv = e(), e()
+        if self.want_c:
+            self.v0, self.xs, self.vs, self.f0 = e(), e(), e(), e()
+        self.xfinal = self.x
+        self._fresh = False
         self.dots = torch.zeros((1 + 2 * S, st.B), dtype=torch.float64, device=st.dev)
         # the fused sweeps (no coefficient cotangents) run on the particles in
         # tile order; forward / backward permute in and out around them
@@ -729,29 +750,28 @@ class BatchLeapfrog:
     def _fwd(self):
         st, sc = self.st, self.sc
         x, v = self.x, self.v
-        x.copy_(self.x0)
-        v.copy_(self.v0)
         if not self.want_c:
-            # every kick carries the drift that follows it (see _run_fused);
-            # checkpoints hold (y, v) just before a kick
-            st.field_kick_drift(x, v, sc.k0, sc.beta[0], self.phi0, x)
+            # every kick carries the drift that follows it, into the buffer
+            # of the positions it makes (see _run_fused): from x0 into the
+            # first checkpoint, then through x into the next; checkpoints
+            # hold (y, v) just before a kick; v starts as the initial
+            # velocity, which forward() writes there
+            st.field_kick_drift(self.x0, v, sc.k0, sc.beta[0], self.phi0, self.ckpts[0][0])
+            y = self.ckpts[0][0]
             for s0 in range(0, self.S, self.k):
                 s1 = min(self.S, s0 + self.k)
-                if s0 == 0:
-                    self.ckpts[0][0].copy_(x)
-                    self.ckpts[0][1].copy_(v)
-                _run_fused(st, x, v, sc.beta, sc.gamma, s0, s1, self.phi, self.keep if s0 == self.last else None,
-                           chain=True)
-                if s1 < self.S:
-                    self.ckpts[s1][0].copy_(x)
-                    self.ckpts[s1][1].copy_(v)
+                if s0 != self.last:
+                    self.ckpts[s0][1].copy_(v)
+                out = self.ckpts[s1][0] if s1 < self.S else None
+                self.xfinal = _run_fused(st, y, v, sc.beta, sc.gamma, s0, s1, self.phi,
+                                         self.keep if s0 == self.last else None, work=x, out=out, chain=True)
+                y = out
             return
+        x.copy_(self.x0)
+        v.copy_(self.v0)
         # the first half kick and the last segment keep their adjoint data
-        if self.want_c:
-            st.forces(x, self.f0, self.phi0, sc.ones)
-            st.axpy(sc.k0, self.f0, v)
-        else:
-            st.field_kick(x, v, sc.k0, self.phi0)
+        st.forces(x, self.f0, self.phi0, sc.ones)
+        st.axpy(sc.k0, self.f0, v)
         for s0 in range(0, self.S, self.k):
             if s0 == 0:
                 self.ckpts[0][0].copy_(x)
@@ -767,20 +787,18 @@ class BatchLeapfrog:
     def _bwd(self):
         st, sc, S = self.st, self.sc, self.S
         lx, lv, dots = self.lx, self.lv, self.dots
-        lx.copy_(self.gx)
-        lv.copy_(self.gv)
         if not self.want_c:
             for s0 in reversed(range(0, S, self.k)):
                 s1 = min(S, s0 + self.k)
+                y = self.ckpts[s0][0]
                 if s0 != self.last:
-                    self.xs.copy_(self.ckpts[s0][0])
-                    self.vs.copy_(self.ckpts[s0][1])
-                    _run_fused(st, self.xs, self.vs, sc.beta, sc.gamma, s0, s1, self.phi, self.keep,
-                               drift_out=False)
+                    # each checkpoint is read once per forward: recompute in
+                    # place on its velocity
+                    _run_fused(st, y, self.ckpts[s0][1], sc.beta, sc.gamma, s0, s1, self.phi, self.keep)
                 for i in reversed(range(s0, s1)):
                     kp = self.keep[i - s0]
-                    st.vjp_field_drift(kp[1], kp[0], lv, sc.gamma[i], sc.gamma_pot[i], sc.beta[i], lx,
-                                       chain=i < S - 1)
+                    st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, sc.gamma[i], sc.gamma_pot[i],
+                                       sc.beta[i], lx, chain=i < S - 1)
             st.vjp_field(self.phi0, self.x0, lv, sc.k0, sc.k0_pot, lx)
             return
         if self.want_c:
@@ -823,40 +841,61 @@ class BatchLeapfrog:
         finally:
             _lib.timing(was)
 
-    def forward(self, x0, v0):
-        """Run the forward sweep from (x0, v0); returns static (x, v)
-        buffers (valid until the next call)."""
+    def forward(self, x0, v0, out=None):
+        """Run the forward sweep from (x0, v0); returns (x, v) in ``out``
+        when given, else in static buffers (valid until the next call)."""
         order = self.order
+        vin = self.v0 if self.want_c else self.v
         if order is not None:
             # the order of this call's initial positions, outside the graph
             self.st.tile_order(x0, order)
-            _ops.permute_rows(order, x0, v0, out=(self.x0, self.v0))
+            _ops.permute_rows(order, x0, v0, out=(self.x0, vin))
         else:
             self.x0.copy_(x0)
-            self.v0.copy_(v0)
+            vin.copy_(v0)
         if self.graphs:
             self.graphs[0].replay()
         else:
             self._fwd()
+        self._fresh = True
+        x = self.x if self.want_c else self.xfinal
         if order is not None:
-            return _ops.permute_rows(order, self.x, self.v, scatter=True, out=(self.xo, self.vo))
+            return _ops.permute_rows(order, x, self.v, scatter=True, out=out if out is not None else (self.xo, self.vo))
+        if out is not None:
+            out[0].copy_(x)
+            out[1].copy_(self.v)
+            return out
+        if x is not self.x:
+            # the fused sweeps end in a kept buffer the adjoint overwrites
+            self.x.copy_(x)
         return self.x, self.v
 
-    def backward(self, gx, gv):
+    def backward(self, gx, gv, out=None):
         """The adjoint of the last forward for output cotangents (gx, gv);
-        returns static (lx, lv, dots) buffers."""
+        returns (lx, lv) in ``out`` when given, else in static buffers, and
+        the static ``dots``.  The fused sweeps recompute on their
+        checkpoints in place, so each forward has one backward."""
+        if not self._fresh:
+            raise RuntimeError("BatchLeapfrog.backward needs a forward since the last backward")
+        self._fresh = False
         order = self.order
         if order is not None:
-            _ops.permute_rows(order, gx, gv, out=(self.gx, self.gv))
+            _ops.permute_rows(order, gx, gv, out=(self.lx, self.lv))
         else:
-            self.gx.copy_(gx)
-            self.gv.copy_(gv)
+            self.lx.copy_(gx)
+            self.lv.copy_(gv)
         if self.graphs:
             self.graphs[1].replay()
         else:
             self._bwd()
         if order is not None:
-            return (*_ops.permute_rows(order, self.lx, self.lv, scatter=True, out=(self.lxo, self.lvo)), self.dots)
+            res = _ops.permute_rows(order, self.lx, self.lv, scatter=True,
+                                    out=out if out is not None else (self.lxo, self.lvo))
+            return (*res, self.dots)
+        if out is not None:
+            out[0].copy_(self.lx)
+            out[1].copy_(self.lv)
+            return (*out, self.dots)
         return self.lx, self.lv, self.dots
 
 
@@ -877,10 +916,11 @@ class BatchNBody(torch.autograd.Function):
             engine.sc.update(coeffs)
             if ctx.needs_input_grad[2] and not engine.want_c:
                 raise ValueError("this BatchLeapfrog was built without coefficient cotangents")
-        x, v = engine.forward(x0.detach().contiguous(), v0.detach().contiguous())
+        x0, v0 = x0.detach().contiguous(), v0.detach().contiguous()
+        x, v = engine.forward(x0, v0, out=(torch.empty_like(x0), torch.empty_like(v0)))
         ctx.engine = engine
         ctx.c_meta = (coeffs.dtype, coeffs.device)
-        return x.clone(), v.clone()
+        return x, v
 
     @staticmethod
     def backward(ctx, gx, gv):
@@ -888,10 +928,11 @@ class BatchNBody(torch.autograd.Function):
         zeros = None
         if gx is None or gv is None:
             zeros = torch.zeros_like(eng.x)
-        lx, lv, dots = eng.backward(gx.contiguous() if gx is not None else zeros,
-                                    gv.contiguous() if gv is not None else zeros)
+        gx = gx.contiguous() if gx is not None else zeros
+        gv = gv.contiguous() if gv is not None else zeros
+        lx, lv, dots = eng.backward(gx, gv, out=(torch.empty_like(gx), torch.empty_like(gv)))
         gc = None
         if ctx.needs_input_grad[2]:
             gc = dots.t().contiguous().to(device=ctx.c_meta[1], dtype=ctx.c_meta[0])
         ctx.engine = None
-        return lx.clone(), lv.clone(), gc, None, None
+        return lx, lv, gc, None, None
