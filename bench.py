@@ -503,6 +503,7 @@ def main():
         "fft_r2c": 4 * M + 8 * H,
         "fft_c2r": 8 * H + 4 * M,
         "fft_potential": 4 * M + 4 * M,                # the mesh in, the potential out
+        "fft_potential[div]": 12 * M + 4 * M,          # three cotangent meshes in (divergence), the potential out
         "abs_max_kernel": 12 * N,
         "permute_rows_kernel": 2 * 24 * N,             # two (N, 3) arrays, read and written
         "record_order_kernel": 16 * N + 4 * N,

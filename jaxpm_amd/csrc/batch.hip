@@ -22,6 +22,8 @@ int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *
                            uint32_t);
 int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
                     uint32_t);
+int potential_solve_div(hipStream_t, int, int, int, int, const float *, size_t, float *, void *, void *, float *, float,
+                        float, const float *, uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
                float *, uint32_t *);
@@ -223,10 +225,9 @@ int batch_force_vjp(hipStream_t s, const SimBatch &bt, const float *phi, const f
   rc = batch_paint(s, bt, b, cot, 3, g3, base + w.paint, w.mesh - w.paint, cot_max_in);
   if (rc) return rc;
   for (const SimClass &c : bt.classes) {
-    if ((rc = fd4_divergence_batch(s, c.nx, c.ny, c.nz, c.count, g3 + c.cell0, size_t(bt.M), 1.0f, rho + c.cell0)))
-      return rc;
-    if ((rc = potential_solve(s, c.nx, c.ny, c.nz, c.count, rho + c.cell0, spec + c.half0, spec2 + c.half0,
-                              rho + c.cell0, r_split, 0.0f, pot_scale + c.first, kflags)))
+    if ((rc = potential_solve_div(s, c.nx, c.ny, c.nz, c.count, g3 + c.cell0, size_t(bt.M), rho + c.cell0,
+                                  spec + c.half0, spec2 + c.half0, rho + c.cell0, r_split, 0.0f, pot_scale + c.first,
+                                  kflags)))
       return rc;
   }
   return batch_grad(s, b, phi, rho, cot, scale, gpos, bt.rho_lds, beta, cot_out, cot_max_out);

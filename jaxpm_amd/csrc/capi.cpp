@@ -112,6 +112,8 @@ int gaussian_smoothing(hipStream_t, int, int, const float *, float *, double, vo
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
 int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
                     uint32_t);
+int potential_solve_div(hipStream_t, int, int, int, int, const float *, size_t, float *, void *, void *, float *, float,
+                        float, const float *, uint32_t);
 int fd4_gradient(hipStream_t, int, int, int, const float *, float, float *);
 int fd4_divergence(hipStream_t, int, int, int, const float *, float, float *);
 
@@ -311,10 +313,8 @@ static int force_vjp_pot(hipStream_t s, int nx, int ny, int nz, const float *phi
   float *rho = reinterpret_cast<float *>(base + w.mesh);
   rc = bins_paint(s, b, cot, 1.0f, 3, g3, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint, cot_max_in);
   if (rc) return rc;
-  rc = fd4_divergence(s, nx, ny, nz, g3, 1.0f, rho);
-  if (rc) return rc;
-  rc = potential_solve(s, nx, ny, nz, 1, rho, base + w.spec, base + w.spec2, rho, r_split,
-                       float(double(scale) / double(m)), nullptr, kflags);
+  rc = potential_solve_div(s, nx, ny, nz, 1, g3, size_t(m), rho, base + w.spec, base + w.spec2, rho, r_split,
+                           float(double(scale) / double(m)), nullptr, kflags);
   if (rc) return rc;
   return bins_grad(s, b, phi, rho, cot, scale, gpos, tuned_rho_lds(flags), beta, cot_out, cot_max_out);
 }

@@ -313,28 +313,79 @@ fft_cols_potential_kernel(float2 *__restrict__ data, int nzh, size_t lstride, si
   }
 }
 
-// z pass of the R2C: rows r = blockIdx.x * KB + c (< nrows) of item
-// blockIdx.y, each nz = 2M reals read as M complex points.
+// The three channels of a vector mesh whose divergence the R2C reads in
+// place of a scalar mesh: channel d of item b at g3 + b * 2 * in_bstride +
+// d * chan floats; nx and ny powers of two, ysh = log2(ny).
+struct DivIn {
+  const float *g3;
+  size_t chan;
+  uint32_t nx, ny, ysh;
+};
+
+// sum_d D_d g_d at cells (x, y, 2l) and (x, y, 2l + 1) of a mesh with
+// nz = 2M: the fourth-order central difference (8 (g[+1] - g[-1]) -
+// (g[+2] - g[-2])) / 12 along each axis, periodic, in the operation order
+// of fd4_divergence_kernel with scale 1.  Each channel is read as float2
+// pairs along z.
 template <int M>
+__device__ __forceinline__ float2 divergence_pair(const float *__restrict__ g, size_t chan, uint32_t nx, uint32_t ny,
+                                                  uint32_t x, uint32_t y, int l) {
+  constexpr float c8 = 8.0f / 12.0f, c1 = 1.0f / 12.0f;
+  const float2 *gx = reinterpret_cast<const float2 *>(g);
+  const float2 *gy = reinterpret_cast<const float2 *>(g + chan);
+  const float2 *gz = reinterpret_cast<const float2 *>(g + 2 * chan);
+  const uint32_t xm = nx - 1u, ym = ny - 1u;
+  auto at = [&](const float2 *ch, uint32_t xi, uint32_t yi, int li) {
+    return ch[(size_t(xi) * ny + yi) * M + uint32_t(li)];
+  };
+  const float2 xp1 = at(gx, (x + 1u) & xm, y, l), xm1 = at(gx, (x + xm) & xm, y, l);
+  const float2 xp2 = at(gx, (x + 2u) & xm, y, l), xm2 = at(gx, (x + nx - 2u) & xm, y, l);
+  const float2 yp1 = at(gy, x, (y + 1u) & ym, l), ym1 = at(gy, x, (y + ym) & ym, l);
+  const float2 yp2 = at(gy, x, (y + 2u) & ym, l), ym2 = at(gy, x, (y + ny - 2u) & ym, l);
+  // z - 2, z - 1 | z, z + 1 | z + 2, z + 3
+  const float2 zl = at(gz, x, y, (l + M - 1) & (M - 1)), zc = at(gz, x, y, l), zr = at(gz, x, y, (l + 1) & (M - 1));
+  const float dx0 = c8 * (xp1.x - xm1.x) - c1 * (xp2.x - xm2.x), dx1 = c8 * (xp1.y - xm1.y) - c1 * (xp2.y - xm2.y);
+  const float dy0 = c8 * (yp1.x - ym1.x) - c1 * (yp2.x - ym2.x), dy1 = c8 * (yp1.y - ym1.y) - c1 * (yp2.y - ym2.y);
+  const float dz0 = c8 * (zc.y - zl.y) - c1 * (zr.x - zl.x), dz1 = c8 * (zr.x - zc.x) - c1 * (zr.y - zl.y);
+  return make_float2((dx0 + dy0) + dz0, (dx1 + dy1) + dz1);
+}
+
+// z pass of the R2C: rows r = blockIdx.x * KB + c (< nrows) of item
+// blockIdx.y, each nz = 2M reals read as M complex points.  With DIV the
+// reals are the divergence of the vector mesh dv, formed as they load.
+template <int M, bool DIV>
 __global__ void __launch_bounds__(kFftThreads)
 fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int nrows, size_t in_bstride,
-                    size_t out_bstride, const float2 *__restrict__ tw_all) {
+                    size_t out_bstride, const float2 *__restrict__ tw_all, DivIn dv) {
   constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB;
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
   load_twiddles<M>(twM, tw_all);
   const float2 *twN = tw_all + (2 * M - 2);
   const int r0 = blockIdx.x * KB;
-  const float2 *src = in + size_t(blockIdx.y) * in_bstride;
   float2 *dst = out + size_t(blockIdx.y) * out_bstride;
   // rows are contiguous: lanes along l
   constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+  if constexpr (DIV) {
+    const float *g = dv.g3 + size_t(blockIdx.y) * 2 * in_bstride;
 #pragma unroll
-  for (int it = 0; it < iters; ++it) {
-    const int f = threadIdx.x + it * kFftThreads;
-    if (total % kFftThreads != 0 && f >= total) break;
-    const int l = f & (M - 1), c = f / M;
-    buf[l * PITCH + c] = r0 + c < nrows ? src[size_t(r0 + c) * M + l] : make_float2(0.0f, 0.0f);
+    for (int it = 0; it < iters; ++it) {
+      const int f = threadIdx.x + it * kFftThreads;
+      if (total % kFftThreads != 0 && f >= total) break;
+      const int l = f & (M - 1), c = f / M;
+      const uint32_t r = uint32_t(r0 + c);
+      buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, r >> dv.ysh, r & (dv.ny - 1u), l)
+                                          : make_float2(0.0f, 0.0f);
+    }
+  } else {
+    const float2 *src = in + size_t(blockIdx.y) * in_bstride;
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+      const int f = threadIdx.x + it * kFftThreads;
+      if (total % kFftThreads != 0 && f >= total) break;
+      const int l = f & (M - 1), c = f / M;
+      buf[l * PITCH + c] = r0 + c < nrows ? src[size_t(r0 + c) * M + l] : make_float2(0.0f, 0.0f);
+    }
   }
   __syncthreads();
   lds_fft<M, KB, PITCH, false>(buf, twM);
@@ -432,17 +483,21 @@ static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int 
   return JPM_OK;
 }
 
+// The R2C's z pass of in, or with dv.g3 set of the divergence of dv's
+// vector mesh.
 static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, float2 *out,
-                    const float2 *tw) {
+                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0, 0}) {
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
   const float2 *i2 = reinterpret_cast<const float2 *>(in);
   switch (M) {
 #define JPM_R2C(N)                                                                                            \
-  case N:                                                                                                     \
-    hipLaunchKernelGGL(fft_r2c_rows_kernel<N>, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch),    \
-                       dim3(kFftThreads), 0, s, i2, out, nrows, Mc, H, tw);                                   \
-    break;
+  case N: {                                                                                                   \
+    auto kern = dv.g3 ? fft_r2c_rows_kernel<N, true> : fft_r2c_rows_kernel<N, false>;                         \
+    hipLaunchKernelGGL(kern, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch), dim3(kFftThreads), 0,  \
+                       s, i2, out, nrows, Mc, H, tw, dv);                                                     \
+    break;                                                                                                    \
+  }
     JPM_FFT_LENGTHS(JPM_R2C)
 #undef JPM_R2C
     default:
@@ -501,9 +556,12 @@ int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float
 // out = C2R(scale G . R2C(in)) for a batch of meshes, the spectrum in spec
 // (H complex per mesh): R2C rows and y columns, the x columns with the
 // multiplier between their forward and inverse transforms, inverse y
-// columns, C2R rows.  in and out may be the same array.
+// columns, C2R rows.  in and out may be the same array.  With g3 the input
+// is instead sum_d D_d g_d of the vector meshes at g3 (channels chan floats
+// apart, items nx ny nz floats apart), formed by the R2C rows as they load.
 int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
-                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv) {
+                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv, const float *g3,
+                     size_t chan) {
   JPM_REQUIRE(native_fft_ok(nx, ny, nz) && nx >= 2 && batch >= 1 && batch <= 65535, "native FFT shape");
   const float2 *tw;
   int rc = twiddles(&tw);
@@ -511,7 +569,8 @@ int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const flo
   const int nzh = nz / 2 + 1;
   const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(spec);
-  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw))) return rc;
+  DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny), uint32_t(__builtin_ctz(uint32_t(ny)))};
+  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv))) return rc;
   if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
   switch (nx) {
 #define JPM_POT(N)                                                                                        \

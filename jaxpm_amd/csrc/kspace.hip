@@ -499,7 +499,10 @@ int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, v
 int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *out);
 bool native_fft_ok(int nx, int ny, int nz);
 int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
-                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv);
+                     const Tabs &t, float r2, float scale, const float *scale_dev, int deconv,
+                     const float *g3 = nullptr, size_t chan = 0);
+int fd4_divergence_batch(hipStream_t s, int nx, int ny, int nz, int batch, const float *g3, size_t chan_stride,
+                         float scale, float *out);
 
 // out = C2R(potential(R2C(in))) for a batch of equal meshes (in and out may
 // be the same array): the forward transform, potential_kernel's multiplier
@@ -524,6 +527,32 @@ int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const floa
   if ((rc = fft_r2c(s, nx, ny, nz, batch, in, spec))) return rc;
   if ((rc = kspace_potential_batch(s, nx, ny, nz, batch, spec, spec2, r_split, scale, scale_dev, flags))) return rc;
   return fft_c2r(s, nx, ny, nz, batch, spec2, out);
+}
+
+// out = C2R(potential(R2C(sum_d D_d g_d))) for a batch of equal meshes: the
+// force VJP's density adjoint from the painted cotangent g (channel d of
+// item b at g3 + d chan + b nx ny nz).  Power-of-two meshes form the
+// divergence inside the R2C's first pass; other shapes run
+// fd4_divergence_kernel into rho (which may be out), then potential_solve.
+int potential_solve_div(hipStream_t s, int nx, int ny, int nz, int batch, const float *g3, size_t chan, float *rho,
+                        void *spec, void *spec2, float *out, float r_split, float scale, const float *scale_dev,
+                        uint32_t flags) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(g3 && rho && spec && spec2 && out, "null pointer argument");
+  JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d meshes", batch);
+  const bool aligned =
+      ((reinterpret_cast<uintptr_t>(g3) | reinterpret_cast<uintptr_t>(out) | uintptr_t(chan * sizeof(float))) & 7u) == 0;
+  if (native_fft_ok(nx, ny, nz) && nx >= 2 && aligned) {
+    Tabs t;
+    rc = get_tabs(s, nx, ny, nz, flags, t);
+    if (rc) return rc;
+    JPM_TIMED(s, "fft_potential[div]");
+    return native_potential(s, nx, ny, nz, batch, nullptr, spec, out, t, r_split * r_split, scale, scale_dev,
+                            int((flags & JPM_K_DECONV) != 0), g3, chan);
+  }
+  if ((rc = fd4_divergence_batch(s, nx, ny, nz, batch, g3, chan, 1.0f, rho))) return rc;
+  return potential_solve(s, nx, ny, nz, batch, rho, spec, spec2, out, r_split, scale, scale_dev, flags);
 }
 
 int kspace_potential(hipStream_t s, int nx, int ny, int nz, const void *dk, void *out, float r_split, float scale,
