@@ -372,7 +372,9 @@ size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n,
  * vel[p] += gamma * read(-D phi, pos[p]) with the 4-point stencil D applied
  * in LDS (write_forces != 0: vel[p] = gamma * forces instead).  pm.py:12-58
  * fused with the kick of ode.py:142-143.  phi is what jpm_force_vjp_pot_f32
- * needs for the adjoint of this evaluation. */
+ * needs for the adjoint of this evaluation.  vel = NULL (with write_forces
+ * = 0) stops after phi and the tile sort: the force evaluation alone, for an
+ * adjoint that needs its data but not its kick. */
 int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos,
                        int64_t n, float *vel, float gamma, float *phi,
                        float r_split, uint32_t flags, int write_forces,
@@ -565,7 +567,9 @@ size_t jpm_batch_dot_workspace_size(const void *batch);
  * simulation's potential, vel_b += gamma[b] F_b (or = gamma[b] F_b with
  * write_forces); gamma is a device array [nsims].  keep_sort (nullable)
  * receives the tile sort for jpm_batch_force_vjp_f32.  Replaces the
- * pm_forces (pm.py:12-58) + kick (ode.py:143) of each simulation. */
+ * pm_forces (pm.py:12-58) + kick (ode.py:143) of each simulation.  vel =
+ * NULL (with write_forces = 0) stops after phi and the sort, as in
+ * jpm_force_kick_f32; gamma may then be NULL too. */
 int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos,
                              float *vel, const float *gamma, float *phi,
                              float r_split, uint32_t kflags, int write_forces,

@@ -172,7 +172,8 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
   JPM_REQUIRE(!xout || (beta && !write_forces), "the fused drift needs beta and a kick");
   JPM_REQUIRE(!counts_out || (xout && !bt.rel), "tile counts of drifted absolute positions only");
   if (int rc = check_kflags(kflags)) return rc;
-  JPM_REQUIRE(pos && vel && phi && gamma, "null pointer argument");
+  JPM_REQUIRE(pos && phi && (gamma || !vel), "null pointer argument");
+  JPM_REQUIRE(vel || !(write_forces || xout), "the force write and the fused drift need vel");
   const BatchWs w = batch_ws(bt, 1);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "batched force_kick needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   char *base = static_cast<char *>(ws);
@@ -195,6 +196,8 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
                               phi + c.cell0, r_split, inv_m, nullptr, kflags)))
       return rc;
   }
+  // without vel the call is the force evaluation alone: potentials and sort
+  if (!vel) return JPM_OK;
   return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out);
 }
 

@@ -222,7 +222,8 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   const uint32_t pflags = paint_flags(flags), kflags = kspace_flags(flags);
   JPM_REQUIRE(sorted_supported(nx, ny, nz, flags), "the sorted pipeline needs the 4-point gradient, the binned paint and 3 meshes within 2^32 cells");
   JPM_REQUIRE(n >= 1, "no particles");
-  JPM_REQUIRE(pos && vel && phi, "null pointer argument");
+  JPM_REQUIRE(pos && phi, "null pointer argument");
+  JPM_REQUIRE(vel || !(write_forces || xout), "the force write and the fused drift need vel");
   const SortedWs w = sorted_ws(nx, ny, nz, n, 1, pflags);
   JPM_REQUIRE(ws && ws_bytes >= w.total, "force_kick needs %zu workspace bytes, got %zu", w.total, ws_bytes);
   char *base = static_cast<char *>(ws);
@@ -242,6 +243,8 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   rc = potential_solve(s, nx, ny, nz, 1, mesh, base + w.spec, base + w.spec2, phi, r_split, float(1.0 / double(m)),
                        nullptr, kflags);
   if (rc) return rc;
+  // without vel the call is the force evaluation alone: potential and sort
+  if (!vel) return JPM_OK;
   return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out);
 }
 

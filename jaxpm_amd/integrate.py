@@ -127,7 +127,8 @@ class Stepper:
 
     def field_kick(self, x, v, gamma, field):
         """v += gamma F(x); ``field`` (from new_field) receives what the
-        adjoint of this evaluation needs."""
+        adjoint of this evaluation needs.  ``v`` None: the force evaluation
+        alone (its field, no kick)."""
         buf, srt = field
         if self.sorted:
             self._timed("field_kick", "jpm_force_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
@@ -135,7 +136,8 @@ class Stepper:
                         self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
         else:
             self.field(x, buf)
-            self.kick(buf, x, v, gamma)
+            if v is not None:
+                self.kick(buf, x, v, gamma)
 
     def forces(self, x, out, field):
         """out = F(x) (the force field_kick would kick with); ``field``
@@ -268,7 +270,8 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
         stepper.field_kick(x, v, gamma[i], phi)
 
 
-def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=None, out=None, chain=False):
+def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=None, out=None, chain=False,
+               kick_last=True):
     """Kicks [start, stop) of the fused form.  On entry ``y`` holds y_start
     (the positions kick ``start`` evaluates its force at, i.e. after drift
     ``start``) and is only read; v is the velocity before that kick and is
@@ -282,7 +285,9 @@ def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=Non
     buffers the drifts write, never copies.  Returns the buffer of the
     positions the segment's last kick evaluated.  ``chain``: y is the output
     of the stepper's previous drift (its tile counts are kept); every later
-    kick of the segment follows a drift of its own."""
+    kick of the segment follows a drift of its own.  Without ``kick_last`` a
+    last kick that drifts nowhere is the force evaluation alone (an adjoint's
+    recompute needs its field, not its velocity)."""
     S = len(beta)
     cur = y
     for i in range(start, stop):
@@ -293,7 +298,7 @@ def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=Non
             if i + 1 < stop:
                 cur = nxt
         else:
-            stepper.field_kick(cur, v, gamma[i], field)
+            stepper.field_kick(cur, v if kick_last or i + 1 < stop else None, gamma[i], field)
     return cur
 
 
@@ -452,7 +457,7 @@ class NBody(torch.autograd.Function):
             y, v = ctx.ckpts.pop(s0)
             if s0 != last:
                 # each checkpoint is restored once: recompute in place
-                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep)
+                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, kick_last=False)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
                 st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, gamma[i], beta[i], lx, chain=i < S - 1)
@@ -573,7 +578,7 @@ class BatchStepper:
 
     def field_kick(self, x, v, gamma, field):
         """v_b += gamma[b] F_b(x_b) for every simulation; gamma a float32
-        device array [B]."""
+        device array [B].  ``v`` None: the force evaluations alone."""
         self._kick(x, v, gamma, field, False)
 
     def forces(self, x, out, field, ones):
@@ -794,7 +799,8 @@ class BatchLeapfrog:
                 if s0 != self.last:
                     # each checkpoint is read once per forward: recompute in
                     # place on its velocity
-                    _run_fused(st, y, self.ckpts[s0][1], sc.beta, sc.gamma, s0, s1, self.phi, self.keep)
+                    _run_fused(st, y, self.ckpts[s0][1], sc.beta, sc.gamma, s0, s1, self.phi, self.keep,
+                               kick_last=False)
                 for i in reversed(range(s0, s1)):
                     kp = self.keep[i - s0]
                     st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, sc.gamma[i], sc.gamma_pot[i],

@@ -33,6 +33,7 @@ class MockStepper:
     def __init__(self, reorder):
         self.reorder = reorder
         self.vjps = 0
+        self.field_only = 0
 
     def tile_order(self, x):
         return torch.argsort(x[:, 0] + 0.37 * x[:, 1]).to(torch.int32)
@@ -54,7 +55,10 @@ class MockStepper:
 
     def field_kick(self, x, v, gamma, field):
         field[0] = x.clone()
-        v += float(gamma) * force(x)
+        if v is None:
+            self.field_only += 1
+        else:
+            v += float(gamma) * force(x)
 
     def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
         self.field_kick(x, v, gamma, field)
@@ -104,6 +108,9 @@ def test_nbody_buffers_and_adjoint(reorder, S, every):
     x, v = integrate.NBody.apply(a, b, coeffs, st, every)
     ((x * gx).sum() + (v * gv).sum()).backward()
     assert st.vjps == S + 1
+    # one force evaluation without its kick per recomputed segment
+    k = S if every is None else every
+    assert st.field_only == len(range(0, S, k)) - 1
     ra, rb = x0.clone().requires_grad_(True), v0.clone().requires_grad_(True)
     xr, vr = plain_leapfrog(ra, rb, k0, beta, gamma)
     ((xr * gx).sum() + (vr * gv).sum()).backward()
@@ -139,7 +146,10 @@ class MockBatchStepper(MockStepper):
 
     def field_kick(self, x, v, gamma, field):
         field[0] = x.clone()
-        v += gamma * force(x)
+        if v is None:
+            self.field_only += 1
+        else:
+            v += gamma * force(x)
 
     def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
         self.field_kick(x, v, gamma, field)
