@@ -24,8 +24,18 @@ constexpr int kFftThreads = 512;
 constexpr int kFftPoints = 4096;   // complex points per workgroup
 constexpr int kFftMaxL = 1024;
 
-// exp(-2 pi i e / L) for e in [0, L), L = 2 .. 2048 (powers of two), each
-// table at offset L - 2 of one device buffer.
+// The transform lengths: powers of two 2 .. 1024 and 3 * 2^k, 6 .. 768.
+constexpr bool fft_len(int L) {
+  return L >= 2 && (((L & (L - 1)) == 0 && L <= kFftMaxL) ||
+                    (L % 3 == 0 && ((L / 3) & (L / 3 - 1)) == 0 && L <= 3 * kFftMaxL / 4));
+}
+
+// exp(-2 pi i e / L) for e in [0, L), one table per length L = 2 .. 2048
+// (powers of two, at offset L - 2 of one device buffer) and L = 6 .. 1536
+// (3 * 2^k, at offset kTwPow2 + L - 6 after them).
+constexpr int kTwPow2 = 4 * kFftMaxL - 2;
+constexpr int tw_off(int L) { return (L & (L - 1)) == 0 ? L - 2 : kTwPow2 + (L - 6); }
+
 static std::mutex g_tw_mu;
 static float2 *g_tw = nullptr;
 
@@ -33,11 +43,14 @@ static int twiddles(const float2 **out) {
   std::lock_guard<std::mutex> lk(g_tw_mu);
   if (!g_tw) {
     std::vector<float2> h;
-    for (int L = 2; L <= 2 * kFftMaxL; L *= 2)
+    auto table = [&](int L) {
       for (int e = 0; e < L; ++e) {
         const double a = -2.0 * M_PI * double(e) / double(L);
         h.push_back(make_float2(float(std::cos(a)), float(std::sin(a))));
       }
+    };
+    for (int L = 2; L <= 2 * kFftMaxL; L *= 2) table(L);
+    for (int L = 6; L <= 3 * kFftMaxL / 2; L *= 2) table(L);
     float *buf = nullptr;
     JPM_HIP(hipMalloc(&buf, h.size() * sizeof(float2)));
     const hipError_t e = hipMemcpy(buf, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
@@ -95,11 +108,20 @@ __device__ __forceinline__ float2 w16(float2 a) {
   }
 }
 
-// R-point DFT of a[0..R) in place, natural order, R = 2, 4, 8 or 16:
+// R-point DFT of a[0..R) in place, natural order, R = 2, 3, 4, 8 or 16:
 // a length-R transform as (R/4 or 2) x 4 with the W_R twiddles between.
 template <int R, bool INV>
 __device__ __forceinline__ void dftR(float2 *a) {
-  if constexpr (R == 2) {
+  if constexpr (R == 3) {
+    // X1, X2 = a0 - (a1 + a2)/2 -+ i sin(2 pi / 3) (a1 - a2) (forward)
+    constexpr float s3 = 0.86602540378443865f;
+    const float2 t = cadd(a[1], a[2]), d = csub(a[1], a[2]);
+    const float2 m = make_float2(a[0].x - 0.5f * t.x, a[0].y - 0.5f * t.y);
+    const float2 r = INV ? make_float2(-s3 * d.y, s3 * d.x) : make_float2(s3 * d.y, -s3 * d.x);   // d * (+-i) s3
+    a[0] = cadd(a[0], t);
+    a[1] = cadd(m, r);
+    a[2] = csub(m, r);
+  } else if constexpr (R == 2) {
     const float2 b0 = cadd(a[0], a[1]), b1 = csub(a[0], a[1]);
     a[0] = b0;
     a[1] = b1;
@@ -147,9 +169,9 @@ __device__ __forceinline__ void dftR(float2 *a) {
   }
 }
 
-// Largest radix of the next stage: 16 while at least 16 points remain, then
-// what is left (8, 4 or 2).
-constexpr int stage_radix(int rem) { return rem >= 16 ? 16 : rem; }
+// Radix of the next stage: 3 while the points left hold a factor 3, then
+// 16 while at least 16 remain, then what is left (8, 4 or 2).
+constexpr int stage_radix(int rem) { return rem % 3 == 0 ? 3 : rem >= 16 ? 16 : rem; }
 
 // FFTs of KB columns of length L held in LDS as buf[l * PITCH + c]
 // (c < KB), in place; L, KB and PITCH are compile-time, so every index is
@@ -170,14 +192,14 @@ __device__ __forceinline__ void fft_stage(float2 *buf, const float2 *tw) {
   for (int it = 0; it < per; ++it) {
     const int f = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads == 0 || f < total) {
-      const int c = f & (KB - 1), j = f / KB, k = j & (NS - 1);
+      const int c = f & (KB - 1), j = f / KB, k = j % NS;
       float2 a[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) a[r] = buf[(j + r * nb) * PITCH + c];
       if constexpr (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[(r * k * step) & (L - 1)];
+          float2 w = tw[r * k * step];   // r k step < R NS step = L
           if (INV) w = conjf2(w);
           a[r] = cmul(a[r], w);
         }
@@ -212,12 +234,14 @@ __device__ __forceinline__ void lds_fft(float2 *buf, const float2 *tw) {
 // orders it before the first stage that reads it.
 template <int L>
 __device__ __forceinline__ void load_twiddles(float2 *tws, const float2 *__restrict__ tw_all) {
-  for (int e = threadIdx.x; e < L; e += kFftThreads) tws[e] = tw_all[(L - 2) + e];
+  for (int e = threadIdx.x; e < L; e += kFftThreads) tws[e] = tw_all[tw_off(L) + e];
 }
 
+// Columns per workgroup: the largest power of two <= min(64, 4096 / L).
+constexpr int pow2_floor(int v) { return v < 2 ? 1 : 2 * pow2_floor(v / 2); }
 template <int L>
 struct FftCols {
-  static constexpr int KB = (kFftPoints / L) > 64 ? 64 : (kFftPoints / L);
+  static constexpr int KB = pow2_floor((kFftPoints / L) > 64 ? 64 : (kFftPoints / L));
 };
 
 // Column FFTs along one axis of a half spectrum, in place: column c of
@@ -315,11 +339,11 @@ fft_cols_potential_kernel(float2 *__restrict__ data, int nzh, size_t lstride, si
 
 // The three channels of a vector mesh whose divergence the R2C reads in
 // place of a scalar mesh: channel d of item b at g3 + b * 2 * in_bstride +
-// d * chan floats; nx and ny powers of two, ysh = log2(ny).
+// d * chan floats.
 struct DivIn {
   const float *g3;
   size_t chan;
-  uint32_t nx, ny, ysh;
+  uint32_t nx, ny;
 };
 
 // sum_d D_d g_d at cells (x, y, 2l) and (x, y, 2l + 1) of a mesh with
@@ -334,16 +358,16 @@ __device__ __forceinline__ float2 divergence_pair(const float *__restrict__ g, s
   const float2 *gx = reinterpret_cast<const float2 *>(g);
   const float2 *gy = reinterpret_cast<const float2 *>(g + chan);
   const float2 *gz = reinterpret_cast<const float2 *>(g + 2 * chan);
-  const uint32_t xm = nx - 1u, ym = ny - 1u;
   auto at = [&](const float2 *ch, uint32_t xi, uint32_t yi, int li) {
     return ch[(size_t(xi) * ny + yi) * M + uint32_t(li)];
   };
-  const float2 xp1 = at(gx, (x + 1u) & xm, y, l), xm1 = at(gx, (x + xm) & xm, y, l);
-  const float2 xp2 = at(gx, (x + 2u) & xm, y, l), xm2 = at(gx, (x + nx - 2u) & xm, y, l);
-  const float2 yp1 = at(gy, x, (y + 1u) & ym, l), ym1 = at(gy, x, (y + ym) & ym, l);
-  const float2 yp2 = at(gy, x, (y + 2u) & ym, l), ym2 = at(gy, x, (y + ny - 2u) & ym, l);
+  // periodic neighbours (nx, ny >= 2)
+  const float2 xp1 = at(gx, (x + 1u) % nx, y, l), xm1 = at(gx, (x + nx - 1u) % nx, y, l);
+  const float2 xp2 = at(gx, (x + 2u) % nx, y, l), xm2 = at(gx, (x + nx - 2u) % nx, y, l);
+  const float2 yp1 = at(gy, x, (y + 1u) % ny, l), ym1 = at(gy, x, (y + ny - 1u) % ny, l);
+  const float2 yp2 = at(gy, x, (y + 2u) % ny, l), ym2 = at(gy, x, (y + ny - 2u) % ny, l);
   // z - 2, z - 1 | z, z + 1 | z + 2, z + 3
-  const float2 zl = at(gz, x, y, (l + M - 1) & (M - 1)), zc = at(gz, x, y, l), zr = at(gz, x, y, (l + 1) & (M - 1));
+  const float2 zl = at(gz, x, y, (l + M - 1) % M), zc = at(gz, x, y, l), zr = at(gz, x, y, (l + 1) % M);
   const float dx0 = c8 * (xp1.x - xm1.x) - c1 * (xp2.x - xm2.x), dx1 = c8 * (xp1.y - xm1.y) - c1 * (xp2.y - xm2.y);
   const float dy0 = c8 * (yp1.x - ym1.x) - c1 * (yp2.x - ym2.x), dy1 = c8 * (yp1.y - ym1.y) - c1 * (yp2.y - ym2.y);
   const float dz0 = c8 * (zc.y - zl.y) - c1 * (zr.x - zl.x), dz1 = c8 * (zr.x - zc.x) - c1 * (zr.y - zl.y);
@@ -361,7 +385,7 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
   load_twiddles<M>(twM, tw_all);
-  const float2 *twN = tw_all + (2 * M - 2);
+  const float2 *twN = tw_all + tw_off(2 * M);
   const int r0 = blockIdx.x * KB;
   float2 *dst = out + size_t(blockIdx.y) * out_bstride;
   // rows are contiguous: lanes along l
@@ -372,9 +396,9 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
     for (int it = 0; it < iters; ++it) {
       const int f = threadIdx.x + it * kFftThreads;
       if (total % kFftThreads != 0 && f >= total) break;
-      const int l = f & (M - 1), c = f / M;
+      const int l = f % M, c = f / M;
       const uint32_t r = uint32_t(r0 + c);
-      buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, r >> dv.ysh, r & (dv.ny - 1u), l)
+      buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, r / dv.ny, r % dv.ny, l)
                                           : make_float2(0.0f, 0.0f);
     }
   } else {
@@ -383,7 +407,7 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
     for (int it = 0; it < iters; ++it) {
       const int f = threadIdx.x + it * kFftThreads;
       if (total % kFftThreads != 0 && f >= total) break;
-      const int l = f & (M - 1), c = f / M;
+      const int l = f % M, c = f / M;
       buf[l * PITCH + c] = r0 + c < nrows ? src[size_t(r0 + c) * M + l] : make_float2(0.0f, 0.0f);
     }
   }
@@ -395,8 +419,8 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   for (int f = threadIdx.x; f < tot2; f += kFftThreads) {
     const int c = f / nzh, k = f - c * nzh;
     if (r0 + c >= nrows) continue;
-    const float2 z = buf[(k & (M - 1)) * PITCH + c];
-    const float2 zm = buf[((M - k) & (M - 1)) * PITCH + c];
+    const float2 z = buf[(k % M) * PITCH + c];
+    const float2 zm = buf[((M - k) % M) * PITCH + c];
     const float2 E = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
     const float2 O = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
     dst[size_t(r0 + c) * nzh + k] = cadd(E, cmul(twN[k], O));
@@ -412,7 +436,7 @@ fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
   load_twiddles<M>(twM, tw_all);
-  const float2 *twN = tw_all + (2 * M - 2);
+  const float2 *twN = tw_all + tw_off(2 * M);
   const int r0 = blockIdx.x * KB;
   const float2 *src = in + size_t(blockIdx.y) * in_bstride;
   float2 *dst = out + size_t(blockIdx.y) * out_bstride;
@@ -423,7 +447,7 @@ fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   for (int it = 0; it < iters; ++it) {
     const int f = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads != 0 && f >= total) break;
-    const int k = f & (M - 1), c = f / M;
+    const int k = f % M, c = f / M;
     float2 z = make_float2(0.0f, 0.0f);
     if (r0 + c < nrows) {
       const float2 *row = src + size_t(r0 + c) * nzh;
@@ -445,22 +469,22 @@ fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   for (int it = 0; it < iters; ++it) {
     const int f = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads != 0 && f >= total) break;
-    const int l = f & (M - 1), c = f / M;
+    const int l = f % M, c = f / M;
     if (r0 + c < nrows) dst[size_t(r0 + c) * M + l] = buf[l * PITCH + c];
   }
 }
 
-static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// The native transforms cover power-of-two meshes with 1 <= nx <= 1024, 2 <=
-// ny <= 1024 and 4 <= nz <= 2048 (nx = 1: 2-D images, no x pass).
+// The native transforms cover meshes whose nx and ny are transform lengths
+// (powers of two 2 .. 1024 or 3 * 2^k 6 .. 768; nx = 1 too: 2-D images, no
+// x pass) and whose nz is twice one (the R2C's complex rows of nz / 2).
 bool native_fft_ok(int nx, int ny, int nz) {
-  return pow2(nx) && pow2(ny) && pow2(nz) && ny >= 2 && nz >= 4 && nx <= kFftMaxL && ny <= kFftMaxL &&
-         nz <= 2 * kFftMaxL;
+  return (nx == 1 || fft_len(nx)) && fft_len(ny) && nz % 2 == 0 && fft_len(nz / 2);
 }
 
-// Launch helpers over the compile-time lengths 2 .. 1024.
-#define JPM_FFT_LENGTHS(X) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024)
+// Launch helpers over the compile-time lengths.
+#define JPM_FFT_LENGTHS(X) \
+  X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(6) X(12) X(24) X(48) X(96) X(192) X(384) X(768)
 
 static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int outer, size_t lstride,
                      size_t ostride, size_t bstride, int batch, const float2 *tw) {
@@ -486,7 +510,7 @@ static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int 
 // The R2C's z pass of in, or with dv.g3 set of the divergence of dv's
 // vector mesh.
 static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, float2 *out,
-                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0, 0}) {
+                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0}) {
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
   const float2 *i2 = reinterpret_cast<const float2 *>(in);
@@ -569,7 +593,7 @@ int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const flo
   const int nzh = nz / 2 + 1;
   const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(spec);
-  DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny), uint32_t(__builtin_ctz(uint32_t(ny)))};
+  DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny)};
   if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv))) return rc;
   if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
   switch (nx) {

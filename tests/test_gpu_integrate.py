@@ -26,9 +26,12 @@ def _state(shape, seed):
     return x0, v0
 
 
-def test_force_vjp_matches_oracle(gpu):
+@pytest.mark.parametrize("shape", [(16, 16, 16), (24, 12, 48)])
+def test_force_vjp_matches_oracle(gpu, shape):
+    """The sorted pipeline's force VJP (paint, divergence inside the R2C,
+    potential, gather) against the oracle; (24, 12, 48) runs radix-3
+    transforms."""
     from jaxpm_amd import integrate
-    shape = (16, 16, 16)
     x, _ = _state(shape, 30)
     u = np.random.default_rng(31).normal(size=x.shape).astype(np.float32)
     st = integrate.Stepper(shape, x.shape[0])
@@ -72,7 +75,7 @@ def test_leapfrog_adjoint(gpu, checkpoint_every):
     assert rel_err(vt.grad.cpu().numpy(), lv) < 2e-4
 
 
-SORTED_SHAPES = [(16, 16, 16), (8, 8, 8), (17, 33, 20), (32, 32, 64)]
+SORTED_SHAPES = [(16, 16, 16), (8, 8, 8), (17, 33, 20), (32, 32, 64), (24, 12, 48)]
 
 
 @pytest.mark.parametrize("shape", SORTED_SHAPES)

@@ -45,11 +45,14 @@ def test_fft_transforms(gpu, shape):
 
 
 @pytest.mark.parametrize("shape,batch", [((2, 2, 4), 1), ((2, 4, 2048), 1), ((1024, 2, 8), 1), ((4, 1024, 4), 1),
-                                         ((64, 32, 128), 1), ((8, 16, 32), 3)])
-def test_power_of_two_fft_shapes(gpu, shape, batch):
-    """Power-of-two meshes take the library's own transforms: each axis
-    length from 2 to 1024 (2048 along z), batched, against numpy in float64,
-    and the inverse of an arbitrary half spectrum against numpy's irfftn."""
+                                         ((64, 32, 128), 1), ((8, 16, 32), 3), ((96, 48, 24), 1), ((6, 12, 12), 2),
+                                         ((768, 6, 12), 1), ((12, 768, 1536), 1), ((24, 384, 192), 1),
+                                         ((96, 96, 96), 2), ((1, 96, 192), 1)])
+def test_native_fft_shapes(gpu, shape, batch):
+    """Meshes whose axes are powers of two (2 to 1024, 2048 along z) or
+    3 * 2^k (6 to 768, 1536 along z) take the library's own transforms,
+    radix-3 stages included: batched, against numpy in float64, and the
+    inverse of an arbitrary half spectrum against numpy's irfftn."""
     from jaxpm_amd import _ops
     print(f"fft {shape} batch={batch}", flush=True)
     r = np.random.default_rng(27)
