@@ -374,12 +374,15 @@ size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n,
  * fused with the kick of ode.py:142-143.  phi is what jpm_force_vjp_pot_f32
  * needs for the adjoint of this evaluation.  vel = NULL (with write_forces
  * = 0) stops after phi and the tile sort: the force evaluation alone, for an
- * adjoint that needs its data but not its kick. */
+ * adjoint that needs its data but not its kick.  counts_in (nullable):
+ * per-tile particle counts of pos from the drift that wrote it
+ * (jpm_force_kick_drift_f32's counts_out), which replace the sort's
+ * counting pass (absolute positions only). */
 int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos,
                        int64_t n, float *vel, float gamma, float *phi,
                        float r_split, uint32_t flags, int write_forces,
                        void *ws, size_t ws_bytes, void *keep_sort,
-                       size_t keep_bytes);
+                       size_t keep_bytes, const uint32_t *counts_in);
 
 /* Bytes of a tile sort of n particles under the pipeline `flags` (their
  * sort tuning bits): the `keep_sort` buffer of jpm_force_kick_f32 (optional;
@@ -569,12 +572,13 @@ size_t jpm_batch_dot_workspace_size(const void *batch);
  * receives the tile sort for jpm_batch_force_vjp_f32.  Replaces the
  * pm_forces (pm.py:12-58) + kick (ode.py:143) of each simulation.  vel =
  * NULL (with write_forces = 0) stops after phi and the sort, as in
- * jpm_force_kick_f32; gamma may then be NULL too. */
+ * jpm_force_kick_f32; gamma may then be NULL too.  counts_in (nullable): the
+ * batch's per-tile counts of pos from the drift that wrote it. */
 int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos,
                              float *vel, const float *gamma, float *phi,
                              float r_split, uint32_t kflags, int write_forces,
                              void *ws, size_t ws_bytes, void *keep_sort,
-                             size_t keep_bytes);
+                             size_t keep_bytes, const uint32_t *counts_in);
 
 /* jpm_force_vjp_pot_f32 for every simulation: gpos_b += scale[b] J_b^T cot_b
  * given the evaluation's phi (and kept sort).  pot_scale[b] must be

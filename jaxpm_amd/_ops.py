@@ -329,7 +329,7 @@ def pm_forces_fused(pos, shape, delta_k=None, r_split=0.0, scale=1.0, relative=F
                 ws = torch.empty(kb, dtype=torch.uint8, device=pos.device)
             phi = torch.empty((nx, ny, nz), dtype=torch.float32, device=pos.device)
             call("jpm_force_kick_f32", stream(), nx, ny, nz, ptr(pos), n, ptr(out), float(scale), ptr(phi),
-                 float(r_split), flags, 1, ptr(ws), ws.numel(), None, 0)
+                 float(r_split), flags, 1, ptr(ws), ws.numel(), None, 0, None)
             return out
     wsb = size("jpm_pm_forces_workspace_size", nx, ny, nz, n, flags)
     if ws is None or ws.numel() < wsb:

@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kTableThreads)
 tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk, uint32_t *__restrict__ tstart,
                   uint32_t *__restrict__ cursor, uint32_t *__restrict__ nchunk, uint32_t *__restrict__ choff,
                   uint32_t *__restrict__ totals, ChunkDesc *__restrict__ desc) {
-  __shared__ uint32_t sp[kTableThreads], sc[kTableThreads];
+  __shared__ uint32_t sp[kTableThreads / 64], sc[kTableThreads / 64];
   const uint32_t t = threadIdx.x;
   const uint32_t per = (K + kTableThreads - 1) / kTableThreads;
   const uint32_t k0 = min(K, t * per), k1 = min(K, k0 + per);
@@ -275,20 +275,31 @@ tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk
     np += c;
     nc += (c + chunk - 1) / chunk;
   }
-  sp[t] = np;
-  sc[t] = nc;
-  __syncthreads();
-  for (uint32_t off = 1; off < kTableThreads; off <<= 1) {
-    const uint32_t x = t >= off ? sp[t - off] : 0u, y = t >= off ? sc[t - off] : 0u;
-    __syncthreads();
-    sp[t] += x;
-    sc[t] += y;
-    __syncthreads();
+  // inclusive block scan of (np, nc): within each wavefront by shuffles,
+  // then across the wavefronts' totals, one barrier
+  const uint32_t lane = t & 63u, wid = t >> 6;
+  uint32_t xp = np, xc = nc;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t yp = __shfl_up(xp, off), yc = __shfl_up(xc, off);
+    if (lane >= uint32_t(off)) {
+      xp += yp;
+      xc += yc;
+    }
   }
-  uint32_t pb = sp[t] - np, cb = sc[t] - nc;
+  if (lane == 63u) {
+    sp[wid] = xp;
+    sc[wid] = xc;
+  }
+  __syncthreads();
+  for (uint32_t w = 0; w < wid; ++w) {
+    xp += sp[w];
+    xc += sc[w];
+  }
+  uint32_t pb = xp - np, cb = xc - nc;
   if (t == kTableThreads - 1) {
-    totals[0] = sp[t];
-    totals[1] = sc[t];
+    totals[0] = xp;
+    totals[1] = xc;
   }
   for (uint32_t k = k0; k < k1; ++k) {
     const uint32_t cnt = count[k], ncnk = (cnt + chunk - 1) / chunk;

@@ -686,10 +686,10 @@ size_t jpm_batch_dot_workspace_size(const void *batch) { return batch ? batch_do
 
 int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos, float *vel, const float *gamma,
                              float *phi, float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes,
-                             void *keep_sort, size_t keep_bytes) {
+                             void *keep_sort, size_t keep_bytes, const uint32_t *counts_in) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, write_forces,
-                             ws, ws_bytes, keep_sort, keep_bytes, nullptr, nullptr, nullptr, nullptr, nullptr));
+                             ws, ws_bytes, keep_sort, keep_bytes, nullptr, nullptr, nullptr, counts_in, nullptr));
 }
 
 int jpm_batch_force_kick_drift_f32(void *stream, const void *batch, const float *pos, float *vel, const float *gamma,
@@ -869,9 +869,9 @@ size_t jpm_force_kick_workspace_size(int nx, int ny, int nz, int64_t n, uint32_t
 
 int jpm_force_kick_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
                        float *phi, float r_split, uint32_t flags, int write_forces, void *ws, size_t ws_bytes,
-                       void *keep_sort, size_t keep_bytes) {
+                       void *keep_sort, size_t keep_bytes, const uint32_t *counts_in) {
   JPM_GUARD(force_kick(as_stream(stream), nx, ny, nz, pos, n, vel, gamma, phi, r_split, flags, write_forces, ws,
-                       ws_bytes, keep_sort, keep_bytes));
+                       ws_bytes, keep_sort, keep_bytes, 0.0f, nullptr, nullptr, counts_in, nullptr));
 }
 
 size_t jpm_bin_sort_size(int nx, int ny, int nz, int64_t n, uint32_t flags) {

@@ -105,6 +105,22 @@ class Stepper:
         tc.reverse()
         return cin, tc[0]
 
+    def counts_save(self, out):
+        """Copy the per-tile counts of the latest drift's positions into
+        ``out`` (a tensor from counts_like); no-op without tile counts."""
+        if self._tile_counts is not None and out is not None:
+            out.copy_(self._tile_counts[0])
+
+    def counts_load(self, src):
+        """Make ``src`` (from counts_save) the counts the next chained sort
+        takes: its positions must be the ones saved then, unchanged."""
+        if self._tile_counts is not None and src is not None:
+            self._tile_counts[0].copy_(src)
+
+    def counts_like(self):
+        """A buffer for counts_save, or None (relative positions)."""
+        return None if self._tile_counts is None else torch.empty_like(self._tile_counts[0])
+
     def tile_order(self, x):
         """order[k] = index of the particle at slot k of the tile sort of x."""
         return _ops.tile_order(x, self.shape, self.ws)
@@ -125,15 +141,19 @@ class Stepper:
             return (pot, srt)
         return (self.new_phi(), None)
 
-    def field_kick(self, x, v, gamma, field):
+    def field_kick(self, x, v, gamma, field, chain=False):
         """v += gamma F(x); ``field`` (from new_field) receives what the
         adjoint of this evaluation needs.  ``v`` None: the force evaluation
-        alone (its field, no kick)."""
+        alone (its field, no kick).  ``chain`` as in field_kick_drift: x is
+        the previous drift's output, whose tile counts replace the counting
+        pass."""
         buf, srt = field
         if self.sorted:
+            tc = self._tile_counts
+            cin = tc[0] if chain and tc is not None else None
             self._timed("field_kick", "jpm_force_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
                         ptr(v), float(gamma), ptr(buf), self.r_split, self.flags, 0, ptr(self.ws),
-                        self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+                        self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel(), ptr(cin))
         else:
             self.field(x, buf)
             if v is not None:
@@ -146,7 +166,7 @@ class Stepper:
         if self.sorted:
             self._timed("forces", "jpm_force_kick_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
                         ptr(out), 1.0, ptr(buf), self.r_split, self.flags, 1, ptr(self.ws), self.ws.numel(),
-                        ptr(srt), 0 if srt is None else srt.numel())
+                        ptr(srt), 0 if srt is None else srt.numel(), None)
         else:
             self.field(x, buf)
             out.zero_()
@@ -298,7 +318,8 @@ def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=Non
             if i + 1 < stop:
                 cur = nxt
         else:
-            stepper.field_kick(cur, v if kick_last or i + 1 < stop else None, gamma[i], field)
+            stepper.field_kick(cur, v if kick_last or i + 1 < stop else None, gamma[i], field,
+                               chain=chain or i > start)
     return cur
 
 
@@ -366,9 +387,15 @@ class NBody(torch.autograd.Function):
             ckpts = {}
             for s0 in range(0, S, k):
                 s1 = min(S, s0 + k)
-                if grad:
-                    # the kept last segment is never recomputed: no velocity
-                    ckpts[s0] = (y, v.clone() if s0 != last else None)
+                if grad and s0 != last:
+                    # the checkpoint's positions, velocity and tile counts (those
+                    # of the drift that wrote y, for the recompute's first sort)
+                    c = st.counts_like()
+                    st.counts_save(c)
+                    ckpts[s0] = (y, v.clone(), c)
+                elif grad:
+                    # the kept last segment is never recomputed
+                    ckpts[s0] = (y, None, None)
                 out = (torch.empty_like(x) if grad else work) if s1 < S else None
                 yl = _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None,
                                 work=work, out=out, chain=True)
@@ -454,10 +481,12 @@ class NBody(torch.autograd.Function):
         phi = st.new_field()
         for s0 in reversed(range(0, S, ctx.k)):
             s1 = min(S, s0 + ctx.k)
-            y, v = ctx.ckpts.pop(s0)
+            y, v, c = ctx.ckpts.pop(s0)
             if s0 != last:
-                # each checkpoint is restored once: recompute in place
-                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, kick_last=False)
+                # each checkpoint is restored once: recompute in place, its
+                # first sort from the saved tile counts
+                st.counts_load(c)
+                _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep, kick_last=False, chain=c is not None)
             for i in reversed(range(s0, s1)):
                 kp = keep[i - s0]
                 st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, gamma[i], beta[i], lx, chain=i < S - 1)
@@ -571,15 +600,23 @@ class BatchStepper:
         srt = torch.empty(self.sort_bytes, dtype=torch.uint8, device=self.dev) if keep_sort else None
         return (pot, srt)
 
-    def _kick(self, x, v, gamma, field, write):
-        buf, srt = field
-        call("jpm_batch_force_kick_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(buf), self.r_split,
-             self.kflags, int(write), ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel())
+    counts_save = Stepper.counts_save
+    counts_load = Stepper.counts_load
+    counts_like = Stepper.counts_like
 
-    def field_kick(self, x, v, gamma, field):
+    def _kick(self, x, v, gamma, field, write, chain=False):
+        buf, srt = field
+        tc = self._tile_counts
+        cin = tc[0] if chain and tc is not None else None
+        call("jpm_batch_force_kick_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(buf), self.r_split,
+             self.kflags, int(write), ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel(),
+             ptr(cin))
+
+    def field_kick(self, x, v, gamma, field, chain=False):
         """v_b += gamma[b] F_b(x_b) for every simulation; gamma a float32
-        device array [B].  ``v`` None: the force evaluations alone."""
-        self._kick(x, v, gamma, field, False)
+        device array [B].  ``v`` None: the force evaluations alone.
+        ``chain`` as in field_kick_drift."""
+        self._kick(x, v, gamma, field, False, chain)
 
     def forces(self, x, out, field, ones):
         """out = F(x) (``ones`` a float32 device array of 1.0, [B])."""
@@ -734,6 +771,8 @@ class BatchLeapfrog:
         # read each segment's first positions from its checkpoint
         fused = not self.want_c
         self.ckpts = {s0: (e(), e() if not (fused and s0 == self.last) else None) for s0 in range(0, S, self.k)}
+        # the fused sweeps' checkpoint tile counts, for the recompute's first sort
+        self.ckpt_counts = {s0: st.counts_like() for s0 in self.ckpts if fused and s0 != self.last}
         self.phi = st.new_field()
         self.phi0 = st.new_field(keep_sort=True)
         self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c, first_pos=not fused)
@@ -767,6 +806,7 @@ class BatchLeapfrog:
                 s1 = min(self.S, s0 + self.k)
                 if s0 != self.last:
                     self.ckpts[s0][1].copy_(v)
+                    st.counts_save(self.ckpt_counts[s0])
                 out = self.ckpts[s1][0] if s1 < self.S else None
                 self.xfinal = _run_fused(st, y, v, sc.beta, sc.gamma, s0, s1, self.phi,
                                          self.keep if s0 == self.last else None, work=x, out=out, chain=True)
@@ -799,8 +839,10 @@ class BatchLeapfrog:
                 if s0 != self.last:
                     # each checkpoint is read once per forward: recompute in
                     # place on its velocity
+                    c = self.ckpt_counts[s0]
+                    st.counts_load(c)
                     _run_fused(st, y, self.ckpts[s0][1], sc.beta, sc.gamma, s0, s1, self.phi, self.keep,
-                               kick_last=False)
+                               kick_last=False, chain=c is not None)
                 for i in reversed(range(s0, s1)):
                     kp = self.keep[i - s0]
                     st.vjp_field_drift(kp[1], y if i == s0 else kp[0], lv, sc.gamma[i], sc.gamma_pot[i],

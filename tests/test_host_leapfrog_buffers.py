@@ -53,12 +53,21 @@ class MockStepper:
     def new_field(self, keep_sort=False):
         return Field([None])
 
-    def field_kick(self, x, v, gamma, field):
+    def field_kick(self, x, v, gamma, field, chain=False):
         field[0] = x.clone()
         if v is None:
             self.field_only += 1
         else:
             v += float(gamma) * force(x)
+
+    def counts_like(self):
+        return None
+
+    def counts_save(self, out):
+        pass
+
+    def counts_load(self, src):
+        pass
 
     def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
         self.field_kick(x, v, gamma, field)
@@ -144,7 +153,7 @@ class MockBatchStepper(MockStepper):
     def tile_order(self, x, order):
         order.copy_(MockStepper.tile_order(self, x))
 
-    def field_kick(self, x, v, gamma, field):
+    def field_kick(self, x, v, gamma, field, chain=False):
         field[0] = x.clone()
         if v is None:
             self.field_only += 1
