@@ -415,14 +415,17 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz,
  * of pos_out (jpm_tile_count(nx, ny, nz) uint32), and a later call whose
  * pos is that pos_out, unchanged, may pass them as counts_in, which skips
  * the counting pass of its tile sort (NULL: count).  The two arrays must
- * differ. */
+ * differ.  vel_out (nullable): the kicked velocities go there instead of
+ * into vel, which is then only read (a checkpoint keeps them without a
+ * copy); vel_out must not overlap vel. */
 int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz,
                              const float *pos, int64_t n, float *vel,
                              float gamma, float beta, float *pos_out,
                              float *pos_keep, float *phi, float r_split,
                              uint32_t flags, void *ws, size_t ws_bytes,
                              void *keep_sort, size_t keep_bytes,
-                             const uint32_t *counts_in, uint32_t *counts_out);
+                             const uint32_t *counts_in, uint32_t *counts_out,
+                             float *vel_out);
 
 /* Tiles (16^3 cells) of a mesh: the length of the tile-count arrays. */
 size_t jpm_tile_count(int nx, int ny, int nz);
@@ -592,8 +595,8 @@ int jpm_batch_force_vjp_f32(void *stream, const void *batch, const float *phi,
 
 /* The fused kick + drift and its adjoint (jpm_force_kick_drift_f32,
  * jpm_force_vjp_pot_drift_f32) for every simulation, beta a device array
- * [nsims]; counts_in / counts_out (jpm_batch_tile_count(batch) uint32) and
- * cot_max_in / cot_max_out as there, over the whole batch. */
+ * [nsims]; counts_in / counts_out (jpm_batch_tile_count(batch) uint32),
+ * vel_out and cot_max_in / cot_max_out as there, over the whole batch. */
 int jpm_batch_force_kick_drift_f32(void *stream, const void *batch,
                                    const float *pos, float *vel,
                                    const float *gamma, const float *beta,
@@ -602,7 +605,8 @@ int jpm_batch_force_kick_drift_f32(void *stream, const void *batch,
                                    size_t ws_bytes, void *keep_sort,
                                    size_t keep_bytes,
                                    const uint32_t *counts_in,
-                                   uint32_t *counts_out);
+                                   uint32_t *counts_out,
+                                   float *vel_out);
 int jpm_batch_force_vjp_drift_f32(void *stream, const void *batch,
                                   const float *phi, const float *pos,
                                   float *cot, const float *scale,

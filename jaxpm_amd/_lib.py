@@ -87,7 +87,7 @@ SIGNATURES = {
     "jpm_batch_force_kick_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _i, _vp, _sz, _vp, _sz, _vp]),
     "jpm_batch_force_vjp_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
     "jpm_batch_force_kick_drift_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp,
-                                            _sz, _vp, _vp]),
+                                            _sz, _vp, _vp, _vp]),
     "jpm_batch_tile_count": (_sz, [_vp]),
     "jpm_batch_force_vjp_drift_f32": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f, _u32, _vp, _sz, _vp,
                                            _sz, _vp, _vp]),
@@ -113,7 +113,7 @@ SIGNATURES = {
     "jpm_force_vjp_pot_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_force_vjp_pot_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _vp, _f, _u32, _vp, _sz, _vp, _sz]),
     "jpm_force_kick_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _f, _vp, _vp, _vp, _f, _u32, _vp, _sz,
-                                      _vp, _sz, _vp, _vp]),
+                                      _vp, _sz, _vp, _vp, _vp]),
     "jpm_tile_count": (_sz, [_i, _i, _i]),
     "jpm_force_vjp_pot_drift_f32": (_i, [_vp, _i, _i, _i, _vp, _vp, _i64, _vp, _f, _f, _vp, _f, _u32, _vp, _sz, _vp,
                                          _sz, _vp, _vp]),

@@ -26,7 +26,7 @@ int potential_solve_div(hipStream_t, int, int, int, int, const float *, size_t, 
                         float, const float *, uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
-               float *, uint32_t *);
+               float *, uint32_t *, float *);
 int kspace_force_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_force_adjoint_batch(hipStream_t, int, int, int, int, const void *, void *, float, uint32_t);
 int kspace_linear_field_batch(hipStream_t, int, int, int, int, void *, const float *, const float *, const float *, int,
@@ -168,7 +168,7 @@ static int check_kflags(uint32_t kflags) {
 int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float *vel, const float *gamma, float *phi,
                      float r_split, uint32_t kflags, int write_forces, void *ws, size_t ws_bytes, void *keep,
                      size_t keep_bytes, const float *beta, float *xout, float *xkeep, const uint32_t *counts_in,
-                     uint32_t *counts_out) {
+                     uint32_t *counts_out, float *vout) {
   JPM_REQUIRE(!xout || (beta && !write_forces), "the fused drift needs beta and a kick");
   JPM_REQUIRE(!counts_out || (xout && !bt.rel), "tile counts of drifted absolute positions only");
   if (int rc = check_kflags(kflags)) return rc;
@@ -198,7 +198,7 @@ int batch_force_kick(hipStream_t s, const SimBatch &bt, const float *pos, float 
   }
   // without vel the call is the force evaluation alone: potentials and sort
   if (!vel) return JPM_OK;
-  return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out);
+  return batch_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out, vout);
 }
 
 // For every simulation b: gpos_b += scale[b] J_b^T cot_b, with pot_scale[b] =

@@ -81,7 +81,7 @@ size_t batch_kick_ws(const SimBatch &);
 size_t batch_vjp_ws(const SimBatch &);
 int batch_force_kick(hipStream_t, const SimBatch &, const float *, float *, const float *, float *, float, uint32_t,
                      int, void *, size_t, void *, size_t, const float *, float *, float *, const uint32_t *,
-                     uint32_t *);
+                     uint32_t *, float *);
 int batch_force_vjp(hipStream_t, const SimBatch &, const float *, const float *, const float *, const float *,
                     const float *, float *, float, uint32_t, void *, size_t, const void *, size_t, const float *,
                     float *, const float *, float *);
@@ -174,7 +174,8 @@ int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, floa
 int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t,
                const float *wmax_in = nullptr);
 size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
-int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *, uint32_t *);
+int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *, uint32_t *,
+              float *);
 int bins_grad(hipStream_t, const Bins &, const float *, const float *, const float *, float, float *, bool, float,
               float *, float *);
 int bins_read(hipStream_t, const Bins &, const float *, int, uint32_t, float *);
@@ -214,7 +215,8 @@ static bool sorted_supported(int nx, int ny, int nz, uint32_t flags) {
 static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, float *vel, float gamma,
                       float *phi, float r_split, uint32_t flags, int write_forces, void *ws, size_t ws_bytes,
                       void *keep_sort, size_t keep_bytes, float beta = 0.0f, float *xout = nullptr,
-                      float *xkeep = nullptr, const uint32_t *counts_in = nullptr, uint32_t *counts_out = nullptr) {
+                      float *xkeep = nullptr, const uint32_t *counts_in = nullptr, uint32_t *counts_out = nullptr,
+                      float *vout = nullptr) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(!xout || !write_forces, "the fused drift follows a kick, not a force write");
@@ -245,7 +247,7 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   if (rc) return rc;
   // without vel the call is the force evaluation alone: potential and sort
   if (!vel) return JPM_OK;
-  return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out);
+  return bins_kick(s, b, phi, gamma, vel, write_forces, beta, xout, xkeep, counts_out, vout);
 }
 
 // cic_read of up to 3 meshes through a tile sort of the positions (absolute
@@ -689,17 +691,19 @@ int jpm_batch_force_kick_f32(void *stream, const void *batch, const float *pos, 
                              void *keep_sort, size_t keep_bytes, const uint32_t *counts_in) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, write_forces,
-                             ws, ws_bytes, keep_sort, keep_bytes, nullptr, nullptr, nullptr, counts_in, nullptr));
+                             ws, ws_bytes, keep_sort, keep_bytes, nullptr, nullptr, nullptr, counts_in, nullptr,
+                             nullptr));
 }
 
 int jpm_batch_force_kick_drift_f32(void *stream, const void *batch, const float *pos, float *vel, const float *gamma,
                                    const float *beta, float *pos_out, float *pos_keep, float *phi, float r_split,
                                    uint32_t kflags, void *ws, size_t ws_bytes, void *keep_sort, size_t keep_bytes,
-                                   const uint32_t *counts_in, uint32_t *counts_out) {
+                                   const uint32_t *counts_in, uint32_t *counts_out, float *vel_out) {
   if (!batch) return fail(JPM_E_INVALID, "null batch");
   if (!pos_out || !beta) return fail(JPM_E_INVALID, "pos_out and beta are required");
   JPM_GUARD(batch_force_kick(as_stream(stream), *as_batch(batch), pos, vel, gamma, phi, r_split, kflags, 0, ws,
-                             ws_bytes, keep_sort, keep_bytes, beta, pos_out, pos_keep, counts_in, counts_out));
+                             ws_bytes, keep_sort, keep_bytes, beta, pos_out, pos_keep, counts_in, counts_out,
+                             vel_out));
 }
 
 size_t jpm_batch_tile_count(const void *batch) { return batch ? as_batch(batch)->K : 0; }
@@ -894,10 +898,10 @@ int jpm_force_vjp_pot_f32(void *stream, int nx, int ny, int nz, const float *phi
 int jpm_force_kick_drift_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, float *vel,
                              float gamma, float beta, float *pos_out, float *pos_keep, float *phi, float r_split,
                              uint32_t flags, void *ws, size_t ws_bytes, void *keep_sort, size_t keep_bytes,
-                             const uint32_t *counts_in, uint32_t *counts_out) {
+                             const uint32_t *counts_in, uint32_t *counts_out, float *vel_out) {
   if (!pos_out) return fail(JPM_E_INVALID, "pos_out is null");
   JPM_GUARD(force_kick(as_stream(stream), nx, ny, nz, pos, n, vel, gamma, phi, r_split, flags, 0, ws, ws_bytes,
-                       keep_sort, keep_bytes, beta, pos_out, pos_keep, counts_in, counts_out));
+                       keep_sort, keep_bytes, beta, pos_out, pos_keep, counts_in, counts_out, vel_out));
 }
 
 size_t jpm_tile_count(int nx, int ny, int nz) {

@@ -256,6 +256,7 @@ struct DriftArgs {
   const float *beta_dev;
   float *xout, *xkeep;
   uint32_t *next_count;
+  float *vout;   // the kicked velocities (MODE 0 and 2); null: in place
 };
 
 // Tile of the floor cell of an absolute position, per axis, and its offset
@@ -360,7 +361,7 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
       }
     }
     const uint32_t idx = __float_as_uint(rec.w);
-    float *v = vel + 3 * size_t(idx);
+    float *v = (dr.vout ? dr.vout : vel) + 3 * size_t(idx);
     if (MODE == 0) {
       v[0] = pv0 + gamma * s[0];
       v[1] = pv1 + gamma * s[1];
@@ -679,9 +680,10 @@ static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const
 }
 
 int batch_kick(hipStream_t s, const BatchBins &b, const float *phi, const float *gamma_dev, float *vel,
-               int write_forces, const float *beta_dev, float *xout, float *xkeep, uint32_t *next_count) {
+               int write_forces, const float *beta_dev, float *xout, float *xkeep, uint32_t *next_count, float *vout) {
   const int mode = write_forces ? 1 : (xout ? 2 : 0);
-  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, mode, DriftArgs{0.0f, beta_dev, xout, xkeep, next_count});
+  return kick_sorted(s, b, phi, 0.0f, gamma_dev, vel, mode,
+                     DriftArgs{0.0f, beta_dev, xout, xkeep, next_count, write_forces ? nullptr : vout});
 }
 
 int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float *rho_bar, const float *cot,
@@ -691,9 +693,10 @@ int batch_grad(hipStream_t s, const BatchBins &b, const float *phi, const float 
 }
 
 int bins_kick(hipStream_t s, const Bins &b, const float *phi, float gamma, float *vel, int write_forces, float beta,
-              float *xout, float *xkeep, uint32_t *next_count) {
+              float *xout, float *xkeep, uint32_t *next_count, float *vout) {
   const int mode = write_forces ? 1 : (xout ? 2 : 0);
-  return kick_sorted(s, b, phi, gamma, nullptr, vel, mode, DriftArgs{beta, nullptr, xout, xkeep, next_count});
+  return kick_sorted(s, b, phi, gamma, nullptr, vel, mode,
+                     DriftArgs{beta, nullptr, xout, xkeep, next_count, write_forces ? nullptr : vout});
 }
 
 int bins_grad(hipStream_t s, const Bins &b, const float *phi, const float *rho_bar, const float *cot, float scale,

@@ -172,21 +172,26 @@ class Stepper:
             out.zero_()
             self.kick(buf, x, out, 1.0)
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False, vout=None):
         """v += gamma F(x), then the drift xout = x + beta v (also into
         ``xkeep`` when given); ``xout`` may be ``x``.  One launch sequence on
         the sorted pipeline (the drift rides on the kick's gather).  With
         absolute positions the kick also counts xout's particles per tile for
         the next sort; ``chain`` says x is the previous field_kick_drift's
-        xout, unchanged since, whose counts then replace the counting pass."""
+        xout, unchanged since, whose counts then replace the counting pass.
+        ``vout`` (not overlapping v): the kicked velocity goes there and v is
+        only read."""
         buf, srt = field
         if self.sorted:
             cin, cout = self._counts(chain)
             self._timed("field_kick", "jpm_force_kick_drift_f32", stream(), self.nx, self.ny, self.nz, ptr(x), self.n,
                         ptr(v), float(gamma), float(beta), ptr(xout), ptr(xkeep), ptr(buf), self.r_split,
                         self.flags, ptr(self.ws), self.ws.numel(), ptr(srt), 0 if srt is None else srt.numel(),
-                        ptr(cin), ptr(cout))
+                        ptr(cin), ptr(cout), ptr(vout))
         else:
+            if vout is not None:
+                vout.copy_(v)
+                v = vout
             self.field_kick(x, v, gamma, field)
             if xout is not x:
                 xout.copy_(x)
@@ -291,7 +296,7 @@ def _run(stepper, x, v, k0, beta, gamma, start, stop, phi, keep=None, forces=Fal
 
 
 def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=None, out=None, chain=False,
-               kick_last=True):
+               kick_last=True, vwork=None, vout=None):
     """Kicks [start, stop) of the fused form.  On entry ``y`` holds y_start
     (the positions kick ``start`` evaluates its force at, i.e. after drift
     ``start``) and is only read; v is the velocity before that kick and is
@@ -307,20 +312,26 @@ def _run_fused(stepper, y, v, beta, gamma, start, stop, phi, keep=None, work=Non
     of the stepper's previous drift (its tile counts are kept); every later
     kick of the segment follows a drift of its own.  Without ``kick_last`` a
     last kick that drifts nowhere is the force evaluation alone (an adjoint's
-    recompute needs its field, not its velocity)."""
+    recompute needs its field, not its velocity).  Velocities: in place on
+    v, or with ``vwork`` v is only read, the kicks writing vwork (in place
+    there after the first) and the segment's last drifting kick ``vout``
+    when given.  Returns (positions of the last kick, velocity buffer)."""
     S = len(beta)
-    cur = y
+    cur, vc = y, v
     for i in range(start, stop):
         field = keep[i - start][1] if keep is not None else phi
         if i + 1 < S and (i + 1 < stop or out is not None):
             nxt = out if i + 1 == stop else (keep[i + 1 - start][0] if keep is not None else work)
-            stepper.field_kick_drift(cur, v, gamma[i], beta[i + 1], field, nxt, chain=chain or i > start)
+            vn = vout if i + 1 == stop and vout is not None else (vwork if vwork is not None else vc)
+            stepper.field_kick_drift(cur, vc, gamma[i], beta[i + 1], field, nxt, chain=chain or i > start,
+                                     vout=None if vn is vc else vn)
+            vc = vn
             if i + 1 < stop:
                 cur = nxt
         else:
-            stepper.field_kick(cur, v if kick_last or i + 1 < stop else None, gamma[i], field,
+            stepper.field_kick(cur, vc if kick_last or i + 1 < stop else None, gamma[i], field,
                                chain=chain or i > start)
-    return cur
+    return cur, vc
 
 
 def _keep_buffers(st, like, count, forces, first_pos=True):
@@ -379,26 +390,33 @@ class NBody(torch.autograd.Function):
             # buffer of the positions it makes; checkpoints hold (y, v) just
             # before a kick, y being the buffer the drift wrote (no copy)
             y = torch.empty_like(x) if grad else x
-            st.field_kick_drift(x, v, k0, beta[0], phi0, y)
+            # checkpoint velocities are the buffers the kicks write, too
+            vckpt = (lambda s: torch.empty_like(v) if grad and s != last else None)  # noqa: E731
+            vc = vckpt(0)
+            st.field_kick_drift(x, v, k0, beta[0], phi0, y, vout=vc)
+            v = vc if vc is not None else v
             x0c = x if grad else None
             keep = _keep_buffers(st, x, min(k, S), False, first_pos=False) if grad else None
             work = torch.empty_like(x) if grad else x
+            vwork = torch.empty_like(v) if grad and last > 0 else None
             phi = st.new_field()
             ckpts = {}
             for s0 in range(0, S, k):
                 s1 = min(S, s0 + k)
-                if grad and s0 != last:
+                kept = s0 == last
+                if grad and not kept:
                     # the checkpoint's positions, velocity and tile counts (those
                     # of the drift that wrote y, for the recompute's first sort)
                     c = st.counts_like()
                     st.counts_save(c)
-                    ckpts[s0] = (y, v.clone(), c)
+                    ckpts[s0] = (y, v, c)
                 elif grad:
                     # the kept last segment is never recomputed
                     ckpts[s0] = (y, None, None)
                 out = (torch.empty_like(x) if grad else work) if s1 < S else None
-                yl = _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep if grad and s0 == last else None,
-                                work=work, out=out, chain=True)
+                yl, v = _run_fused(st, y, v, beta, gamma, s0, s1, phi, keep if grad and kept else None, work=work,
+                                   out=out, chain=True, vwork=None if kept else vwork,
+                                   vout=vckpt(s1) if s1 < S else None)
                 y = out
             ctx.ckpts = ckpts
             ctx.kept = (keep, phi0, x0c, last)
@@ -622,15 +640,15 @@ class BatchStepper:
         """out = F(x) (``ones`` a float32 device array of 1.0, [B])."""
         self._kick(x, out, ones, field, True)
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False, vout=None):
         """v_b += gamma[b] F_b(x_b), then xout_b = x_b + beta[b] v_b (also
-        into ``xkeep``); gamma, beta float32 device arrays [B]; ``chain`` as
-        in Stepper.field_kick_drift."""
+        into ``xkeep``); gamma, beta float32 device arrays [B]; ``chain`` and
+        ``vout`` as in Stepper.field_kick_drift."""
         buf, srt = field
         cin, cout = Stepper._counts(self, chain)
         call("jpm_batch_force_kick_drift_f32", stream(), self.handle, ptr(x), ptr(v), ptr(gamma), ptr(beta),
              ptr(xout), ptr(xkeep), ptr(buf), self.r_split, self.kflags, ptr(self.ws), self.ws.numel(), ptr(srt),
-             0 if srt is None else srt.numel(), ptr(cin), ptr(cout))
+             0 if srt is None else srt.numel(), ptr(cin), ptr(cout), ptr(vout))
 
     def vjp_field_drift(self, field, x, cot, scale, pot_scale, beta, gpos, chain=False):
         """vjp_field, then the drift adjoint cot_b += beta[b] gpos_b; ``chain``
@@ -777,6 +795,9 @@ class BatchLeapfrog:
         self.phi0 = st.new_field(keep_sort=True)
         self.keep = _keep_buffers(st, self.x, min(self.k, S), self.want_c, first_pos=not fused)
         self.lx, self.lv = e(), e()
+        # the fused forward's velocity work buffer (checkpoints are read only)
+        self.vw = e() if fused and self.last > 0 else None
+        self.vfinal = self.v
         if self.want_c:
             self.v0, self.xs, self.vs, self.f0 = e(), e(), e(), e()
         self.xfinal = self.x
@@ -800,17 +821,20 @@ class BatchLeapfrog:
             # first checkpoint, then through x into the next; checkpoints
             # hold (y, v) just before a kick; v starts as the initial
             # velocity, which forward() writes there
-            st.field_kick_drift(self.x0, v, sc.k0, sc.beta[0], self.phi0, self.ckpts[0][0])
-            y = self.ckpts[0][0]
+            vck = lambda s: self.ckpts[s][1] if s < self.S else None  # noqa: E731
+            st.field_kick_drift(self.x0, v, sc.k0, sc.beta[0], self.phi0, self.ckpts[0][0], vout=vck(0))
+            y, v = self.ckpts[0][0], (vck(0) if vck(0) is not None else v)
             for s0 in range(0, self.S, self.k):
                 s1 = min(self.S, s0 + self.k)
-                if s0 != self.last:
-                    self.ckpts[s0][1].copy_(v)
+                kept = s0 == self.last
+                if not kept:
                     st.counts_save(self.ckpt_counts[s0])
                 out = self.ckpts[s1][0] if s1 < self.S else None
-                self.xfinal = _run_fused(st, y, v, sc.beta, sc.gamma, s0, s1, self.phi,
-                                         self.keep if s0 == self.last else None, work=x, out=out, chain=True)
+                self.xfinal, v = _run_fused(st, y, v, sc.beta, sc.gamma, s0, s1, self.phi,
+                                            self.keep if kept else None, work=x, out=out, chain=True,
+                                            vwork=None if kept else self.vw, vout=vck(s1))
                 y = out
+            self.vfinal = v
             return
         x.copy_(self.x0)
         v.copy_(self.v0)
@@ -906,17 +930,17 @@ class BatchLeapfrog:
         else:
             self._fwd()
         self._fresh = True
-        x = self.x if self.want_c else self.xfinal
+        x, v = (self.x, self.v) if self.want_c else (self.xfinal, self.vfinal)
         if order is not None:
-            return _ops.permute_rows(order, x, self.v, scatter=True, out=out if out is not None else (self.xo, self.vo))
+            return _ops.permute_rows(order, x, v, scatter=True, out=out if out is not None else (self.xo, self.vo))
         if out is not None:
             out[0].copy_(x)
-            out[1].copy_(self.v)
+            out[1].copy_(v)
             return out
         if x is not self.x:
             # the fused sweeps end in a kept buffer the adjoint overwrites
             self.x.copy_(x)
-        return self.x, self.v
+        return self.x, v
 
     def backward(self, gx, gv, out=None):
         """The adjoint of the last forward for output cotangents (gx, gv);

@@ -69,7 +69,11 @@ class MockStepper:
     def counts_load(self, src):
         pass
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False, vout=None):
+        if vout is not None:
+            assert vout is not v
+            vout.copy_(v)
+            v = vout
         self.field_kick(x, v, gamma, field)
         y = x + float(beta) * v
         xout.copy_(y)
@@ -160,7 +164,11 @@ class MockBatchStepper(MockStepper):
         else:
             v += gamma * force(x)
 
-    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False):
+    def field_kick_drift(self, x, v, gamma, beta, field, xout, xkeep=None, chain=False, vout=None):
+        if vout is not None:
+            assert vout is not v
+            vout.copy_(v)
+            v = vout
         self.field_kick(x, v, gamma, field)
         xout.copy_(x + beta * v)
 
