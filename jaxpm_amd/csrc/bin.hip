@@ -251,13 +251,13 @@ chunk_desc_kernel(const uint32_t *__restrict__ tile_start, const uint32_t *__res
   }
 }
 
-// The whole tile table of a sort of at most kTableMaxK tiles in one
-// single-workgroup launch: tile starts and scatter cursors (exclusive scan of
-// the counts), chunks per tile and their offsets (exclusive scan), the chunk
-// descriptors, and the totals [particles, chunks].  Thread t owns the
-// consecutive tiles [t per, (t + 1) per); both scans run over the threads'
-// sums in LDS.  The same values as count -> scans -> tile_chunks ->
-// chunk_desc, which larger tile grids use.
+// The whole tile table of a sort of at most kTableMaxK tiles in one launch
+// of ceil(K / 1024) workgroups: tile starts and scatter cursors (exclusive
+// scan of the counts), chunks per tile and their offsets (exclusive scan),
+// the chunk descriptors, and the totals [particles, chunks].  Each thread
+// owns one tile; each workgroup sums the counts before its range itself and
+// scans its own by wavefront shuffles.  The same values as count -> scans ->
+// tile_chunks -> chunk_desc, which larger tile grids use.
 constexpr int kTableThreads = 1024;
 constexpr uint32_t kTableMaxK = 32768;
 
@@ -265,19 +265,21 @@ __global__ void __launch_bounds__(kTableThreads)
 tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk, uint32_t *__restrict__ tstart,
                   uint32_t *__restrict__ cursor, uint32_t *__restrict__ nchunk, uint32_t *__restrict__ choff,
                   uint32_t *__restrict__ totals, ChunkDesc *__restrict__ desc) {
+  // workgroup b owns tiles [b T, (b + 1) T), one per thread, T = kTableThreads
   __shared__ uint32_t sp[kTableThreads / 64], sc[kTableThreads / 64];
-  const uint32_t t = threadIdx.x;
-  const uint32_t per = (K + kTableThreads - 1) / kTableThreads;
-  const uint32_t k0 = min(K, t * per), k1 = min(K, k0 + per);
-  uint32_t np = 0, nc = 0;
-  for (uint32_t k = k0; k < k1; ++k) {
-    const uint32_t c = count[k];
-    np += c;
-    nc += (c + chunk - 1) / chunk;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wid = t >> 6;
+  const uint32_t first = blockIdx.x * kTableThreads, k = first + t;
+  // particles and chunks of the tiles before this workgroup's (summed by
+  // every workgroup for itself: at most K counts, read from L2)
+  uint32_t bp = 0, bc = 0;
+  for (uint32_t j = t; j < first; j += kTableThreads) {
+    const uint32_t c = count[j];
+    bp += c;
+    bc += (c + chunk - 1) / chunk;
   }
-  // inclusive block scan of (np, nc): within each wavefront by shuffles,
-  // then across the wavefronts' totals, one barrier
-  const uint32_t lane = t & 63u, wid = t >> 6;
+  const uint32_t np = k < K ? count[k] : 0u, nc = (np + chunk - 1) / chunk;
+  // one pass of wavefront shuffles: the base sums reduced, this
+  // workgroup's (np, nc) scanned inclusively
   uint32_t xp = np, xc = nc;
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
@@ -286,36 +288,45 @@ tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk
       xp += yp;
       xc += yc;
     }
+    bp += __shfl_xor(bp, off);
+    bc += __shfl_xor(bc, off);
   }
+  __shared__ uint32_t bsp[kTableThreads / 64], bsc[kTableThreads / 64];
   if (lane == 63u) {
     sp[wid] = xp;
     sc[wid] = xc;
+    bsp[wid] = bp;
+    bsc[wid] = bc;
   }
   __syncthreads();
-  for (uint32_t w = 0; w < wid; ++w) {
-    xp += sp[w];
-    xc += sc[w];
+  uint32_t basep = 0, basec = 0;
+  for (uint32_t w = 0; w < kTableThreads / 64; ++w) {
+    basep += bsp[w];
+    basec += bsc[w];
+    if (w < wid) {
+      xp += sp[w];
+      xc += sc[w];
+    }
   }
-  uint32_t pb = xp - np, cb = xc - nc;
-  if (t == kTableThreads - 1) {
-    totals[0] = xp;
-    totals[1] = xc;
+  const uint32_t pb = basep + xp - np;
+  uint32_t cb = basec + xc - nc;
+  if (k == K - 1) {
+    totals[0] = pb + np;
+    totals[1] = cb + nc;
   }
-  for (uint32_t k = k0; k < k1; ++k) {
-    const uint32_t cnt = count[k], ncnk = (cnt + chunk - 1) / chunk;
+  if (k < K) {
     tstart[k] = pb;
     cursor[k] = pb;
-    nchunk[k] = ncnk;
+    nchunk[k] = nc;
     choff[k] = cb;
-    for (uint32_t o = 0; o < cnt; o += chunk, ++cb) {
+    for (uint32_t o = 0; o < np; o += chunk, ++cb) {
       ChunkDesc d;
       d.tile = k;
       d.start = pb + o;
-      d.count = min(chunk, cnt - o);
-      d.pad = cnt > chunk ? 1u : 0u;
+      d.count = min(chunk, np - o);
+      d.pad = np > chunk ? 1u : 0u;
       desc[cb] = d;
     }
-    pb += cnt;
   }
 }
 
@@ -896,7 +907,8 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
   }
   // 2. tile starts, scatter cursors, chunk table
   if (d.K <= kTableMaxK) {
-    hipLaunchKernelGGL(tile_table_kernel, dim3(1), dim3(kTableThreads), 0, s, count, d.K, d.chunk, tstart, cursor,
+    hipLaunchKernelGGL(tile_table_kernel, dim3((d.K + kTableThreads - 1) / kTableThreads), dim3(kTableThreads), 0, s,
+                       count, d.K, d.chunk, tstart, cursor,
                        nchunk, choff, totals, reinterpret_cast<ChunkDesc *>(base + w.desc));
     JPM_LAUNCHED("tile_table_kernel");
   } else {
