@@ -246,3 +246,37 @@ def test_tile_ordered_sweeps_give_the_same_bits(gpu, shape):
     for other in out[1:]:
         for a, b in zip(out[0], other):
             assert np.array_equal(a, b)
+
+
+def test_kick_drift_velocity_out_and_field_only(gpu):
+    """jpm_force_kick_drift_f32 with vel_out gives the in-place call's bits
+    and leaves vel unchanged; jpm_force_kick_f32 with a null vel leaves the
+    same potential as a full kick and no velocity change."""
+    from jaxpm_amd import integrate
+    shape = (32, 32, 32)
+    x0, v0 = _state(shape, 36)
+    n = x0.shape[0]
+    st = integrate.Stepper(shape, n)
+    x = torch.as_tensor(x0, device="cuda")
+    v_in = torch.as_tensor(v0, device="cuda")
+    # in place
+    v_a, xo_a = v_in.clone(), torch.empty_like(x)
+    f_a = st.new_field(keep_sort=True)
+    st.field_kick_drift(x, v_a, 0.3, 0.2, f_a, xo_a)
+    # into vel_out
+    v_b, vo_b, xo_b = v_in.clone(), torch.empty_like(x), torch.empty_like(x)
+    f_b = st.new_field(keep_sort=True)
+    st.field_kick_drift(x, v_b, 0.3, 0.2, f_b, xo_b, vout=vo_b)
+    torch.cuda.synchronize()
+    assert torch.equal(vo_b, v_a) and torch.equal(xo_b, xo_a)
+    assert torch.equal(v_b, v_in)
+    assert torch.equal(f_a[0], f_b[0])
+    # the force evaluation alone: same potential, no kick
+    f_c = st.new_field(keep_sort=True)
+    st.field_kick(x, None, 0.3, f_c)
+    v_d = v_in.clone()
+    f_d = st.new_field(keep_sort=True)
+    st.field_kick(x, v_d, 0.3, f_d)
+    torch.cuda.synchronize()
+    assert torch.equal(f_c[0], f_d[0])
+    assert not torch.equal(v_d, v_in)
