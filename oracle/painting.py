@@ -23,18 +23,25 @@ def abs_corners(positions, shape, dtype=np.float64):
 
     painting.py:21-37: floor, corner = floor + offset, weight
     prod(1 - |pos - corner|), int32 cast of the corner, then ``mod shape``.
-    Returns (idx [8, P] int64, w [8, P] dtype).
+    Returns (idx [8, P] int64, w [8, P] dtype).  The per-axis factors are
+    formed once for each offset and multiplied in axis order, the same float
+    operations per corner as forming them corner by corner.
     """
     pos = np.asarray(positions, dtype).reshape(-1, 3)
     fl = np.floor(pos)
-    shp = np.array(shape, np.int64)
+    kk, cc = [], []
+    for d in range(3):
+        kd, cd = [], []
+        for o in (0, 1):
+            c = fl[:, d] + dtype(o)
+            kd.append(dtype(1) - np.abs(pos[:, d] - c))
+            cd.append(np.mod(c.astype(np.int32).astype(np.int64), np.int64(shape[d])))
+        kk.append(kd)
+        cc.append(cd)
     idxs, ws = [], []
     for o in ABS_CORNERS:
-        c = fl + o.astype(dtype)
-        k = dtype(1) - np.abs(pos - c)
-        ws.append(k[:, 0] * k[:, 1] * k[:, 2])
-        ci = np.mod(c.astype(np.int32).astype(np.int64), shp)
-        idxs.append(_flat(ci, shape))
+        ws.append(kk[0][o[0]] * kk[1][o[1]] * kk[2][o[2]])
+        idxs.append((cc[0][o[0]] * shape[1] + cc[1][o[1]]) * shape[2] + cc[2][o[2]])
     return np.stack(idxs), np.stack(ws)
 
 
@@ -95,11 +102,13 @@ def cic_paint(grid_mesh, positions, weight=1.0, dtype=np.float64):
     return (mesh.astype(np.float64) + _deposit(shape, idx, w)).astype(dtype)
 
 
-def cic_read(grid_mesh, positions, dtype=np.float64):
-    """Absolute CIC read, painting.py:78-106: sum_o mesh[idx_o] * W_o."""
+def cic_read(grid_mesh, positions, dtype=np.float64, corners=None):
+    """Absolute CIC read, painting.py:78-106: sum_o mesh[idx_o] * W_o.
+    ``corners``: the (idx, w) of ``abs_corners`` for these positions, when
+    several meshes are read at the same positions."""
     mesh = np.asarray(grid_mesh, dtype)
     pos = np.asarray(positions)
-    idx, w = abs_corners(pos, mesh.shape, dtype)
+    idx, w = abs_corners(pos, mesh.shape, dtype) if corners is None else corners
     vals = mesh.reshape(-1)[idx].astype(np.float64) * w
     return vals.sum(0).astype(dtype).reshape(pos.shape[:-1])
 

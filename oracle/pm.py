@@ -8,7 +8,7 @@ import numpy as np
 
 from . import cosmology as cc
 from .kernels import fftk, gradient_kernel, invlaplace_kernel, longrange_kernel
-from .painting import cic_paint, cic_paint_dx, cic_read, cic_read_dx
+from .painting import abs_corners, cic_paint, cic_paint_dx, cic_read, cic_read_dx
 
 
 def fft3d(x):
@@ -51,7 +51,8 @@ def pm_forces(positions, mesh_shape=None, delta=None, r_split=0,
         delta_k = np.asarray(delta)
     pots = force_potentials(delta_k, r_split)
     if paint_absolute_pos:
-        comps = [cic_read(p, positions) for p in pots]
+        corners = abs_corners(positions, pots[0].shape)
+        comps = [cic_read(p, positions, corners=corners) for p in pots]
     else:
         comps = [cic_read_dx(p, positions) for p in pots]
     return np.stack(comps, axis=-1)

@@ -1,0 +1,161 @@
+"""The public helpers around the PM step against the oracle: the ODE
+siblings of make_ode_fn (jaxpm/ode.py:13-176), uniform_particles
+(jaxpm/distributed.py:168-190) and the Fourier multipliers of
+jaxpm/kernels.py:10-136."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cosmology as oc
+from oracle import kernels as ok
+from oracle import ode as oo
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (16, 16, 16)
+
+
+def _state(shape, seed):
+    r = np.random.default_rng(seed)
+    grid = np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1)
+    x = (grid + r.normal(scale=0.6, size=(*shape, 3))).astype(np.float32)
+    v = r.normal(scale=0.3, size=(*shape, 3)).astype(np.float32)
+    return x, v
+
+
+def rel_err(got, ref):
+    return np.abs(np.asarray(got, np.float64) - ref).max() / np.abs(ref).max()
+
+
+def test_make_diffrax_ode_and_symplectic_ode(gpu):
+    from jaxpm_amd import cosmology as pc
+    from jaxpm_amd import ode
+    x, v = _state(SHAPE, 1)
+    a = 0.37
+    xt, vt = torch.as_tensor(x, device=gpu), torch.as_tensor(v, device=gpu)
+    dpos, dvel = oo.make_ode_fn(SHAPE)((x, v), a, oc.Cosmology())
+    st = ode.make_diffrax_ode(SHAPE)(a, torch.stack([xt, vt]), pc.Cosmology())
+    assert tuple(st.shape) == (2, *SHAPE, 3)
+    assert rel_err(st[0].cpu().numpy(), dpos) < 1e-6
+    assert rel_err(st[1].cpu().numpy(), dvel) < 2e-5
+    drift, kick = ode.symplectic_ode(SHAPE, pc.Cosmology())
+    assert rel_err(drift(a, vt, None).cpu().numpy(), dpos) < 1e-6
+    assert rel_err(kick(a, xt, None).cpu().numpy(), dvel) < 2e-5
+
+
+@pytest.mark.parametrize("paint_absolute_pos", [True, False])
+def test_symplectic_fpm_ode_matches_oracle(gpu, paint_absolute_pos):
+    from jaxpm_amd import cosmology as pc
+    from jaxpm_amd import ode
+    x, v = _state(SHAPE, 2)
+    if not paint_absolute_pos:
+        grid = np.stack(np.meshgrid(*[np.arange(s) for s in SHAPE], indexing="ij"), -1)
+        x = (x - grid).astype(np.float32)
+    a, dt0 = 0.25, 0.09
+    cosmo = dict(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    d, k, fk = ode.symplectic_fpm_ode(SHAPE, dt0, pc.Cosmology(**cosmo), paint_absolute_pos=paint_absolute_pos)
+    od, okk, ofk = oo.symplectic_fpm_ode(SHAPE, dt0, oc.Cosmology(**cosmo), paint_absolute_pos=paint_absolute_pos)
+    xt, vt = torch.as_tensor(x, device=gpu), torch.as_tensor(v, device=gpu)
+    assert rel_err(d(a, vt, None).cpu().numpy(), od(a, v.astype(np.float64), None)) < 1e-6
+    assert rel_err(k(a, xt, None).cpu().numpy(), okk(a, x.astype(np.float64), None)) < 2e-5
+    assert rel_err(fk(a, xt, None).cpu().numpy(), ofk(a, x.astype(np.float64), None)) < 2e-5
+
+
+def test_fastpm_integration_matches_oracle(gpu):
+    """Six FastPM steps from 1LPT initial conditions at a = 0.1, driven by the
+    product's symplectic_fpm_ode with GPU forces, against the oracle's float64
+    FastPM run."""
+    from jaxpm_amd import cosmology as pc
+    from jaxpm_amd import ode, pm
+    from oracle import pm as opm
+    cosmo = dict(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    shape, box = SHAPE, 64.0
+    r = np.random.default_rng(3)
+    noise = r.standard_normal(shape).astype(np.float32)
+    k = np.logspace(-3, 1, 128)
+    pk = oc.linear_matter_power(oc.Cosmology(**cosmo), k)
+    ic = opm.linear_field(shape, [box] * 3, lambda q: np.interp(q, k, pk), noise)
+    grid = np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), -1).astype(np.float64)
+    a0, n = 0.1, 6
+    dt0 = (1.0 - a0) / n
+    dx, p, _ = opm.lpt(oc.Cosmology(**cosmo), ic, grid.reshape(-1, 3), a0)
+    x0 = (grid.reshape(-1, 3) + dx).astype(np.float32)
+    p0 = p.astype(np.float32)
+    rx, rv = oo.fastpm_leapfrog(x0, p0, a0, dt0, n, oc.Cosmology(**cosmo), shape)
+    d, kk, fk = ode.symplectic_fpm_ode(shape, dt0, pc.Cosmology(**cosmo))
+    x = torch.as_tensor(x0, device=gpu)
+    v = torch.as_tensor(p0, device=gpu)
+    v = v + dt0 * fk(a0, x, None)
+    a = a0
+    for _ in range(n):
+        x = x + dt0 * d(a, v, None)
+        v = v + dt0 * kk(a, x, None)
+        a = a + dt0
+    assert rel_err(x.cpu().numpy() - x0, rx - x0) < 1e-4
+    assert rel_err(v.cpu().numpy(), rv) < 1e-4
+    # the final density through the public paint
+    from jaxpm_amd import painting
+    rho = painting.cic_paint(torch.zeros(shape, device=gpu), x)
+    assert abs(float(rho.sum()) - np.prod(shape)) < 1e-3 * np.prod(shape)
+    del pm
+
+
+@pytest.mark.parametrize("shape", [(4, 5, 6), (16, 16, 16), (1, 8, 3)])
+def test_uniform_particles(gpu, shape):
+    """distributed.py:187-190: stack(meshgrid(arange), 'ij') as int32."""
+    from jaxpm_amd import distributed
+    u = distributed.uniform_particles(shape)
+    assert u.dtype == torch.int32
+    assert tuple(u.shape) == (*shape, 3)
+    want = np.stack(np.meshgrid(*[np.arange(s) for s in shape], indexing="ij"), axis=-1)
+    np.testing.assert_array_equal(u.cpu().numpy(), want)
+    with pytest.raises(NotImplementedError):
+        distributed.uniform_particles(shape, sharding=object())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 12), (5, 7, 9)])
+def test_kernel_helpers_match_oracle(gpu, shape):
+    """fftk, gradient_kernel (orders 0 and 1), invlaplace_kernel (plain and
+    fd), longrange_kernel and cic_compensation against the oracle's float64
+    closed forms, at float32 rounding."""
+    from jaxpm_amd import kernels
+    kv = kernels.fftk(shape)
+    okv = ok.fftk(shape)
+    for a, b in zip(kv, okv):
+        assert a.dtype == torch.float32 and tuple(a.shape) == b.shape
+        np.testing.assert_allclose(a.cpu().numpy(), b, rtol=1e-6, atol=1e-7)
+    # a spectrum argument gives the same vectors
+    kv2 = kernels.fftk(torch.zeros(shape, dtype=torch.complex64, device=gpu))
+    for a, b in zip(kv, kv2):
+        assert torch.equal(a, b)
+    for d in range(3):
+        for order in (0, 1):
+            g = kernels.gradient_kernel(kv, d, order=order)
+            assert g.is_complex()
+            np.testing.assert_allclose(g.cpu().numpy(), ok.gradient_kernel(okv, d, order=order), rtol=1e-5,
+                                       atol=2e-7)
+    for fd in (False, True):
+        lap = kernels.invlaplace_kernel(kv, fd=fd).cpu().numpy()
+        want = ok.invlaplace_kernel(okv, fd=fd)
+        np.testing.assert_allclose(lap, want, rtol=2e-6)
+        assert lap.reshape(-1)[0] == 0.0
+    assert kernels.longrange_kernel(kv, 0) == 1.0
+    np.testing.assert_allclose(kernels.longrange_kernel(kv, 1.3).cpu().numpy(), ok.longrange_kernel(okv, 1.3),
+                               rtol=2e-6, atol=1e-30)
+    np.testing.assert_allclose(kernels.cic_compensation(kv).cpu().numpy(), ok.cic_compensation(okv), rtol=3e-6)
+
+
+def test_kernel_helpers_compose_pm_forces(gpu):
+    """The multipliers composed by hand as jaxpm/pm.py:41-58 does (c2c FFT of
+    the painted density, -g_i * invlaplace * longrange, inverse FFT, read)
+    give pm_forces."""
+    from jaxpm_amd import distributed, kernels, painting, pm
+    x, _ = _state(SHAPE, 5)
+    xt = torch.as_tensor(x, device=gpu)
+    delta_k = distributed.fft3d(painting.cic_paint(torch.zeros(SHAPE, device=gpu), xt))
+    kv = kernels.fftk(delta_k)
+    pot_k = delta_k * kernels.invlaplace_kernel(kv) * kernels.longrange_kernel(kv, r_split=0)
+    f = torch.stack([painting.cic_read(distributed.ifft3d(-kernels.gradient_kernel(kv, i) * pot_k), xt)
+                     for i in range(3)], dim=-1)
+    ref = pm.pm_forces(xt, mesh_shape=SHAPE)
+    assert rel_err(f.cpu().numpy(), ref.cpu().numpy().astype(np.float64)) < 2e-5
