@@ -28,14 +28,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from hardware counters (FETCH_SIZE and WRITE_SIZE
-# passes of `python -m jaxpm_amd.traffic_probe 256 <tuning>`, each scaled by
-# the same counter's reading for a plain copy of a known 64 MiB: x1.999 for
-# reads, x1.000 for writes), by kernel, mesh side and where the force-VJP
-# gather reads the density adjoint ("lds" / "caches"; None: either).  The
-# commands and the counter values are in profiles/sorted_gathers_256_traffic.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256, None): 1267.8e6,
-                     ("bin_grad_kernel[drift]", 256, "caches"): 2121.3e6,
-                     ("bin_grad_kernel[drift]", 256, "lds"): 1879.4e6}
+# passes over `python bench.py --workload traffic`, each scaled by the same
+# counter's reading for a plain copy of a known 64 MiB: x1.999 for reads,
+# x1.000 for writes), by timed kernel name and mesh side.  The commands and
+# the counter values are in profiles/sorted_gathers_256_traffic.md.
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256): 1267.8e6,
+                     ("bin_grad_kernel[drift]", 256): 1879.4e6}
 
 
 def fixed_tuning(args):
@@ -55,10 +53,13 @@ def parse():
     ap.add_argument("--box", type=float, default=1024.0)
     ap.add_argument("--checkpoint", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the float64-oracle error checks")
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48", "large", "kernels"], default="pm",
+    ap.add_argument("--workload", choices=["pm", "diffsim", "batch48", "large", "kernels", "traffic"], default="pm",
                     help="pm: the default forward+adjoint step benchmark; diffsim: the differentiable "
-                         "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8)")
+                         "128^3 simulation with gradients w.r.t. the noise and (Omega_c, sigma8); batch48: 48 "
+                         "batched simulations; large: 512^3 particles on a 1024^3 mesh; kernels: the paint / "
+                         "read / pm_forces microbenchmark; traffic: fixed launches for hardware-counter passes")
     ap.add_argument("--no-tune", action="store_true", help="skip the sort autotuner (default widths)")
     ap.add_argument("--tuning", type=str, default=None,
                     help="fixed sort tuning 'chunk,slice,rho_global,xcd_slabs' codes instead of the autotuner")
@@ -83,20 +84,92 @@ def kernel_time_ms(fn, reps):
     return s.elapsed_time(e) / reps
 
 
-def cpu_baseline(n, box_cells_per_side, x, v):
-    """One forward + adjoint step of the float32 NumPy/SciPy port on the host."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n, x, v, runs=3):
+    """One forward + adjoint step of the float32 NumPy/SciPy port on the
+    host: one warm-up step, then the median of ``runs`` timed steps."""
     from oracle import cpu_step
     cores = len(os.sched_getaffinity(0))
     shape = (n, n, n)
     r = np.random.default_rng(0)
     cx = r.normal(size=x.shape).astype(np.float32)
     cv = r.normal(size=x.shape).astype(np.float32)
-    t0 = time.perf_counter()
-    cpu_step.step_fwd_adj(x, v, cx, cv, shape, 0.01, 0.01, cores)
-    dt = time.perf_counter() - t0
+    times = []
+    for i in range(runs + 1):
+        t0 = time.perf_counter()
+        cpu_step.step_fwd_adj(x, v, cx, cv, shape, 0.01, 0.01, cores)
+        if i:
+            times.append(time.perf_counter() - t0)
+        print(f"bench: cpu baseline step {i} ({'timed' if i else 'warm-up'}) {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+    dt = float(np.median(times))
     return {"value": 1.0 / dt, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": f"1 forward+adjoint leapfrog step at {n}^3 (drift, force, kick, recomputed force, force "
-                      f"VJP, drift adjoint) of oracle/cpu_step.py, float32 NumPy/SciPy c2c FFT, {dt:.1f} s"}
+            "cpu_model": cpu_model(),
+            "sample": f"forward+adjoint leapfrog steps at {n}^3 (drift, force, kick, recomputed force, force VJP, "
+                      f"drift adjoint) of oracle/cpu_step.py, float32 NumPy/SciPy c2c FFT with {cores} workers: "
+                      f"1 warm-up step, then the median of {runs} steps "
+                      f"({', '.join(f'{t:.1f}' for t in times)} s)"}
+
+
+def config1_forward(dev):
+    """The CPU-runnable reference case: 32^3 particles on a 32^3 mesh, box
+    256 Mpc/h, 1LPT at a=0.1 and 10 leapfrog steps to a=1, CIC paint of the
+    final positions.  Times the GPU forward and the float64 oracle's on the
+    same noise (default_rng(0)), and reports the per-cell RMS difference of
+    the final densities relative to the oracle's density contrast."""
+    import torch
+
+    from jaxpm_amd import integrate, painting, pm
+    from jaxpm_amd import cosmology as pc
+    from oracle import cosmology as oc
+    from oracle import ode as oo
+    from oracle import pm as opm
+    shape, box = (32, 32, 32), (256.0,) * 3
+    cp = dict(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    steps = np.linspace(0.1, 1.0, 11)
+    noise = np.random.default_rng(0).standard_normal(shape, dtype=np.float32)
+    ktab = np.logspace(-4, 1, 128)
+
+    def gpu_run():
+        cosmo = pc.Cosmology(**cp)
+        with torch.no_grad():
+            ic = pm.linear_field(shape, box, pm.TabulatedPk(ktab, pc.linear_matter_power(cosmo, ktab)),
+                                 torch.as_tensor(noise, device=dev))
+            lat = torch.stack(torch.meshgrid(*[torch.arange(32, device=dev, dtype=torch.float32)] * 3,
+                                             indexing="ij"), -1)
+            dx, p, _ = pm.lpt(cosmo, ic, lat, a=0.1, order=1)
+            x, _ = integrate.leapfrog((lat + dx).reshape(-1, 3), p.reshape(-1, 3), steps, cosmo, shape)
+            return painting.cic_paint(torch.zeros(shape, device=dev), x)
+
+    gpu_run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rho = gpu_run()
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    c = oc.Cosmology(**cp)
+    pk = oc.linear_matter_power(c, ktab)
+    ic = opm.linear_field(shape, box, lambda k: np.interp(k, ktab, pk), noise.astype(np.float64))
+    parts = opm.uniform_particles(shape).astype(np.float64)
+    dx, p, _ = opm.lpt(c, ic, parts, a=0.1, order=1)
+    x, _ = oo.leapfrog((parts + dx).reshape(-1, 3), p.reshape(-1, 3), steps, c, shape, reuse_force=True)
+    ref = oo.final_density(x, shape)
+    t_cpu = time.perf_counter() - t0
+    r = rho.double().cpu().numpy()
+    return {"workload": "32^3 particles on 32^3 mesh, box 256 Mpc/h, 1LPT a=0.1 + 10 leapfrog steps + CIC paint",
+            "gpu_s": round(t_gpu, 4), "cpu_oracle_fp64_s": round(t_cpu, 3),
+            "density_rms_rel_diff_vs_fp64": float(np.sqrt(np.mean((r - ref) ** 2)) / np.sqrt(np.mean((ref - 1) ** 2)))}
 
 
 def diffsim(args):
@@ -406,6 +479,62 @@ def large(args):
     print(json.dumps(out))
 
 
+def traffic(args):
+    """A fixed set of launches for hardware-counter passes (rocprofv3 --pmc
+    FETCH_SIZE, then WRITE_SIZE): a plain device copy of 4 n^3 bytes (the
+    calibration launch), then on the n^3 workload the tile-sorted force
+    evaluations of a 10-step leapfrog from 2LPT initial conditions to the
+    clustered state at a=1, on the particles in the tile order of the
+    initial positions as NBody runs them, each kick carrying the next drift
+    (the last one a plain kick), and 5 force VJPs with the drift adjoint
+    there: the launches of the leapfrog's forward and adjoint sweeps.  The
+    per-dispatch counters of a kernel divided by those of the copy give its
+    HBM traffic in units of the copy's known byte count."""
+    import torch
+
+    from jaxpm_amd import _lib, integrate, pm
+    from jaxpm_amd import cosmology as pc
+    n = args.mesh
+    tuning = fixed_tuning(args) or (0, 0)
+    dev = _lib.device()
+    shape = (n, n, n)
+    N = n ** 3
+    a = torch.randn(N, device=dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    cosmo = pc.Cosmology(Omega_c=0.25, Omega_b=0.05, h=0.7, n_s=0.96, sigma8=0.8)
+    k, pk = pc.pk_table(cosmo)
+    gen = torch.Generator(device=dev).manual_seed(2)
+    with torch.no_grad():
+        ic = pm.linear_field(shape, (4.0 * n,) * 3, pm.TabulatedPk(k, pk), torch.randn(shape, generator=gen,
+                                                                                      device=dev))
+        lattice = torch.stack(torch.meshgrid(*[torch.arange(n, device=dev, dtype=torch.float32)] * 3,
+                                             indexing="ij"), -1)
+        dx, p, _ = pm.lpt(cosmo, ic, lattice, a=0.1, order=2)
+        x = (lattice + dx).reshape(-1, 3).contiguous()
+        v = p.reshape(-1, 3).contiguous()
+        st = integrate.Stepper(shape, N, sort_tuning=tuning)
+        if st.reorder:
+            x, v = st.permute(st.tile_order(x), x, v)
+        k0, beta, gamma = integrate.kdk_schedule(cosmo, np.linspace(0.1, 1.0, 11))
+        field = st.new_field(keep_sort=True)
+        S = len(beta)
+        st.field_kick_drift(x, v, k0, beta[0], field, x)
+        for i in range(S):
+            if i + 1 < S:
+                st.field_kick_drift(x, v, gamma[i], beta[i + 1], field, x)
+            else:
+                st.field_kick(x, v, gamma[i], field)
+        cot = torch.randn((N, 3), generator=gen, device=dev)
+        gpos = torch.zeros_like(x)
+        for _ in range(5):
+            st.vjp_field_drift(field, x, cot, float(gamma[-1]), float(beta[-1]), gpos)
+    torch.cuda.synchronize()
+    print(f"traffic launches done: copy of {4 * N} bytes, {S + 1} force evaluations and 5 force VJPs at {n}^3, "
+          f"sort tuning {tuning}", flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -420,6 +549,8 @@ def main():
         return large(args)
     if args.workload == "kernels":
         return kernels(args)
+    if args.workload == "traffic":
+        return traffic(args)
     import torch
 
     from jaxpm_amd import _lib, _ops, integrate, painting, pm
@@ -478,6 +609,7 @@ def main():
     per_kernel = _lib.timing_report()
     assert torch.isfinite(g0).all(), "non-finite gradient"
     steps_per_s = args.steps / elapsed
+    print(f"bench: timed region {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
     # average launch durations over the timed region: HIP events recorded by
     # the library on the launch stream around each of its kernels
     M = N
@@ -500,15 +632,23 @@ def main():
         "axpy_kernel": 36 * N,
         "hipfft_r2c": 4 * M + 8 * H,
         "hipfft_c2r": 8 * H + 4 * M,
-        "fft_r2c": 4 * M + 8 * H,
-        "fft_c2r": 8 * H + 4 * M,
-        "fft_potential": 4 * M + 4 * M,                # the mesh in, the potential out
-        "fft_potential[div]": 12 * M + 4 * M,          # three cotangent meshes in (divergence), the potential out
+        # the library's FFT, pass by pass
+        "fft_r2c_rows_kernel": 4 * M + 8 * H,          # reals in, half spectrum out
+        "fft_r2c_rows_kernel[div]": 12 * M + 8 * H,    # three cotangent meshes in (divergence), half spectrum out
+        "fft_cols_kernel": 16 * H,                     # in place
+        "fft_cols_potential_kernel": 16 * H,           # in place
+        "fft_c2r_rows_kernel": 8 * H + 4 * M,
+        # aggregates of the passes above (not kernels; never the roofline kernel)
+        "potential_solve (5 passes)": 8 * M + 64 * H,
+        "potential_solve[div] (5 passes)": 16 * M + 64 * H,
+        "native_r2c (3 passes)": 4 * M + 40 * H,
+        "native_c2r (3 passes)": 40 * H + 4 * M,
         "abs_max_kernel": 12 * N,
         "permute_rows_kernel": 2 * 24 * N,             # two (N, 3) arrays, read and written
         "record_order_kernel": 16 * N + 4 * N,
     }
     kinfo = {}
+    groups = {}
     for name, (cnt, tot) in per_kernel.items():
         avg = tot / cnt
         entry = {"ms": round(avg, 4), "launches_per_step": round(cnt / args.steps, 2),
@@ -516,20 +656,74 @@ def main():
         if name in algo:
             gbs = algo[name] / (avg * 1e-3) / 1e9
             entry.update({"GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)})
+            if " (" not in name:
+                g = groups.setdefault(name.split("[")[0], {"launches": 0, "ms": 0.0, "bytes": 0.0, "variants": {}})
+                g["launches"] += cnt
+                g["ms"] += tot
+                g["bytes"] += cnt * algo[name]
+                g["variants"][name] = cnt
         kinfo[name] = entry
-    # dominant hand-written kernel by total time in the timed region
-    dom = max((k for k in per_kernel if k in algo and not k.startswith(("hipfft", "fft_"))),
-              key=lambda k: per_kernel[k][1])
-    t_dom = per_kernel[dom][1] / per_kernel[dom][0]
-    achieved = algo[dom] / (t_dom * 1e-3) / 1e9
-    # the paint and k-space pass alone at the final state, for the record
-    mesh = torch.zeros(shape, device=dev)
-    dk = _ops.r2c(mesh)
+    # the roofline kernel: the kernel function (its variants together) with
+    # the most GPU time in the timed region
+    dom = max(groups, key=lambda k: groups[k]["ms"])
+    gd = groups[dom]
+    t_dom = gd["ms"] / gd["launches"]
+    bytes_dom = gd["bytes"] / gd["launches"]
+    achieved = bytes_dom / (t_dom * 1e-3) / 1e9
+    traffic = None
+    meas = [PMC_TRAFFIC_BYTES.get((v, n)) for v in gd["variants"]]
+    if all(m is not None for m in meas):
+        traffic = sum(m * c for m, c in zip(meas, gd["variants"].values())) / gd["launches"]
+    kernel_groups = {k: {"launches_per_step": round(g["launches"] / args.steps, 2),
+                         "ms_per_step": round(g["ms"] / args.steps, 4),
+                         "ms_per_launch": round(g["ms"] / g["launches"], 4),
+                         "GB_s": round(g["bytes"] / (g["ms"] * 1e-3) / 1e9, 1),
+                         "frac_hbm": round(g["bytes"] / (g["ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                     for k, g in sorted(groups.items(), key=lambda kv: -kv[1]["ms"])}
+    # cic_paint and cic_read alone (the public API, forward), on uniform
+    # random positions and on the clustered final state (2LPT + the timed
+    # steps), with their float64-oracle errors
     reps = args.kernel_reps
-    t_paint = kernel_time_ms(lambda: _ops.paint(mesh, xS), reps)
-    kinfo["cic_paint (sort + windows, all passes)"] = {
-        "ms": round(t_paint, 4), "GB_s": round((12 * N + 4 * M) / (t_paint * 1e-3) / 1e9, 1),
-        "frac_hbm": round((12 * N + 4 * M) / (t_paint * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    paint_bytes, read_bytes = 12 * N + 4 * M, 12 * N + 4 * M + 4 * N
+    uniform = torch.rand((N, 3), generator=gen, device=dev) * n
+    paint_read = {}
+    for name, pos in (("uniform", uniform), ("clustered", xS)):
+        mesh = torch.zeros(shape, device=dev)
+        t_p = kernel_time_ms(lambda: painting.cic_paint(mesh, pos), reps)
+        rho = painting.cic_paint(mesh, pos)
+        t_r = kernel_time_ms(lambda: painting.cic_read(rho, pos), reps)
+        paint_read[name] = {
+            "cic_paint": {"ms": round(t_p, 4), "GB_s": round(paint_bytes / (t_p * 1e-3) / 1e9, 1),
+                          "frac_hbm": round(paint_bytes / (t_p * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "cic_read": {"ms": round(t_r, 4), "GB_s": round(read_bytes / (t_r * 1e-3) / 1e9, 1),
+                         "frac_hbm": round(read_bytes / (t_r * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+        if not args.no_oracle:
+            from oracle import painting as op
+            p64 = pos.double().cpu().numpy()
+            ref = op.cic_paint(np.zeros(shape), p64)
+            got = rho.double().cpu().numpy()
+            paint_read[name]["cic_paint"]["fp64_oracle_max_rel_err"] = float(np.abs(got - ref).max() /
+                                                                            np.abs(ref).max())
+            paint_read[name]["cic_paint"]["fp64_oracle_rms_rel_err"] = float(
+                np.sqrt(np.mean((got - ref) ** 2)) / np.sqrt(np.mean((ref - ref.mean()) ** 2)))
+            rd = painting.cic_read(rho, pos).double().cpu().numpy()
+            refr = op.cic_read(got, p64)
+            paint_read[name]["cic_read"]["fp64_oracle_max_rel_err"] = float(np.abs(rd - refr).max() /
+                                                                           np.abs(refr).max())
+            del ref, got, rd, refr, p64
+    del uniform
+    print("bench: paint / read microbenchmark done", file=sys.stderr, flush=True)
+    fp64 = None
+    if not args.no_oracle:
+        from oracle import pm as opm
+        f = pm.pm_forces(xS, mesh_shape=shape).double().cpu().numpy()
+        ref = opm.pm_forces(xS.double().cpu().numpy(), mesh_shape=shape)
+        fp64 = {"pm_forces_max_rel_err": float(np.abs(f - ref).max() / np.abs(ref).max()),
+                "gradient": "the adjoint is checked against float64 central differences of the oracle in "
+                            "tests/test_gpu_diffsim.py and tests/test_gpu_configs.py"}
+        del f, ref
+        fp64["config1"] = config1_forward(dev)
+        print("bench: float64 oracle checks done", file=sys.stderr, flush=True)
 
     # the plain per-simulation form: the same leapfrog as differentiable
     # tensor expressions, autograd over the stored trajectory
@@ -570,9 +764,8 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(n, n, x0.cpu().numpy(), v0.cpu().numpy())
+        cpu = cpu_baseline(n, x0.cpu().numpy(), v0.cpu().numpy())
 
-    rho_mode = "caches" if len(tuning) > 2 and tuning[2] else "lds"
     out = {
         "metric": "fwd+adjoint PM steps/s at 256^3 mesh; CIC paint/read GB/s as % HBM peak",
         "value": round(steps_per_s, 3),
@@ -595,9 +788,14 @@ def main():
                                    "xcd_placement": "slabs" if len(tuning) > 3 and tuning[3] else "round-robin"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": PMC_TRAFFIC_BYTES.get((dom, n, rho_mode), PMC_TRAFFIC_BYTES.get((dom, n, None))),
-                     "algorithmic_bytes": int(algo[dom]), "ms_per_launch": round(t_dom, 4)},
+                     "traffic": None if traffic is None else round(traffic),
+                     "algorithmic_bytes": int(bytes_dom), "ms_per_launch": round(t_dom, 4),
+                     "launches_per_step": round(gd["launches"] / args.steps, 2),
+                     "variants": {k: round(v / args.steps, 2) for k, v in gd["variants"].items()}},
+        "kernel_functions": kernel_groups,
         "kernels": kinfo,
+        "paint_read_256": paint_read,
+        "fp64_oracle": fp64,
         "plain_autodiff": plain,
         "cpu_baseline": cpu,
     }

@@ -84,6 +84,16 @@ int jpm_timing_report(char *buf, size_t len);
 
 /* ---------------------------------------------------------------- painting */
 
+/* Non-finite input.  Every paint gives a deposit that is NaN or +-Inf (a
+ * non-finite weight, or a NaN / +-Inf position, whose corner weights are
+ * NaN) to the cells it touches as a float, as the reference's float
+ * scatter_add does: a touched cell becomes NaN, +Inf or -Inf (NaN where
+ * both infinities land).  The fixed-point sums of the deterministic paints
+ * cover the finite deposits only, scaled by the largest finite weight.  A
+ * non-finite position indexes the cells its floor would if it were 0
+ * (absolute mode; the reference's int32 cast of it is implementation-defined)
+ * and relative mode drops it, as enmesh's mode='drop' scatter does. */
+
 /* Bytes of workspace jpm_cic_paint_f32 needs for this call (0 unless
  * JPM_DETERMINISTIC: 8 bytes per mesh cell). */
 size_t jpm_cic_paint_workspace_size(int nx, int ny, int nz, int64_t n,

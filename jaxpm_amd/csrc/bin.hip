@@ -313,6 +313,7 @@ tile_table_kernel(const uint32_t *__restrict__ count, uint32_t K, uint32_t chunk
   if (k == K - 1) {
     totals[0] = pb + np;
     totals[1] = cb + nc;
+    totals[2] = 0u;   // the non-finite flag of the paints of this sort
   }
   if (k < K) {
     tstart[k] = pb;
@@ -492,12 +493,47 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz) {
   return lx == kTile ? ly * kWin + lz : (ly == kTile ? kH0 + lx * kWin + lz : kH1 + lx * kTile + ly);
 }
 
+// The window corners of a sorted particle (record rec) of a tile with
+// origin (ox, oy, oz): per axis the local window index of its two corners
+// (relative mode: -2 when enmesh drops the corner, -1 when rounding put it
+// outside the window), their weights and (relative mode) global cells.
+template <bool REL>
+__device__ __forceinline__ void window_corners(const float4 &rec, const TileGrid &t, uint32_t part0, int ox, int oy,
+                                               int oz, int lx[2], int ly[2], int lz[2], int gx[2], int gy[2],
+                                               int gz[2], float wx[2], float wy[2], float wz[2]) {
+  const float x = rec.x, y = rec.y, z = rec.z;
+  if (REL) {
+    const uint32_t local = __float_as_uint(rec.w) - part0;
+    uint32_t r = t.div_nz.div(local);
+    uint32_t k = local - r * uint32_t(t.nz);
+    uint32_t i = t.div_ny.div(r);
+    uint32_t j = r - i * uint32_t(t.ny);
+    const float px = float(i) + x, py = float(j) + y, pz = float(k) + z;
+    lcorner_rel(px, 0.0f, t.nx, ox, lx[0], wx[0], gx[0]);
+    lcorner_rel(px, 1.0f, t.nx, ox, lx[1], wx[1], gx[1]);
+    lcorner_rel(py, 0.0f, t.ny, oy, ly[0], wy[0], gy[0]);
+    lcorner_rel(py, 1.0f, t.ny, oy, ly[1], wy[1], gy[1]);
+    lcorner_rel(pz, 0.0f, t.nz, oz, lz[0], wz[0], gz[0]);
+    lcorner_rel(pz, 1.0f, t.nz, oz, lz[1], wz[1], gz[1]);
+  } else {
+    LAxis ax = laxis_abs(x, ox, t.nx), ay = laxis_abs(y, oy, t.ny), az = laxis_abs(z, oz, t.nz);
+    lx[0] = ax.l0; lx[1] = ax.l1; wx[0] = ax.w0; wx[1] = ax.w1;
+    ly[0] = ay.l0; ly[1] = ay.l1; wy[0] = ay.w0; wy[1] = ay.w1;
+    lz[0] = az.l0; lz[1] = az.l1; wz[0] = az.w0; wz[1] = az.w1;
+  }
+}
+
+// A deposit that is NaN or +-Inf (a non-finite weight or position) does not
+// enter the fixed-point window: it sets *nonfinite, and bin_reduce_kernel
+// adds it to the finished mesh as a float, so it reaches exactly the cells
+// it touches, as in a float scatter-add.
 template <bool REL, class G>
 __global__ void __launch_bounds__(kWindowThreads)
 bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weight, int nc, float wscalar,
                  const ChunkDesc *__restrict__ desc, const uint32_t *__restrict__ nchunks_total, G g, uint32_t M,
                  const float *__restrict__ wmax_dev, float wmax_host, uint32_t n, float *__restrict__ partial,
-                 float *__restrict__ overflow, int accumulate, float *__restrict__ mesh, uint32_t per, bool slabs) {
+                 float *__restrict__ overflow, int accumulate, float *__restrict__ mesh, uint32_t per, bool slabs,
+                 uint32_t *__restrict__ nonfinite) {
   __shared__ unsigned long long win[kWinCells];
   // (chunk, channel) of this workgroup: the nc channels of one chunk go to
   // one XCD back to back (workgroups b, b + 8, ...), so their reads of the
@@ -524,6 +560,7 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
   if (weight && threadIdx.x < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
   for (int i = threadIdx.x; i < kWinCells; i += blockDim.x) win[i] = 0ull;
   const double scale = fixed_point_scale(weight ? *wmax_dev : wmax_host, n);
+  bool hit = false;
   __syncthreads();
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
@@ -531,29 +568,9 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
     rec1 = rec2;
     if (weight && q + B < cnt) nw = weight[size_t(__float_as_uint(rec1.w)) * nc + ch];
     if (q + 2 * B < cnt) rec2 = load_global(spos + (d.start + q + 2 * B));
-    const float x = rec.x, y = rec.y, z = rec.z;
-    const uint32_t pidx = __float_as_uint(rec.w);
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
     float wx[2], wy[2], wz[2];
-    if (REL) {
-      const uint32_t local = pidx - part0;
-      uint32_t r = t.div_nz.div(local);
-      uint32_t k = local - r * uint32_t(t.nz);
-      uint32_t i = t.div_ny.div(r);
-      uint32_t j = r - i * uint32_t(t.ny);
-      const float px = float(i) + x, py = float(j) + y, pz = float(k) + z;
-      lcorner_rel(px, 0.0f, t.nx, ox, lx[0], wx[0], gx[0]);
-      lcorner_rel(px, 1.0f, t.nx, ox, lx[1], wx[1], gx[1]);
-      lcorner_rel(py, 0.0f, t.ny, oy, ly[0], wy[0], gy[0]);
-      lcorner_rel(py, 1.0f, t.ny, oy, ly[1], wy[1], gy[1]);
-      lcorner_rel(pz, 0.0f, t.nz, oz, lz[0], wz[0], gz[0]);
-      lcorner_rel(pz, 1.0f, t.nz, oz, lz[1], wz[1], gz[1]);
-    } else {
-      LAxis ax = laxis_abs(x, ox, t.nx), ay = laxis_abs(y, oy, t.ny), az = laxis_abs(z, oz, t.nz);
-      lx[0] = ax.l0; lx[1] = ax.l1; wx[0] = ax.w0; wx[1] = ax.w1;
-      ly[0] = ay.l0; ly[1] = ay.l1; wy[0] = ay.w0; wy[1] = ay.w1;
-      lz[0] = az.l0; lz[1] = az.l1; wz[0] = az.w0; wz[1] = az.w1;
-    }
+    window_corners<REL>(rec, t, part0, ox, oy, oz, lx, ly, lz, gx, gy, gz, wx, wy, wz);
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
@@ -562,6 +579,10 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
       if (REL && (Lx == -2 || Ly == -2 || Lz == -2)) continue;   // dropped corner
       const float v = wp * wc;
       if (!REL || (Lx >= 0 && Ly >= 0 && Lz >= 0)) {
+        if (!isfinite(v)) {
+          hit = true;
+          continue;
+        }
         atomicAdd(&win[(Lx * kWin + Ly) * kWin + Lz], (unsigned long long)(long long)rint(double(v) * scale));
       } else {
         // rounding put a relative corner outside the window: global path
@@ -570,6 +591,7 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
       }
     }
   }
+  if (hit) atomicOr(nonfinite, 1u);
   __syncthreads();
   // A tile with a single chunk stores its window rounded once to float.  A
   // tile split over several chunks keeps the exact int64 windows: which
@@ -641,12 +663,19 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
 // chunks are summed in a fixed order.  With DIRECT, a single-chunk tile's
 // own deposits are already in the mesh (the accumulation wrote them), and
 // only its low-face cells take the neighbours' halos, in the same order.
+//
+// When the accumulation flagged a non-finite deposit, the workgroup then
+// walks the particles of the same (up to 8) source tiles and adds their
+// NaN / +-Inf deposits that land in its own tile to the finished cells as
+// floats (only this workgroup writes them, so nothing races).
 template <class G, bool DIRECT>
 __global__ void __launch_bounds__(256)
 bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict__ chunk_off,
                   const uint32_t *__restrict__ nchunk, G g, uint32_t M, int nc, const float *__restrict__ overflow,
                   int has_overflow, const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n,
-                  int accumulate, float *__restrict__ mesh, uint32_t K, uint32_t per, bool slabs) {
+                  int accumulate, float *__restrict__ mesh, uint32_t K, uint32_t per, bool slabs,
+                  const uint32_t *__restrict__ nonfinite, const float4 *__restrict__ spos,
+                  const ChunkDesc *__restrict__ desc, const float *__restrict__ weight, float wscalar) {
   const uint32_t gtile = slab_item(blockIdx.x, per, slabs);
   if (gtile >= K) return;
   const unsigned long long *ipart = reinterpret_cast<const unsigned long long *>(partial);
@@ -666,13 +695,14 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
   const int PLx = tile_len(px, t.tx, t.nx), PLy = tile_len(py, t.ty, t.ny), PLz = tile_len(pz, t.tz, t.nz);
   // the (up to 8) source tiles, q = sx | sy << 1 | sz << 2 (s = 1: the
   // lower neighbour along that axis): first chunk and chunk count
-  __shared__ uint32_t src_off[8], src_cnt[8];
+  __shared__ uint32_t src_off[8], src_cnt[8], src_id[8];
   if (threadIdx.x < 8) {
     const int q = int(threadIdx.x);
     const int jx = (q & 1) ? px : itx, jy = (q & 2) ? py : ity, jz = (q & 4) ? pz : itz;
     const uint32_t src = tile0 + (uint32_t(jx) * uint32_t(t.ty) + uint32_t(jy)) * uint32_t(t.tz) + uint32_t(jz);
     src_off[q] = chunk_off[src];
     src_cnt[q] = nchunk[src];
+    src_id[q] = src;
   }
   __syncthreads();
   const bool faces_only = DIRECT && src_cnt[0] == 1;
@@ -740,6 +770,50 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
     float v = acc;
     if (has_overflow) v += overflow[size_t(ch) * M + cell];
     *dst = accumulate ? *dst + v : v;
+  }
+  if (*nonfinite == 0u) return;
+  __threadfence();
+  __syncthreads();
+  float *meshc = mesh + size_t(ch) * M + cell0;
+  for (int q = 0; q < 8; ++q) {
+    bool dup = false;
+    for (int r = 0; r < q; ++r) dup |= src_id[r] == src_id[q];
+    if (dup) continue;   // a tile grid one or two tiles long along an axis
+    for (uint32_t k = 0; k < src_cnt[q]; ++k) {
+      const ChunkDesc dk = desc[src_off[q] + k];
+      int sx, sy, sz;
+      tile_origin(dk.tile - tile0, t, sx, sy, sz);
+      const int ox = sx * kTile, oy = sy * kTile, oz = sz * kTile;
+      for (uint32_t i = threadIdx.x; i < dk.count; i += blockDim.x) {
+        const float4 rec = spos[dk.start + i];
+        const float wp = weight ? weight[size_t(__float_as_uint(rec.w)) * nc + ch] : wscalar;
+        int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
+        float wx[2], wy[2], wz[2];
+        window_corners<!DIRECT>(rec, t, sim.part0, ox, oy, oz, lx, ly, lz, gx, gy, gz, wx, wy, wz);
+        for (int o = 0; o < 8; ++o) {
+          const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+          const float v = wp * (wx[bx] * wy[by] * wz[bz]);
+          if (isfinite(v)) continue;
+          int cx, cy, cz;
+          if (DIRECT) {
+            cx = ox + lx[bx];
+            cy = oy + ly[by];
+            cz = oz + lz[bz];
+            cx -= cx >= t.nx ? t.nx : 0;
+            cy -= cy >= t.ny ? t.ny : 0;
+            cz -= cz >= t.nz ? t.nz : 0;
+          } else {
+            // dropped corners, and those the overflow mesh carried
+            if (lx[bx] < 0 || ly[by] < 0 || lz[bz] < 0) continue;
+            cx = gx[bx];
+            cy = gy[by];
+            cz = gz[bz];
+          }
+          if (cx / kTile != itx || cy / kTile != ity || cz / kTile != itz) continue;
+          atomicAdd(&meshc[(uint32_t(cx) * uint32_t(t.ny) + uint32_t(cy)) * uint32_t(t.nz) + uint32_t(cz)], v);
+        }
+      }
+    }
   }
 }
 
@@ -853,6 +927,7 @@ static BinsT<G> view_of(const G &g, const SortDims &d, uint32_t M, bool rel, con
   b.nchunk = reinterpret_cast<const uint32_t *>(base + w.nchunk);
   b.choff = reinterpret_cast<const uint32_t *>(base + w.choff);
   b.total_chunks = reinterpret_cast<const uint32_t *>(base + w.totals) + 1;
+  b.nonfinite = const_cast<uint32_t *>(reinterpret_cast<const uint32_t *>(base + w.totals) + 2);
   return b;
 }
 
@@ -924,6 +999,8 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
     hipLaunchKernelGGL(chunk_desc_kernel, dim3(grid_for(d.K, 256)), dim3(256), 0, s, tstart, count, choff, d.K,
                        d.chunk, reinterpret_cast<ChunkDesc *>(base + w.desc));
     JPM_LAUNCHED("chunk_desc_kernel");
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, totals + 2, 1u);
+    JPM_LAUNCHED("zero_u32_kernel");
   }
   // 3. scatter particles into tile order
   if (lds && (d.slice == 8u * kScatterRegThreads || d.slice == 16u * kScatterRegThreads)) {
@@ -1013,7 +1090,7 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     rc = zero_f32(s, reinterpret_cast<float *>(base + oover), nc * M);
     if (rc) return rc;
   }
-  const float wmax_host = fabsf(wscalar);
+  const float wmax_host = finite_abs(wscalar);
   if (weight && !wmax_in) {
     rc = abs_max(s, weight, uint32_t(int64_t(b.n) * nc), reinterpret_cast<float *>(base + owmax));
     if (rc) return rc;
@@ -1029,7 +1106,7 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     const unsigned grid = 8u * per * unsigned(nc);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kWindowThreads), 0, s, b.spos, weight, nc, wscalar, b.desc,
                        b.total_chunks, b.g, b.M, wmax_dev, wmax_host, b.n, partial, overflow, int(accumulate), mesh,
-                       per, b.slabs);
+                       per, b.slabs, b.nonfinite);
   }
   JPM_LAUNCHED("bin_accum_kernel");
   {
@@ -1043,7 +1120,7 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     const unsigned grid = slab_grid(b.K, true, per);
     hipLaunchKernelGGL(kern, dim3(grid, nc), dim3(256), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
                        int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh, b.K, per,
-                       true);
+                       true, b.nonfinite, b.spos, b.desc, weight, wscalar);
   }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;

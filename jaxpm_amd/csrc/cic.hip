@@ -34,8 +34,8 @@ __device__ __forceinline__ Axis axis_abs(float x, int n) {
   a.w1 = 1.0f - fabsf(u1);
   a.d0 = u0 >= 0.0f ? -1.0f : 1.0f;
   a.d1 = u1 >= 0.0f ? -1.0f : 1.0f;
-  a.i0 = wrap_index(int(f), n);
-  a.i1 = wrap_index(int(f1), n);
+  a.i0 = wrap_index(cell_int(f), n);
+  a.i1 = wrap_index(cell_int(f1), n);
   return a;
 }
 
@@ -164,23 +164,55 @@ paint_atomic_kernel(const float *__restrict__ pos, const float *__restrict__ wei
 // Deterministic paint: every deposit is rounded to a 64-bit fixed-point
 // integer with a power-of-two scale and summed with integer atomics, which
 // are associative, so the result does not depend on the order in which
-// particles land.  The scale 2^e keeps sum |w| * 2^e below 2^62.
+// particles land.  The scale 2^e keeps sum |w| * 2^e below 2^62 (w over the
+// finite weights).  A deposit that is NaN or +-Inf (a non-finite weight or
+// position) skips the fixed-point sum and sets *nonfinite; nonfinite_add_kernel
+// then adds it to the converted mesh as a float.
 template <bool REL, bool HAS_W>
 __global__ void __launch_bounds__(256)
 paint_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weight, float wscalar,
                    uint32_t n, Grid g, const float *__restrict__ wmax_dev, float wmax_host,
-                   unsigned long long *__restrict__ acc) {
+                   unsigned long long *__restrict__ acc, uint32_t *__restrict__ nonfinite) {
   float wmax = HAS_W ? *wmax_dev : wmax_host;
   double scale = fixed_point_scale(wmax, n);
+  bool hit = false;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     Cic c;
     particle_cic<REL>(pos, p, g, c);
     float wp = HAS_W ? weight[p] : wscalar;
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
-      double v = double(wp * c.w[o]) * scale;
-      long long q = (long long)rint(v);
+      const float d = wp * c.w[o];
+      if (!isfinite(d)) {
+        hit = true;
+        continue;
+      }
+      long long q = (long long)rint(double(d) * scale);
       atomicAdd(&acc[c.idx[o]], (unsigned long long)q);
+    }
+  }
+  if (hit) atomicOr(nonfinite, 1u);
+}
+
+// The non-finite deposits of a deterministic paint, added as floats to the
+// converted mesh (nc channels, channel c of particle p weighted by
+// weight[p * nc + c], or wscalar with no weights).  Exits at once unless the
+// paint flagged one.
+template <bool REL>
+__global__ void __launch_bounds__(256)
+nonfinite_add_kernel(const float *__restrict__ pos, const float *__restrict__ weight, float wscalar, int nc,
+                     uint32_t n, Grid g, uint32_t m, const uint32_t *__restrict__ nonfinite, float *__restrict__ mesh) {
+  if (*nonfinite == 0u) return;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    Cic c;
+    particle_cic<REL>(pos, p, g, c);
+    for (int ch = 0; ch < nc; ++ch) {
+      const float wp = weight ? weight[size_t(p) * nc + ch] : wscalar;
+#pragma unroll
+      for (int o = 0; o < 8; ++o) {
+        const float d = wp * c.w[o];
+        if (!isfinite(d)) unsafeAtomicAdd(&mesh[size_t(ch) * m + c.idx[o]], d);
+      }
     }
   }
 }
@@ -197,10 +229,10 @@ abs_max_kernel(const float *__restrict__ w, uint32_t n, float *__restrict__ out)
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   for (uint32_t p = tid; p < nv; p += stride) {
     const float4 q = w4[p];
-    m = fmaxf(m, fmaxf(fmaxf(fabsf(q.x), fabsf(q.y)), fmaxf(fabsf(q.z), fabsf(q.w))));
+    m = fmaxf(m, fmaxf(fmaxf(finite_abs(q.x), finite_abs(q.y)), fmaxf(finite_abs(q.z), finite_abs(q.w))));
   }
-  for (uint32_t p = tid; p < h; p += stride) m = fmaxf(m, fabsf(w[p]));
-  for (uint32_t p = h + 4u * nv + tid; p < n; p += stride) m = fmaxf(m, fabsf(w[p]));
+  for (uint32_t p = tid; p < h; p += stride) m = fmaxf(m, finite_abs(w[p]));
+  for (uint32_t p = h + 4u * nv + tid; p < n; p += stride) m = fmaxf(m, finite_abs(w[p]));
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
@@ -274,8 +306,10 @@ paint_multi_atomic_kernel(const float *__restrict__ pos, const float *__restrict
 template <bool REL, int NC>
 __global__ void __launch_bounds__(256)
 paint_multi_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weight, uint32_t n, Grid g,
-                         uint32_t m, const float *__restrict__ wmax_dev, unsigned long long *__restrict__ acc) {
+                         uint32_t m, const float *__restrict__ wmax_dev, unsigned long long *__restrict__ acc,
+                         uint32_t *__restrict__ nonfinite) {
   const double scale = fixed_point_scale(*wmax_dev, n);
+  bool hit = false;
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     Cic c;
     particle_cic<REL>(pos, p, g, c);
@@ -284,11 +318,17 @@ paint_multi_fixed_kernel(const float *__restrict__ pos, const float *__restrict_
       const float wp = weight[size_t(p) * NC + ch];
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
-        long long q = (long long)rint(double(wp * c.w[o]) * scale);
+        const float d = wp * c.w[o];
+        if (!isfinite(d)) {
+          hit = true;
+          continue;
+        }
+        long long q = (long long)rint(double(d) * scale);
         atomicAdd(&acc[size_t(ch) * m + c.idx[o]], (unsigned long long)q);
       }
     }
   }
+  if (hit) atomicOr(nonfinite, 1u);
 }
 
 // Gradient of CIC sampling with respect to the particle coordinate:
@@ -460,21 +500,23 @@ int cic_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n
   JPM_REQUIRE(ws != nullptr && ws_bytes >= need, "deterministic paint needs %zu workspace bytes, got %zu",
               need, ws_bytes);
   auto *acc = static_cast<unsigned long long *>(ws);
+  // [0] max |finite weight|, [1] the non-finite flag
   float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(m) * 8));
+  uint32_t *nonfinite = reinterpret_cast<uint32_t *>(wmax + 1);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, acc, uint32_t(m));
   JPM_LAUNCHED("zero_u64_kernel");
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 2u);
   if (hw) {
-    hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
     {
       JPM_TIMED(s, "abs_max_kernel");
       hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, weight, uint32_t(n), wmax);
     }
     JPM_LAUNCHED("abs_max_kernel");
   }
-  const float wh = fabsf(wscalar);
+  const float wh = finite_abs(wscalar);
 #define JPM_PAINT(R, W)                                                                          \
   hipLaunchKernelGGL((paint_fixed_kernel<R, W>), dim3(grid), dim3(256), 0, s, pos, weight, wscalar,  \
-                     uint32_t(n), g, wmax, wh, acc)
+                     uint32_t(n), g, wmax, wh, acc, nonfinite)
   if (rel) {
     if (hw) JPM_PAINT(true, true); else JPM_PAINT(true, false);
   } else {
@@ -485,6 +527,13 @@ int cic_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, acc, uint32_t(m),
                      uint32_t(n), wmax, wh, int(hw), int(!(flags & JPM_ZERO_MESH)), mesh);
   JPM_LAUNCHED("fixed_to_float_kernel");
+  if (rel)
+    hipLaunchKernelGGL(nonfinite_add_kernel<true>, dim3(grid), dim3(256), 0, s, pos, weight, wscalar, 1,
+                       uint32_t(n), g, uint32_t(m), nonfinite, mesh);
+  else
+    hipLaunchKernelGGL(nonfinite_add_kernel<false>, dim3(grid), dim3(256), 0, s, pos, weight, wscalar, 1,
+                       uint32_t(n), g, uint32_t(m), nonfinite, mesh);
+  JPM_LAUNCHED("nonfinite_add_kernel");
   return JPM_OK;
 }
 
@@ -573,9 +622,11 @@ int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int
   JPM_REQUIRE(ws != nullptr && ws_bytes >= need, "deterministic paint needs %zu workspace bytes, got %zu", need,
               ws_bytes);
   auto *acc = static_cast<unsigned long long *>(ws);
+  // [0] max |finite weight|, [1] the non-finite flag
   float *wmax = reinterpret_cast<float *>(static_cast<char *>(ws) + align_up(size_t(nc) * size_t(m) * 8));
+  uint32_t *nonfinite = reinterpret_cast<uint32_t *>(wmax + 1);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m));
-  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 1u);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, s, wmax, 2u);
   {
     JPM_TIMED(s, "abs_max_kernel");
     hipLaunchKernelGGL(abs_max_kernel, dim3(grid_for(n * nc, 256, 2048)), dim3(256), 0, s, weight,
@@ -584,7 +635,7 @@ int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int
   JPM_LAUNCHED("abs_max_kernel");
 #define JPM_PM(R, C)                                                                                  \
   hipLaunchKernelGGL((paint_multi_fixed_kernel<R, C>), dim3(grid), dim3(256), 0, s, pos, weight, uint32_t(n), \
-                     g, uint32_t(m), wmax, acc)
+                     g, uint32_t(m), wmax, acc, nonfinite)
 #define JPM_PM_NC(R)              \
   switch (nc) {                   \
     case 1: JPM_PM(R, 1); break;  \
@@ -603,6 +654,13 @@ int cic_paint_multi(hipStream_t s, int nx, int ny, int nz, const float *pos, int
   hipLaunchKernelGGL(fixed_to_float_kernel, dim3(grid_for(nc * m, 256)), dim3(256), 0, s, acc, uint32_t(nc * m),
                      uint32_t(n), wmax, 0.0f, 1, int(!(flags & JPM_ZERO_MESH)), mesh);
   JPM_LAUNCHED("fixed_to_float_kernel");
+  if (rel)
+    hipLaunchKernelGGL(nonfinite_add_kernel<true>, dim3(grid), dim3(256), 0, s, pos, weight, 0.0f, nc, uint32_t(n),
+                       g, uint32_t(m), nonfinite, mesh);
+  else
+    hipLaunchKernelGGL(nonfinite_add_kernel<false>, dim3(grid), dim3(256), 0, s, pos, weight, 0.0f, nc, uint32_t(n),
+                       g, uint32_t(m), nonfinite, mesh);
+  JPM_LAUNCHED("nonfinite_add_kernel");
   return JPM_OK;
 }
 

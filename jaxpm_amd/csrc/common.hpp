@@ -103,6 +103,22 @@ struct FastDiv {
 
 inline size_t align_up(size_t x, size_t a = 256) { return (x + a - 1) / a * a; }
 
+// int32(f) of a floor value f, with NaN, +-Inf and values outside the int32
+// range mapped to 0 (a C++ float-to-int conversion of those is undefined):
+// non-finite positions then index valid cells, with NaN weights.
+__host__ __device__ inline int cell_int(float f) {
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? int(f) : 0;
+}
+
+// |x| for finite x and 0 for NaN or +-Inf: the bound of the fixed-point
+// scale covers the finite deposits only (non-finite ones bypass the
+// fixed-point sums and reach the mesh as floats, so NaN and Inf propagate to
+// the cells they touch as in a float scatter-add).
+__host__ __device__ inline float finite_abs(float x) {
+  const float a = fabsf(x);
+  return a < INFINITY ? a : 0.0f;
+}
+
 // 2^e with e the largest exponent such that n * wmax * 2^e < 2^62: the scale
 // of the 64-bit fixed-point sums of the deterministic paint.  Every particle
 // spreads |w| over its corners (the CIC weights sum to 1), so no cell of the

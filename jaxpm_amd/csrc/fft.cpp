@@ -94,7 +94,7 @@ int fft_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, v
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
   if (use_native(nx, ny, nz, in)) {
-    JPM_TIMED(s, "fft_r2c");
+    JPM_TIMED(s, "native_r2c (3 passes)");
     return native_r2c(s, nx, ny, nz, batch, in, out);
   }
   const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));
@@ -121,7 +121,7 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch %d out of range", batch);
   JPM_REQUIRE(in && out, "null FFT buffer");
   if (use_native(nx, ny, nz, out)) {
-    JPM_TIMED(s, "fft_c2r");
+    JPM_TIMED(s, "native_c2r (3 passes)");
     return native_c2r(s, nx, ny, nz, batch, in, out);
   }
   const size_t M = size_t(cells(nx, ny, nz)), H = size_t(half_cells(nx, ny, nz));

@@ -488,6 +488,7 @@ bool native_fft_ok(int nx, int ny, int nz) {
 
 static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int outer, size_t lstride,
                      size_t ostride, size_t bstride, int batch, const float2 *tw) {
+  JPM_TIMED(s, "fft_cols_kernel");
   switch (L) {
 #define JPM_COLS(N)                                                                                    \
   case N: {                                                                                            \
@@ -514,6 +515,7 @@ static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const floa
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
   const float2 *i2 = reinterpret_cast<const float2 *>(in);
+  JPM_TIMED(s, dv.g3 ? "fft_r2c_rows_kernel[div]" : "fft_r2c_rows_kernel");
   switch (M) {
 #define JPM_R2C(N)                                                                                            \
   case N: {                                                                                                   \
@@ -535,6 +537,7 @@ static int rows_c2r(hipStream_t s, int nx, int ny, int nz, int batch, float2 *in
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
   float2 *o2 = reinterpret_cast<float2 *>(out);
+  JPM_TIMED(s, "fft_c2r_rows_kernel");
   switch (M) {
 #define JPM_C2R(N)                                                                                            \
   case N:                                                                                                     \
@@ -596,20 +599,23 @@ int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const flo
   DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny)};
   if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv))) return rc;
   if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
-  switch (nx) {
+  {
+    JPM_TIMED(s, "fft_cols_potential_kernel");
+    switch (nx) {
 #define JPM_POT(N)                                                                                        \
-  case N: {                                                                                               \
-    const uint32_t nkzb = (nzh + FftCols<N>::KB - 1) / FftCols<N>::KB;                                   \
-    const uint32_t nblk = nkzb * uint32_t(ny) * uint32_t(batch), per = (nblk + 7) / 8;                    \
-    hipLaunchKernelGGL(fft_cols_potential_kernel<N>, dim3(8 * per), dim3(kFftThreads), 0, s, h, nzh,      \
-                       size_t(ny) * nzh, size_t(nzh), H, tw, nkzb, uint32_t(ny), nblk, per, t, r2, scale, \
-                       scale_dev, deconv);                                                                \
-    break;                                                                                                \
-  }
-    JPM_FFT_LENGTHS(JPM_POT)
+    case N: {                                                                                             \
+      const uint32_t nkzb = (nzh + FftCols<N>::KB - 1) / FftCols<N>::KB;                                 \
+      const uint32_t nblk = nkzb * uint32_t(ny) * uint32_t(batch), per = (nblk + 7) / 8;                  \
+      hipLaunchKernelGGL(fft_cols_potential_kernel<N>, dim3(8 * per), dim3(kFftThreads), 0, s, h, nzh,    \
+                         size_t(ny) * nzh, size_t(nzh), H, tw, nkzb, uint32_t(ny), nblk, per, t, r2,      \
+                         scale, scale_dev, deconv);                                                       \
+      break;                                                                                              \
+    }
+      JPM_FFT_LENGTHS(JPM_POT)
 #undef JPM_POT
-    default:
-      return fail(JPM_E_INVALID, "native FFT length %d", nx);
+      default:
+        return fail(JPM_E_INVALID, "native FFT length %d", nx);
+    }
   }
   JPM_LAUNCHED("fft_cols_potential_kernel");
   if ((rc = cols_pass(s, true, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;

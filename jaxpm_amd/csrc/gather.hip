@@ -212,7 +212,7 @@ __device__ __forceinline__ void corners_global_abs(const float *__restrict__ mes
   uint32_t lo[3], hi[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int fi = int(floorf(p[a]));
+    const int fi = cell_int(floorf(p[a]));
     const int c = uint32_t(fi) < uint32_t(n[a]) ? fi : wrapi(fi, n[a]);
     lo[a] = uint32_t(c);
     hi[a] = c + 1 == n[a] ? 0u : uint32_t(c + 1);
@@ -570,7 +570,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
       co[0] = c0;
       co[1] = c1;
       co[2] = c2;
-      cmax = fmaxf(cmax, fmaxf(fabsf(c0), fmaxf(fabsf(c1), fabsf(c2))));
+      cmax = fmaxf(cmax, fmaxf(finite_abs(c0), fmaxf(finite_abs(c1), finite_abs(c2))));
     }
   }
   if (cot_max) {

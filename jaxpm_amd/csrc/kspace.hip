@@ -520,7 +520,7 @@ int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const floa
     Tabs t;
     rc = get_tabs(s, nx, ny, nz, flags, t);
     if (rc) return rc;
-    JPM_TIMED(s, "fft_potential");
+    JPM_TIMED(s, "potential_solve (5 passes)");
     return native_potential(s, nx, ny, nz, batch, in, spec, out, t, r_split * r_split, scale, scale_dev,
                             int((flags & JPM_K_DECONV) != 0));
   }
@@ -547,7 +547,7 @@ int potential_solve_div(hipStream_t s, int nx, int ny, int nz, int batch, const 
     Tabs t;
     rc = get_tabs(s, nx, ny, nz, flags, t);
     if (rc) return rc;
-    JPM_TIMED(s, "fft_potential[div]");
+    JPM_TIMED(s, "potential_solve[div] (5 passes)");
     return native_potential(s, nx, ny, nz, batch, nullptr, spec, out, t, r_split * r_split, scale, scale_dev,
                             int((flags & JPM_K_DECONV) != 0), g3, chan);
   }

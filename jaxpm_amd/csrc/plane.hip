@@ -36,7 +36,7 @@ __device__ __forceinline__ void cic2(float x, float y, int nx, int ny, Cic2 &c) 
     c.w[o] = kx * ky;
     c.dx[o] = (ex >= 0.0f ? -1.0f : 1.0f) * ky;
     c.dy[o] = kx * (ey >= 0.0f ? -1.0f : 1.0f);
-    c.idx[o] = uint32_t(wrapi(int(cx), nx)) * uint32_t(ny) + uint32_t(wrapi(int(cy), ny));
+    c.idx[o] = uint32_t(wrapi(cell_int(cx), nx)) * uint32_t(ny) + uint32_t(wrapi(cell_int(cy), ny));
   }
 }
 

@@ -93,7 +93,7 @@ __device__ __forceinline__ int wrapi(int c, int n) {
 // Floor cell of a particle: absolute -> int32(floor(x)) mod n (painting.py:
 // 23, 35-37); relative -> the low corner of enmesh (painting_utils.py:48-57),
 // or, when that corner is dropped, the high corner minus one.
-__device__ __forceinline__ int floor_cell_abs(float x, int n) { return wrapi(int(floorf(x)), n); }
+__device__ __forceinline__ int floor_cell_abs(float x, int n) { return wrapi(cell_int(floorf(x)), n); }
 
 __device__ __forceinline__ float jmod(float x, float L) {
   float m = fmodf(x, L);
@@ -164,6 +164,9 @@ struct BinsT {
   const float4 *spos;
   const ChunkDesc *desc;
   const uint32_t *nchunk, *choff, *total_chunks;
+  // set by a paint of these bins that met a NaN or +-Inf deposit; cleared by
+  // the sort
+  uint32_t *nonfinite;
 };
 using Bins = BinsT<OneSim>;
 using BatchBins = BinsT<SimTable>;
@@ -222,7 +225,7 @@ __device__ __forceinline__ LAxis laxis_abs(float x, int origin, int n) {
   const float f1 = f + 1.0f;
   a.w0 = 1.0f - fabsf(x - f);
   a.w1 = 1.0f - fabsf(x - f1);
-  const int c0 = wrapi(int(f), n);
+  const int c0 = wrapi(cell_int(f), n);
   a.l0 = c0 - origin;  // the floor cell lies in the tile by construction
   a.l1 = a.l0 + 1;
   return a;

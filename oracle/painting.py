@@ -125,7 +125,9 @@ def cic_paint_dx(displacements, weight=1.0, dtype=np.float64):
     wt = np.asarray(weight, dtype).reshape(-1)
     if wt.size > 1 and wt.size != int(np.prod(shape)):
         raise ValueError("Weight shape must match particle shape")
-    w = w * (wt if wt.size > 1 else wt[0]) * valid
+    # dropped corners deposit nothing, not even a NaN weight (the
+    # scatter's out-of-bounds mode drops the update)
+    w = np.where(valid, w * (wt if wt.size > 1 else wt[0]), 0)
     return _deposit(shape, idx, w).astype(dtype)
 
 
