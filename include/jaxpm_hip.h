@@ -119,6 +119,31 @@ int jpm_cic_read_f32(void *stream, int nx, int ny, int nz, const float *mesh,
                      int ncomp, int64_t comp_stride, const float *pos,
                      int64_t n, float *out, uint32_t flags);
 
+/* Float64 CIC paint and read: jpm_cic_paint_f32 / jpm_cic_read_f32 in double
+ * precision, the reference's computation under jax_enable_x64 (its
+ * equivalence tests run in float64, tests/test_distributed_pm.py:30).
+ * Deposits are summed with float64 atomics; JPM_DETERMINISTIC is rejected
+ * (a float32 mode).  Flags: JPM_RELATIVE, JPM_ZERO_MESH.  Replaces
+ * painting.py:15-45 / 78-106 and, relative, 161-189 / 218-236. */
+int jpm_cic_paint_f64(void *stream, int nx, int ny, int nz, const double *pos,
+                      int64_t n, const double *weight, double wscalar,
+                      double *mesh, uint32_t flags);
+int jpm_cic_read_f64(void *stream, int nx, int ny, int nz, const double *mesh,
+                     int ncomp, int64_t comp_stride, const double *pos,
+                     int64_t n, double *out, uint32_t flags);
+
+/* Float64 pm_forces (pm.py:12-58 under jax_enable_x64): paint of the n
+ * positions (or the real density `delta` when not NULL), hipFFT D2Z, the
+ * force multipliers -g_d * invlaplace * longrange (k-space flags as for
+ * jpm_kspace_force_f32) in double, three Z2D, read at the positions into
+ * out3[n][3].  flags: JPM_RELATIVE for displacements.  Workspace:
+ * jpm_pm_forces_f64_workspace_size bytes. */
+size_t jpm_pm_forces_f64_workspace_size(int nx, int ny, int nz);
+int jpm_pm_forces_f64(void *stream, int nx, int ny, int nz, const double *pos,
+                      int64_t n, const double *delta, double r_split,
+                      uint32_t flags, uint32_t kflags, double *out3,
+                      void *ws, size_t ws_bytes);
+
 /* Bytes of workspace jpm_cic_paint_multi_f32 needs (0 unless deterministic). */
 size_t jpm_cic_paint_multi_workspace_size(int nx, int ny, int nz, int nc,
                                           uint32_t flags);

@@ -6,9 +6,9 @@ Modules mirror the reference package layout (``jaxpm.painting``,
 ``libjaxpm_hip.so`` (hand-written HIP kernels + hipFFT), loaded lazily on
 first use.  Build it with ``python -m jaxpm_amd.build``.
 """
-from . import cosmology, growth  # noqa: F401 (host-only, import eagerly)
+from . import config, cosmology, growth  # noqa: F401 (host-only, import eagerly)
 
-__all__ = ["painting", "kernels", "pm", "ode", "growth", "cosmology", "distributed", "utils", "integrate"]
+__all__ = ["config", "painting", "kernels", "pm", "ode", "growth", "cosmology", "distributed", "utils", "integrate"]
 
 
 def __getattr__(name):

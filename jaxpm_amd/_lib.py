@@ -56,6 +56,10 @@ SIGNATURES = {
     "jpm_cic_paint_multi_workspace_size": (_sz, [_i, _i, _i, _i, _u32]),
     "jpm_cic_paint_multi_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _i, _vp, _u32, _vp, _sz]),
     "jpm_cic_grad_pos_f32": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _vp, _f, _vp, _u32, _i]),
+    "jpm_cic_paint_f64": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _d, _vp, _u32]),
+    "jpm_cic_read_f64": (_i, [_vp, _i, _i, _i, _vp, _i, _i64, _vp, _i64, _vp, _u32]),
+    "jpm_pm_forces_f64_workspace_size": (_sz, [_i, _i, _i]),
+    "jpm_pm_forces_f64": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _d, _u32, _u32, _vp, _vp, _sz]),
     "jpm_fft_r2c_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "jpm_fft_c2r_f32": (_i, [_vp, _i, _i, _i, _i, _vp, _vp]),
     "jpm_fft_c2c_workspace_size": (_sz, [_i, _i, _i, _i]),

@@ -79,6 +79,39 @@ def paint(mesh, pos, weight=None, wscalar=1.0, relative=False, deterministic=Fal
     return mesh
 
 
+def paint_f64(mesh, pos, weight=None, wscalar=1.0, relative=False, zero=False):
+    """Float64 paint into ``mesh`` (float64, updated in place)."""
+    nx, ny, nz = _dims(mesh.shape)
+    n = pos.shape[0]
+    flags = (_lib.RELATIVE if relative else 0) | (_lib.ZERO_MESH if zero else 0)
+    call("jpm_cic_paint_f64", stream(), nx, ny, nz, ptr(pos), n, ptr(weight), float(wscalar), ptr(mesh), flags)
+    return mesh
+
+
+def read_f64(mesh4, pos, relative=False):
+    """Float64 read of nc meshes (nc, nx, ny, nz) at pos (n, 3) -> (n, nc)."""
+    nc = mesh4.shape[0]
+    nx, ny, nz = _dims(mesh4.shape[1:])
+    n = pos.shape[0]
+    out = torch.empty((n, nc), dtype=torch.float64, device=pos.device)
+    call("jpm_cic_read_f64", stream(), nx, ny, nz, ptr(mesh4), nc, nx * ny * nz, ptr(pos), n, ptr(out),
+         _lib.RELATIVE if relative else 0)
+    return out
+
+
+def pm_forces_f64(pos, shape, delta=None, r_split=0.0, relative=False, kf=0):
+    """Float64 pm_forces: (n, 3) forces at pos from its paint (or the real
+    density ``delta``)."""
+    nx, ny, nz = _dims(shape)
+    n = pos.shape[0]
+    wsb = size("jpm_pm_forces_f64_workspace_size", nx, ny, nz)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=pos.device)
+    out = torch.empty((n, 3), dtype=torch.float64, device=pos.device)
+    call("jpm_pm_forces_f64", stream(), nx, ny, nz, ptr(pos), n, ptr(delta), float(r_split),
+         _lib.RELATIVE if relative else 0, kf, ptr(out), ptr(ws), wsb)
+    return out
+
+
 def paint_multi(shape, pos, weights, relative=False, deterministic=False, method="binned"):
     """nc meshes (nc,nx,ny,nz): channel c painted with weights[:, c]."""
     nx, ny, nz = _dims(shape)

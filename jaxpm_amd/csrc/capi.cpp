@@ -61,6 +61,12 @@ int kspace_linear_field(hipStream_t, int, int, int, void *, float, float, float,
                         int, float);
 int kspace_shear6(hipStream_t, int, int, int, const void *, void *, float, uint32_t);
 int lpt2_source(hipStream_t, int64_t, const float *, float *);
+int cic_paint_f64(hipStream_t, int, int, int, const double *, int64_t, const double *, double, double *, uint32_t);
+int cic_read_f64(hipStream_t, int, int, int, const double *, int, int64_t, const double *, int64_t, double *,
+                 uint32_t);
+size_t pm_forces_f64_workspace_size(int, int, int);
+int pm_forces_f64(hipStream_t, int, int, int, const double *, int64_t, const double *, double, uint32_t, uint32_t,
+                  double *, void *, size_t);
 int cic_paint_multi(hipStream_t, int, int, int, const float *, int64_t, const float *, int, float *, uint32_t, void *,
                     size_t);
 int cic_grad_pos(hipStream_t, int, int, int, const float *, int, int64_t, const float *, int64_t, const float *,
@@ -567,6 +573,23 @@ int jpm_cic_paint_f32(void *stream, int nx, int ny, int nz, const float *pos, in
 int jpm_cic_read_f32(void *stream, int nx, int ny, int nz, const float *mesh, int ncomp, int64_t comp_stride,
                      const float *pos, int64_t n, float *out, uint32_t flags) {
   JPM_GUARD(cic_read(as_stream(stream), nx, ny, nz, mesh, ncomp, comp_stride, pos, n, out, flags));
+}
+
+int jpm_cic_paint_f64(void *stream, int nx, int ny, int nz, const double *pos, int64_t n, const double *weight,
+                      double wscalar, double *mesh, uint32_t flags) {
+  JPM_GUARD(cic_paint_f64(as_stream(stream), nx, ny, nz, pos, n, weight, wscalar, mesh, flags));
+}
+
+int jpm_cic_read_f64(void *stream, int nx, int ny, int nz, const double *mesh, int ncomp, int64_t comp_stride,
+                     const double *pos, int64_t n, double *out, uint32_t flags) {
+  JPM_GUARD(cic_read_f64(as_stream(stream), nx, ny, nz, mesh, ncomp, comp_stride, pos, n, out, flags));
+}
+
+size_t jpm_pm_forces_f64_workspace_size(int nx, int ny, int nz) { return pm_forces_f64_workspace_size(nx, ny, nz); }
+
+int jpm_pm_forces_f64(void *stream, int nx, int ny, int nz, const double *pos, int64_t n, const double *delta,
+                      double r_split, uint32_t flags, uint32_t kflags, double *out3, void *ws, size_t ws_bytes) {
+  JPM_GUARD(pm_forces_f64(as_stream(stream), nx, ny, nz, pos, n, delta, r_split, flags, kflags, out3, ws, ws_bytes));
 }
 
 int jpm_fft_r2c_f32(void *stream, int nx, int ny, int nz, int batch, const float *in, void *out) {

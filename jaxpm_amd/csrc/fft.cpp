@@ -45,8 +45,9 @@ int plan_locked(int nx, int ny, int nz, hipfftType type, int batch, hipfftHandle
   if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftCreate: %s", fft_err(r));
   int n[3] = {nx, ny, nz};
   const long long M = cells(nx, ny, nz), H = half_cells(nx, ny, nz);
-  const int idist = int(type == HIPFFT_C2R ? H : M);
-  const int odist = int(type == HIPFFT_R2C ? H : M);
+  const bool half_in = type == HIPFFT_C2R || type == HIPFFT_Z2D, half_out = type == HIPFFT_R2C || type == HIPFFT_D2Z;
+  const int idist = int(half_in ? H : M);
+  const int odist = int(half_out ? H : M);
   size_t work = 0;
   r = hipfftMakePlanMany(h, 3, n, nullptr, 1, idist, nullptr, 1, odist, type, batch, &work);
   if (r != HIPFFT_SUCCESS) {
@@ -138,6 +139,36 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
       r = hipfftExecC2R(h, static_cast<hipfftComplex *>(in) + size_t(b0) * H, out + size_t(b0) * M);
     if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecC2R: %s", fft_err(r));
   }
+  return JPM_OK;
+}
+
+// Float64 transforms (hipFFT D2Z / Z2D) of a batch of meshes: the float64
+// pm_forces.  Only these real-to-complex and complex-to-real plans are made.
+int fft_d2z(hipStream_t s, int nx, int ny, int nz, int batch, const double *in, void *out) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(batch >= 1 && batch <= 3, "batch %d out of range", batch);
+  JPM_REQUIRE(in && out, "null FFT buffer");
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipfftHandle h;
+  if ((rc = plan_locked(nx, ny, nz, HIPFFT_D2Z, batch, h))) return rc;
+  hipfftResult r = hipfftSetStream(h, s);
+  if (r == HIPFFT_SUCCESS) r = hipfftExecD2Z(h, const_cast<hipfftDoubleReal *>(in), static_cast<hipfftDoubleComplex *>(out));
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecD2Z: %s", fft_err(r));
+  return JPM_OK;
+}
+
+int fft_z2d(hipStream_t s, int nx, int ny, int nz, int batch, void *in, double *out) {
+  int rc = check_dims(nx, ny, nz);
+  if (rc) return rc;
+  JPM_REQUIRE(batch >= 1 && batch <= 3, "batch %d out of range", batch);
+  JPM_REQUIRE(in && out, "null FFT buffer");
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipfftHandle h;
+  if ((rc = plan_locked(nx, ny, nz, HIPFFT_Z2D, batch, h))) return rc;
+  hipfftResult r = hipfftSetStream(h, s);
+  if (r == HIPFFT_SUCCESS) r = hipfftExecZ2D(h, static_cast<hipfftDoubleComplex *>(in), out);
+  if (r != HIPFFT_SUCCESS) return fail(JPM_E_FFT, "hipfftExecZ2D: %s", fft_err(r));
   return JPM_OK;
 }
 

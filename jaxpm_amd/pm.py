@@ -10,7 +10,7 @@ import torch
 
 from . import _autograd, _ops
 from . import growth as gr
-from ._convert import single_device, to_device
+from ._convert import forward_only, single_device, to_device, wants_f64
 from .distributed import normal_field
 
 __all__ = ["pm_forces", "lpt", "linear_field", "TabulatedPk", "lpt_coefficients", "lpt_fields", "lpt_combine"]
@@ -46,10 +46,20 @@ def pm_forces(positions, mesh_shape=None, delta=None, r_split=0, paint_absolute_
         assert delta is not None, "If mesh_shape is not provided, delta should be provided"
         mesh_shape = tuple(delta.shape)
     mesh_shape = tuple(int(s) for s in mesh_shape)
-    pos = to_device(positions)
-    out_shape = pos.shape
     rel = not paint_absolute_pos
     kf = _ops.kflags(grad_order, lapl_fd, compensate)
+    if wants_f64(positions, delta):
+        pos = to_device(positions, torch.float64)
+        forward_only(pos, delta)
+        field = None
+        if delta is not None:
+            if delta.is_complex() if isinstance(delta, torch.Tensor) else np.iscomplexobj(delta):
+                raise NotImplementedError("the float64 pm_forces takes a real density (delta), not its spectrum")
+            field = to_device(delta, torch.float64)
+        return _ops.pm_forces_f64(pos.reshape(-1, 3), mesh_shape, field, float(r_split), rel,
+                                  kf).reshape(pos.shape)
+    pos = to_device(positions)
+    out_shape = pos.shape
     pflat = pos.reshape(-1, 3)
     if delta is None:
         if _needs_grad(pos):
