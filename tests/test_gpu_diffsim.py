@@ -55,7 +55,8 @@ def test_gradients_noise_and_cosmology(gpu, order, absolute):
     rho = model(nt, params)
     loss = ((rho - 1.0 - dev(target)) ** 2).sum() / 0.1 ** 2
     ref_loss = loss_np(oracle_density(noise, shape, box, a_steps, order, absolute), target)
-    assert abs(float(loss) - ref_loss) < 1e-4 * ref_loss
+    print(f"loss {float(loss.detach()):.8e} vs oracle {ref_loss:.8e}", flush=True)
+    assert abs(float(loss.detach()) - ref_loss) < 1e-4 * ref_loss
     loss.backward()
     gn = nt.grad.cpu().numpy().astype(np.float64)
     gp = params.grad.numpy()
@@ -74,10 +75,13 @@ def test_gradients_noise_and_cosmology(gpu, order, absolute):
         fd.append((L(noise + eps * d) - L(noise - eps * d)) / (2 * eps))
         an.append(float(np.sum(gn * d)))
     an, fd = np.array(an), np.array(fd)
+    print(f"noise directions: max |adjoint - FD| / max |FD| = {np.abs(an - fd).max() / np.abs(fd).max():.2e}",
+          flush=True)
     assert np.abs(an - fd).max() < 2e-3 * np.abs(fd).max(), (an, fd)
     for j, (nm, h) in enumerate((("Omega_c", 1e-6), ("sigma8", 1e-6))):
         v = BASE[nm]
         fdp = (L(noise, **{nm: v + h}) - L(noise, **{nm: v - h})) / (2 * h)
+        print(f"{nm}: adjoint {gp[j]:.8e}, FD {fdp:.8e}, rel {abs(gp[j] - fdp) / abs(fdp):.2e}", flush=True)
         assert abs(gp[j] - fdp) < 2e-3 * abs(fdp), (nm, gp[j], fdp)
 
 
