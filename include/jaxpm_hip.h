@@ -176,6 +176,27 @@ int jpm_cic_paint_binned_f32(void *stream, int nx, int ny, int nz,
                              float wscalar, int nc, float *mesh,
                              uint32_t flags, void *ws, size_t ws_bytes);
 
+/* Bytes of device scratch the allocation-free paint holds per GPU (64 MiB). */
+#define JPM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* jpm_cic_paint_binned_f32 without a caller workspace and without any device
+ * allocation: the tile sort, windows and counters live in a file-scope device
+ * array of JPM_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load to
+ * the end of the process (its address is looked up once per device, in
+ * jpm_init or on the first call on that device).  Same arguments, flags,
+ * results and bits as jpm_cic_paint_binned_f32.  A call whose workspace
+ * (jpm_cic_paint_binned_workspace_size) exceeds the scratch returns
+ * JPM_E_INVALID with the sizes in jpm_last_error(); 128^3 particles on a 128^3
+ * mesh fit for nc = 1.  Every call on one GPU uses the same scratch, so calls
+ * that overlap on one GPU (several streams or threads) race; calls ordered on
+ * one stream do not.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). */
+int jpm_cic_paint_binned_noalloc_f32(void *stream, int nx, int ny, int nz,
+                                     const float *pos, int64_t n,
+                                     const float *weights, float wscalar,
+                                     int nc, float *mesh, uint32_t flags);
+
 /* Position cotangent of CIC sampling:
  *   gpos[p][e] (+)= w_p * sum_c cot[p*ncomp+c] * sum_o mesh_c[idx_o] dW_o/dx_e
  * with dW/dx from d|u|/du = +1 at u >= 0 (the jnp.abs convention).  cot NULL

@@ -135,6 +135,7 @@ SIGNATURES = {
     "jpm_pm_forces_workspace_size": (_sz, [_i, _i, _i, _i64, _u32]),
     "jpm_cic_paint_binned_workspace_size": (_sz, [_i, _i, _i, _i64, _i, _u32]),
     "jpm_cic_paint_binned_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _i, _vp, _u32, _vp, _sz]),
+    "jpm_cic_paint_binned_noalloc_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _f, _i, _vp, _u32]),
     "jpm_pm_forces_f32": (_i, [_vp, _i, _i, _i, _vp, _i64, _vp, _vp, _f, _f, _u32, _vp, _sz]),
 }
 

@@ -19,7 +19,7 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libjaxpm_hip.so")
 ARCH = os.environ.get("JPM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["cic.hip", "bin.hip", "stencil.hip", "gather.hip", "kspace.hip", "spectrum.hip", "reduce.hip", "batch.hip", "fftk.hip", "plane.hip", "f64.hip", "fft.cpp", "capi.cpp"]
+SOURCES = ["cic.hip", "bin.hip", "stencil.hip", "gather.hip", "kspace.hip", "spectrum.hip", "reduce.hip", "batch.hip", "fftk.hip", "plane.hip", "f64.hip", "noalloc.hip", "fft.cpp", "capi.cpp"]
 
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

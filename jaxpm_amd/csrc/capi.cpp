@@ -37,7 +37,12 @@ struct TimedLaunch {
 };
 static std::vector<TimedLaunch> g_launches;
 
-bool timing_enabled() { return g_timing.load(std::memory_order_relaxed); }
+// Calls that must not allocate switch the timing off on their thread (its
+// events are created per launch).
+static thread_local int t_timing_paused = 0;
+bool timing_enabled() { return t_timing_paused == 0 && g_timing.load(std::memory_order_relaxed); }
+TimingPause::TimingPause() { ++t_timing_paused; }
+TimingPause::~TimingPause() { --t_timing_paused; }
 
 void timing_record(hipStream_t, const char *name, hipEvent_t begin, hipEvent_t end) {
   std::lock_guard<std::mutex> lk(g_timing_mu);
@@ -55,6 +60,9 @@ int fft_c2c(hipStream_t, int, int, int, int, const void *, void *, int, void *, 
 size_t fft_c2c_workspace_size(int, int, int, int);
 void free_plans();
 void free_tabs();
+int noalloc_init_current_device();
+int binned_paint_noalloc(hipStream_t, int, int, int, const float *, int64_t, const float *, float, int, float *,
+                         uint32_t);
 int kspace_force(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
 int kspace_compensate(hipStream_t, int, int, int, void *, float);
 int kspace_linear_field(hipStream_t, int, int, int, void *, float, float, float, const float *, const float *,
@@ -504,7 +512,8 @@ int jpm_init(void) {
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0) return fail(JPM_E_HIP, "no HIP device available");
-  return JPM_OK;
+  // the allocation-free paint's scratch address on the current device
+  return noalloc_init_current_device();
 }
 
 int jpm_shutdown(void) {
@@ -964,6 +973,11 @@ int jpm_cic_paint_binned_f32(void *stream, int nx, int ny, int nz, const float *
 int jpm_pm_forces_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n, const void *delta_k,
                       float *forces, float r_split, float scale, uint32_t flags, void *ws, size_t ws_bytes) {
   JPM_GUARD(pm_forces(as_stream(stream), nx, ny, nz, pos, n, delta_k, forces, r_split, scale, flags, ws, ws_bytes));
+}
+
+int jpm_cic_paint_binned_noalloc_f32(void *stream, int nx, int ny, int nz, const float *pos, int64_t n,
+                                     const float *weights, float wscalar, int nc, float *mesh, uint32_t flags) {
+  JPM_GUARD(binned_paint_noalloc(as_stream(stream), nx, ny, nz, pos, n, weights, wscalar, nc, mesh, flags));
 }
 
 }  // extern "C"

@@ -43,6 +43,13 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 // wrapped in JPM_TIMED record a pair of HIP events on their stream; the
 // report sums the elapsed times per kernel name.
 bool timing_enabled();
+// While one is alive, the calling thread records no timing events.
+struct TimingPause {
+  TimingPause();
+  ~TimingPause();
+  TimingPause(const TimingPause &) = delete;
+  TimingPause &operator=(const TimingPause &) = delete;
+};
 void timing_record(hipStream_t s, const char *name, hipEvent_t begin, hipEvent_t end);
 
 struct TimedScope {
