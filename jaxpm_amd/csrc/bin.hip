@@ -571,19 +571,50 @@ bin_accum_kernel(const float4 *__restrict__ spos, const float *__restrict__ weig
     int lx[2], ly[2], lz[2], gx[2], gy[2], gz[2];
     float wx[2], wy[2], wz[2];
     window_corners<REL>(rec, t, part0, ox, oy, oz, lx, ly, lz, gx, gy, gz, wx, wy, wz);
+    if (!REL) {
+      // Absolute positions: the 8 corners are the 2^3 block at the floor
+      // cell's window index, so each deposit's LDS address is one base plus a
+      // compile-time offset.  A particle with a finite weight and position
+      // has finite deposits (the CIC weights lie in [0, 1]); the others test
+      // each deposit.
+      unsigned long long *w0 =
+          win + (__umul24(__umul24(uint32_t(lx[0]), uint32_t(kWin)) + uint32_t(ly[0]), uint32_t(kWin)) +
+                 uint32_t(lz[0]));
+      const bool finite = isfinite(wp) && isfinite(rec.x) && isfinite(rec.y) && isfinite(rec.z);
+      if (finite) {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+          const float v = wp * (wx[bx] * wy[by] * wz[bz]);
+          atomicAdd(w0 + (bx * kWin + by) * kWin + bz, fixed_point_q(v, scale));
+        }
+      } else {
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+          const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
+          const float v = wp * (wx[bx] * wy[by] * wz[bz]);
+          if (!isfinite(v)) {
+            hit = true;
+            continue;
+          }
+          atomicAdd(w0 + (bx * kWin + by) * kWin + bz, fixed_point_q(v, scale));
+        }
+      }
+      continue;
+    }
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
       const float wc = wx[bx] * wy[by] * wz[bz];
       const int Lx = lx[bx], Ly = ly[by], Lz = lz[bz];
-      if (REL && (Lx == -2 || Ly == -2 || Lz == -2)) continue;   // dropped corner
+      if (Lx == -2 || Ly == -2 || Lz == -2) continue;   // dropped corner
       const float v = wp * wc;
-      if (!REL || (Lx >= 0 && Ly >= 0 && Lz >= 0)) {
+      if (Lx >= 0 && Ly >= 0 && Lz >= 0) {
         if (!isfinite(v)) {
           hit = true;
           continue;
         }
-        atomicAdd(&win[(Lx * kWin + Ly) * kWin + Lz], (unsigned long long)(long long)rint(double(v) * scale));
+        atomicAdd(&win[(Lx * kWin + Ly) * kWin + Lz], fixed_point_q(v, scale));
       } else {
         // rounding put a relative corner outside the window: global path
         const uint32_t cell = (uint32_t(gx[bx]) * uint32_t(t.ny) + uint32_t(gy[by])) * uint32_t(t.nz) + uint32_t(gz[bz]);
@@ -710,13 +741,20 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
   // faces: the a = 0 plane, then b = 0 with a > 0, then c = 0 with a, b > 0
   const int nf0 = Ly * Lz, nf1 = (Lx - 1) * Lz, nf2 = (Lx - 1) * (Ly - 1);
   const int cells_in_tile = faces_only ? nf0 + nf1 + nf2 : Lx * Ly * Lz;
-  for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
-    int a, b, c;
-    if (full) {
-      // the same enumeration with the divisors known at compile time
-      if (!faces_only) {
-        a = e >> 8; b = (e >> 4) & 15; c = e & 15;
-      } else if (e < 256) {
+  uint32_t src_max = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) src_max = src_max > src_cnt[q] ? src_max : src_cnt[q];
+  if (faces_only && full && PLx == kTile && PLy == kTile && PLz == kTile && src_max <= 1) {
+    // The common case: a full single-chunk tile whose lower neighbours are
+    // full and hold at most one chunk each.  Each face cell loads its own
+    // value and all 7 neighbour halo entries it may take at once (those it
+    // does not take from the tile's own slot, a valid address), then adds
+    // the ones it takes in the order of the general loop below: sources
+    // (sx, sy, sz) lexicographically, sz fastest.  Same sums, same bits.
+    constexpr int kFaces = kTile * kTile + (kTile - 1) * kTile + (kTile - 1) * (kTile - 1);
+    for (int e = threadIdx.x; e < kFaces; e += blockDim.x) {
+      int a, b, c;
+      if (e < 256) {
         a = 0; b = e >> 4; c = e & 15;
       } else if (e < 496) {
         const int f = e - 256;
@@ -725,51 +763,92 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
         const int f = e - 496;
         a = 1 + f / 15; b = 1 + f % 15; c = 0;
       }
-    } else if (!faces_only) {
-      c = e % Lz;
-      const int ab = e / Lz;
-      b = ab % Ly;
-      a = ab / Ly;
-    } else if (e < nf0) {
-      a = 0; b = e / Lz; c = e % Lz;
-    } else if (e < nf0 + nf1) {
-      const int f = e - nf0;
-      a = 1 + f / Lz; b = 0; c = f % Lz;
-    } else {
-      const int f = e - nf0 - nf1;
-      a = 1 + f / (Ly - 1); b = 1 + f % (Ly - 1); c = 0;
-    }
-    const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
-                                      uint32_t(t.nz) + uint32_t(itz * kTile + c);
-    float *dst = mesh + size_t(ch) * M + cell;
-    float acc = faces_only ? *dst : 0.0f;
-    const int nxs = a == 0 ? 2 : 1, nys = b == 0 ? 2 : 1, nzs = c == 0 ? 2 : 1;
-    for (int sx = 0; sx < nxs; ++sx)
-      for (int sy = 0; sy < nys; ++sy)
-        for (int sz = 0; sz < nzs; ++sz) {
-          if (faces_only && !(sx | sy | sz)) continue;  // own deposits: already in acc
-          const int q = sx | (sy << 1) | (sz << 2);
-          const int la = sx ? PLx : a, lb = sy ? PLy : b, lc = sz ? PLz : c;
-          const int li = (la * kWin + lb) * kWin + lc;
-          const uint32_t c0 = src_off[q], cn = src_cnt[q];
-          if (cn == 1) {
-            // a full single-chunk source (absolute mode) holds its halo compactly
-            const bool compact = DIRECT && (sx ? PLx : Lx) == kTile && (sy ? PLy : Ly) == kTile &&
-                                 (sz ? PLz : Lz) == kTile;
-            acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + (compact ? halo_index(la, lb, lc) : li)];
-          } else if (cn > 1) {
-            unsigned long long sum = 0ull;
-            for (uint32_t k = 0; k < cn; ++k) sum += ipart[(size_t(c0 + k) * nc + ch) * kWinCells + li];
-            acc += float(double((long long)sum) * inv);
-          }
-        }
-    if (faces_only) {
+      const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
+                                        uint32_t(t.nz) + uint32_t(itz * kTile + c);
+      float *dst = mesh + size_t(ch) * M + cell;
+      float h[8];
+      h[0] = *dst;
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const int sx = k >> 2, sy = (k >> 1) & 1, sz = k & 1;
+        const int q = sx | (sy << 1) | (sz << 2);
+        const bool take = (!sx || a == 0) && (!sy || b == 0) && (!sz || c == 0) && src_cnt[q] == 1;
+        const uint32_t slot = take ? src_off[q] : src_off[0];
+        h[k] = partial[((size_t(slot) * nc + ch) * kWinCells) * 2 +
+                       halo_index(sx ? kTile : a, sy ? kTile : b, sz ? kTile : c)];
+      }
+      float acc = h[0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) {
+        const int sx = k >> 2, sy = (k >> 1) & 1, sz = k & 1;
+        const int q = sx | (sy << 1) | (sz << 2);
+        const bool take = (!sx || a == 0) && (!sy || b == 0) && (!sz || c == 0) && src_cnt[q] == 1;
+        acc = take ? acc + h[k] : acc;
+      }
       *dst = acc;
-      continue;
     }
-    float v = acc;
-    if (has_overflow) v += overflow[size_t(ch) * M + cell];
-    *dst = accumulate ? *dst + v : v;
+  } else {
+    for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
+      int a, b, c;
+      if (full) {
+        // the same enumeration with the divisors known at compile time
+        if (!faces_only) {
+          a = e >> 8; b = (e >> 4) & 15; c = e & 15;
+        } else if (e < 256) {
+          a = 0; b = e >> 4; c = e & 15;
+        } else if (e < 496) {
+          const int f = e - 256;
+          a = 1 + (f >> 4); b = 0; c = f & 15;
+        } else {
+          const int f = e - 496;
+          a = 1 + f / 15; b = 1 + f % 15; c = 0;
+        }
+      } else if (!faces_only) {
+        c = e % Lz;
+        const int ab = e / Lz;
+        b = ab % Ly;
+        a = ab / Ly;
+      } else if (e < nf0) {
+        a = 0; b = e / Lz; c = e % Lz;
+      } else if (e < nf0 + nf1) {
+        const int f = e - nf0;
+        a = 1 + f / Lz; b = 0; c = f % Lz;
+      } else {
+        const int f = e - nf0 - nf1;
+        a = 1 + f / (Ly - 1); b = 1 + f % (Ly - 1); c = 0;
+      }
+      const uint32_t cell = cell0 + (uint32_t(itx * kTile + a) * uint32_t(t.ny) + uint32_t(ity * kTile + b)) *
+                                        uint32_t(t.nz) + uint32_t(itz * kTile + c);
+      float *dst = mesh + size_t(ch) * M + cell;
+      float acc = faces_only ? *dst : 0.0f;
+      const int nxs = a == 0 ? 2 : 1, nys = b == 0 ? 2 : 1, nzs = c == 0 ? 2 : 1;
+      for (int sx = 0; sx < nxs; ++sx)
+        for (int sy = 0; sy < nys; ++sy)
+          for (int sz = 0; sz < nzs; ++sz) {
+            if (faces_only && !(sx | sy | sz)) continue;  // own deposits: already in acc
+            const int q = sx | (sy << 1) | (sz << 2);
+            const int la = sx ? PLx : a, lb = sy ? PLy : b, lc = sz ? PLz : c;
+            const int li = (la * kWin + lb) * kWin + lc;
+            const uint32_t c0 = src_off[q], cn = src_cnt[q];
+            if (cn == 1) {
+              // a full single-chunk source (absolute mode) holds its halo compactly
+              const bool compact = DIRECT && (sx ? PLx : Lx) == kTile && (sy ? PLy : Ly) == kTile &&
+                                   (sz ? PLz : Lz) == kTile;
+              acc += partial[((size_t(c0) * nc + ch) * kWinCells) * 2 + (compact ? halo_index(la, lb, lc) : li)];
+            } else if (cn > 1) {
+              unsigned long long sum = 0ull;
+              for (uint32_t k = 0; k < cn; ++k) sum += ipart[(size_t(c0 + k) * nc + ch) * kWinCells + li];
+              acc += float(double((long long)sum) * inv);
+            }
+          }
+      if (faces_only) {
+        *dst = acc;
+        continue;
+      }
+      float v = acc;
+      if (has_overflow) v += overflow[size_t(ch) * M + cell];
+      *dst = accumulate ? *dst + v : v;
+    }
   }
   if (*nonfinite == 0u) return;
   __threadfence();

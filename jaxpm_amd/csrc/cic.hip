@@ -187,8 +187,7 @@ paint_fixed_kernel(const float *__restrict__ pos, const float *__restrict__ weig
         hit = true;
         continue;
       }
-      long long q = (long long)rint(double(d) * scale);
-      atomicAdd(&acc[c.idx[o]], (unsigned long long)q);
+      atomicAdd(&acc[c.idx[o]], fixed_point_q(d, scale));
     }
   }
   if (hit) atomicOr(nonfinite, 1u);
@@ -323,8 +322,7 @@ paint_multi_fixed_kernel(const float *__restrict__ pos, const float *__restrict_
           hit = true;
           continue;
         }
-        long long q = (long long)rint(double(d) * scale);
-        atomicAdd(&acc[size_t(ch) * m + c.idx[o]], (unsigned long long)q);
+        atomicAdd(&acc[size_t(ch) * m + c.idx[o]], fixed_point_q(d, scale));
       }
     }
   }

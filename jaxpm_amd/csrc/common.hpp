@@ -126,16 +126,33 @@ __host__ __device__ inline float finite_abs(float x) {
   return a < INFINITY ? a : 0.0f;
 }
 
-// 2^e with e the largest exponent such that n * wmax * 2^e < 2^62: the scale
-// of the 64-bit fixed-point sums of the deterministic paint.  Every particle
-// spreads |w| over its corners (the CIC weights sum to 1), so no cell of the
-// fixed-point sum can leave the int64 range.
+// 2^e with e the largest exponent such that n * wmax * 2^e < 2^62 and
+// wmax * 2^e < 2^51: the scale of the 64-bit fixed-point sums of the
+// deterministic paint.  Every particle spreads |w| over its corners (the CIC
+// weights sum to 1), so no cell of the fixed-point sum can leave the int64
+// range, and every single deposit stays below 2^51 in magnitude, the range
+// fixed_point_q converts exactly (it binds only below 2^11 particles).
 __host__ __device__ inline double fixed_point_scale(float wmax, uint32_t n) {
   double bound = double(wmax) * double(n > 0 ? n : 1);
   if (!(bound > 0.0) || !(bound < 1e300)) return 1.0;
-  int e;
-  frexp(bound, &e);  // bound < 2^e
-  return ldexp(1.0, 62 - e);
+  int e, ew;
+  frexp(bound, &e);          // bound < 2^e
+  frexp(double(wmax), &ew);  // wmax < 2^ew
+  return ldexp(1.0, (62 - e) < (51 - ew) ? (62 - e) : (51 - ew));
+}
+
+// rint(double(v) * scale) as a 64-bit two's complement integer, for a power
+// of two `scale` and |v| * scale < 2^51.  The product is exact, and adding
+// 1.5 * 2^52 in the same fused operation rounds it to the nearest integer
+// (ties to even, as rint) inside the binade [2^52, 2^53), whose doubles are
+// the consecutive integers: the result's bit pattern minus that of 1.5 * 2^52
+// (0x4338000000000000, low word zero) is the integer itself.  One fma and one
+// 32-bit subtract instead of a rounding and a double-to-int64 conversion.
+__device__ __forceinline__ unsigned long long fixed_point_q(float v, double scale) {
+  const double t = fma(double(v), scale, 6755399441055744.0);
+  const uint32_t lo = uint32_t(__double2loint(t));
+  const uint32_t hi = uint32_t(__double2hiint(t)) - 0x43380000u;
+  return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
 
 // Number of blocks for a grid-stride loop over `n` items, capped so every
