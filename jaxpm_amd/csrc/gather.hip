@@ -4,11 +4,12 @@
 //
 // One workgroup per chunk of a tile's particles.  The workgroup loads the
 // potential over the tile plus a 2-cell stencil halo, [o-2, o+19)^3 (21^3
-// floats, periodic), into LDS; each particle's 8 CIC corners lie in
-// [o, o+17) so every stencil tap -D_d phi(c) = -(8 (phi(c+1) - phi(c-1)) -
-// (phi(c+2) - phi(c-2))) / 12 is an LDS read.  Results go back to the
-// particle's original index.  Relative-mode corners that rounding puts
-// outside the window (see lcorner_rel) read the mesh directly.
+// floats, periodic), into LDS and turns it into the force window: the
+// 4-point gradient -D_d phi(c) = -(8 (phi(c+1) - phi(c-1)) - (phi(c+2) -
+// phi(c-2))) / 12 at every cell c of [o, o+17)^3, where each particle's 8
+// CIC corners lie.  Results go back to the particle's original index.
+// Relative-mode corners that rounding puts outside the window (see
+// lcorner_rel) read the mesh directly.
 #include "tiles.hpp"
 
 namespace jpm {
@@ -82,29 +83,52 @@ __device__ __forceinline__ int wrap_near(int c, int n) { return c < 0 ? c + n : 
 // [-n, 2n) once each mesh edge is at least 21 cells.
 __device__ __forceinline__ bool near_wrap_ok(const TileGrid &t) { return t.nx >= kPw && t.ny >= kPw && t.nz >= kPw; }
 
-// Load phi over [o - halo, o - halo + edge)^3 (periodic) into win.
-__device__ __forceinline__ void load_window(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
-                                            int halo, int edge, float *win) {
-  const int cells = edge * edge * edge;
-  const bool near = near_wrap_ok(t);
-#pragma unroll 4
-  for (int e = threadIdx.x; e < cells; e += blockDim.x) {
-    const int c = e % edge;
-    const int ab = e / edge;
-    const int b = ab % edge;
-    const int a = ab / edge;
-    int gx, gy, gz;
-    if (near) {
-      gx = wrap_near(ox - halo + a, t.nx);
-      gy = wrap_near(oy - halo + b, t.ny);
-      gz = wrap_near(oz - halo + c, t.nz);
+// Load phi over [o - HALO, o - HALO + EDGE)^3 (periodic) into win, with
+// every load of a thread in flight together: the thread computes the
+// addresses of all its cells, loads them into registers, and only then
+// writes them to LDS.  (Writing each cell as it arrives made the compiler
+// wait on every load before issuing the next one.)  The launch has
+// kWindowThreads threads.  With edges of at least kPw cells the periodic wrap
+// is one compare-and-add per axis, without branches.
+template <int EDGE, int HALO, bool NEAR>
+__device__ __forceinline__ void load_window_batched(const float *__restrict__ phi, const TileGrid &t, int ox, int oy,
+                                                    int oz, float *win) {
+  constexpr int CELLS = EDGE * EDGE * EDGE;
+  constexpr int PER = (CELLS + kWindowThreads - 1) / kWindowThreads;
+  const uint32_t ny = uint32_t(t.ny), nz = uint32_t(t.nz);
+  float v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e = e0 < CELLS ? e0 : CELLS - 1;   // the last pass repeats a cell it does not store
+    const int c = e % EDGE, ab = e / EDGE;
+    const int b = ab % EDGE, a = ab / EDGE;
+    int gx = ox - HALO + a, gy = oy - HALO + b, gz = oz - HALO + c;
+    if (NEAR) {
+      gx = wrap_near(gx, t.nx);
+      gy = wrap_near(gy, t.ny);
+      gz = wrap_near(gz, t.nz);
     } else {
-      gx = wrapn(ox - halo + a, t.nx);
-      gy = wrapn(oy - halo + b, t.ny);
-      gz = wrapn(oz - halo + c, t.nz);
+      gx = wrapn(gx, t.nx);
+      gy = wrapn(gy, t.ny);
+      gz = wrapn(gz, t.nz);
     }
-    win[e] = phi[(uint32_t(gx) * uint32_t(t.ny) + uint32_t(gy)) * uint32_t(t.nz) + uint32_t(gz)];
+    v[i] = phi[(uint32_t(gx) * ny + uint32_t(gy)) * nz + uint32_t(gz)];
   }
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    if (e0 < CELLS) win[e0] = v[i];
+  }
+}
+
+template <int EDGE, int HALO>
+__device__ __forceinline__ void load_window_t(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
+                                              float *win) {
+  if (near_wrap_ok(t))
+    load_window_batched<EDGE, HALO, true>(phi, t, ox, oy, oz, win);
+  else
+    load_window_batched<EDGE, HALO, false>(phi, t, ox, oy, oz, win);
 }
 
 constexpr float kC8 = 8.0f / 12.0f;
@@ -119,86 +143,58 @@ __device__ __forceinline__ void stencil_lds(const float *win, int a, int b, int 
   F[2] = -(kC8 * (win[e + 1] - win[e - 1]) - kC1 * (win[e + 2] - win[e - 2]));
 }
 
-// -D phi at all 8 CIC corners (a..a+1, b..b+1, c..c+1) of one particle
-// (window indices already offset by the halo), F[d][o] with o = bx | by << 1
-// | bz << 2.  The 24 stencils touch 56 distinct cells: the 2^3 corner cells
-// and, per axis and corner column, the cells at -2, -1, +2, +3.  Each F is
-// the same expression as stencil_lds, so the results agree bit for bit.
-__device__ __forceinline__ void stencil8_lds(const float *win, int a, int b, int c, float F[3][8]) {
-  const int sx = kPw * kPw, sy = kPw;
-  const int e0 = (a * kPw + b) * kPw + c;
-  float C[8];
+// The force window: -D_d phi at every cell of the tile's 17^3 window,
+// fw[d * kWinCells + (a * kWin + b) * kWin + c], from the 21^3 potential
+// window held at the start of the same LDS buffer (3 * kWinCells floats).
+// Each force is stencil_lds's expression, so a particle reading its 8
+// corners here gets the bits it would get from the stencils at its corners,
+// while the stencils run once per cell instead of 24 times per particle.
+// Every thread computes its cells into registers; after a barrier (all
+// potential reads done) it stores them over the potential window; a second
+// barrier publishes them.  The launch has kWindowThreads threads.
+__device__ __forceinline__ void build_force_window(float *lds) {
+  constexpr int PER = (kWinCells + kWindowThreads - 1) / kWindowThreads;
+  float f[PER][3];
 #pragma unroll
-  for (int o = 0; o < 8; ++o) C[o] = win[e0 + (o & 1) * sx + ((o >> 1) & 1) * sy + ((o >> 2) & 1)];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // x: column (b + j, c + k); corners o = 0|q<<1 and 1|q<<1
-    const int j = q & 1, k = q >> 1;
-    const int ex = e0 + j * sy + k;
-    const float m2 = win[ex - 2 * sx], m1 = win[ex - sx], p2 = win[ex + 2 * sx], p3 = win[ex + 3 * sx];
-    const float c0 = C[(j << 1) | (k << 2)], c1 = C[1 | (j << 1) | (k << 2)];
-    F[0][(j << 1) | (k << 2)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    F[0][1 | (j << 1) | (k << 2)] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  for (int i = 0; i < PER; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e = e0 < kWinCells ? e0 : kWinCells - 1;
+    const int c = e % kWin, ab = e / kWin;
+    const int b = ab % kWin, a = ab / kWin;
+    stencil_lds(lds, a + kHalo, b + kHalo, c + kHalo, f[i]);
   }
+  __syncthreads();
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // y: column (a + i, c + k)
-    const int i = q & 1, k = q >> 1;
-    const int ey = e0 + i * sx + k;
-    const float m2 = win[ey - 2 * sy], m1 = win[ey - sy], p2 = win[ey + 2 * sy], p3 = win[ey + 3 * sy];
-    const float c0 = C[i | (k << 2)], c1 = C[i | 2 | (k << 2)];
-    F[1][i | (k << 2)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    F[1][i | 2 | (k << 2)] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  for (int i = 0; i < PER; ++i) {
+    const int e = int(threadIdx.x) + i * kWindowThreads;
+    if (e < kWinCells) {
+      lds[e] = f[i][0];
+      lds[kWinCells + e] = f[i][1];
+      lds[2 * kWinCells + e] = f[i][2];
+    }
   }
+  __syncthreads();
+}
+
+// The forces at a particle's 8 corners from the force window, corner o =
+// bx | by << 1 | bz << 2 at window cell (a + bx, b + by, c + bz).
+__device__ __forceinline__ void corners_fw(const float *fw, int a, int b, int c, float F[3][8]) {
+  const float *p = fw + (a * kWin + b) * kWin + c;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // z: column (a + i, b + j)
-    const int i = q & 1, j = q >> 1;
-    const int ez = e0 + i * sx + j * sy;
-    const float m2 = win[ez - 2], m1 = win[ez - 1], p2 = win[ez + 2], p3 = win[ez + 3];
-    const float c0 = C[i | (j << 1)], c1 = C[i | (j << 1) | 4];
-    F[2][i | (j << 1)] = -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    F[2][i | (j << 1) | 4] = -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
+  for (int o = 0; o < 8; ++o) {
+    const int off = (o & 1) * kWin * kWin + ((o >> 1) & 1) * kWin + ((o >> 2) & 1);
+    F[0][o] = p[off];
+    F[1][o] = p[kWinCells + off];
+    F[2][o] = p[2 * kWinCells + off];
   }
 }
 
-// u . (-D phi) at the 8 CIC corners, f[o] = (u0 F[0][o] + u1 F[1][o]) + u2
-// F[2][o] with F as stencil8_lds computes it: the same float operations in
-// the same order, accumulated axis by axis so that only 8 sums stay live.
-__device__ __forceinline__ void stencil8_dot_lds(const float *win, int a, int b, int c, float u0, float u1, float u2,
-                                                 float f[8]) {
-  const int sx = kPw * kPw, sy = kPw;
-  const int e0 = (a * kPw + b) * kPw + c;
-  float C[8];
-#pragma unroll
-  for (int o = 0; o < 8; ++o) C[o] = win[e0 + (o & 1) * sx + ((o >> 1) & 1) * sy + ((o >> 2) & 1)];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int j = q & 1, k = q >> 1;
-    const int ex = e0 + j * sy + k;
-    const float m2 = win[ex - 2 * sx], m1 = win[ex - sx], p2 = win[ex + 2 * sx], p3 = win[ex + 3 * sx];
-    const float c0 = C[(j << 1) | (k << 2)], c1 = C[1 | (j << 1) | (k << 2)];
-    f[(j << 1) | (k << 2)] = u0 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    f[1 | (j << 1) | (k << 2)] = u0 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int i = q & 1, k = q >> 1;
-    const int ey = e0 + i * sx + k;
-    const float m2 = win[ey - 2 * sy], m1 = win[ey - sy], p2 = win[ey + 2 * sy], p3 = win[ey + 3 * sy];
-    const float c0 = C[i | (k << 2)], c1 = C[i | 2 | (k << 2)];
-    f[i | (k << 2)] = f[i | (k << 2)] + u1 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    f[i | 2 | (k << 2)] = f[i | 2 | (k << 2)] + u1 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int i = q & 1, j = q >> 1;
-    const int ez = e0 + i * sx + j * sy;
-    const float m2 = win[ez - 2], m1 = win[ez - 1], p2 = win[ez + 2], p3 = win[ez + 3];
-    const float c0 = C[i | (j << 1)], c1 = C[i | (j << 1) | 4];
-    f[i | (j << 1)] = f[i | (j << 1)] + u2 * -(kC8 * (c1 - m1) - kC1 * (p2 - m2));
-    f[i | (j << 1) | 4] = f[i | (j << 1) | 4] + u2 * -(kC8 * (p2 - c0) - kC1 * (p3 - m1));
-  }
+// -D phi at one window cell from the force window.
+__device__ __forceinline__ void force_fw(const float *fw, int a, int b, int c, float F[3]) {
+  const int e = (a * kWin + b) * kWin + c;
+  F[0] = fw[e];
+  F[1] = fw[kWinCells + e];
+  F[2] = fw[2 * kWinCells + e];
 }
 
 // The 8 corner values of a mesh around an absolute position (cells floor(x)
@@ -289,7 +285,7 @@ template <bool REL, int MODE, class G>
 __global__ void __launch_bounds__(kWindowThreads)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
                 float *__restrict__ vel, uint32_t per, DriftArgs dr) {
-  __shared__ float win[kPwCells];
+  __shared__ float win[3 * kWinCells];   // the potential window, then the force window
   __shared__ uint32_t ncount[27];   // drifted particles per neighbour tile
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
@@ -318,8 +314,9 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
     const float *v = vel + 3 * size_t(__float_as_uint(rec1.w));
     nv0 = v[0]; nv1 = v[1]; nv2 = v[2];
   }
-  load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
+  load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
   __syncthreads();
+  build_force_window(win);
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
     const float pv0 = nv0, pv1 = nv1, pv2 = nv2;
@@ -334,7 +331,7 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
     float s[3] = {0.0f, 0.0f, 0.0f};
     if (!REL || corners_in_window(cs)) {
       float F8[3][8];
-      stencil8_lds(win, cs.l[0][0] + kHalo, cs.l[1][0] + kHalo, cs.l[2][0] + kHalo, F8);
+      corners_fw(win, cs.l[0][0], cs.l[1][0], cs.l[2][0], F8);
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
@@ -352,7 +349,7 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
         const float wc = cs.w[0][bx] * cs.w[1][by] * cs.w[2][bz];
         float F[3];
         if (la >= 0 && lb >= 0 && lc >= 0)
-          stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+          force_fw(win, la, lb, lc, F);
         else
           stencil_global(phi, t, cs.g[0][bx], cs.g[1][by], cs.g[2][bz], F);
         s[0] += F[0] * wc;
@@ -436,7 +433,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
                 const float *cot, float scale_s, const float *__restrict__ scale_dev,
                 float *__restrict__ gpos, uint32_t per, float beta_s, const float *__restrict__ beta_dev,
                 float *cot_out, float *__restrict__ cot_max) {
-  __shared__ float win[kPwCells];
+  __shared__ float win[3 * kWinCells];   // the potential window, then the force window
   __shared__ float rwin[RLDS ? kWinCells : 1];
   __shared__ float wmax[kWindowThreads / 64];
   const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
@@ -471,11 +468,12 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
     if (PF) corners_global_abs(rho_bar, t, rec1, nr);
   }
-  load_window(phi, t, ox, oy, oz, kHalo, kPw, win);
-  if (RLDS) load_window(rho_bar, t, ox, oy, oz, 0, kWin, rwin);
+  load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
+  if (RLDS) load_window_t<kWin, 0>(rho_bar, t, ox, oy, oz, rwin);
   const bool near = near_wrap_ok(t);
   float cmax = 0.0f;
   __syncthreads();
+  build_force_window(win);
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
     const uint32_t idx = __float_as_uint(rec.w);
@@ -498,7 +496,17 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     if (!REL || corners_in_window(cs)) {
       float f8[8];
       const int la0 = cs.l[0][0], lb0 = cs.l[1][0], lc0 = cs.l[2][0];
-      stencil8_dot_lds(win, la0 + kHalo, lb0 + kHalo, lc0 + kHalo, u0, u1, u2, f8);
+      {
+        // u . (-D phi) at the 8 corners, summed as (u0 Fx + u1 Fy) + u2 Fz
+        float F8[3][8];
+        corners_fw(win, la0, lb0, lc0, F8);
+#pragma unroll
+        for (int o = 0; o < 8; ++o) f8[o] = u0 * F8[0][o];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) f8[o] = f8[o] + u1 * F8[1][o];
+#pragma unroll
+        for (int o = 0; o < 8; ++o) f8[o] = f8[o] + u2 * F8[2][o];
+      }
 #pragma unroll
       for (int o = 0; o < 8; ++o) {
         const int bx = o & 1, by = (o >> 1) & 1, bz = (o >> 2) & 1;
@@ -534,7 +542,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
         if (la == -2 || lb == -2 || lc == -2) continue;
         float F[3], rv;
         if (la >= 0 && lb >= 0 && lc >= 0) {
-          stencil_lds(win, la + kHalo, lb + kHalo, lc + kHalo, F);
+          force_fw(win, la, lb, lc, F);
           if (RLDS) {
             rv = rwin[(la * kWin + lb) * kWin + lc];
           } else {
@@ -607,7 +615,7 @@ bin_read_kernel(BinsT<G> bins, const float *__restrict__ mesh_all, uint32_t cstr
   tile_origin(d.tile - sim.tile0, t, itx, ity, itz);
   const int ox = itx * kTile, oy = ity * kTile, oz = itz * kTile;
 #pragma unroll
-  for (int ch = 0; ch < NC; ++ch) load_window(mesh + size_t(ch) * cstride, t, ox, oy, oz, 0, kWin, win + ch * kWinCells);
+  for (int ch = 0; ch < NC; ++ch) load_window_t<kWin, 0>(mesh + size_t(ch) * cstride, t, ox, oy, oz, win + ch * kWinCells);
   __syncthreads();
   const int xs[8] = {0, 1, 0, 0, 1, 1, 0, 1};
   const int ys[8] = {0, 0, 1, 0, 1, 0, 1, 1};
