@@ -230,12 +230,29 @@ __device__ __forceinline__ void lds_fft(float2 *buf, const float2 *tw) {
   if constexpr (L > 1) fft_stage<L, KB, PITCH, INV, 1>(buf, tw);
 }
 
-// The length-L twiddle table into LDS; the barrier after the data load
-// orders it before the first stage that reads it.
-template <int L>
-__device__ __forceinline__ void load_twiddles(float2 *tws, const float2 *__restrict__ tw_all) {
-  for (int e = threadIdx.x; e < L; e += kFftThreads) tws[e] = tw_all[tw_off(L) + e];
-}
+// A twiddle table into LDS in two halves, so that its loads share their
+// latency with the data loads: load() reads entries [off, off + N) of the
+// device table into registers, store() writes them to LDS (the barrier after
+// the data stores orders them before the first stage that reads them).
+template <int N>
+struct TwStage {
+  static constexpr int I = (N + kFftThreads - 1) / kFftThreads;
+  float2 v[I];
+  __device__ __forceinline__ void load(const float2 *__restrict__ tw_all, int off) {
+#pragma unroll
+    for (int i = 0; i < I; ++i) {
+      const int e = int(threadIdx.x) + i * kFftThreads;
+      v[i] = tw_all[off + (e < N ? e : N - 1)];
+    }
+  }
+  __device__ __forceinline__ void store(float2 *tws) const {
+#pragma unroll
+    for (int i = 0; i < I; ++i) {
+      const int e = int(threadIdx.x) + i * kFftThreads;
+      if (e < N) tws[e] = v[i];
+    }
+  }
+};
 
 // Columns per workgroup: the largest power of two <= min(64, 4096 / L).
 constexpr int pow2_floor(int v) { return v < 2 ? 1 : 2 * pow2_floor(v / 2); }
@@ -243,6 +260,34 @@ template <int L>
 struct FftCols {
   static constexpr int KB = pow2_floor((kFftPoints / L) > 64 ? 64 : (kFftPoints / L));
 };
+
+// The KB columns of length L at kz0 .. kz0 + KB - 1 into buf[l * KB + c]:
+// every load of a thread is issued before any LDS store, so they are in
+// flight together.  A column past nzh loads a copy of column nzh - 1 (a valid
+// address): columns transform independently and such a column is never
+// stored back.
+// The length-L twiddles load with them, into tw.
+template <int L, int KB>
+__device__ __forceinline__ void load_cols(const float2 *__restrict__ base, size_t lstride, int kz0, int nzh,
+                                          float2 *buf, const float2 *__restrict__ tw_all, float2 *tw) {
+  constexpr int total = L * KB, iters = (total + kFftThreads - 1) / kFftThreads;
+  float2 v[iters];
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f0 = threadIdx.x + it * kFftThreads;
+    const int f = (total % kFftThreads != 0 && f0 >= total) ? total - 1 : f0;
+    const int c = f & (KB - 1), l = f / KB;
+    v[it] = base[size_t(l) * lstride + (kz0 + c < nzh ? kz0 + c : nzh - 1)];
+  }
+  TwStage<L> tws;
+  tws.load(tw_all, tw_off(L));
+  tws.store(tw);
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads == 0 || f < total) buf[f] = v[it];
+  }
+}
 
 // Column FFTs along one axis of a half spectrum, in place: column c of
 // workgroup (blockIdx.x, outer = blockIdx.y, batch item blockIdx.z) is kz =
@@ -261,17 +306,10 @@ fft_cols_kernel(float2 *__restrict__ data, int nzh, size_t lstride, size_t ostri
   const uint32_t item = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   if (item >= nblk) return;
   const uint32_t kzb = item % nkzb, rest = item / nkzb, outer = rest % nouter, bi = rest / nouter;
-  load_twiddles<L>(tw, tw_all);
   float2 *base = data + size_t(bi) * bstride + size_t(outer) * ostride;
   const int kz0 = int(kzb) * KB;
   constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
-#pragma unroll
-  for (int it = 0; it < iters; ++it) {
-    const int f = threadIdx.x + it * kFftThreads;
-    if (total % kFftThreads != 0 && f >= total) break;
-    const int c = f & (KB - 1), l = f / KB;
-    buf[f] = kz0 + c < nzh ? base[size_t(l) * lstride + kz0 + c] : make_float2(0.0f, 0.0f);
-  }
+  load_cols<L, KB>(base, lstride, kz0, nzh, buf, tw_all, tw);
   __syncthreads();
   lds_fft<L, KB, KB, INV>(buf, tw);
 #pragma unroll
@@ -300,10 +338,14 @@ fft_cols_potential_kernel(float2 *__restrict__ data, int nzh, size_t lstride, si
   const uint32_t item = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
   if (item >= nblk) return;
   const uint32_t kzb = item % nkzb, rest = item / nkzb, outer = rest % nouter, bi = rest / nouter;
-  load_twiddles<L>(tw, tw_all);
   float2 *base = data + size_t(bi) * bstride + size_t(outer) * ostride;
   const int kz0 = int(kzb) * KB;
   constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+  // Element by element: batching these loads as the other passes do raised
+  // its registers past 64 and cut the resident waves from 8 to 6 per SIMD
+  TwStage<L> tws;
+  tws.load(tw_all, tw_off(L));
+  tws.store(tw);
 #pragma unroll
   for (int it = 0; it < iters; ++it) {
     const int f = threadIdx.x + it * kFftThreads;
@@ -384,8 +426,11 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB;
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
-  load_twiddles<M>(twM, tw_all);
-  const float2 *twN = tw_all + tw_off(2 * M);
+  __shared__ float2 twN[M + 1];
+  // twM, and W_{2M}^k for the split, k = 0 .. M: loaded with the data and
+  // ordered before their use by the barrier after it
+  TwStage<M> tsM;
+  TwStage<M + 1> tsN;
   const int r0 = blockIdx.x * KB;
   float2 *dst = out + size_t(blockIdx.y) * out_bstride;
   // rows are contiguous: lanes along l
@@ -401,16 +446,33 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
       buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, r / dv.ny, r % dv.ny, l)
                                           : make_float2(0.0f, 0.0f);
     }
+    tsM.load(tw_all, tw_off(M));
+    tsN.load(tw_all, tw_off(2 * M));
   } else {
     const float2 *src = in + size_t(blockIdx.y) * in_bstride;
+    // all loads of a thread first, then the LDS stores; a row past nrows
+    // loads a copy of row nrows - 1 (rows transform independently and such a
+    // row is never stored)
+    float2 v[iters];
+#pragma unroll
+    for (int it = 0; it < iters; ++it) {
+      const int f0 = threadIdx.x + it * kFftThreads;
+      const int f = (total % kFftThreads != 0 && f0 >= total) ? total - 1 : f0;
+      const int l = f % M, c = f / M;
+      v[it] = src[size_t(r0 + c < nrows ? r0 + c : nrows - 1) * M + l];
+    }
+    tsM.load(tw_all, tw_off(M));
+    tsN.load(tw_all, tw_off(2 * M));
 #pragma unroll
     for (int it = 0; it < iters; ++it) {
       const int f = threadIdx.x + it * kFftThreads;
       if (total % kFftThreads != 0 && f >= total) break;
       const int l = f % M, c = f / M;
-      buf[l * PITCH + c] = r0 + c < nrows ? src[size_t(r0 + c) * M + l] : make_float2(0.0f, 0.0f);
+      buf[l * PITCH + c] = v[it];
     }
   }
+  tsM.store(twM);
+  tsN.store(twN);
   __syncthreads();
   lds_fft<M, KB, PITCH, false>(buf, twM);
   // X[k] = E[k] + W_{2M}^k O[k], E = (Z[k] + conj Z[M-k]) / 2,
@@ -435,33 +497,43 @@ fft_c2r_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
   constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB, nzh = M + 1;
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
-  load_twiddles<M>(twM, tw_all);
+  TwStage<M> tsM;
   const float2 *twN = tw_all + tw_off(2 * M);
   const int r0 = blockIdx.x * KB;
   const float2 *src = in + size_t(blockIdx.y) * in_bstride;
   float2 *dst = out + size_t(blockIdx.y) * out_bstride;
   // Z'[k] = E' + i O', E' = X[k] + conj X[M-k], O' = (X[k] - conj X[M-k]) W_{2M}^{-k};
-  // X[0] and X[M] enter with their real parts only
+  // X[0] and X[M] enter with their real parts only.  Every load of a thread
+  // is issued before the first use; a row past nrows loads a copy of row
+  // nrows - 1 (rows transform independently and such a row is never stored).
   constexpr int iters = (total + kFftThreads - 1) / kFftThreads;
+  float2 va[iters], vb[iters], vt[iters];
+#pragma unroll
+  for (int it = 0; it < iters; ++it) {
+    const int f0 = threadIdx.x + it * kFftThreads;
+    const int f = (total % kFftThreads != 0 && f0 >= total) ? total - 1 : f0;
+    const int k = f % M, c = f / M;
+    const float2 *row = src + size_t(r0 + c < nrows ? r0 + c : nrows - 1) * nzh;
+    va[it] = row[k];
+    vb[it] = row[M - k];
+    vt[it] = twN[k];
+  }
+  tsM.load(tw_all, tw_off(M));
+  tsM.store(twM);
 #pragma unroll
   for (int it = 0; it < iters; ++it) {
     const int f = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads != 0 && f >= total) break;
     const int k = f % M, c = f / M;
-    float2 z = make_float2(0.0f, 0.0f);
-    if (r0 + c < nrows) {
-      const float2 *row = src + size_t(r0 + c) * nzh;
-      float2 a = row[k], b = row[M - k];
-      if (k == 0) {
-        a.y = 0.0f;
-        b.y = 0.0f;
-      }
-      const float2 bc = conjf2(b);
-      const float2 E = cadd(a, bc);
-      const float2 O = cmul(csub(a, bc), conjf2(twN[k]));
-      z = make_float2(E.x - O.y, E.y + O.x);
+    float2 a = va[it], b = vb[it];
+    if (k == 0) {
+      a.y = 0.0f;
+      b.y = 0.0f;
     }
-    buf[k * PITCH + c] = z;
+    const float2 bc = conjf2(b);
+    const float2 E = cadd(a, bc);
+    const float2 O = cmul(csub(a, bc), conjf2(vt[it]));
+    buf[k * PITCH + c] = make_float2(E.x - O.y, E.y + O.x);
   }
   __syncthreads();
   lds_fft<M, KB, PITCH, true>(buf, twM);
