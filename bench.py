@@ -582,11 +582,11 @@ def main():
         tuning, _ = autotune.tune_stepper(shape, x0)
     stepper = integrate.Stepper(shape, N, sort_tuning=tuning)
 
-    def fwd_adj(S, trace=None):
+    def fwd_adj(S, trace=None, x_init=None):
         steps = np.linspace(0.1, 1.0, S + 1)
         sched = integrate.schedule_tensor(integrate.kdk_schedule(cosmo, steps))
         stepper.trace = trace
-        xt = x0.clone().requires_grad_(True)
+        xt = (x0 if x_init is None else x_init).clone().requires_grad_(True)
         vt = v0.clone().requires_grad_(True)
         x, v = integrate.NBody.apply(xt, vt, sched, stepper, args.checkpoint)
         rho = painting.cic_paint(torch.zeros(shape, device=dev), x)
@@ -670,10 +670,10 @@ def main():
     t_dom = gd["ms"] / gd["launches"]
     bytes_dom = gd["bytes"] / gd["launches"]
     achieved = bytes_dom / (t_dom * 1e-3) / 1e9
-    traffic = None
+    pmc_traffic = None
     meas = [PMC_TRAFFIC_BYTES.get((v, n)) for v in gd["variants"]]
     if all(m is not None for m in meas):
-        traffic = sum(m * c for m, c in zip(meas, gd["variants"].values())) / gd["launches"]
+        pmc_traffic = sum(m * c for m, c in zip(meas, gd["variants"].values())) / gd["launches"]
     kernel_groups = {k: {"launches_per_step": round(g["launches"] / args.steps, 2),
                          "ms_per_step": round(g["ms"] / args.steps, 4),
                          "ms_per_launch": round(g["ms"] / g["launches"], 4),
@@ -735,29 +735,44 @@ def main():
             sched = integrate.schedule_tensor(integrate.kdk_schedule(cosmo, np.linspace(0.1, 1.0, P + 1)))
             xt = x0.clone().requires_grad_(True)
             vt = v0.clone().requires_grad_(True)
-            x, v = integrate.leapfrog_autodiff(xt, vt, sched, shape)
+            x, v = integrate.leapfrog_autodiff(xt, vt, sched, shape, trajectory=traj)
             rho = painting.cic_paint(torch.zeros(shape, device=dev), x)
             (((rho - target) ** 2).sum() / sigma ** 2).backward()
-            return xt.grad
+            return x.detach(), xt.grad
 
+        traj = None
         plain_fwd_adj()
         torch.cuda.synchronize()
         base_mem = torch.cuda.memory_allocated()
         torch.cuda.reset_peak_memory_stats()
         t0 = time.perf_counter()
-        gp = plain_fwd_adj()
+        xp, gp = plain_fwd_adj()
         torch.cuda.synchronize()
         t_plain = time.perf_counter() - t0
         mem_plain = torch.cuda.max_memory_allocated() - base_mem
         torch.cuda.reset_peak_memory_stats()
-        _, gc = fwd_adj(P)
+        xc, gc = fwd_adj(P)
         torch.cuda.synchronize()
         mem_custom = torch.cuda.max_memory_allocated() - base_mem
         rel = float((gp - gc).abs().max() / gc.abs().max())
         rms = float((gp - gc).pow(2).mean().sqrt() / gc.pow(2).mean().sqrt())
+        # where the two gradients differ: the plain form rerun, keeping the
+        # positions of its force evaluations (outside the timed runs)
+        traj = []
+        plain_fwd_adj()
+        gap = integrate.gradient_gap(xp, xc, gp, gc, trajectory=traj)
+        traj = None
+        # the custom adjoint against itself, with every initial coordinate
+        # moved by one float32 ulp: the float32 sensitivity of this gradient
+        xn, gn = fwd_adj(P, x_init=torch.nextafter(x0, torch.full_like(x0, float("inf"))))
+        gap["custom_vs_custom_1ulp"] = {
+            "max_rel": float((gn - gc).abs().max() / gc.abs().max()),
+            "rms_rel": float((gn - gc).pow(2).mean().sqrt() / gc.pow(2).mean().sqrt()),
+            "final_position_max_diff_cells": float((xn - xc).abs().max())}
+        del xn, gn
         plain = {"steps_per_s": round(P / t_plain, 3), "steps": P,
                  "speedup_of_custom_adjoint": round(steps_per_s / (P / t_plain), 3),
-                 "grad_max_rel_diff": rel, "grad_rms_rel_diff": rms,
+                 "grad_max_rel_diff": rel, "grad_rms_rel_diff": rms, "grad_gap": gap,
                  "extra_mem_gb_plain": round(mem_plain / 1e9, 2), "extra_mem_gb_custom": round(mem_custom / 1e9, 2),
                  "what": "same leapfrog as tensor expressions over pm_forces (per-op adjoints), autograd over "
                          "the stored trajectory, no checkpointing; grad w.r.t. x0 compared with the custom adjoint"}
@@ -788,7 +803,7 @@ def main():
                                    "xcd_placement": "slabs" if len(tuning) > 3 and tuning[3] else "round-robin"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None if traffic is None else round(traffic),
+                     "traffic": None if pmc_traffic is None else round(pmc_traffic),
                      "algorithmic_bytes": int(bytes_dom), "ms_per_launch": round(t_dom, 4),
                      "launches_per_step": round(gd["launches"] / args.steps, 2),
                      "variants": {k: round(v / args.steps, 2) for k, v in gd["variants"].items()}},

@@ -515,18 +515,21 @@ class NBody(torch.autograd.Function):
         return lx, lv, None, None, None
 
 
-def leapfrog_autodiff(x0, v0, coeffs, mesh_shape, paint_absolute_pos=True):
+def leapfrog_autodiff(x0, v0, coeffs, mesh_shape, paint_absolute_pos=True, trajectory=None):
     """The plain form of the same leapfrog, for comparison with
     :class:`NBody`: every step is a differentiable tensor expression
     (``pm_forces`` through the per-operation adjoints of ``_autograd``, kicks
     and drifts as tensor arithmetic) and autograd differentiates it over the
     stored trajectory -- the analogue of jax.grad through the reference's
     integrator with all intermediates kept (no checkpointing, no fused
-    kernels)."""
+    kernels).  ``trajectory``: a list that receives the (detached) positions
+    of every force evaluation, the initial ones first."""
     from .pm import pm_forces
     k0, beta, gamma = _split(coeffs)
 
     def force(x):
+        if trajectory is not None:
+            trajectory.append(x.detach())
         return pm_forces(x, mesh_shape=mesh_shape, paint_absolute_pos=paint_absolute_pos)
 
     x = x0
@@ -535,6 +538,50 @@ def leapfrog_autodiff(x0, v0, coeffs, mesh_shape, paint_absolute_pos=True):
         x = x + float(beta[i]) * v
         v = v + float(gamma[i]) * force(x)
     return x, v
+
+
+def gradient_gap(xa, xb, ga, gb, trajectory=None, top=100):
+    """Where two float32 evaluations of the same leapfrog gradient differ.
+
+    ``xa``, ``xb``: the final (absolute, grid-unit) positions each evaluation
+    reached; ``ga``, ``gb``: its gradients with respect to the initial
+    positions, all ``(N, 3)``; ``trajectory``: optionally, the positions of
+    every force evaluation of evaluation a.  CIC weights are piecewise
+    linear, so the forces' and the painted loss's derivatives jump where a
+    particle crosses a cell face: two evaluations whose trajectories differ by
+    float rounding put a particle that comes within that rounding of a face on
+    different pieces, and its gradient then differs at O(1).  A particle is
+    "at a face" when its final cells differ between the two evaluations, or a
+    coordinate lies within ``tol`` of an integer at the end of either or at
+    any force evaluation of the trajectory, with ``tol`` ten times the largest
+    final position difference.  Returns the maximum difference relative to
+    max |gb| over all particles and over the others, and how many of the
+    ``top`` largest differences sit at a face."""
+    xa, xb = xa.detach().double(), xb.detach().double()
+    ga, gb = ga.detach().double(), gb.detach().double()
+    dx = float((xa - xb).abs().max())
+    tol = max(10.0 * dx, 1e-6)
+
+    def face_dist(x):
+        f = x - torch.floor(x)
+        return torch.minimum(f, 1.0 - f)
+
+    at_face = ((torch.floor(xa) != torch.floor(xb)).any(1) | (face_dist(xa) < tol).any(1) |
+               (face_dist(xb) < tol).any(1))
+    for xt in trajectory or ():
+        at_face |= (face_dist(xt.detach().double().reshape(-1, 3)) < tol).any(1)
+    scale = float(gb.abs().max())
+    diff = (ga - gb).abs().amax(1)
+    k = min(int(top), diff.numel())
+    idx = torch.topk(diff, k).indices
+    away = diff[~at_face]
+    return {"max_rel": float(diff.max()) / scale,
+            "max_rel_away_from_faces": (float(away.max()) / scale) if away.numel() else 0.0,
+            "rms_rel_away_from_faces": (float(((ga - gb)[~at_face] ** 2).mean().sqrt()) /
+                                        float((gb ** 2).mean().sqrt())) if away.numel() else 0.0,
+            "final_position_max_diff_cells": dx, "face_tol_cells": tol,
+            "particles_at_faces": int(at_face.sum()), "particles": int(diff.numel()),
+            f"top{k}_at_faces": int(at_face[idx].sum())}
 
 
 def leapfrog(pos, vel, a_steps, cosmo, mesh_shape, paint_absolute_pos=True, deterministic=False, r_split=0.0,

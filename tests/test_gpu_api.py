@@ -140,8 +140,12 @@ def test_kernel_helpers_match_oracle(gpu, shape):
         np.testing.assert_allclose(lap, want, rtol=2e-6)
         assert lap.reshape(-1)[0] == 0.0
     assert kernels.longrange_kernel(kv, 0) == 1.0
-    np.testing.assert_allclose(kernels.longrange_kernel(kv, 1.3).cpu().numpy(), ok.longrange_kernel(okv, 1.3),
-                               rtol=2e-6, atol=1e-30)
+    # exp(-kk r^2) in float32: the argument's rounding (a few ulp of kk r^2,
+    # up to ~50 here) is a relative error of the result of that size
+    got = kernels.longrange_kernel(kv, 1.3).cpu().numpy().astype(np.float64)
+    want = ok.longrange_kernel(okv, 1.3)
+    arg = sum(np.asarray(k, dtype=np.float64) ** 2 for k in okv) * 1.3 ** 2
+    assert np.all(np.abs(got - want) <= want * (2e-6 + 4 * 2.0 ** -24 * arg))
     np.testing.assert_allclose(kernels.cic_compensation(kv).cpu().numpy(), ok.cic_compensation(okv), rtol=3e-6)
 
 

@@ -111,7 +111,7 @@ def test_batch48_matches_per_simulation(gpu, batch48, b):
     g_rms = np.sqrt(np.mean((gb - g1) ** 2)) / np.sqrt(np.mean(g1 ** 2))
     print(f"class {n}^3: density rms rel diff {rho_rms:.2e}, gradient rms rel diff {g_rms:.2e}", flush=True)
     assert rho_rms < 1e-4
-    assert g_rms < 2e-3
+    assert g_rms < 1e-3
 
 
 def _oracle_density(noise, n, params):
@@ -152,7 +152,7 @@ def test_batch48_64_against_oracle(batch48):
         fd = (lp - lm) / (2 * eps)
         an = float(np.sum(g * d))
         print(f"direction {i}: adjoint {an:.6e}, central difference {fd:.6e}", flush=True)
-        assert abs(an - fd) < 2e-3 * abs(fd), (an, fd)
+        assert abs(an - fd) < 1e-3 * abs(fd), (an, fd)
 
 
 def test_large_1024_mesh_512_particles(gpu):

@@ -188,7 +188,7 @@ def test_pm_forces_gradient_matches_finite_differences(gpu):
         lm = np.sum(opm.pm_forces(pos - eps * d, mesh_shape=shape) * u)
         fd = (lp - lm) / (2 * eps)
         an = np.sum(grad * d)
-        assert abs(an - fd) < 2e-3 * (abs(fd) + 1e-3), (j, an, fd)
+        assert abs(an - fd) < 1e-3 * (abs(fd) + 1e-3), (j, an, fd)
 
 
 @pytest.mark.parametrize("shape", [(16, 16, 32), (8, 6, 5), (1, 2, 4)])
