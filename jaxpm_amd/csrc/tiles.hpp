@@ -18,7 +18,7 @@ constexpr uint32_t kSlice = 16384;              // default particles per count /
 // allow 4 workgroups per CU, so 512 threads keep 32 waves resident.
 constexpr int kWindowThreads = 512;
 
-// Sort tuning carried in bits 16..24 of the pipeline flags (0 = default):
+// Sort tuning carried in bits 16..25 of the pipeline flags (0 = default):
 // chunk = 2^(9 + c) particles for c = bits 16..19 in 1..6 (1024..32768),
 // slice = 2^(11 + s) particles for s = bits 20..23 in 1..7 (4096..262144).
 inline uint32_t tuned_chunk(uint32_t flags) {
