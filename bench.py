@@ -30,10 +30,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch from hardware counters (FETCH_SIZE and WRITE_SIZE
 # passes over `python bench.py --workload traffic`, each scaled by the same
 # counter's reading for a plain copy of a known 64 MiB: x1.999 for reads,
-# x1.000 for writes), by timed kernel name and mesh side.  The commands and
-# the counter values are in profiles/sorted_gathers_256_traffic.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256): 1267.8e6,
-                     ("bin_grad_kernel[drift]", 256): 1879.4e6}
+# x1.000 for writes), by timed kernel name and mesh side, with the bench's
+# sort tuning.  The commands and the counter values are in
+# profiles/sorted_gathers_256_traffic.md and kernel_bounds_256_counters.md.
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256): 1300.8e6,
+                     ("bin_grad_kernel[drift]", 256): 1917.0e6,
+                     ("bin_accum_kernel[nc=1]", 256): 350.0e6,
+                     ("bin_accum_kernel[nc=3]", 256): 817.4e6}
 
 
 def fixed_tuning(args):
