@@ -287,7 +287,7 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
                 float *__restrict__ vel, uint32_t per, DriftArgs dr) {
   __shared__ float win[3 * kWinCells];   // the potential window, then the force window
   __shared__ uint32_t ncount[27];   // drifted particles per neighbour tile
-  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
+  const uint32_t c = gather_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const bool count_next = MODE == 2 && !REL && dr.next_count;
   if (count_next && threadIdx.x < 27) ncount[threadIdx.x] = 0u;
@@ -436,7 +436,7 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   __shared__ float win[3 * kWinCells];   // the potential window, then the force window
   __shared__ float rwin[RLDS ? kWinCells : 1];
   __shared__ float wmax[kWindowThreads / 64];
-  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
+  const uint32_t c = gather_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
   const SimRef &sim = bins.g.tile_sim(d.tile);
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(kWindowThreads)
 bin_read_kernel(BinsT<G> bins, const float *__restrict__ mesh_all, uint32_t cstride, float *__restrict__ out,
                 uint32_t per) {
   __shared__ float win[NC * kWinCells];
-  const uint32_t c = slab_item(blockIdx.x, per, bins.slabs);
+  const uint32_t c = gather_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
   const SimRef &sim = bins.g.tile_sim(d.tile);
@@ -652,7 +652,7 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
     if (rc) return rc;
   }
   uint32_t per;
-  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, mode == 2 ? "bin_kick_kernel[drift]" : mode == 1 ? "bin_kick_kernel[forces]" : "bin_kick_kernel");
   auto kern = b.rel ? (mode == 1 ? bin_kick_kernel<true, 1, G> : mode == 2 ? bin_kick_kernel<true, 2, G>
                                                                           : bin_kick_kernel<true, 0, G>)
@@ -677,7 +677,7 @@ static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const
     if (rc) return rc;
   }
   uint32_t per;
-  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, cot_out ? "bin_grad_kernel[drift]" : "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
@@ -717,7 +717,7 @@ int bins_read(hipStream_t s, const Bins &b, const float *mesh, int nc, uint32_t 
   JPM_REQUIRE(!b.rel, "the tile-sorted read takes absolute positions");
   JPM_REQUIRE(nc >= 1 && nc <= 3, "the tile-sorted read takes 1 to 3 meshes (got %d)", nc);
   uint32_t per;
-  const dim3 grid(slab_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
   JPM_TIMED(s, "bin_read_kernel");
   auto kern = nc == 1 ? bin_read_kernel<1, OneSim> : nc == 2 ? bin_read_kernel<2, OneSim> : bin_read_kernel<3, OneSim>;
   hipLaunchKernelGGL(kern, grid, block, 0, s, b, mesh, cstride, out, per);

@@ -46,6 +46,27 @@ inline unsigned slab_grid(uint32_t items, bool slabs, uint32_t &per) {
 __device__ __forceinline__ uint32_t slab_item(uint32_t b, uint32_t per, bool slabs) {
   return slabs ? (b & 7u) * per + (b >> 3) : b;
 }
+
+// XCD groups for the window gathers (without slabs): of each 64 consecutive
+// workgroups b = 64k + 8q + x (the dispatcher deals b to XCD x), chunk
+// 64k + 8x + q, so each run of 8 consecutive chunks -- z-neighbouring tiles
+// when single-chunk -- runs on one XCD at once, and the 64-byte lines that
+// their potential windows' rows overlap in (a window row of 21 floats spans
+// three such lines) are fetched into its L2 once.  Runs still alternate over
+// the XCDs, which keeps clustered tiles spread over them (the slab placement,
+// one contiguous eighth of the tiles per XCD, does not).  256^3: force+kick
+// 0.239 -> 0.229 ms, force VJP 0.293 -> 0.282 ms.
+constexpr uint32_t kXcdRun = 8;
+inline unsigned gather_grid(uint32_t items, bool slabs, uint32_t &per) {
+  const unsigned g = slab_grid(items, slabs, per);
+  return slabs ? g : (g + 8u * kXcdRun - 1u) / (8u * kXcdRun) * (8u * kXcdRun);
+}
+
+__device__ __forceinline__ uint32_t gather_item(uint32_t b, uint32_t per, bool slabs) {
+  if (slabs) return slab_item(b, per, true);
+  const uint32_t blk = b / (8u * kXcdRun), r = b % (8u * kXcdRun);
+  return blk * (8u * kXcdRun) + (r & 7u) * kXcdRun + (r >> 3);
+}
 inline uint32_t tuned_slice(uint32_t flags) {
   const uint32_t v = (flags >> 20) & 0xfu;
   return (v >= 1 && v <= 7) ? (1u << (11 + v)) : kSlice;
