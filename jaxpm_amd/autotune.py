@@ -18,7 +18,7 @@ import torch
 from . import _lib, _ops, integrate
 
 CHUNK_CODES = (2, 3, 4, 5)      # 2048, 4096, 8192, 16384 particles
-SLICE_CODES = (2, 3, 4, 5)      # 8192, 16384, 32768, 65536 particles
+SLICE_CODES = (1, 2, 3, 4, 5)   # 4096, 8192, 16384, 32768, 65536 particles
 # Where the force-VJP gather reads the density adjoint: only the LDS window
 # (0) is searched.  With 512-thread workgroups both forms keep 4 waves per
 # SIMD resident and the window moves a fifth fewer bytes; the cache form (1)

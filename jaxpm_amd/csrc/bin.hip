@@ -378,13 +378,14 @@ bin_scatter_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t 
   }
 }
 
-// bin_scatter_kernel with the slice held in registers: 1024 threads, PER
-// particles each (slice = 1024 PER), loaded and keyed once for both the
-// histogram and the scatter, so the positions are read once.
+// bin_scatter_kernel with the slice held in registers: THREADS threads, PER
+// particles each (slice = THREADS PER), loaded and keyed once for both the
+// histogram and the scatter, so the positions are read once.  1024 x 8 and
+// 1024 x 16 hold one workgroup per CU (their registers); 512 x 8, three.
 constexpr int kScatterRegThreads = 1024;
 
-template <bool REL, class G, int PER>
-__global__ void __launch_bounds__(kScatterRegThreads)
+template <bool REL, class G, int PER, int THREADS>
+__global__ void __launch_bounds__(THREADS)
 bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint32_t *__restrict__ cursor,
                        float4 *__restrict__ spos) {
   extern __shared__ uint32_t cur[];
@@ -398,7 +399,7 @@ bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint3
   uint32_t key[PER];
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const uint32_t p = p0 + uint32_t(it) * THREADS + threadIdx.x;
     key[it] = 0xffffffffu;
     x[it] = y[it] = z[it] = 0.0f;
     if (p < p1) {
@@ -409,7 +410,7 @@ bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint3
   }
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const uint32_t p = p0 + uint32_t(it) * THREADS + threadIdx.x;
     if (p < p1) {
       int cx, cy, cz;
       floor_cells_xyz<REL>(x[it], y[it], z[it], p - s.part0, s.t, cx, cy, cz);
@@ -420,7 +421,7 @@ bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint3
   // LDS histogram of the slice, one atomic per run of equal keys in a wavefront
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const uint32_t p = p0 + uint32_t(it) * THREADS + threadIdx.x;
     const bool active = p < p1;
     const unsigned long long act = __ballot(active);
     const unsigned long long heads = run_heads(key[it], active);
@@ -432,7 +433,7 @@ bin_scatter_reg_kernel(const float *__restrict__ pos, uint32_t slice, G g, uint3
   __syncthreads();
 #pragma unroll
   for (int it = 0; it < PER; ++it) {
-    const uint32_t p = p0 + uint32_t(it) * kScatterRegThreads + threadIdx.x;
+    const uint32_t p = p0 + uint32_t(it) * THREADS + threadIdx.x;
     const bool active = p < p1;
     const unsigned long long act = __ballot(active);
     const unsigned long long heads = run_heads(key[it], active);
@@ -1119,15 +1120,18 @@ static int sort_into(hipStream_t s, const G &g, const SortDims &d, uint32_t M, b
     JPM_LAUNCHED("zero_u32_kernel");
   }
   // 3. scatter particles into tile order
-  if (lds && (d.slice == 8u * kScatterRegThreads || d.slice == 16u * kScatterRegThreads)) {
-    const bool p8 = d.slice == 8u * kScatterRegThreads;
-    auto kern = rel ? (p8 ? bin_scatter_reg_kernel<true, G, 8> : bin_scatter_reg_kernel<true, G, 16>)
-                    : (p8 ? bin_scatter_reg_kernel<false, G, 8> : bin_scatter_reg_kernel<false, G, 16>);
+  if (lds && (d.slice == 4u * kScatterRegThreads || d.slice == 8u * kScatterRegThreads ||
+              d.slice == 16u * kScatterRegThreads)) {
+    const int v = d.slice == 4u * kScatterRegThreads ? 0 : d.slice == 8u * kScatterRegThreads ? 1 : 2;
+    auto kern = rel ? (v == 0 ? bin_scatter_reg_kernel<true, G, 8, 512>
+                              : v == 1 ? bin_scatter_reg_kernel<true, G, 8, 1024> : bin_scatter_reg_kernel<true, G, 16, 1024>)
+                    : (v == 0 ? bin_scatter_reg_kernel<false, G, 8, 512>
+                              : v == 1 ? bin_scatter_reg_kernel<false, G, 8, 1024> : bin_scatter_reg_kernel<false, G, 16, 1024>);
     rc = set_lds(kern, lds_hist);
     if (rc) return rc;
     JPM_TIMED(s, "bin_scatter_kernel");
-    hipLaunchKernelGGL(kern, dim3(d.G), dim3(kScatterRegThreads), lds_hist, s, pos, d.slice, g, cursor,
-                       reinterpret_cast<float4 *>(base + w.spos));
+    hipLaunchKernelGGL(kern, dim3(d.G), dim3(v == 0 ? 512 : kScatterRegThreads), lds_hist, s, pos, d.slice, g,
+                       cursor, reinterpret_cast<float4 *>(base + w.spos));
     JPM_LAUNCHED("bin_scatter_reg_kernel");
   } else if (lds) {
     auto kern = rel ? bin_scatter_kernel<true, G> : bin_scatter_kernel<false, G>;
