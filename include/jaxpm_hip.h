@@ -54,7 +54,7 @@ extern "C" {
  * tuning in bits 16..25 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
  * accumulation/gather chunk (c = 1..6, default 8192), JPM_TUNE_SLICE(s):
  * 2^(11+s) particles per counting/scatter workgroup (s = 1..7, default
- * 16384).  The choice changes speed, never results beyond float rounding;
+ * 8192).  The choice changes speed, never results beyond float rounding;
  * a kept sort must be read with the flags it was built with. */
 #define JPM_TUNE_CHUNK(c) ((uint32_t)(c) << 16)
 #define JPM_TUNE_SLICE(s) ((uint32_t)(s) << 20)

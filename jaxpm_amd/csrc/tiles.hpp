@@ -13,7 +13,7 @@ constexpr int kWin = kTile + 1;                 // window edge (CIC +1 corner)
 constexpr int kWinCells = kWin * kWin * kWin;   // 4913
 constexpr int kChunk = 8192;                    // default max particles per chunk
 constexpr int kMaxKeysLds = 32768;              // LDS histogram limit (128 KiB)
-constexpr uint32_t kSlice = 16384;              // default particles per count / scatter workgroup
+constexpr uint32_t kSlice = 8192;               // default particles per count / scatter workgroup
 // Accumulation / force-gather workgroup size: their LDS windows (38-39 KiB)
 // allow 4 workgroups per CU, so 512 threads keep 32 waves resident.
 constexpr int kWindowThreads = 512;
