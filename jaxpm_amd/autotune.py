@@ -96,13 +96,21 @@ def _cached(key, use_cache):
 # reloading its windows (4096 against 16384 particles: 234 against 270
 # batch48 simulations/s), while timing noise alone can pick the narrow one.
 TIE = 1.03
+# Among those, slices within this factor of the fastest count as tied too,
+# and ties go to the narrowest slice: on the 256^3 initial state slices of
+# 4096 and 8192 particles time within 0.1% of each other, while over the
+# whole 20-step leapfrog the 4096-particle scatter (512-thread workgroups,
+# three per CU) runs 0.097 against 0.109 ms, 1% of the step.
+SLICE_TIE = 1.01
 
 
 def _finish(key, timings):
     fastest = min(timings.values())
     tied = [t for t, ms in timings.items() if ms <= TIE * fastest]
     widest = max(t[0] for t in tied)
-    best = min((t for t in tied if t[0] == widest), key=timings.get)
+    cands = [t for t in tied if t[0] == widest]
+    near = min(timings[t] for t in cands) * SLICE_TIE
+    best = min((t for t in cands if timings[t] <= near), key=lambda t: (t[1], timings[t]))
     _memo[key] = {"best": list(best), "ms": timings[best]}
     _store(key, _memo[key])
     return best, timings

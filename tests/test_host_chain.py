@@ -41,10 +41,13 @@ def test_cosmology_quantities_are_smooth():
 
 
 def test_autotune_ties_go_to_the_widest_chunk(tmp_path, monkeypatch):
-    """Candidates within 3% of the fastest are tied; the widest chunk wins,
-    then the fastest among those; a clear winner keeps its place."""
+    """Candidates within 3% of the fastest are tied; the widest chunk wins;
+    among its slices, those within 1% of its fastest are tied and the
+    narrowest wins; a clear winner keeps its place."""
     monkeypatch.setenv("XDG_CACHE_HOME", str(tmp_path))
     from jaxpm_amd import autotune
-    t = {(3, 2, 0, 0): 1.00, (5, 2, 0, 0): 1.02, (5, 3, 0, 1): 1.01, (4, 2, 0, 0): 0.99, (2, 2, 0, 0): 1.2}
+    t = {(3, 2, 0, 0): 1.00, (5, 2, 0, 0): 1.025, (5, 3, 0, 1): 1.01, (4, 2, 0, 0): 0.99, (2, 2, 0, 0): 1.2}
     assert autotune._finish("tie", t)[0] == (5, 3, 0, 1)
+    t = {(5, 1, 0, 0): 1.008, (5, 2, 0, 0): 1.00, (5, 3, 0, 0): 1.001, (4, 1, 0, 0): 0.99}
+    assert autotune._finish("slice", t)[0] == (5, 1, 0, 0)
     assert autotune._finish("clear", {(3, 2, 0, 0): 1.0, (5, 2, 0, 0): 1.2})[0] == (3, 2, 0, 0)
