@@ -116,7 +116,7 @@ def _finish(key, timings):
     return best, timings
 
 
-def tune_stepper(mesh_shape, pos, relative=False, reps=3, use_cache=True):
+def tune_stepper(mesh_shape, pos, relative=False, reps=8, use_cache=True):
     """Best (chunk_code, slice_code, rho_global, xcd_slabs) for a single simulation at
     positions ``pos`` ([n, 3] device tensor); returns (codes, {codes: ms})."""
     shape = tuple(int(s) for s in mesh_shape)
