@@ -33,10 +33,10 @@ HBM_PEAK_GBS = 8000.0
 # x1.000 for writes), by timed kernel name and mesh side, with the bench's
 # sort tuning.  The commands and the counter values are in
 # profiles/sorted_gathers_256_traffic.md and kernel_bounds_256_counters.md.
-PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256): 1300.8e6,
-                     ("bin_grad_kernel[drift]", 256): 1917.0e6,
-                     ("bin_accum_kernel[nc=1]", 256): 350.0e6,
-                     ("bin_accum_kernel[nc=3]", 256): 817.4e6}
+PMC_TRAFFIC_BYTES = {("bin_kick_kernel[drift]", 256): 1038.1e6,
+                     ("bin_grad_kernel[drift]", 256): 1468.0e6,
+                     ("bin_accum_kernel[nc=1]", 256): 361.8e6,
+                     ("bin_accum_kernel[nc=3]", 256): 848.1e6}
 
 
 def fixed_tuning(args):
