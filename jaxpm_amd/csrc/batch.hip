@@ -20,10 +20,6 @@ int fft_r2c(hipStream_t, int, int, int, int, const float *, void *);
 int fft_c2r(hipStream_t, int, int, int, int, void *, float *);
 int kspace_potential_batch(hipStream_t, int, int, int, int, const void *, void *, float, float, const float *,
                            uint32_t);
-int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
-                    uint32_t);
-int potential_solve_div(hipStream_t, int, int, int, int, const float *, size_t, float *, void *, void *, float *, float,
-                        float, const float *, uint32_t);
 int fd4_divergence_batch(hipStream_t, int, int, int, int, const float *, size_t, float, float *);
 int batch_kick(hipStream_t, const BatchBins &, const float *, const float *, float *, int, const float *, float *,
                float *, uint32_t *, float *);

@@ -501,7 +501,6 @@ __device__ __forceinline__ int halo_index(int lx, int ly, int lz) {
 // reduce reads them there, contiguous, instead of from the mesh, where the
 // lz = 0 cells lie single floats a mesh row apart (their reads cost a
 // 64-byte line each: 0.064 -> 0.046 ms for one channel at 256^3).
-constexpr int kFaces = kTile * kTile + (kTile - 1) * kTile + (kTile - 1) * (kTile - 1);
 static_assert(kH + kFaces <= 2 * kWinCells, "halo and own faces fit a window slot (kWinCells int64)");
 
 __device__ __forceinline__ void face_cell(int e, int &a, int &b, int &c) {
@@ -747,9 +746,12 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
                   int has_overflow, const float *__restrict__ wmax_dev, float wmax_host, int has_w, uint32_t n,
                   int accumulate, float *__restrict__ mesh, uint32_t K, uint32_t per, bool slabs,
                   const uint32_t *__restrict__ nonfinite, const float4 *__restrict__ spos,
-                  const ChunkDesc *__restrict__ desc, const float *__restrict__ weight, float wscalar) {
+                  const ChunkDesc *__restrict__ desc, const float *__restrict__ weight, float wscalar,
+                  FaceCells fc) {
   const uint32_t gtile = slab_item(blockIdx.x, per, slabs);
   if (gtile >= K) return;
+  // with fc.fb the finished face cells go there instead of the mesh
+  float *fbt = fc.fb ? fc.fb + size_t(blockIdx.y) * fc.chan + size_t(gtile) * kFaceStride : nullptr;
   const unsigned long long *ipart = reinterpret_cast<const unsigned long long *>(partial);
   const double inv = 1.0 / fixed_point_scale(has_w ? *wmax_dev : wmax_host, n);
   const SimRef &sim = g.tile_sim(gtile);
@@ -826,7 +828,10 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
       float acc = h[i][0];
 #pragma unroll
       for (int k = 1; k < 8; ++k) acc = takes[i][k] ? acc + h[i][k] : acc;
-      *dsts[i] = acc;
+      if (fbt)
+        fbt[int(threadIdx.x) + i * kReduceThreads] = acc;
+      else
+        *dsts[i] = acc;
     }
   } else {
     for (int e = threadIdx.x; e < cells_in_tile; e += blockDim.x) {
@@ -880,12 +885,18 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
             }
           }
       if (faces_only) {
-        *dst = acc;
+        if (fbt)
+          fbt[e] = acc;   // a full tile: e is the face index
+        else
+          *dst = acc;
         continue;
       }
       float v = acc;
       if (has_overflow) v += overflow[size_t(ch) * M + cell];
-      *dst = accumulate ? *dst + v : v;
+      if (fbt && (a == 0 || b == 0 || c == 0))
+        fbt[face_index(a, b, c)] = v;
+      else
+        *dst = accumulate ? *dst + v : v;
     }
   }
   if (*nonfinite == 0u) return;
@@ -927,7 +938,11 @@ bin_reduce_kernel(const float *__restrict__ partial, const uint32_t *__restrict_
             cz = gz[bz];
           }
           if (cx / kTile != itx || cy / kTile != ity || cz / kTile != itz) continue;
-          atomicAdd(&meshc[(uint32_t(cx) * uint32_t(t.ny) + uint32_t(cy)) * uint32_t(t.nz) + uint32_t(cz)], v);
+          const int fa = cx - itx * kTile, fbb = cy - ity * kTile, fcc = cz - itz * kTile;
+          if (fbt && (fa == 0 || fbb == 0 || fcc == 0))
+            atomicAdd(&fbt[face_index(fa, fbb, fcc)], v);
+          else
+            atomicAdd(&meshc[(uint32_t(cx) * uint32_t(t.ny) + uint32_t(cy)) * uint32_t(t.nz) + uint32_t(cz)], v);
         }
       }
     }
@@ -1193,7 +1208,8 @@ template int bins_order<SimTable>(hipStream_t, const BatchBins &, uint32_t *);
 // fixed-point scale's bound), which skips that pass.
 template <class G>
 static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, float wscalar, int nc, float *mesh,
-                        uint32_t flags, void *ws, size_t ws_bytes, const float *wmax_in = nullptr) {
+                        uint32_t flags, void *ws, size_t ws_bytes, const float *wmax_in = nullptr,
+                        const FaceCells *faces = nullptr) {
   SortDims d{};
   d.maxchunks = b.maxchunks;
   JPM_REQUIRE(nc == 1 || nc == 3, "binned paint supports 1 or 3 channels (got %d)", nc);
@@ -1204,6 +1220,9 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
   const size_t need = accum_ws(d, size_t(M), nc, b.rel, &opart, &oover, &owmax);
   JPM_REQUIRE(ws && ws_bytes >= need, "binned paint needs %zu workspace bytes, got %zu", need, ws_bytes);
   const bool accumulate = !(flags & JPM_ZERO_MESH);
+  const FaceCells fc = faces ? *faces : FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)};
+  JPM_REQUIRE(!fc.fb || (!b.rel && !accumulate && size_t(b.K) * kFaceStride <= fc.chan),
+              "face cells apart need absolute positions, a replaced mesh and room for every tile");
   char *base = static_cast<char *>(ws);
   int rc;
   if (b.rel) {
@@ -1240,15 +1259,15 @@ static int paint_sorted(hipStream_t s, const BinsT<G> &b, const float *weight, f
     const unsigned grid = slab_grid(b.K, true, per);
     hipLaunchKernelGGL(kern, dim3(grid, nc), dim3(kReduceThreads), 0, s, partial, b.choff, b.nchunk, b.g, b.M, nc, overflow,
                        int(b.rel), wmax_dev, wmax_host, int(weight != nullptr), b.n, int(accumulate), mesh, b.K, per,
-                       true, b.nonfinite, b.spos, b.desc, weight, wscalar);
+                       true, b.nonfinite, b.spos, b.desc, weight, wscalar, fc);
   }
   JPM_LAUNCHED("bin_reduce_kernel");
   return JPM_OK;
 }
 
 int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
-               void *ws, size_t ws_bytes, const float *wmax_in) {
-  return paint_sorted(s, b, weight, wscalar, nc, mesh, flags, ws, ws_bytes, wmax_in);
+               void *ws, size_t ws_bytes, const float *wmax_in, const FaceCells *faces) {
+  return paint_sorted(s, b, weight, wscalar, nc, mesh, flags, ws, ws_bytes, wmax_in, faces);
 }
 
 int binned_paint(hipStream_t s, int nx, int ny, int nz, const float *pos, int64_t n, const float *weight,

@@ -124,10 +124,6 @@ int density_plane_vjp(hipStream_t, const float *, int64_t, int, int, double, dou
 size_t gaussian_smoothing_ws(int, int);
 int gaussian_smoothing(hipStream_t, int, int, const float *, float *, double, void *, size_t);
 int kspace_potential(hipStream_t, int, int, int, const void *, void *, float, float, uint32_t);
-int potential_solve(hipStream_t, int, int, int, int, const float *, void *, void *, float *, float, float, const float *,
-                    uint32_t);
-int potential_solve_div(hipStream_t, int, int, int, int, const float *, size_t, float *, void *, void *, float *, float,
-                        float, const float *, uint32_t);
 int fd4_gradient(hipStream_t, int, int, int, const float *, float, float *);
 int fd4_divergence(hipStream_t, int, int, int, const float *, float, float *);
 
@@ -185,8 +181,6 @@ static int pipeline_paint(hipStream_t s, int nx, int ny, int nz, const float *po
 }
 int kick(hipStream_t, int, int, int, const float *, const float *, int64_t, float, float *, uint32_t);
 
-int bins_paint(hipStream_t, const Bins &, const float *, float, int, float *, uint32_t, void *, size_t,
-               const float *wmax_in = nullptr);
 size_t bins_paint_workspace_size(int, int, int, int64_t, int, uint32_t);
 int bins_kick(hipStream_t, const Bins &, const float *, float, float *, int, float, float *, float *, uint32_t *,
               float *);
@@ -205,12 +199,22 @@ int bins_read(hipStream_t, const Bins &, const float *, int, uint32_t, float *);
 //     both the direct term (on phi) and the density term (on rho_bar).
 
 struct SortedWs {
-  size_t sort, paint, mesh, g3, spec, spec2, total;
+  size_t sort, paint, mesh, g3, spec, spec2, faces, total;
+  bool apart;   // the paint's face cells kept apart for the potential solve
 };
+
+// Absolute positions on a mesh of full tiles that the library's FFT takes:
+// the paint hands its face cells to the potential solve's first pass in a
+// buffer of their own (FaceCells) instead of writing them into the mesh.
+static bool faces_apart(int nx, int ny, int nz, uint32_t pflags) {
+  return !(pflags & JPM_RELATIVE) && potential_faces_ok(nx, ny, nz);
+}
 
 static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pflags) {
   const size_t m = size_t(cells(nx, ny, nz)), h = size_t(half_cells(nx, ny, nz));
   SortedWs w;
+  w.apart = nc == 1 && faces_apart(nx, ny, nz, pflags);
+  const size_t tiles = w.apart ? size_t(nx / kTile) * size_t(ny / kTile) * size_t(nz / kTile) : 0;
   size_t o = 0;
   w.sort = o;  o += align_up(bin_sort_workspace_size(nx, ny, nz, n, pflags));
   w.paint = o; o += align_up(bins_paint_workspace_size(nx, ny, nz, n, nc, pflags));
@@ -218,8 +222,15 @@ static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pf
   w.g3 = o;    o += (nc == 3) ? align_up(3 * m * 4) : 0;
   w.spec = o;  o += align_up(h * 8);
   w.spec2 = o; o += align_up(h * 8);
+  w.faces = o; o += (nc == 1) ? align_up(tiles * kFaceStride * 4) : 0;
   w.total = o;
   return w;
+}
+
+static FaceCells sorted_faces(const SortedWs &w, char *base, int nx, int ny, int nz) {
+  const size_t tiles = size_t(nx / kTile) * size_t(ny / kTile) * size_t(nz / kTile);
+  return FaceCells{reinterpret_cast<float *>(base + w.faces), uint32_t(ny), uint32_t(ny / kTile),
+                   uint32_t(nz / kTile), tiles * kFaceStride, FastDiv(uint32_t(ny))};
 }
 
 static bool sorted_supported(int nx, int ny, int nz, uint32_t flags) {
@@ -254,10 +265,12 @@ static int force_kick(hipStream_t s, int nx, int ny, int nz, const float *pos, i
   }
   if (rc) return rc;
   float *mesh = reinterpret_cast<float *>(base + w.mesh);
-  rc = bins_paint(s, b, nullptr, 1.0f, 1, mesh, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint);
+  const FaceCells fc = sorted_faces(w, base, nx, ny, nz);
+  const FaceCells *fcp = w.apart ? &fc : nullptr;
+  rc = bins_paint(s, b, nullptr, 1.0f, 1, mesh, JPM_ZERO_MESH, base + w.paint, w.mesh - w.paint, nullptr, fcp);
   if (rc) return rc;
   rc = potential_solve(s, nx, ny, nz, 1, mesh, base + w.spec, base + w.spec2, phi, r_split, float(1.0 / double(m)),
-                       nullptr, kflags);
+                       nullptr, kflags, fcp);
   if (rc) return rc;
   // without vel the call is the force evaluation alone: potential and sort
   if (!vel) return JPM_OK;

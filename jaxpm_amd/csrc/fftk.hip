@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "kspace.hpp"
+#include "tiles.hpp"
 
 namespace jpm {
 
@@ -422,7 +423,7 @@ __device__ __forceinline__ float2 divergence_pair(const float *__restrict__ g, s
 template <int M, bool DIV>
 __global__ void __launch_bounds__(kFftThreads)
 fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int nrows, size_t in_bstride,
-                    size_t out_bstride, const float2 *__restrict__ tw_all, DivIn dv) {
+                    size_t out_bstride, const float2 *__restrict__ tw_all, DivIn dv, FaceCells fc) {
   constexpr int KB = FftCols<M>::KB, PITCH = KB + 1, total = M * KB;
   __shared__ float2 buf[M * PITCH];
   __shared__ float2 twM[M];
@@ -454,12 +455,33 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
     // loads a copy of row nrows - 1 (rows transform independently and such a
     // row is never stored)
     float2 v[iters];
+    // with face cells apart: a pair in a tile's lx = 0 or ly = 0 plane loads
+    // from the face buffer instead of the mesh; the lz = 0 cell of any other
+    // pair (its x) is loaded from there too, by every lane (a partner lane's
+    // line), and selected after all loads, so that no load waits on a branch
+    float zf[iters];
+    bool zsel[iters];
 #pragma unroll
     for (int it = 0; it < iters; ++it) {
       const int f0 = threadIdx.x + it * kFftThreads;
       const int f = (total % kFftThreads != 0 && f0 >= total) ? total - 1 : f0;
       const int l = f % M, c = f / M;
-      v[it] = src[size_t(r0 + c < nrows ? r0 + c : nrows - 1) * M + l];
+      const uint32_t r = uint32_t(r0 + c < nrows ? r0 + c : nrows - 1);
+      const float2 *p = src + size_t(r) * M + l;
+      zsel[it] = false;
+      if (fc.fb) {
+        const uint32_t x = fc.div_ny.div(r), y = r - x * fc.ny;
+        const int a = int(x & uint32_t(kTile - 1)), b = int(y & uint32_t(kTile - 1)), z0 = 2 * l,
+                  cz = z0 & (kTile - 1);
+        const float *ft =
+            fc.fb + ((size_t(x / kTile) * fc.ty + y / kTile) * fc.tz + uint32_t(z0 / kTile)) * kFaceStride;
+        const bool plane = a == 0 || b == 0;
+        // face_index of (a, b, cz) and (a, b, cz + 1), cz even: consecutive, even
+        if (plane) p = reinterpret_cast<const float2 *>(ft + face_index(a, b, cz));
+        zf[it] = ft[face_index(a, b, 0)];
+        zsel[it] = !plane && cz == 0;
+      }
+      v[it] = *p;
     }
     tsM.load(tw_all, tw_off(M));
     tsN.load(tw_all, tw_off(2 * M));
@@ -468,6 +490,7 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
       const int f = threadIdx.x + it * kFftThreads;
       if (total % kFftThreads != 0 && f >= total) break;
       const int l = f % M, c = f / M;
+      if (zsel[it]) v[it].x = zf[it];
       buf[l * PITCH + c] = v[it];
     }
   }
@@ -583,7 +606,11 @@ static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int 
 // The R2C's z pass of in, or with dv.g3 set of the divergence of dv's
 // vector mesh.
 static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, float2 *out,
-                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0}) {
+                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0},
+                    const FaceCells &fc = FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)}) {
+  JPM_REQUIRE(!fc.fb || (!dv.g3 && batch == 1 && nx % kTile == 0 && ny % kTile == 0 && nz % kTile == 0 &&
+                         fc.ny == uint32_t(ny) && fc.ty == uint32_t(ny / kTile) && fc.tz == uint32_t(nz / kTile)),
+              "face cells apart need one mesh of full 16^3 tiles");
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
   const float2 *i2 = reinterpret_cast<const float2 *>(in);
@@ -593,7 +620,7 @@ static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const floa
   case N: {                                                                                                   \
     auto kern = dv.g3 ? fft_r2c_rows_kernel<N, true> : fft_r2c_rows_kernel<N, false>;                         \
     hipLaunchKernelGGL(kern, dim3((nrows + FftCols<N>::KB - 1) / FftCols<N>::KB, batch), dim3(kFftThreads), 0,  \
-                       s, i2, out, nrows, Mc, H, tw, dv);                                                     \
+                       s, i2, out, nrows, Mc, H, tw, dv, fc);                                                 \
     break;                                                                                                    \
   }
     JPM_FFT_LENGTHS(JPM_R2C)
@@ -660,7 +687,7 @@ int native_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float
 // apart, items nx ny nz floats apart), formed by the R2C rows as they load.
 int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
                      const Tabs &t, float r2, float scale, const float *scale_dev, int deconv, const float *g3,
-                     size_t chan) {
+                     size_t chan, const FaceCells *faces) {
   JPM_REQUIRE(native_fft_ok(nx, ny, nz) && nx >= 2 && batch >= 1 && batch <= 65535, "native FFT shape");
   const float2 *tw;
   int rc = twiddles(&tw);
@@ -669,7 +696,8 @@ int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const flo
   const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(spec);
   DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny)};
-  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv))) return rc;
+  const FaceCells fc = faces ? *faces : FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)};
+  if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv, fc))) return rc;
   if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
   {
     JPM_TIMED(s, "fft_cols_potential_kernel");

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "kspace.hpp"
+#include "tiles.hpp"
 
 namespace jpm {
 
@@ -500,7 +501,7 @@ int fft_c2r(hipStream_t s, int nx, int ny, int nz, int batch, void *in, float *o
 bool native_fft_ok(int nx, int ny, int nz);
 int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, float *out,
                      const Tabs &t, float r2, float scale, const float *scale_dev, int deconv,
-                     const float *g3 = nullptr, size_t chan = 0);
+                     const float *g3 = nullptr, size_t chan = 0, const FaceCells *faces = nullptr);
 int fd4_divergence_batch(hipStream_t s, int nx, int ny, int nz, int batch, const float *g3, size_t chan_stride,
                          float scale, float *out);
 
@@ -509,20 +510,30 @@ int fd4_divergence_batch(hipStream_t s, int nx, int ny, int nz, int batch, const
 // and the inverse transform.  spec and spec2 hold H complex values per mesh;
 // power-of-two meshes apply the multiplier inside the FFT's x pass and use
 // spec only, other shapes run hipFFT, potential_kernel into spec2, hipFFT.
+// The shapes on which the potential solves run the library's own FFT, and
+// on which they can read a paint's face cells kept apart (FaceCells): one
+// mesh of full 16^3 tiles.
+bool potential_faces_ok(int nx, int ny, int nz) {
+  return native_fft_ok(nx, ny, nz) && nx >= 2 && nx % kTile == 0 && ny % kTile == 0 && nz % kTile == 0;
+}
+
 int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, void *spec2,
-                    float *out, float r_split, float scale, const float *scale_dev, uint32_t flags) {
+                    float *out, float r_split, float scale, const float *scale_dev, uint32_t flags,
+                    const FaceCells *faces) {
   int rc = check_dims(nx, ny, nz);
   if (rc) return rc;
   JPM_REQUIRE(in && spec && spec2 && out, "null pointer argument");
   JPM_REQUIRE(batch >= 1 && batch <= 65535, "batch of %d meshes", batch);
   const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 7u) == 0;
+  JPM_REQUIRE(!faces || (potential_faces_ok(nx, ny, nz) && aligned && batch == 1),
+              "face cells apart need the library's FFT on one mesh of full tiles");
   if (native_fft_ok(nx, ny, nz) && nx >= 2 && aligned) {
     Tabs t;
     rc = get_tabs(s, nx, ny, nz, flags, t);
     if (rc) return rc;
     JPM_TIMED(s, "potential_solve (5 passes)");
     return native_potential(s, nx, ny, nz, batch, in, spec, out, t, r_split * r_split, scale, scale_dev,
-                            int((flags & JPM_K_DECONV) != 0));
+                            int((flags & JPM_K_DECONV) != 0), nullptr, 0, faces);
   }
   if ((rc = fft_r2c(s, nx, ny, nz, batch, in, spec))) return rc;
   if ((rc = kspace_potential_batch(s, nx, ny, nz, batch, spec, spec2, r_split, scale, scale_dev, flags))) return rc;

@@ -74,6 +74,35 @@ inline uint32_t tuned_slice(uint32_t flags) {
   return (v >= 1 && v <= 7) ? (1u << (11 + v)) : kSlice;
 }
 
+// A tile's low-face cells (lx = 0, or ly = 0, or lz = 0), numbered in face
+// order: the lx = 0 plane (ly * 16 + lz), then ly = 0 with lx > 0
+// (256 + (lx - 1) * 16 + lz), then lz = 0 with lx, ly > 0
+// (496 + (lx - 1) * 15 + ly - 1) -- 721 cells.
+constexpr int kFaces = kTile * kTile + (kTile - 1) * kTile + (kTile - 1) * (kTile - 1);
+
+__host__ __device__ __forceinline__ int face_index(int a, int b, int c) {
+  return a == 0 ? b * kTile + c
+                : (b == 0 ? kTile * kTile + (a - 1) * kTile + c
+                          : kTile * kTile + (kTile - 1) * kTile + (a - 1) * (kTile - 1) + (b - 1));
+}
+
+// The low-face cells of a paint kept apart from its mesh.  A tile's face
+// cells take the lower neighbours' halos, which the face reduce adds; the
+// force paths let it write the finished face values here, contiguous per
+// tile, and the potential solve's first pass reads them in place of the
+// mesh's (stale) face cells.  Written into the mesh, the lz = 0 face cells
+// are single floats a mesh row apart, each write a partial 64-byte line.
+// Channel ch, tile T at fb + ch * chan + T * kFaceStride, in face order.
+// Needs one simulation whose mesh sides are multiples of 16 (every tile
+// full), absolute positions and a paint that replaces the mesh.
+constexpr int kFaceStride = 724;   // 721 cells, padded to 16 bytes
+struct FaceCells {
+  float *fb;            // null: the mesh holds every cell
+  uint32_t ny, ty, tz;  // mesh cells along y; tiles along y and z
+  size_t chan;          // floats per channel
+  FastDiv div_ny;       // row -> (x, y)
+};
+
 // A load through the global address space.  For some pointers held in
 // structs the compiler otherwise emits a flat load, which also counts
 // against the LDS wait counter: every LDS wait after it would wait for the
@@ -231,6 +260,17 @@ size_t batch_paint_workspace_size(const SimBatch &bt, int nc);
 int batch_sort(hipStream_t s, const SimBatch &bt, const float *pos, void *ws, size_t ws_bytes, BatchBins &out,
                const uint32_t *counts_in = nullptr);
 int zero_u32(hipStream_t s, uint32_t *p, uint32_t n);
+int bins_paint(hipStream_t s, const Bins &b, const float *weight, float wscalar, int nc, float *mesh, uint32_t flags,
+               void *ws, size_t ws_bytes, const float *wmax_in = nullptr, const FaceCells *faces = nullptr);
+// The potential solves (kspace.hip); faces: the painted mesh's face cells
+// kept apart, on the shapes potential_faces_ok accepts, for a one-mesh solve.
+bool potential_faces_ok(int nx, int ny, int nz);
+int potential_solve(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, void *spec, void *spec2,
+                    float *out, float r_split, float scale, const float *scale_dev, uint32_t flags,
+                    const FaceCells *faces = nullptr);
+int potential_solve_div(hipStream_t s, int nx, int ny, int nz, int batch, const float *g3, size_t chan, float *rho,
+                        void *spec, void *spec2, float *out, float r_split, float scale, const float *scale_dev,
+                        uint32_t flags);
 BatchBins batch_view(const SimBatch &bt, const void *ws);
 int batch_paint(hipStream_t s, const SimBatch &bt, const BatchBins &b, const float *weight, int nc, float *mesh,
                 void *ws, size_t ws_bytes, const float *wmax_in = nullptr);
