@@ -387,6 +387,7 @@ struct DivIn {
   const float *g3;
   size_t chan;
   uint32_t nx, ny;
+  FastDiv div_ny;   // row -> (x, y)
 };
 
 // sum_d D_d g_d at cells (x, y, 2l) and (x, y, 2l + 1) of a mesh with
@@ -404,11 +405,16 @@ __device__ __forceinline__ float2 divergence_pair(const float *__restrict__ g, s
   auto at = [&](const float2 *ch, uint32_t xi, uint32_t yi, int li) {
     return ch[(size_t(xi) * ny + yi) * M + uint32_t(li)];
   };
-  // periodic neighbours (nx, ny >= 2)
-  const float2 xp1 = at(gx, (x + 1u) % nx, y, l), xm1 = at(gx, (x + nx - 1u) % nx, y, l);
-  const float2 xp2 = at(gx, (x + 2u) % nx, y, l), xm2 = at(gx, (x + nx - 2u) % nx, y, l);
-  const float2 yp1 = at(gy, x, (y + 1u) % ny, l), ym1 = at(gy, x, (y + ny - 1u) % ny, l);
-  const float2 yp2 = at(gy, x, (y + 2u) % ny, l), ym2 = at(gy, x, (y + ny - 2u) % ny, l);
+  // periodic neighbours (nx, ny >= 2, x < nx, y < ny): i + d wrapped by one
+  // compare, d in {-2, -1, 1, 2}
+  auto wrap = [](uint32_t i, int d, uint32_t n) {
+    const uint32_t v = i + n + uint32_t(d);
+    return v >= 2u * n ? v - 2u * n : (v >= n ? v - n : v);
+  };
+  const float2 xp1 = at(gx, wrap(x, 1, nx), y, l), xm1 = at(gx, wrap(x, -1, nx), y, l);
+  const float2 xp2 = at(gx, wrap(x, 2, nx), y, l), xm2 = at(gx, wrap(x, -2, nx), y, l);
+  const float2 yp1 = at(gy, x, wrap(y, 1, ny), l), ym1 = at(gy, x, wrap(y, -1, ny), l);
+  const float2 yp2 = at(gy, x, wrap(y, 2, ny), l), ym2 = at(gy, x, wrap(y, -2, ny), l);
   // z - 2, z - 1 | z, z + 1 | z + 2, z + 3
   const float2 zl = at(gz, x, y, (l + M - 1) % M), zc = at(gz, x, y, l), zr = at(gz, x, y, (l + 1) % M);
   const float dx0 = c8 * (xp1.x - xm1.x) - c1 * (xp2.x - xm2.x), dx1 = c8 * (xp1.y - xm1.y) - c1 * (xp2.y - xm2.y);
@@ -444,7 +450,8 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
       if (total % kFftThreads != 0 && f >= total) break;
       const int l = f % M, c = f / M;
       const uint32_t r = uint32_t(r0 + c);
-      buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, r / dv.ny, r % dv.ny, l)
+      const uint32_t x = dv.div_ny.div(r);
+      buf[l * PITCH + c] = r0 + c < nrows ? divergence_pair<M>(g, dv.chan, dv.nx, dv.ny, x, r - x * dv.ny, l)
                                           : make_float2(0.0f, 0.0f);
     }
     tsM.load(tw_all, tw_off(M));
@@ -606,7 +613,7 @@ static int cols_pass(hipStream_t s, bool inv, float2 *data, int L, int nzh, int 
 // The R2C's z pass of in, or with dv.g3 set of the divergence of dv's
 // vector mesh.
 static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const float *in, float2 *out,
-                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0},
+                    const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0, FastDiv(1)},
                     const FaceCells &fc = FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)}) {
   JPM_REQUIRE(!fc.fb || (!dv.g3 && batch == 1 && nx % kTile == 0 && ny % kTile == 0 && nz % kTile == 0 &&
                          fc.ny == uint32_t(ny) && fc.ty == uint32_t(ny / kTile) && fc.tz == uint32_t(nz / kTile)),
@@ -695,7 +702,7 @@ int native_potential(hipStream_t s, int nx, int ny, int nz, int batch, const flo
   const int nzh = nz / 2 + 1;
   const size_t H = size_t(nx) * ny * nzh;
   float2 *h = static_cast<float2 *>(spec);
-  DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny)};
+  DivIn dv{g3, chan, uint32_t(nx), uint32_t(ny), FastDiv(uint32_t(ny))};
   const FaceCells fc = faces ? *faces : FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)};
   if ((rc = rows_r2c(s, nx, ny, nz, batch, in, h, tw, dv, fc))) return rc;
   if ((rc = cols_pass(s, false, h, ny, nzh, nx, size_t(nzh), size_t(ny) * nzh, H, batch, tw))) return rc;
