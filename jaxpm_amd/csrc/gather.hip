@@ -122,6 +122,53 @@ __device__ __forceinline__ void load_window_batched(const float *__restrict__ ph
   }
 }
 
+// Two meshes' windows with all loads of a thread in flight together: a
+// over [o - HA, o - HA + EA)^3 into wa, b over [o - HB, o - HB + EB)^3
+// into wb, the same cells and order as two load_window_batched calls.
+template <int EA, int HA, int EB, int HB, bool NEAR>
+__device__ __forceinline__ void load_window_pair(const float *__restrict__ a, float *wa, const float *__restrict__ b,
+                                                 float *wb, const TileGrid &t, int ox, int oy, int oz) {
+  constexpr int CA = EA * EA * EA, CB = EB * EB * EB;
+  constexpr int PA = (CA + kWindowThreads - 1) / kWindowThreads, PB = (CB + kWindowThreads - 1) / kWindowThreads;
+  const uint32_t ny = uint32_t(t.ny), nz = uint32_t(t.nz);
+  auto cell = [&](int e, int EDGE, int HALO) {
+    const int c = e % EDGE, ab = e / EDGE;
+    const int bb = ab % EDGE, aa = ab / EDGE;
+    int gx = ox - HALO + aa, gy = oy - HALO + bb, gz = oz - HALO + c;
+    if (NEAR) {
+      gx = wrap_near(gx, t.nx);
+      gy = wrap_near(gy, t.ny);
+      gz = wrap_near(gz, t.nz);
+    } else {
+      gx = wrapn(gx, t.nx);
+      gy = wrapn(gy, t.ny);
+      gz = wrapn(gz, t.nz);
+    }
+    return (uint32_t(gx) * ny + uint32_t(gy)) * nz + uint32_t(gz);
+  };
+  float va[PA], vb[PB];
+#pragma unroll
+  for (int i = 0; i < PA; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    va[i] = a[cell(e0 < CA ? e0 : CA - 1, EA, HA)];
+  }
+#pragma unroll
+  for (int i = 0; i < PB; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    vb[i] = b[cell(e0 < CB ? e0 : CB - 1, EB, HB)];
+  }
+#pragma unroll
+  for (int i = 0; i < PA; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    if (e0 < CA) wa[e0] = va[i];
+  }
+#pragma unroll
+  for (int i = 0; i < PB; ++i) {
+    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    if (e0 < CB) wb[e0] = vb[i];
+  }
+}
+
 template <int EDGE, int HALO>
 __device__ __forceinline__ void load_window_t(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
                                               float *win) {
@@ -468,9 +515,13 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
     ng[0] = gpos[i3]; ng[1] = gpos[i3 + 1]; ng[2] = gpos[i3 + 2];
     if (PF) corners_global_abs(rho_bar, t, rec1, nr);
   }
-  load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
-  if (RLDS) load_window_t<kWin, 0>(rho_bar, t, ox, oy, oz, rwin);
   const bool near = near_wrap_ok(t);
+  if (!RLDS)
+    load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
+  else if (near)
+    load_window_pair<kPw, kHalo, kWin, 0, true>(phi, win, rho_bar, rwin, t, ox, oy, oz);
+  else
+    load_window_pair<kPw, kHalo, kWin, 0, false>(phi, win, rho_bar, rwin, t, ox, oy, oz);
   float cmax = 0.0f;
   __syncthreads();
   build_force_window(win);
