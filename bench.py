@@ -647,6 +647,7 @@ def main():
     M = N
     H = n * n * (n // 2 + 1)
     win = 4 * M * (17 / 16) ** 3   # tile windows (16^3 tiles + 1-cell CIC halo)
+    faces = 4 * (M // 16 ** 3) * (2 * 721 + 1156)
     algo = {
         "bin_kick_kernel": 36 * N + 4 * M,             # pos, vel r+w, potential
         "bin_kick_kernel[drift]": 48 * N + 4 * M,      # ... and the drifted positions
@@ -657,8 +658,11 @@ def main():
         "bin_scatter_kernel": 12 * N + 16 * N,
         "bin_accum_kernel[nc=1]": 16 * N + win,
         "bin_accum_kernel[nc=3]": 16 * N + 12 * N + 3 * win,
-        "bin_reduce_kernel[nc=1]": win + 4 * M,
-        "bin_reduce_kernel[nc=3]": 3 * (win + 4 * M),
+        # the face reduce of single-chunk tiles (every tile here): per 16^3
+        # tile and channel its 721 low-face cells read and written, and the
+        # 1156 halo cells of its window slot read (by the upper neighbours)
+        "bin_reduce_kernel[nc=1]": faces,
+        "bin_reduce_kernel[nc=3]": 3 * faces,
         "potential_kernel": 16 * H,
         "fd4_divergence_kernel": 16 * M,
         "axpy_kernel": 36 * N,
