@@ -477,15 +477,22 @@ fft_r2c_rows_kernel(const float2 *__restrict__ in, float2 *__restrict__ out, int
       const float2 *p = src + size_t(r) * M + l;
       zsel[it] = false;
       if (fc.fb) {
+        // cells (x, y, 2l) and (x, y, 2l + 1): tile (x/16, y/16, l/8), local
+        // (a, b, cz) and (a, b, cz + 1) with cz = 2 (l mod 8); face offsets
+        // in 32 bits (the host checks the buffer's size)
         const uint32_t x = fc.div_ny.div(r), y = r - x * fc.ny;
-        const int a = int(x & uint32_t(kTile - 1)), b = int(y & uint32_t(kTile - 1)), z0 = 2 * l,
-                  cz = z0 & (kTile - 1);
-        const float *ft =
-            fc.fb + ((size_t(x / kTile) * fc.ty + y / kTile) * fc.tz + uint32_t(z0 / kTile)) * kFaceStride;
+        const uint32_t a = x & uint32_t(kTile - 1), b = y & uint32_t(kTile - 1), cz = (uint32_t(l) & 7u) * 2u;
+        const uint32_t tile = ((x >> 4) * fc.ty + (y >> 4)) * fc.tz + (uint32_t(l) >> 3);
+        // face_index(a, b, 0): in a plane (a == 0 or b == 0) the row's first
+        // cell, whose pair at cz is consecutive there; else the lz = 0 cell
+        const uint32_t row = a == 0 ? b * uint32_t(kTile)
+                                    : (b == 0 ? uint32_t(kTile * kTile) + (a - 1u) * uint32_t(kTile)
+                                              : uint32_t(kTile * kTile + (kTile - 1) * kTile) +
+                                                    (a - 1u) * uint32_t(kTile - 1) + (b - 1u));
+        const float *ft = fc.fb + (tile * uint32_t(kFaceStride) + row);
         const bool plane = a == 0 || b == 0;
-        // face_index of (a, b, cz) and (a, b, cz + 1), cz even: consecutive, even
-        if (plane) p = reinterpret_cast<const float2 *>(ft + face_index(a, b, cz));
-        zf[it] = ft[face_index(a, b, 0)];
+        if (plane) p = reinterpret_cast<const float2 *>(ft + cz);
+        zf[it] = *ft;
         zsel[it] = !plane && cz == 0;
       }
       v[it] = *p;
@@ -616,7 +623,9 @@ static int rows_r2c(hipStream_t s, int nx, int ny, int nz, int batch, const floa
                     const float2 *tw, const DivIn &dv = DivIn{nullptr, 0, 0, 0, FastDiv(1)},
                     const FaceCells &fc = FaceCells{nullptr, 0, 0, 0, 0, FastDiv(1)}) {
   JPM_REQUIRE(!fc.fb || (!dv.g3 && batch == 1 && nx % kTile == 0 && ny % kTile == 0 && nz % kTile == 0 &&
-                         fc.ny == uint32_t(ny) && fc.ty == uint32_t(ny / kTile) && fc.tz == uint32_t(nz / kTile)),
+                         fc.ny == uint32_t(ny) && fc.ty == uint32_t(ny / kTile) && fc.tz == uint32_t(nz / kTile) &&
+                         size_t(nx / kTile) * size_t(ny / kTile) * size_t(nz / kTile) * kFaceStride <
+                             (size_t(1) << 32)),
               "face cells apart need one mesh of full 16^3 tiles");
   const int M = nz / 2, nzh = M + 1, nrows = nx * ny;
   const size_t H = size_t(nx) * ny * nzh, Mc = size_t(nx) * ny * M;
