@@ -51,7 +51,7 @@ extern "C" {
 
 /* Pipeline flags (jpm_pm_forces_f32, jpm_force_*, jpm_batch_create): paint
  * flags in bits 0..7, k-space flags shifted left by 8, and the tile sort's
- * tuning in bits 16..25 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
+ * tuning in bits 16..26 -- JPM_TUNE_CHUNK(c): at most 2^(9+c) particles per
  * accumulation/gather chunk (c = 1..6, default 8192), JPM_TUNE_SLICE(s):
  * 2^(11+s) particles per counting/scatter workgroup (s = 1..7, default
  * 8192).  The choice changes speed, never results beyond float rounding;
@@ -62,6 +62,12 @@ extern "C" {
                                            read through the caches, not LDS */
 #define JPM_TUNE_XCD_SLABS (1u << 25)   /* tile workgroups placed so each XCD
                                            walks one contiguous run of tiles */
+#define JPM_TUNE_FACES_IN_MESH (1u << 26) /* jpm_force_kick*: the paint writes
+                                           its tiles' low-face cells into the
+                                           mesh instead of a buffer of their
+                                           own read by the potential solve
+                                           (absolute positions on meshes of
+                                           full 16^3 tiles); same bits */
 
 /* k-space flags */
 #define JPM_K_GRAD_EXACT 1u  /* gradient_kernel order=0 (i k, len//2 zeroed)

@@ -21,14 +21,17 @@ K_LAPL_FD = 2
 K_DECONV = 4
 
 
-def tune_flags(chunk_code=0, slice_code=0, rho_global=0, xcd_slabs=0):
+def tune_flags(chunk_code=0, slice_code=0, rho_global=0, xcd_slabs=0, faces_in_mesh=0):
     """Sort / gather tuning bits of the pipeline flags (JPM_TUNE_CHUNK,
-    JPM_TUNE_SLICE, JPM_TUNE_RHO_GLOBAL, JPM_TUNE_XCD_SLABS): chunk = 2^(9 +
-    chunk_code), slice = 2^(11 + slice_code), 0 = default; rho_global reads
-    the force-VJP density adjoint through the caches instead of LDS;
-    xcd_slabs gives each XCD one contiguous run of tiles."""
+    JPM_TUNE_SLICE, JPM_TUNE_RHO_GLOBAL, JPM_TUNE_XCD_SLABS,
+    JPM_TUNE_FACES_IN_MESH): chunk = 2^(9 + chunk_code), slice = 2^(11 +
+    slice_code), 0 = default; rho_global reads the force-VJP density adjoint
+    through the caches instead of LDS; xcd_slabs gives each XCD one
+    contiguous run of tiles; faces_in_mesh has the force evaluations' paint
+    write its tiles' face cells into the mesh instead of a buffer of their
+    own (the same bits)."""
     return ((int(chunk_code) << 16) | (int(slice_code) << 20) | (int(bool(rho_global)) << 24)
-            | (int(bool(xcd_slabs)) << 25))
+            | (int(bool(xcd_slabs)) << 25) | (int(bool(faces_in_mesh)) << 26))
 
 E_INVALID = -1
 

@@ -157,7 +157,7 @@ bool binned_paint_supported(int, int, int);
 // the mesh has more tiles than the binned paint's LDS histogram holds.
 // Pipeline flags: paint flags in bits 0..7, k-space flags in bits 8..15,
 // sort and gather tuning in bits 16..25 (which travel with the paint flags).
-static uint32_t paint_flags(uint32_t flags) { return flags & 0x03ff00ffu; }
+static uint32_t paint_flags(uint32_t flags) { return flags & 0x07ff00ffu; }
 static uint32_t kspace_flags(uint32_t flags) { return (flags >> 8) & 0xffu; }
 
 static bool use_atomic(int nx, int ny, int nz, uint32_t pflags) {
@@ -205,9 +205,10 @@ struct SortedWs {
 
 // Absolute positions on a mesh of full tiles that the library's FFT takes:
 // the paint hands its face cells to the potential solve's first pass in a
-// buffer of their own (FaceCells) instead of writing them into the mesh.
+// buffer of their own (FaceCells) instead of writing them into the mesh,
+// unless the flags ask for the mesh (JPM_TUNE_FACES_IN_MESH).
 static bool faces_apart(int nx, int ny, int nz, uint32_t pflags) {
-  return !(pflags & JPM_RELATIVE) && potential_faces_ok(nx, ny, nz);
+  return !(pflags & (JPM_RELATIVE | kTuneFacesInMesh)) && potential_faces_ok(nx, ny, nz);
 }
 
 static SortedWs sorted_ws(int nx, int ny, int nz, int64_t n, int nc, uint32_t pflags) {

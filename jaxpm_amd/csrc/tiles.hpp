@@ -18,7 +18,7 @@ constexpr uint32_t kSlice = 8192;               // default particles per count /
 // allow 4 workgroups per CU, so 512 threads keep 32 waves resident.
 constexpr int kWindowThreads = 512;
 
-// Sort tuning carried in bits 16..25 of the pipeline flags (0 = default):
+// Sort tuning carried in bits 16..26 of the pipeline flags (0 = default):
 // chunk = 2^(9 + c) particles for c = bits 16..19 in 1..6 (1024..32768),
 // slice = 2^(11 + s) particles for s = bits 20..23 in 1..7 (4096..262144).
 inline uint32_t tuned_chunk(uint32_t flags) {
@@ -35,6 +35,10 @@ inline bool tuned_rho_lds(uint32_t flags) { return !(flags & kTuneRhoGlobal); }
 // windows overlap, share that XCD's L2.
 constexpr uint32_t kTuneXcdSlabs = 1u << 25;
 inline bool tuned_slabs(uint32_t flags) { return (flags & kTuneXcdSlabs) != 0; }
+// Bit 26: the force evaluations' one-channel paint writes its finished
+// low-face cells into the mesh, not into a buffer of their own that the
+// potential solve's first pass reads (FaceCells); the same bits either way.
+constexpr uint32_t kTuneFacesInMesh = 1u << 26;
 
 // Workgroups for `items` work items and the per-XCD run length.
 inline unsigned slab_grid(uint32_t items, bool slabs, uint32_t &per) {

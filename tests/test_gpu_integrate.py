@@ -248,6 +248,35 @@ def test_tile_ordered_sweeps_give_the_same_bits(gpu, shape):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("shape", [(32, 32, 32), (48, 32, 64)])
+def test_face_cells_apart_give_the_same_bits(gpu, shape):
+    """The force evaluations' paint hands its tiles' low-face cells to the
+    potential solve in a buffer of their own (absolute positions, meshes of
+    full 16^3 tiles); JPM_TUNE_FACES_IN_MESH writes them into the mesh
+    instead.  A leapfrog and its adjoint give the same bits either way, with
+    a heavy tile split over several chunks (the reduce's general path)."""
+    from jaxpm_amd import cosmology as pc
+    from jaxpm_amd import integrate
+    x0, v0 = _state(shape, 43)
+    x0[:6000] = (10.0 + np.random.default_rng(44).normal(scale=0.8, size=(6000, 3))).astype(np.float32)
+    r = np.random.default_rng(45)
+    gx = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    gv = torch.as_tensor(r.normal(size=x0.shape).astype(np.float32), device="cuda")
+    sched = integrate.schedule_tensor(integrate.kdk_schedule(pc.Cosmology(), np.linspace(0.1, 1.0, 6)))
+    out = []
+    for faces_in_mesh in (0, 1):
+        print(f"shape {shape} faces_in_mesh {faces_in_mesh}", flush=True)
+        st = integrate.Stepper(shape, x0.shape[0], sort_tuning=(0, 0, 0, 0, faces_in_mesh))
+        xt = torch.as_tensor(x0, device="cuda").requires_grad_(True)
+        vt = torch.as_tensor(v0, device="cuda").requires_grad_(True)
+        x, v = integrate.NBody.apply(xt, vt, sched, st, 2)
+        ((x * gx).sum() + (v * gv).sum()).backward()
+        out.append([t.detach().cpu().numpy() for t in (x, v, xt.grad, vt.grad)])
+    for a, b in zip(out[0], out[1]):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b)
+
+
 def test_kick_drift_velocity_out_and_field_only(gpu):
     """jpm_force_kick_drift_f32 with vel_out gives the in-place call's bits
     and leaves vel unchanged; jpm_force_kick_f32 with a null vel leaves the
