@@ -90,16 +90,16 @@ __device__ __forceinline__ bool near_wrap_ok(const TileGrid &t) { return t.nx >=
 // wait on every load before issuing the next one.)  The launch has
 // kWindowThreads threads.  With edges of at least kPw cells the periodic wrap
 // is one compare-and-add per axis, without branches.
-template <int EDGE, int HALO, bool NEAR>
+template <int EDGE, int HALO, bool NEAR, int T = kWindowThreads>
 __device__ __forceinline__ void load_window_batched(const float *__restrict__ phi, const TileGrid &t, int ox, int oy,
                                                     int oz, float *win) {
   constexpr int CELLS = EDGE * EDGE * EDGE;
-  constexpr int PER = (CELLS + kWindowThreads - 1) / kWindowThreads;
+  constexpr int PER = (CELLS + T - 1) / T;
   const uint32_t ny = uint32_t(t.ny), nz = uint32_t(t.nz);
   float v[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     const int e = e0 < CELLS ? e0 : CELLS - 1;   // the last pass repeats a cell it does not store
     const int c = e % EDGE, ab = e / EDGE;
     const int b = ab % EDGE, a = ab / EDGE;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void load_window_batched(const float *__restrict__ ph
   }
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     if (e0 < CELLS) win[e0] = v[i];
   }
 }
@@ -169,13 +169,13 @@ __device__ __forceinline__ void load_window_pair(const float *__restrict__ a, fl
   }
 }
 
-template <int EDGE, int HALO>
+template <int EDGE, int HALO, int T = kWindowThreads>
 __device__ __forceinline__ void load_window_t(const float *__restrict__ phi, const TileGrid &t, int ox, int oy, int oz,
                                               float *win) {
   if (near_wrap_ok(t))
-    load_window_batched<EDGE, HALO, true>(phi, t, ox, oy, oz, win);
+    load_window_batched<EDGE, HALO, true, T>(phi, t, ox, oy, oz, win);
   else
-    load_window_batched<EDGE, HALO, false>(phi, t, ox, oy, oz, win);
+    load_window_batched<EDGE, HALO, false, T>(phi, t, ox, oy, oz, win);
 }
 
 constexpr float kC8 = 8.0f / 12.0f;
@@ -198,13 +198,14 @@ __device__ __forceinline__ void stencil_lds(const float *win, int a, int b, int 
 // while the stencils run once per cell instead of 24 times per particle.
 // Every thread computes its cells into registers; after a barrier (all
 // potential reads done) it stores them over the potential window; a second
-// barrier publishes them.  The launch has kWindowThreads threads.
+// barrier publishes them.  The launch has T threads.
+template <int T = kWindowThreads>
 __device__ __forceinline__ void build_force_window(float *lds) {
-  constexpr int PER = (kWinCells + kWindowThreads - 1) / kWindowThreads;
+  constexpr int PER = (kWinCells + T - 1) / T;
   float f[PER][3];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     const int e = e0 < kWinCells ? e0 : kWinCells - 1;
     const int c = e % kWin, ab = e / kWin;
     const int b = ab % kWin, a = ab / kWin;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void build_force_window(float *lds) {
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int e = int(threadIdx.x) + i * kWindowThreads;
+    const int e = int(threadIdx.x) + i * T;
     if (e < kWinCells) {
       lds[e] = f[i][0];
       lds[kWinCells + e] = f[i][1];
@@ -328,8 +329,15 @@ __device__ __forceinline__ bool tile_offset(float x, float y, float z, const Til
 // MODE 0 followed by the drift x[idx] = x + beta vel[idx], x being the
 // sorted copy of the positions the forces were computed at.  gamma is per
 // simulation (gamma_dev[sim]) when gamma_dev is given.
+// Threads of a kick workgroup, two workgroups per CU (59 KiB of LDS each):
+// 1024 (16 waves, at most 64 registers a thread) for absolute positions,
+// 768 (12 waves, at most 80 registers) for relative ones, whose corner
+// maths needs more registers.
+template <bool REL>
+constexpr int kick_threads() { return REL ? 768 : 1024; }
+
 template <bool REL, int MODE, class G>
-__global__ void __launch_bounds__(kWindowThreads)
+__global__ void __launch_bounds__(kick_threads<REL>(), REL ? 6 : 8)
 bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s, const float *__restrict__ gamma_dev,
                 float *__restrict__ vel, uint32_t per, DriftArgs dr) {
   __shared__ float win[3 * kWinCells];   // the potential window, then the force window
@@ -361,9 +369,9 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
     const float *v = vel + 3 * size_t(__float_as_uint(rec1.w));
     nv0 = v[0]; nv1 = v[1]; nv2 = v[2];
   }
-  load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
+  load_window_t<kPw, kHalo, kick_threads<REL>()>(phi, t, ox, oy, oz, win);
   __syncthreads();
-  build_force_window(win);
+  build_force_window<kick_threads<REL>()>(win);
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
     const float pv0 = nv0, pv1 = nv1, pv2 = nv2;
@@ -703,7 +711,7 @@ static int kick_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, float
     if (rc) return rc;
   }
   uint32_t per;
-  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(b.rel ? kick_threads<true>() : kick_threads<false>());
   JPM_TIMED(s, mode == 2 ? "bin_kick_kernel[drift]" : mode == 1 ? "bin_kick_kernel[forces]" : "bin_kick_kernel");
   auto kern = b.rel ? (mode == 1 ? bin_kick_kernel<true, 1, G> : mode == 2 ? bin_kick_kernel<true, 2, G>
                                                                           : bin_kick_kernel<true, 0, G>)
