@@ -125,11 +125,11 @@ __device__ __forceinline__ void load_window_batched(const float *__restrict__ ph
 // Two meshes' windows with all loads of a thread in flight together: a
 // over [o - HA, o - HA + EA)^3 into wa, b over [o - HB, o - HB + EB)^3
 // into wb, the same cells and order as two load_window_batched calls.
-template <int EA, int HA, int EB, int HB, bool NEAR>
+template <int EA, int HA, int EB, int HB, bool NEAR, int T = kWindowThreads>
 __device__ __forceinline__ void load_window_pair(const float *__restrict__ a, float *wa, const float *__restrict__ b,
                                                  float *wb, const TileGrid &t, int ox, int oy, int oz) {
   constexpr int CA = EA * EA * EA, CB = EB * EB * EB;
-  constexpr int PA = (CA + kWindowThreads - 1) / kWindowThreads, PB = (CB + kWindowThreads - 1) / kWindowThreads;
+  constexpr int PA = (CA + T - 1) / T, PB = (CB + T - 1) / T;
   const uint32_t ny = uint32_t(t.ny), nz = uint32_t(t.nz);
   auto cell = [&](int e, int EDGE, int HALO) {
     const int c = e % EDGE, ab = e / EDGE;
@@ -149,22 +149,22 @@ __device__ __forceinline__ void load_window_pair(const float *__restrict__ a, fl
   float va[PA], vb[PB];
 #pragma unroll
   for (int i = 0; i < PA; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     va[i] = a[cell(e0 < CA ? e0 : CA - 1, EA, HA)];
   }
 #pragma unroll
   for (int i = 0; i < PB; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     vb[i] = b[cell(e0 < CB ? e0 : CB - 1, EB, HB)];
   }
 #pragma unroll
   for (int i = 0; i < PA; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     if (e0 < CA) wa[e0] = va[i];
   }
 #pragma unroll
   for (int i = 0; i < PB; ++i) {
-    const int e0 = int(threadIdx.x) + i * kWindowThreads;
+    const int e0 = int(threadIdx.x) + i * T;
     if (e0 < CB) wb[e0] = vb[i];
   }
 }
@@ -482,15 +482,22 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
 // With cot_out (the same array as cot, so neither is restrict), the adjoint
 // of the drift that preceded the kick follows: cot[idx] += beta gpos[idx]
 // with the new gpos (beta per simulation, beta_dev[sim], when given).
+// Threads of a force-VJP workgroup, two workgroups per CU: 768 (12 waves,
+// at most 80 registers a thread) for absolute positions with the LDS density
+// window, 512 for the other forms, whose registers would not fit.
+template <bool REL, bool RLDS>
+constexpr int grad_threads() { return !REL && RLDS ? 768 : 512; }
+
 template <bool REL, class G, bool RLDS>
-__global__ void __launch_bounds__(kWindowThreads)
+__global__ void __launch_bounds__((grad_threads<REL, RLDS>()), ((grad_threads<REL, RLDS>()) == 768 ? 6 : 1))
 bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *__restrict__ rho_all,
                 const float *cot, float scale_s, const float *__restrict__ scale_dev,
                 float *__restrict__ gpos, uint32_t per, float beta_s, const float *__restrict__ beta_dev,
                 float *cot_out, float *__restrict__ cot_max) {
   __shared__ float win[3 * kWinCells];   // the potential window, then the force window
   __shared__ float rwin[RLDS ? kWinCells : 1];
-  __shared__ float wmax[kWindowThreads / 64];
+  constexpr int T = grad_threads<REL, RLDS>();
+  __shared__ float wmax[T / 64];
   const uint32_t c = gather_item(blockIdx.x, per, bins.slabs);
   if (c >= *bins.total_chunks) return;
   const ChunkDesc d = bins.desc[c];
@@ -525,14 +532,14 @@ bin_grad_kernel(BinsT<G> bins, const float *__restrict__ phi_all, const float *_
   }
   const bool near = near_wrap_ok(t);
   if (!RLDS)
-    load_window_t<kPw, kHalo>(phi, t, ox, oy, oz, win);
+    load_window_t<kPw, kHalo, T>(phi, t, ox, oy, oz, win);
   else if (near)
-    load_window_pair<kPw, kHalo, kWin, 0, true>(phi, win, rho_bar, rwin, t, ox, oy, oz);
+    load_window_pair<kPw, kHalo, kWin, 0, true, T>(phi, win, rho_bar, rwin, t, ox, oy, oz);
   else
-    load_window_pair<kPw, kHalo, kWin, 0, false>(phi, win, rho_bar, rwin, t, ox, oy, oz);
+    load_window_pair<kPw, kHalo, kWin, 0, false, T>(phi, win, rho_bar, rwin, t, ox, oy, oz);
   float cmax = 0.0f;
   __syncthreads();
-  build_force_window(win);
+  build_force_window<T>(win);
   for (uint32_t q = threadIdx.x; q < cnt; q += B) {
     const float4 rec = rec1;
     const uint32_t idx = __float_as_uint(rec.w);
@@ -736,7 +743,9 @@ static int grad_sorted(hipStream_t s, const BinsT<G> &b, const float *phi, const
     if (rc) return rc;
   }
   uint32_t per;
-  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(kWindowThreads);
+  const int threads = b.rel ? (rho_lds ? grad_threads<true, true>() : grad_threads<true, false>())
+                            : (rho_lds ? grad_threads<false, true>() : grad_threads<false, false>());
+  const dim3 grid(gather_grid(b.maxchunks, b.slabs, per)), block(threads);
   JPM_TIMED(s, cot_out ? "bin_grad_kernel[drift]" : "bin_grad_kernel");
   auto kern = b.rel ? (rho_lds ? bin_grad_kernel<true, G, true> : bin_grad_kernel<true, G, false>)
                     : (rho_lds ? bin_grad_kernel<false, G, true> : bin_grad_kernel<false, G, false>);
