@@ -475,10 +475,10 @@ bin_kick_kernel(BinsT<G> bins, const float *__restrict__ phi_all, float gamma_s,
 // gpos[idx] += scale * sum_d cot_d dread(F_d)/dx + dread(rho_bar)/dx, scale
 // per simulation (scale_dev[sim]) when scale_dev is given.
 // RLDS: the density-adjoint window in LDS (true) or read from the mesh
-// through the caches (false: 37 KiB of LDS per workgroup instead of 57 KiB).
-// With 512-thread workgroups both forms keep 4 waves per SIMD resident (two
-// workgroups per CU; the cache form is held there by its 84 VGPRs), so the
-// LDS window, which moves a fifth fewer bytes, is usually the faster.
+// through the caches (false: 37 KiB of LDS per workgroup instead of 77 KiB).
+// Both forms hold two workgroups per CU (the cache form by its registers);
+// the LDS window, which moves a fifth fewer bytes and runs in 768-thread
+// workgroups, is usually the faster.
 // With cot_out (the same array as cot, so neither is restrict), the adjoint
 // of the drift that preceded the kick follows: cot[idx] += beta gpos[idx]
 // with the new gpos (beta per simulation, beta_dev[sim], when given).
