@@ -850,7 +850,7 @@ def main():
                                    "slice": 2 ** (11 + tuning[1]) if tuning[1] else 8192,
                                    "vjp_density_window": "caches" if len(tuning) > 2 and tuning[2] else "lds",
                                    "xcd_placement": "slabs" if len(tuning) > 3 and tuning[3] else
-                                   "round-robin (accumulation), runs of 16 chunks (gathers)"}},
+                                   "round-robin (accumulation), runs of 32 chunks (gathers)"}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None if pmc_traffic is None else round(pmc_traffic),

@@ -54,15 +54,17 @@ __device__ __forceinline__ uint32_t slab_item(uint32_t b, uint32_t per, bool sla
 // XCD runs for the window gathers (without slabs): of each 8 R consecutive
 // workgroups b = 8 R k + 8 q + x (the dispatcher deals b to XCD x), chunk
 // 8 R k + R x + q, so each run of R consecutive chunks -- z-neighbouring
-// tiles when single-chunk, a z row of tiles at 256^3 -- runs on one XCD at
-// once, and the 64-byte lines that their potential windows' rows overlap in
-// (a window row of 21 floats spans three such lines) are fetched into its L2
-// once.  Runs still alternate over the XCDs, which keeps clustered tiles
-// spread over them (the slab placement, one contiguous eighth of the tiles
-// per XCD, does not).  At 256^3, against dispatch order: force+kick 0.239 ->
-// 0.229 ms, force VJP 0.293 -> 0.280 ms (runs of 2: 0.235 / 0.286; of 8:
-// 0.230 / 0.282; of 32: 0.227 / 0.280).
-constexpr uint32_t kXcdRun = 16;
+// tiles when single-chunk, two y-adjacent z rows of tiles at 256^3 -- runs
+// on one XCD at once, and the 64-byte lines that their potential windows'
+// rows overlap in (a window row of 21 floats spans three such lines) are
+// fetched into its L2 once.  Runs still alternate over the XCDs, which keeps
+// clustered tiles spread over them (the slab placement, one contiguous eighth
+// of the tiles per XCD, does not).  At 256^3 with 512-thread gathers, against
+// dispatch order: force+kick 0.239 -> 0.227 ms, force VJP 0.293 -> 0.280 ms
+// (runs of 2: 0.235 / 0.286; of 8: 0.230 / 0.282; of 16: 0.229 / 0.280);
+// with the 1024- and 768-thread gathers runs of 32 against 16 and 64:
+// 0.2174 / 0.2693 ms against 0.2184 / 0.2703 and 0.2199 / 0.2715.
+constexpr uint32_t kXcdRun = 32;
 inline unsigned gather_grid(uint32_t items, bool slabs, uint32_t &per) {
   const unsigned g = slab_grid(items, slabs, per);
   return slabs ? g : (g + 8u * kXcdRun - 1u) / (8u * kXcdRun) * (8u * kXcdRun);
