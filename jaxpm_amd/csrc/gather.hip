@@ -87,8 +87,8 @@ __device__ __forceinline__ bool near_wrap_ok(const TileGrid &t) { return t.nx >=
 // every load of a thread in flight together: the thread computes the
 // addresses of all its cells, loads them into registers, and only then
 // writes them to LDS.  (Writing each cell as it arrives made the compiler
-// wait on every load before issuing the next one.)  The launch has
-// kWindowThreads threads.  With edges of at least kPw cells the periodic wrap
+// wait on every load before issuing the next one.)  The launch has T
+// threads.  With edges of at least kPw cells the periodic wrap
 // is one compare-and-add per axis, without branches.
 template <int EDGE, int HALO, bool NEAR, int T = kWindowThreads>
 __device__ __forceinline__ void load_window_batched(const float *__restrict__ phi, const TileGrid &t, int ox, int oy,
